@@ -1,0 +1,653 @@
+// arch_kernels.hip -- CDNA4 kernels behind the lafem_hip_* C ABI: the device counterparts of
+// FEAT3's LAFEM::Arch::<Op><Mem_> statics (lafem/arch/apply.hpp:24-303 and siblings).
+//
+// Every entry point takes an explicit hipStream_t (as void*), raw device pointers and sizes,
+// launches asynchronously and returns 0 or a feat3 error code.  Host-returning reductions
+// (the *_value variants) synchronise the stream, as the reference's cuBLAS scalar returns
+// do; the plain variants leave their result in device memory (graph-capturable).
+
+#include "device_common.hpp"
+#include "runtime.hpp"
+#include "feat3_hip.h"
+
+#include <cmath>
+#include <limits>
+
+using namespace feat3_amd;
+using feat3_amd::dev::half;
+typedef std::uint32_t u32;
+typedef std::uint64_t u64;
+
+namespace
+{
+  constexpr int kBlock = 256;
+
+  inline unsigned int blocks_for(u64 work, int per_block = kBlock)
+  {
+    const u64 b = (work + u64(per_block) - 1) / u64(per_block);
+    return unsigned(b == 0 ? 1 : b);
+  }
+
+  // lanes per row for the CSR kernels: the smallest power of two >= the mean row length,
+  // clamped to [4, 64]; an explicit choice (autotuner) overrides it
+  int lanes_for(u64 rows, u64 nnz)
+  {
+    const int forced = rt::csr_lanes_override();
+    if(forced > 0) return forced;
+    const double mean = rows ? double(nnz) / double(rows) : 1.0;
+    int g = 4;
+    while(g < 64 && double(g) < mean) g *= 2;
+    return g;
+  }
+
+  // -------------------------------------------------------------------------------------
+  // CSR: r = a*A*x + b*y (apply_generic.hpp:24-69); G lanes per row, group reduction
+  // -------------------------------------------------------------------------------------
+  template<int G, typename DT, typename VT, typename IT>
+  __global__ __launch_bounds__(kBlock) void k_csr(DT* __restrict__ r, DT a, const DT* __restrict__ x, DT b,
+                                                  const DT* y, const VT* __restrict__ val, const IT* __restrict__ col,
+                                                  const IT* __restrict__ rp, u64 rows, int use_y)
+  {
+    const unsigned lb = dev::xcd_remap(blockIdx.x, gridDim.x);
+    const u64 row = (u64(lb) * kBlock + threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    if(row >= rows) return;
+    const IT beg = rp[row], end = rp[row + 1];
+    DT sum = DT(0);
+    for(IT i = beg + IT(lane); i < end; i += IT(G))
+      sum += dev::ld<DT>(val, i) * x[col[i]];
+    sum = dev::group_sum<G>(sum);
+    if(lane == 0)
+      r[row] = use_y ? (sum * a) + (b * y[row]) : (sum * a);
+  }
+
+  template<typename DT, typename VT, typename IT>
+  int launch_csr(hipStream_t s, DT* r, DT a, const DT* x, DT b, const DT* y, const VT* val, const IT* col,
+                 const IT* rp, u64 rows, u64 nnz)
+  {
+    if(rows == 0) return 0;
+    const int use_y = std::abs(double(b)) < double(std::numeric_limits<DT>::epsilon()) ? 0 : 1;
+    const int g = lanes_for(rows, nnz);
+    const unsigned nb = blocks_for(rows * u64(g));
+#define F3_CSR_CASE(GG) case GG: k_csr<GG, DT, VT, IT><<<nb, kBlock, 0, s>>>(r, a, x, b, y, val, col, rp, rows, use_y); break;
+    switch(g)
+    {
+      F3_CSR_CASE(4) F3_CSR_CASE(8) F3_CSR_CASE(16) F3_CSR_CASE(32) F3_CSR_CASE(64)
+      default: return rt::kErrArg;
+    }
+#undef F3_CSR_CASE
+    return rt::check_launch();
+  }
+
+  // transposed CSR: r = a*(b/a*y + A^T x) with atomic scatter (apply_generic.hpp:39-55)
+  template<typename DT>
+  __global__ void k_scale_copy(DT* r, const DT* y, DT s, int use_y, u64 n)
+  {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i < n) r[i] = use_y ? s * y[i] : DT(0);
+  }
+  template<typename DT, typename VT, typename IT>
+  __global__ void k_csr_scatter(DT* r, const DT* x, const VT* val, const IT* col, const IT* rp, u64 rows)
+  {
+    const u64 row = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(row >= rows) return;
+    const DT xr = x[row];
+    for(IT i = rp[row]; i < rp[row + 1]; ++i)
+      atomicAdd(&r[col[i]], dev::ld<DT>(val, i) * xr);
+  }
+  template<typename DT>
+  __global__ void k_scale_inplace(DT* r, DT s, u64 n)
+  {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i < n) r[i] = s * r[i];
+  }
+
+  template<typename DT, typename VT, typename IT>
+  int launch_csr_t(hipStream_t s, DT* r, DT a, const DT* x, DT b, const DT* y, const VT* val, const IT* col,
+                   const IT* rp, u64 rows, u64 cols)
+  {
+    const int use_y = std::abs(double(b)) < double(std::numeric_limits<DT>::epsilon()) ? 0 : 1;
+    k_scale_copy<DT><<<blocks_for(cols), kBlock, 0, s>>>(r, y, b / a, use_y, cols);
+    k_csr_scatter<DT, VT, IT><<<blocks_for(rows), kBlock, 0, s>>>(r, x, val, col, rp, rows);
+    k_scale_inplace<DT><<<blocks_for(cols), kBlock, 0, s>>>(r, a, cols);
+    return rt::check_launch();
+  }
+
+  // -------------------------------------------------------------------------------------
+  // SELL-C (FEAT's SparseMatrixELL layout, sparse_matrix_ell.hpp:41-70,865-960) and its
+  // sigma-sorted variant: one lane per row, chunk-column-major val/col (coalesced).
+  // perm == nullptr: position p is row p (sigma = 1).  rl != nullptr: skip row padding.
+  // -------------------------------------------------------------------------------------
+  template<typename DT, typename VT, typename IT>
+  __global__ __launch_bounds__(kBlock) void k_sell(DT* __restrict__ r, DT a, const DT* __restrict__ x, DT b,
+                                                   const DT* y, const VT* __restrict__ val, const IT* __restrict__ col,
+                                                   const IT* __restrict__ cs, const IT* __restrict__ cl,
+                                                   const IT* __restrict__ rl, const IT* __restrict__ perm,
+                                                   u64 C, u64 rows, int use_y)
+  {
+    const unsigned lb = dev::xcd_remap(blockIdx.x, gridDim.x);
+    const u64 p = u64(lb) * kBlock + threadIdx.x;
+    if(p >= rows) return;
+    const u64 chunk = p / C, loc = p % C;
+    const u64 base = u64(cs[chunk]) + loc;
+    const u64 len = rl ? u64(rl[p]) : u64(cl[chunk]);
+    DT sum = DT(0);
+#pragma unroll 4
+    for(u64 k = 0; k < len; ++k)
+      sum += dev::ld<DT>(val, base + k * C) * x[col[base + k * C]];
+    const u64 row = perm ? u64(perm[p]) : p;
+    r[row] = use_y ? (sum * a) + (b * y[row]) : (sum * a);
+  }
+
+  // -------------------------------------------------------------------------------------
+  // BCSR with BH x BW row-major blocks (apply_generic.hpp:121-154): G lanes per block row
+  // -------------------------------------------------------------------------------------
+  template<int G, int BH, int BW, typename DT, typename IT>
+  __global__ __launch_bounds__(kBlock) void k_csrb(DT* __restrict__ r, DT a, const DT* __restrict__ x, DT b,
+                                                   const DT* y, const DT* __restrict__ val, const IT* __restrict__ col,
+                                                   const IT* __restrict__ rp, u64 rows, int use_y)
+  {
+    const unsigned lb = dev::xcd_remap(blockIdx.x, gridDim.x);
+    const u64 row = (u64(lb) * kBlock + threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    if(row >= rows) return;
+    DT s[BH];
+#pragma unroll
+    for(int h = 0; h < BH; ++h) s[h] = DT(0);
+    for(IT i = rp[row] + IT(lane); i < rp[row + 1]; i += IT(G))
+    {
+      const DT* v = val + u64(i) * (BH * BW);
+      const DT* xb = x + u64(col[i]) * BW;
+      DT xv[BW];
+#pragma unroll
+      for(int w = 0; w < BW; ++w) xv[w] = xb[w];
+#pragma unroll
+      for(int h = 0; h < BH; ++h)
+#pragma unroll
+        for(int w = 0; w < BW; ++w)
+          s[h] += v[h * BW + w] * xv[w];
+    }
+#pragma unroll
+    for(int h = 0; h < BH; ++h)
+      s[h] = dev::group_sum<G>(s[h]);
+    if(lane == 0)
+    {
+#pragma unroll
+      for(int h = 0; h < BH; ++h)
+        r[row * BH + h] = use_y ? (s[h] * a) + (b * y[row * BH + h]) : (s[h] * a);
+    }
+  }
+
+  template<int BH, int BW, typename DT, typename IT>
+  int launch_csrb(hipStream_t s, DT* r, DT a, const DT* x, DT b, const DT* y, const DT* val, const IT* col,
+                  const IT* rp, u64 rows, u64 nnzb)
+  {
+    if(rows == 0) return 0;
+    const int use_y = std::abs(double(b)) < double(std::numeric_limits<DT>::epsilon()) ? 0 : 1;
+    const int g = lanes_for(rows, nnzb);
+    const unsigned nb = blocks_for(rows * u64(g));
+#define F3_CSRB_CASE(GG) case GG: k_csrb<GG, BH, BW, DT, IT><<<nb, kBlock, 0, s>>>(r, a, x, b, y, val, col, rp, rows, use_y); break;
+    switch(g)
+    {
+      F3_CSRB_CASE(4) F3_CSRB_CASE(8) F3_CSRB_CASE(16) F3_CSRB_CASE(32) F3_CSRB_CASE(64)
+      default: return rt::kErrArg;
+    }
+#undef F3_CSRB_CASE
+    return rt::check_launch();
+  }
+
+  // scalar CSR times a vector of BS-blocks (csrsb, apply_generic.hpp:156-181)
+  template<int G, int BS, typename DT, typename VT, typename IT>
+  __global__ __launch_bounds__(kBlock) void k_csrsb(DT* __restrict__ r, DT a, const DT* __restrict__ x, DT b,
+                                                    const DT* y, const VT* __restrict__ val, const IT* __restrict__ col,
+                                                    const IT* __restrict__ rp, u64 rows, int use_y)
+  {
+    const unsigned lb = dev::xcd_remap(blockIdx.x, gridDim.x);
+    const u64 row = (u64(lb) * kBlock + threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    if(row >= rows) return;
+    DT s[BS];
+#pragma unroll
+    for(int h = 0; h < BS; ++h) s[h] = DT(0);
+    for(IT i = rp[row] + IT(lane); i < rp[row + 1]; i += IT(G))
+    {
+      const DT v = dev::ld<DT>(val, i);
+      const DT* xb = x + u64(col[i]) * BS;
+#pragma unroll
+      for(int h = 0; h < BS; ++h) s[h] += v * xb[h];
+    }
+#pragma unroll
+    for(int h = 0; h < BS; ++h) s[h] = dev::group_sum<G>(s[h]);
+    if(lane == 0)
+    {
+#pragma unroll
+      for(int h = 0; h < BS; ++h)
+        r[row * BS + h] = use_y ? (s[h] * a) + (b * y[row * BS + h]) : (s[h] * a);
+    }
+  }
+
+  template<int BS, typename DT, typename VT, typename IT>
+  int launch_csrsb(hipStream_t s, DT* r, DT a, const DT* x, DT b, const DT* y, const VT* val, const IT* col,
+                   const IT* rp, u64 rows, u64 nnz)
+  {
+    if(rows == 0) return 0;
+    const int use_y = std::abs(double(b)) < double(std::numeric_limits<DT>::epsilon()) ? 0 : 1;
+    const int g = lanes_for(rows, nnz);
+    const unsigned nb = blocks_for(rows * u64(g));
+#define F3_CSRSB_CASE(GG) case GG: k_csrsb<GG, BS, DT, VT, IT><<<nb, kBlock, 0, s>>>(r, a, x, b, y, val, col, rp, rows, use_y); break;
+    switch(g)
+    {
+      F3_CSRSB_CASE(4) F3_CSRSB_CASE(8) F3_CSRSB_CASE(16) F3_CSRSB_CASE(32) F3_CSRSB_CASE(64)
+      default: return rt::kErrArg;
+    }
+#undef F3_CSRSB_CASE
+    return rt::check_launch();
+  }
+
+  // -------------------------------------------------------------------------------------
+  // BLAS-1 and friends (axpy_generic.hpp, component_*_generic.hpp, scale_generic.hpp)
+  // -------------------------------------------------------------------------------------
+  template<typename DT>
+  __global__ void k_axpy(DT* r, DT a, const DT* x, const DT* y, u64 n)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      r[i] = (a * x[i]) + y[i];
+  }
+  template<typename DT>
+  __global__ void k_cprod(DT* r, const DT* x, const DT* y, u64 n)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      r[i] = x[i] * y[i];
+  }
+  template<typename DT>
+  __global__ void k_cinv(DT* r, const DT* x, DT s, u64 n)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      r[i] = s / x[i];
+  }
+  template<typename DT>
+  __global__ void k_scale(DT* r, const DT* x, DT s, u64 n)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      r[i] = x[i] * s;
+  }
+  template<typename T>
+  __global__ void k_set(T* r, T v, u64 n)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      r[i] = v;
+  }
+  template<typename DO, typename DI>
+  __global__ void k_convert(DO* r, const DI* x, u64 n)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      r[i] = DO(x[i]);
+  }
+
+  inline unsigned stream_blocks(u64 n)
+  {
+    const u64 b = (n + kBlock - 1) / kBlock;
+    return unsigned(std::max<u64>(1, std::min<u64>(b, 8192)));
+  }
+
+  // dot / squared norm: deterministic grid reduction into a device scalar
+  template<typename DT, bool SQRT>
+  __global__ __launch_bounds__(kBlock) void k_dot(const DT* x, const DT* y, u64 n, DT* out, DT* partials, unsigned* ticket)
+  {
+    DT s = DT(0);
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      s += x[i] * y[i];
+    dev::grid_reduce(s, partials, ticket, [&](DT t) { *out = SQRT ? DT(sqrt(t)) : t; });
+  }
+
+  template<typename DT, bool SQRT>
+  int launch_dot(hipStream_t s, const DT* x, const DT* y, u64 n, DT* out)
+  {
+    rt::Workspace* ws = rt::workspace();
+    if(!ws) return rt::kErrRuntime;
+    const unsigned nb = unsigned(std::min<u64>(stream_blocks(n), rt::Workspace::kMaxPartials));
+    k_dot<DT, SQRT><<<nb, kBlock, 0, s>>>(x, y, n, out, reinterpret_cast<DT*>(ws->partials), ws->ticket);
+    return rt::check_launch();
+  }
+
+  template<typename DT, bool SQRT>
+  int dot_value(hipStream_t s, const DT* x, const DT* y, u64 n, DT* host_out)
+  {
+    rt::Workspace* ws = rt::workspace();
+    if(!ws) return rt::kErrRuntime;
+    DT* dres = reinterpret_cast<DT*>(ws->scalar);
+    int rc = launch_dot<DT, SQRT>(s, x, y, n, dres);
+    if(rc) return rc;
+    if(hipMemcpyAsync(host_out, dres, sizeof(DT), hipMemcpyDeviceToHost, s) != hipSuccess) return rt::kErrRuntime;
+    if(hipStreamSynchronize(s) != hipSuccess) return rt::kErrRuntime;
+    return 0;
+  }
+
+  // diagonal extraction (diagonal_generic.hpp:20-81)
+  template<typename DT, typename IT>
+  __global__ void k_diag_csr(DT* d, const DT* val, const IT* col, const IT* rp, u64 rows)
+  {
+    const u64 row = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(row >= rows) return;
+    DT v = DT(0);
+    for(IT i = rp[row]; i < rp[row + 1]; ++i)
+      if(u64(col[i]) == row) { v = val[i]; break; }
+    d[row] = v;
+  }
+  template<typename DT, typename IT>
+  __global__ void k_diag_ell(DT* d, const DT* val, const IT* col, const IT* cs, u64 C, u64 rows)
+  {
+    const u64 row = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(row >= rows) return;
+    const u64 chunk = row / C;
+    DT v = DT(0);
+    for(u64 p = u64(cs[chunk]) + row % C; p < u64(cs[chunk + 1]); p += C)
+      if(u64(col[p]) == row) { v = val[p]; break; }
+    d[row] = v;
+  }
+  template<typename DT, typename IT>
+  __global__ void k_diag_csrb(DT* d, const DT* val, const IT* col, const IT* rp, u64 rows, int bh, int bw)
+  {
+    const u64 row = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(row >= rows) return;
+    for(int i = 0; i < bh; ++i) d[row * bh + i] = DT(0);
+    for(IT c = rp[row]; c < rp[row + 1]; ++c)
+      if(u64(col[c]) == row)
+        for(int i = 0; i < bh; ++i) d[row * bh + i] = val[u64(c) * bh * bw + i + i * bw];
+  }
+
+  // unit filter (unit_filter_generic.hpp:21-37, unit_filter_blocked_generic.hpp)
+  template<typename DT, typename IT>
+  __global__ void k_filter_rhs(DT* v, const DT* sv, const IT* idx, u64 ue, int bs)
+  {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i >= ue * u64(bs)) return;
+    const u64 k = i / bs, c = i % bs;
+    v[u64(idx[k]) * bs + c] = sv[k * bs + c];
+  }
+  template<typename DT, typename IT>
+  __global__ void k_filter_def(DT* v, const IT* idx, u64 ue, int bs)
+  {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i >= ue * u64(bs)) return;
+    v[u64(idx[i / bs]) * bs + i % bs] = DT(0);
+  }
+} // namespace
+
+// =========================================================================================
+// C ABI
+// =========================================================================================
+#define S_ static_cast<hipStream_t>(stream)
+#define F3_EPS(DT) std::numeric_limits<DT>::epsilon()
+
+extern "C" {
+
+// ---- Apply<Mem::HIP>::csr  (apply.hpp:271-274) ----------------------------------------
+#define F3_DEF_CSR(SUF, DT, VT, IT)                                                                          \
+  int lafem_hip_apply_csr_##SUF(void* stream, DT* r, DT a, const DT* x, DT b, const DT* y, const VT* val,      \
+                                const IT* col_ind, const IT* row_ptr, u64 rows, u64 columns, u64 used_elements, \
+                                int transposed)                                                               \
+  {                                                                                                           \
+    if(!r || !row_ptr || (used_elements > 0 && (!x || !val || !col_ind))) return rt::kErrArg;                 \
+    if(transposed) return launch_csr_t<DT, VT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, columns);  \
+    return launch_csr<DT, VT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements);             \
+  }
+F3_DEF_CSR(f64_u32, double, double, u32)
+F3_DEF_CSR(f64_u64, double, double, u64)
+F3_DEF_CSR(f32_u32, float, float, u32)
+F3_DEF_CSR(f32_u64, float, float, u64)
+#undef F3_DEF_CSR
+
+int lafem_hip_apply_csr_h16f32_u32(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                   const void* val, const u32* col_ind, const u32* row_ptr, u64 rows, u64 columns,
+                                   u64 used_elements)
+{
+  if(!r || !row_ptr || (used_elements > 0 && (!x || !val || !col_ind))) return rt::kErrArg;
+  (void)columns;
+  return launch_csr<float, half, u32>(S_, r, a, x, b, y, static_cast<const half*>(val), col_ind, row_ptr, rows, used_elements);
+}
+
+// ---- Apply<Mem::HIP>::ell  (apply.hpp:296) and the sigma-sorted SELL variant -------------
+#define F3_DEF_ELL(SUF, DT, IT)                                                                              \
+  int lafem_hip_apply_ell_##SUF(void* stream, DT* r, DT a, const DT* x, DT b, const DT* y, const DT* val,      \
+                                const IT* col_ind, const IT* cs, const IT* cl, u64 C, u64 rows)               \
+  {                                                                                                           \
+    if(!r || !x || !val || !col_ind || !cs || !cl || C == 0) return rt::kErrArg;                              \
+    if(rows == 0) return 0;                                                                                   \
+    const int use_y = std::abs(double(b)) < double(F3_EPS(DT)) ? 0 : 1;                                       \
+    k_sell<DT, DT, IT><<<blocks_for(rows), kBlock, 0, S_>>>(r, a, x, b, y, val, col_ind, cs, cl, nullptr,    \
+                                                            nullptr, C, rows, use_y);                         \
+    return rt::check_launch();                                                                                \
+  }                                                                                                           \
+  int lafem_hip_apply_sell_##SUF(void* stream, DT* r, DT a, const DT* x, DT b, const DT* y, const DT* val,     \
+                                 const IT* col_ind, const IT* cs, const IT* cl, const IT* rl, const IT* perm,  \
+                                 u64 C, u64 rows)                                                             \
+  {                                                                                                           \
+    if(!r || !x || !val || !col_ind || !cs || !cl || C == 0) return rt::kErrArg;                              \
+    if(rows == 0) return 0;                                                                                   \
+    const int use_y = std::abs(double(b)) < double(F3_EPS(DT)) ? 0 : 1;                                       \
+    k_sell<DT, DT, IT><<<blocks_for(rows), kBlock, 0, S_>>>(r, a, x, b, y, val, col_ind, cs, cl, rl, perm, C, \
+                                                            rows, use_y);                                     \
+    return rt::check_launch();                                                                                \
+  }
+F3_DEF_ELL(f64_u32, double, u32)
+F3_DEF_ELL(f64_u64, double, u64)
+F3_DEF_ELL(f32_u32, float, u32)
+F3_DEF_ELL(f32_u64, float, u64)
+#undef F3_DEF_ELL
+
+int lafem_hip_apply_sell_h16f32_u32(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                    const void* val, const u32* col_ind, const u32* cs, const u32* cl,
+                                    const u32* rl, const u32* perm, u64 C, u64 rows)
+{
+  if(!r || !x || !val || !col_ind || !cs || !cl || C == 0) return rt::kErrArg;
+  if(rows == 0) return 0;
+  const int use_y = std::abs(b) < F3_EPS(float) ? 0 : 1;
+  k_sell<float, half, u32><<<blocks_for(rows), kBlock, 0, S_>>>(r, a, x, b, y, static_cast<const half*>(val), col_ind,
+                                                                cs, cl, rl, perm, C, rows, use_y);
+  return rt::check_launch();
+}
+
+// ---- Apply<Mem::HIP>::csrb  (apply.hpp:277) --------------------------------------------
+#define F3_DEF_CSRB(SUF, DT, IT)                                                                             \
+  int lafem_hip_apply_csrb_##SUF(void* stream, DT* r, DT a, const DT* x, DT b, const DT* y, const DT* val,     \
+                                 const IT* col_ind, const IT* row_ptr, u64 rows, u64 columns,                 \
+                                 u64 used_elements, int block_height, int block_width)                        \
+  {                                                                                                           \
+    (void)columns;                                                                                            \
+    if(!r || !row_ptr || (used_elements > 0 && (!x || !val || !col_ind))) return rt::kErrArg;                 \
+    if(block_height != block_width) return rt::kErrArg;                                                       \
+    switch(block_height)                                                                                      \
+    {                                                                                                         \
+      case 1: return launch_csrb<1, 1, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
+      case 2: return launch_csrb<2, 2, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
+      case 3: return launch_csrb<3, 3, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
+      case 4: return launch_csrb<4, 4, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
+      default: return rt::kErrArg;                                                                            \
+    }                                                                                                         \
+  }
+F3_DEF_CSRB(f64_u32, double, u32)
+F3_DEF_CSRB(f64_u64, double, u64)
+F3_DEF_CSRB(f32_u32, float, u32)
+F3_DEF_CSRB(f32_u64, float, u64)
+#undef F3_DEF_CSRB
+
+// ---- Apply<Mem::HIP>::csrsb  (apply.hpp:283) -------------------------------------------
+#define F3_DEF_CSRSB(SUF, DT, IT)                                                                            \
+  int lafem_hip_apply_csrsb_##SUF(void* stream, DT* r, DT a, const DT* x, DT b, const DT* y, const DT* val,    \
+                                  const IT* col_ind, const IT* row_ptr, u64 rows, u64 columns,                \
+                                  u64 used_elements, int block_size)                                          \
+  {                                                                                                           \
+    (void)columns;                                                                                            \
+    if(!r || !row_ptr || (used_elements > 0 && (!x || !val || !col_ind))) return rt::kErrArg;                 \
+    switch(block_size)                                                                                        \
+    {                                                                                                         \
+      case 1: return launch_csrsb<1, DT, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
+      case 2: return launch_csrsb<2, DT, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
+      case 3: return launch_csrsb<3, DT, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
+      default: return rt::kErrArg;                                                                            \
+    }                                                                                                         \
+  }
+F3_DEF_CSRSB(f64_u32, double, u32)
+F3_DEF_CSRSB(f64_u64, double, u64)
+F3_DEF_CSRSB(f32_u32, float, u32)
+F3_DEF_CSRSB(f32_u64, float, u64)
+#undef F3_DEF_CSRSB
+
+// ---- BLAS-1 ---------------------------------------------------------------------------
+#define F3_DEF_BLAS1(SUF, DT)                                                                                \
+  int lafem_hip_axpy_##SUF(void* stream, DT* r, DT a, const DT* x, const DT* y, u64 n)                        \
+  {                                                                                                           \
+    if(n == 0) return 0;                                                                                      \
+    if(!r || !x || !y) return rt::kErrArg;                                                                    \
+    k_axpy<DT><<<stream_blocks(n), kBlock, 0, S_>>>(r, a, x, y, n);                                           \
+    return rt::check_launch();                                                                                \
+  }                                                                                                           \
+  int lafem_hip_dot_##SUF(void* stream, const DT* x, const DT* y, u64 n, DT* result_dev)                      \
+  {                                                                                                           \
+    if(!result_dev || (n > 0 && (!x || !y))) return rt::kErrArg;                                              \
+    return launch_dot<DT, false>(S_, x, y, n, result_dev);                                                    \
+  }                                                                                                           \
+  int lafem_hip_dot_value_##SUF(void* stream, const DT* x, const DT* y, u64 n, DT* result_host)               \
+  {                                                                                                           \
+    if(!result_host || (n > 0 && (!x || !y))) return rt::kErrArg;                                             \
+    return dot_value<DT, false>(S_, x, y, n, result_host);                                                    \
+  }                                                                                                           \
+  int lafem_hip_norm2_##SUF(void* stream, const DT* x, u64 n, DT* result_dev)                                 \
+  {                                                                                                           \
+    if(!result_dev || (n > 0 && !x)) return rt::kErrArg;                                                      \
+    return launch_dot<DT, true>(S_, x, x, n, result_dev);                                                     \
+  }                                                                                                           \
+  int lafem_hip_norm2_value_##SUF(void* stream, const DT* x, u64 n, DT* result_host)                          \
+  {                                                                                                           \
+    if(!result_host || (n > 0 && !x)) return rt::kErrArg;                                                     \
+    return dot_value<DT, true>(S_, x, x, n, result_host);                                                     \
+  }                                                                                                           \
+  int lafem_hip_component_product_##SUF(void* stream, DT* r, const DT* x, const DT* y, u64 n)                 \
+  {                                                                                                           \
+    if(n == 0) return 0;                                                                                      \
+    if(!r || !x || !y) return rt::kErrArg;                                                                    \
+    k_cprod<DT><<<stream_blocks(n), kBlock, 0, S_>>>(r, x, y, n);                                             \
+    return rt::check_launch();                                                                                \
+  }                                                                                                           \
+  int lafem_hip_component_invert_##SUF(void* stream, DT* r, const DT* x, DT s, u64 n)                         \
+  {                                                                                                           \
+    if(n == 0) return 0;                                                                                      \
+    if(!r || !x) return rt::kErrArg;                                                                          \
+    k_cinv<DT><<<stream_blocks(n), kBlock, 0, S_>>>(r, x, s, n);                                              \
+    return rt::check_launch();                                                                                \
+  }                                                                                                           \
+  int lafem_hip_scale_##SUF(void* stream, DT* r, const DT* x, DT s, u64 n)                                    \
+  {                                                                                                           \
+    if(n == 0) return 0;                                                                                      \
+    if(!r || !x) return rt::kErrArg;                                                                          \
+    k_scale<DT><<<stream_blocks(n), kBlock, 0, S_>>>(r, x, s, n);                                             \
+    return rt::check_launch();                                                                                \
+  }
+F3_DEF_BLAS1(f64, double)
+F3_DEF_BLAS1(f32, float)
+#undef F3_DEF_BLAS1
+
+// ---- Diagonal<Mem::HIP> -----------------------------------------------------------------
+#define F3_DEF_DIAG(SUF, DT, IT)                                                                             \
+  int lafem_hip_diagonal_csr_##SUF(void* stream, DT* diag, const DT* val, const IT* col_ind, const IT* row_ptr, \
+                                   u64 rows)                                                                  \
+  {                                                                                                           \
+    if(rows == 0) return 0;                                                                                   \
+    if(!diag || !val || !col_ind || !row_ptr) return rt::kErrArg;                                             \
+    k_diag_csr<DT, IT><<<blocks_for(rows), kBlock, 0, S_>>>(diag, val, col_ind, row_ptr, rows);               \
+    return rt::check_launch();                                                                                \
+  }                                                                                                           \
+  int lafem_hip_diagonal_ell_##SUF(void* stream, DT* diag, const DT* val, const IT* col_ind, const IT* cs,    \
+                                   const IT* cl, u64 C, u64 rows)                                             \
+  {                                                                                                           \
+    (void)cl;                                                                                                 \
+    if(rows == 0) return 0;                                                                                   \
+    if(!diag || !val || !col_ind || !cs || C == 0) return rt::kErrArg;                                        \
+    k_diag_ell<DT, IT><<<blocks_for(rows), kBlock, 0, S_>>>(diag, val, col_ind, cs, C, rows);                 \
+    return rt::check_launch();                                                                                \
+  }                                                                                                           \
+  int lafem_hip_diagonal_csrb_##SUF(void* stream, DT* diag, const DT* val, const IT* col_ind,                 \
+                                    const IT* row_ptr, u64 rows, int block_height, int block_width)           \
+  {                                                                                                           \
+    if(rows == 0) return 0;                                                                                   \
+    if(!diag || !val || !col_ind || !row_ptr || block_height < 1 || block_height > 16) return rt::kErrArg;    \
+    k_diag_csrb<DT, IT><<<blocks_for(rows), kBlock, 0, S_>>>(diag, val, col_ind, row_ptr, rows, block_height, \
+                                                             block_width);                                    \
+    return rt::check_launch();                                                                                \
+  }
+F3_DEF_DIAG(f64_u32, double, u32)
+F3_DEF_DIAG(f64_u64, double, u64)
+F3_DEF_DIAG(f32_u32, float, u32)
+F3_DEF_DIAG(f32_u64, float, u64)
+#undef F3_DEF_DIAG
+
+// ---- UnitFilter<Mem::HIP> / UnitFilterBlocked<Mem::HIP> -------------------------------
+#define F3_DEF_FILTER(SUF, DT, IT)                                                                           \
+  int lafem_hip_unit_filter_rhs_##SUF(void* stream, DT* v, const DT* sv_elements, const IT* sv_indices, u64 ue, \
+                                      int block_size)                                                         \
+  {                                                                                                           \
+    if(ue == 0) return 0;                                                                                     \
+    if(!v || !sv_elements || !sv_indices || block_size < 1) return rt::kErrArg;                               \
+    k_filter_rhs<DT, IT><<<blocks_for(ue * u64(block_size)), kBlock, 0, S_>>>(v, sv_elements, sv_indices, ue, \
+                                                                              block_size);                    \
+    return rt::check_launch();                                                                                \
+  }                                                                                                           \
+  int lafem_hip_unit_filter_def_##SUF(void* stream, DT* v, const IT* sv_indices, u64 ue, int block_size)      \
+  {                                                                                                           \
+    if(ue == 0) return 0;                                                                                     \
+    if(!v || !sv_indices || block_size < 1) return rt::kErrArg;                                               \
+    k_filter_def<DT, IT><<<blocks_for(ue * u64(block_size)), kBlock, 0, S_>>>(v, sv_indices, ue, block_size); \
+    return rt::check_launch();                                                                                \
+  }
+F3_DEF_FILTER(f64_u32, double, u32)
+F3_DEF_FILTER(f64_u64, double, u64)
+F3_DEF_FILTER(f32_u32, float, u32)
+F3_DEF_FILTER(f32_u64, float, u64)
+#undef F3_DEF_FILTER
+
+// ---- MemoryPool<Mem::HIP>::set_memory / convert  (memory_pool.cu:26-41,176-227) --------
+#define F3_DEF_SET(SUF, T)                                                                                   \
+  int lafem_hip_set_memory_##SUF(void* stream, T* dst, T value, u64 n)                                        \
+  {                                                                                                           \
+    if(n == 0) return 0;                                                                                      \
+    if(!dst) return rt::kErrArg;                                                                              \
+    k_set<T><<<stream_blocks(n), kBlock, 0, S_>>>(dst, value, n);                                             \
+    return rt::check_launch();                                                                                \
+  }
+F3_DEF_SET(f64, double)
+F3_DEF_SET(f32, float)
+F3_DEF_SET(u32, u32)
+F3_DEF_SET(u64, u64)
+#undef F3_DEF_SET
+
+#define F3_DEF_CONV(SUF, DO, DI)                                                                             \
+  int lafem_hip_convert_##SUF(void* stream, DO* dst, const DI* src, u64 n)                                    \
+  {                                                                                                           \
+    if(n == 0) return 0;                                                                                      \
+    if(!dst || !src) return rt::kErrArg;                                                                      \
+    k_convert<DO, DI><<<stream_blocks(n), kBlock, 0, S_>>>(dst, src, n);                                      \
+    return rt::check_launch();                                                                                \
+  }
+F3_DEF_CONV(f32_f64, float, double)
+F3_DEF_CONV(f64_f32, double, float)
+F3_DEF_CONV(u32_u64, u32, u64)
+F3_DEF_CONV(u64_u32, u64, u32)
+#undef F3_DEF_CONV
+
+int lafem_hip_convert_h16_f32(void* stream, void* dst, const float* src, u64 n)
+{
+  if(n == 0) return 0;
+  if(!dst || !src) return rt::kErrArg;
+  k_convert<half, float><<<stream_blocks(n), kBlock, 0, S_>>>(static_cast<half*>(dst), src, n);
+  return rt::check_launch();
+}
+int lafem_hip_convert_f32_h16(void* stream, float* dst, const void* src, u64 n)
+{
+  if(n == 0) return 0;
+  if(!dst || !src) return rt::kErrArg;
+  k_convert<float, half><<<stream_blocks(n), kBlock, 0, S_>>>(dst, static_cast<const half*>(src), n);
+  return rt::check_launch();
+}
+
+} // extern "C"

@@ -1,0 +1,113 @@
+// device_common.hpp -- device helpers shared by the CDNA4 kernels: value loads with
+// precision conversion, wave64 reductions, deterministic grid reductions and the
+// XCD-aware block remap.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace feat3_amd
+{
+namespace dev
+{
+  typedef _Float16 half;
+
+  constexpr int kWave = 64;
+
+  // ---- loads with conversion into the arithmetic type ----------------------------------
+  template<typename DT_, typename VT_>
+  __device__ __forceinline__ DT_ ld(const VT_* p, std::uint64_t i) { return DT_(p[i]); }
+  template<>
+  __device__ __forceinline__ float ld<float, half>(const half* p, std::uint64_t i) { return float(p[i]); }
+  template<>
+  __device__ __forceinline__ double ld<double, half>(const half* p, std::uint64_t i) { return double(float(p[i])); }
+
+  // nontemporal streaming loads for matrix arrays that are read exactly once per sweep
+  template<typename T>
+  __device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
+  template<>
+  __device__ __forceinline__ half ld_stream<half>(const half* p) { return *p; }
+
+  // ---- sub-wave reductions (G lanes, G a power of two <= 64) ----------------------------
+  template<int G, typename T>
+  __device__ __forceinline__ T group_sum(T v)
+  {
+#pragma unroll
+    for(int off = G / 2; off > 0; off >>= 1)
+      v += __shfl_xor(v, off, G);
+    return v;
+  }
+
+  template<typename T>
+  __device__ __forceinline__ T wave_sum(T v) { return group_sum<kWave>(v); }
+
+  // block sum (blockDim.x a multiple of 64, at most 1024); result valid in thread 0
+  template<typename T>
+  __device__ __forceinline__ T block_sum(T v)
+  {
+    __shared__ T red[16];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    if(lane == 0) red[wid] = v;
+    __syncthreads();
+    const int nw = (blockDim.x + kWave - 1) / kWave;
+    T s = T(0);
+    if(wid == 0)
+    {
+      s = lane < nw ? red[lane] : T(0);
+      s = wave_sum(s);
+    }
+    __syncthreads();
+    return s;
+  }
+
+  // Deterministic grid reduction: every block stores its partial, the last block to arrive
+  // (ticket) sums the partials in block order and calls fin(total) in thread 0.  The ticket
+  // is reset by the last block, so the counter is reusable across launches and graph
+  // replays without a memset.  Returns true in the thread that ran fin().
+  template<typename T, typename Fin>
+  __device__ __forceinline__ void grid_reduce(T v, T* partials, unsigned int* ticket, Fin fin)
+  {
+    __shared__ bool last;
+    const T bs = block_sum(v);
+    if(threadIdx.x == 0)
+    {
+      partials[blockIdx.x] = bs;
+      __threadfence();
+      const unsigned int t = atomicAdd(ticket, 1u);
+      last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if(!last) return;
+    __threadfence();
+    // the last block sums all partials in fixed order
+    T s = T(0);
+    for(unsigned int i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+      s += __builtin_nontemporal_load(&partials[i]);
+    s = block_sum(s);
+    if(threadIdx.x == 0)
+    {
+      fin(s);
+      *ticket = 0u;
+    }
+  }
+
+  // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8), each XCD with
+  // its own L2.  Remapping gives every XCD a contiguous range of logical blocks, so the x
+  // entries gathered by neighbouring rows are fetched into one L2 instead of eight.
+  __device__ __forceinline__ unsigned int xcd_remap(unsigned int b, unsigned int nb)
+  {
+    constexpr unsigned int kXcd = 8;
+    const unsigned int q = nb / kXcd, r = nb % kXcd;
+    const unsigned int x = b % kXcd, i = b / kXcd;
+    return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+  }
+
+  // Dirichlet rows as a bitmask (bit i%32 of word i/32)
+  __device__ __forceinline__ bool masked(const std::uint32_t* mask, std::uint64_t i)
+  {
+    return mask != nullptr && ((mask[i >> 5] >> (i & 31u)) & 1u);
+  }
+} // namespace dev
+} // namespace feat3_amd

@@ -1,0 +1,422 @@
+// mg_kernels.hpp -- fused kernels of the device V-cycle and the device-resident PCG.
+//
+// The V-cycle follows MultiGrid::_apply_cycle_v (solver/multigrid.hpp:1333-1351,1711-2000)
+// with Richardson(omega_R, Jacobi(omega_J)) smoothers (richardson.hpp:187-228,
+// jacobi_precond.hpp:137-165).  Instead of FEAT's op-by-op sequence each smoothing step is
+// one kernel:  x_out = x_in + dinv .* (rhs - A x_in),  dinv = omega_R*omega_J/diag(A) with
+// dinv = 0 on Dirichlet rows (which is the filter_cor of the Jacobi correction).
+// Post-smoothing runs directly on the level solution (Richardson on A c = rhs - A x from
+// c = 0 is the same iteration as Richardson on A x = rhs started from x).  Every other
+// SpMV carries the unit-filter as a row bitmask in its epilogue.
+//
+// Matrix formats are "row evaluators": CsrRows<G> (G lanes per row, sub-wave reduction) and
+// SellRows (one lane per row of a sigma-sorted SELL-C chunk).
+#pragma once
+
+#include "device_common.hpp"
+
+namespace feat3_amd
+{
+namespace mg
+{
+  typedef std::uint32_t u32;
+  typedef std::uint64_t u64;
+
+  enum : int { kUndefined = 0, kProgress = 1, kSuccess = 2, kAborted = 3, kDiverged = 4, kMaxIter = 5, kStagnated = 6 };
+
+  // Row evaluators.  row_sum(p, lane, x) returns sum_j a_pj x_j for the row at position p,
+  // reduced over the G lanes of the calling group (valid in every lane); row(p) maps the
+  // position to the vector index (identity for CSR, the sigma permutation for SELL).
+  template<typename DT_, typename VT_, int G_>
+  struct CsrRows
+  {
+    static constexpr int G = G_;
+    const u32* rp;
+    const u32* col;
+    const VT_* val;
+    u64 rows;
+    __device__ __forceinline__ u64 row(u64 p) const { return p; }
+    __device__ __forceinline__ DT_ row_sum(u64 p, int lane, const DT_* __restrict__ x) const
+    {
+      const u32 end = rp[p + 1];
+      DT_ s = DT_(0);
+      for(u32 i = rp[p] + u32(lane); i < end; i += u32(G_))
+        s += dev::ld<DT_>(val, i) * x[col[i]];
+      return dev::group_sum<G_>(s);
+    }
+  };
+
+  // column storage of the derived SELL format: absolute u32 indices, or 16-bit signed offsets
+  // from the row index (usable when every |col - row| < 2^15, e.g. lexicographic 3D grids up
+  // to 129^3 nodes); a padding slot holds offset 0 / value 0
+  template<typename CI_> struct ColDecode;
+  template<> struct ColDecode<u32>
+  {
+    __device__ static __forceinline__ u64 get(u32 c, u64) { return c; }
+  };
+  template<> struct ColDecode<std::int16_t>
+  {
+    __device__ static __forceinline__ u64 get(std::int16_t d, u64 r) { return u64(std::int64_t(r) + std::int64_t(d)); }
+  };
+
+  // SELL-C-sigma: one lane per row; a row's k-th entry of chunk c sits at cs[c] + (p % C) + k*C,
+  // so a wave's loads are contiguous.  The loop runs to the chunk width cl[c] (uniform over the
+  // chunk, padded with zeros), which keeps the trip count wave-uniform and lets it unroll.
+  template<typename DT_, typename VT_, typename CI_, bool PERM_>
+  struct SellRows
+  {
+    static constexpr int G = 1;
+    const u32* cs;
+    const u32* cl;
+    const u32* perm;
+    const CI_* col;
+    const VT_* val;
+    u64 rows, C;
+    __device__ __forceinline__ u64 row(u64 p) const { return PERM_ ? u64(perm[p]) : p; }
+    __device__ __forceinline__ DT_ row_sum(u64 p, int, const DT_* __restrict__ x) const
+    {
+      const u64 ch = p / C;
+      const u64 base = u64(cs[ch]) + (p - ch * C);
+      const u32 len = cl[ch];
+      const u64 r = row(p);
+      DT_ s = DT_(0);
+#pragma unroll 4
+      for(u32 k = 0; k < len; ++k)
+      {
+        const u64 e = base + u64(k) * C;
+        s += dev::ld<DT_>(val, e) * x[ColDecode<CI_>::get(col[e], r)];
+      }
+      return s;
+    }
+  };
+
+  // Launch shape of the row kernels: every block owns a contiguous range of `per` row
+  // positions (a multiple of 64), its 256/G lane groups stride through that range, and the
+  // blocks are XCD-remapped so that neighbouring ranges share an L2.
+  struct RowGrid
+  {
+    u64 per;
+    unsigned blocks;
+  };
+
+  inline RowGrid row_grid(u64 rows, int G, unsigned max_blocks = 8192)
+  {
+    const u64 groups_per_block = 256 / u64(G);
+    u64 nb = (rows + groups_per_block - 1) / groups_per_block;
+    nb = nb < 1 ? 1 : (nb > max_blocks ? max_blocks : nb);
+    u64 per = (rows + nb - 1) / nb;
+    per = (per + 63) / 64 * 64;
+    if(per == 0) per = 64;
+    RowGrid g;
+    g.per = per;
+    g.blocks = unsigned((rows + per - 1) / per);
+    if(g.blocks == 0) g.blocks = 1;
+    return g;
+  }
+
+  // calls f(position, row_sum, is_group_leader) for every row position of this block
+  template<typename Rows_, typename DT_, typename F>
+  __device__ __forceinline__ void for_rows(const Rows_& A, u64 per, const DT_* __restrict__ x, F&& f)
+  {
+    constexpr int G = Rows_::G;
+    const u64 b = dev::xcd_remap(blockIdx.x, gridDim.x);
+    const u64 beg = b * per;
+    const u64 end = beg + per < A.rows ? beg + per : A.rows;
+    const int lane = int(threadIdx.x % G);
+    const u64 ngrp = blockDim.x / G;
+    for(u64 p = beg + threadIdx.x / G; p < end; p += ngrp)
+    {
+      const DT_ s = A.row_sum(p, lane, x);
+      f(p, s, lane == 0);
+    }
+  }
+
+  __device__ __forceinline__ bool stopped(const int* status) { return status != nullptr && *status != kProgress; }
+
+  // ---- V-cycle kernels -------------------------------------------------------------------
+
+  // MG entry: rhs = def (converted to the inner precision, convert_precond.hpp:104),
+  // first Richardson step from zero: x = dinv .* rhs
+  template<typename DT_>
+  __global__ void k_mg_entry(u64 n, const double* __restrict__ def, DT_* __restrict__ rhs, DT_* __restrict__ x,
+                             const DT_* __restrict__ dinv, const int* status)
+  {
+    if(stopped(status)) return;
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+    {
+      const DT_ r = DT_(def[i]);
+      rhs[i] = r;
+      x[i] = dinv[i] * r;
+    }
+  }
+
+  // damped Jacobi step  x_out = x_in + dinv .* (rhs - A x_in)
+  template<typename Rows_, typename DT_>
+  __global__ __launch_bounds__(256) void k_sweep(Rows_ A, u64 per, const DT_* __restrict__ dinv, const DT_* __restrict__ rhs,
+                                                 const DT_* __restrict__ xin, DT_* __restrict__ xout, const int* status)
+  {
+    if(stopped(status)) return;
+    for_rows(A, per, xin, [&](u64 p, DT_ s, bool w) {
+      const u64 row = A.row(p);
+      if(w) xout[row] = xin[row] + dinv[row] * (rhs[row] - s);
+    });
+  }
+
+  // defect  def = filter(rhs - A x)
+  template<typename Rows_, typename DT_>
+  __global__ __launch_bounds__(256) void k_defect(Rows_ A, u64 per, const u32* __restrict__ mask, const DT_* __restrict__ rhs,
+                                                  const DT_* __restrict__ x, DT_* __restrict__ def, const int* status)
+  {
+    if(stopped(status)) return;
+    for_rows(A, per, x, [&](u64 p, DT_ s, bool w) {
+      const u64 row = A.row(p);
+      if(w) def[row] = dev::masked(mask, row) ? DT_(0) : rhs[row] - s;
+    });
+  }
+
+  // restriction  rhs_c = filter_c(R def_f);  optional first smoothing step x_c = dinv_c .* rhs_c
+  template<typename Rows_, typename DT_>
+  __global__ __launch_bounds__(256) void k_restrict(Rows_ R, u64 per, const u32* __restrict__ mask_c, const DT_* __restrict__ def_f,
+                                                    DT_* __restrict__ rhs_c, DT_* __restrict__ x_c,
+                                                    const DT_* __restrict__ dinv_c, const int* status)
+  {
+    if(stopped(status)) return;
+    for_rows(R, per, def_f, [&](u64 p, DT_ s, bool w) {
+      const u64 row = R.row(p);
+      if(w)
+      {
+        const DT_ r = dev::masked(mask_c, row) ? DT_(0) : s;
+        rhs_c[row] = r;
+        if(x_c) x_c[row] = dinv_c[row] * r;
+      }
+    });
+  }
+
+  // prolongation and coarse-grid correction  x_f += filter_f(P x_c)
+  template<typename Rows_, typename DT_>
+  __global__ __launch_bounds__(256) void k_prolong(Rows_ P, u64 per, const u32* __restrict__ mask_f, const DT_* __restrict__ x_c,
+                                                   DT_* __restrict__ x_f, const int* status)
+  {
+    if(stopped(status)) return;
+    for_rows(P, per, x_c, [&](u64 p, DT_ s, bool w) {
+      const u64 row = P.row(p);
+      if(w && !dev::masked(mask_f, row)) x_f[row] = x_f[row] + s;
+    });
+  }
+
+  // exact coarse solve with the FP64 inverse of the unit-filtered coarse matrix, one
+  // workgroup: x = filter(Ainv * rhs)
+  template<typename DT_>
+  __global__ __launch_bounds__(256) void k_coarse(u64 n, const double* __restrict__ ainv, const u32* __restrict__ mask,
+                                                  const DT_* __restrict__ rhs, DT_* __restrict__ x, const int* status)
+  {
+    if(stopped(status)) return;
+    __shared__ double sr[1024];
+    for(u64 j = threadIdx.x; j < n; j += blockDim.x) sr[j] = double(rhs[j]);
+    __syncthreads();
+    for(u64 i = threadIdx.x; i < n; i += blockDim.x)
+    {
+      double s = 0.0;
+      for(u64 j = 0; j < n; ++j) s += ainv[i * n + j] * sr[j];
+      x[i] = dev::masked(mask, i) ? DT_(0) : DT_(s);
+    }
+  }
+
+  // ---- setup kernels -----------------------------------------------------------------------
+
+  // dinv = omega_R * (omega_J / diag), 0 on Dirichlet rows (diagonal_generic.hpp:20-37,
+  // component_invert_generic.hpp:21-36)
+  template<typename DT_, typename VT_>
+  __global__ void k_make_dinv(u64 rows, const u32* rp, const u32* col, const VT_* val, const u32* mask,
+                              DT_ omega_j, DT_ omega_r, DT_* dinv)
+  {
+    const u64 row = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(row >= rows) return;
+    DT_ d = DT_(0);
+    for(u32 i = rp[row]; i < rp[row + 1]; ++i)
+      if(u64(col[i]) == row) { d = dev::ld<DT_>(val, i); break; }
+    const DT_ inv = omega_j / d;
+    dinv[row] = dev::masked(mask, row) ? DT_(0) : omega_r * inv;
+  }
+
+  // ---- PCG state and kernels (FP64) ----------------------------------------------------------
+
+  // device-resident PCG scalars and the criteria of IterativeSolver (iterative.hpp:213-237)
+  struct PcgState
+  {
+    double gamma, alpha, beta, def_init, def_cur, def_prev;
+    double tol_rel, tol_abs, tol_abs_low, div_rel, div_abs, stag_rate;
+    u64 num_iter, min_iter, max_iter, min_stag_iter, num_stag_iter, hist_cap;
+    int status;
+    int pad;
+  };
+
+  // _set_initial_defect (iterative.hpp:744-766)
+  __device__ inline void set_initial_defect(PcgState* st, double* hist, double d)
+  {
+    st->def_init = st->def_cur = st->def_prev = d;
+    st->num_iter = 0;
+    st->num_stag_iter = 0;
+    if(st->hist_cap > 0) hist[0] = d;
+    const double eps = 2.220446049250313e-16;
+    if(!isfinite(d)) st->status = kAborted;
+    else if(d <= eps * eps) st->status = kSuccess;
+    else st->status = kProgress;
+  }
+
+  // _set_new_defect / _analyse_defect (iterative.hpp:791-879)
+  __device__ inline void set_new_defect(PcgState* st, double* hist, double d)
+  {
+    ++st->num_iter;
+    st->def_prev = st->def_cur;
+    // with min_iter == max_iter and no stagnation test the reference skips the norm
+    if((st->min_iter < st->max_iter) || (st->min_stag_iter > 0))
+      st->def_cur = d;
+    d = st->def_cur;
+    if(st->num_iter < st->hist_cap) hist[st->num_iter] = d;
+    int s = kProgress;
+    if(!isfinite(d)) s = kAborted;
+    else if((d > st->div_abs) || (d > st->div_rel * st->def_init)) s = kDiverged;
+    else if(st->num_iter < st->min_iter) s = kProgress;
+    else if((d <= st->tol_abs) && ((d <= st->tol_rel * st->def_init) || (d <= st->tol_abs_low))) s = kSuccess;
+    else if(st->num_iter >= st->max_iter) s = kMaxIter;
+    else if(st->min_stag_iter > 0)
+    {
+      if(d >= st->stag_rate * st->def_prev)
+      {
+        if(++st->num_stag_iter >= st->min_stag_iter) s = kStagnated;
+      }
+      else
+        st->num_stag_iter = 0;
+    }
+    st->status = s;
+  }
+
+  // r = filter(b - A x)  (pcg.hpp:159-173 correct) fused with ||r|| -> _set_initial_defect
+  template<typename Rows_>
+  __global__ __launch_bounds__(256) void k_pcg_defect_norm(Rows_ A, u64 per, const u32* __restrict__ mask, const double* __restrict__ b,
+                                                           const double* __restrict__ x, double* __restrict__ r,
+                                                           PcgState* st, double* hist, double* partials, unsigned* ticket)
+  {
+    double v = 0.0;
+    for_rows(A, per, x, [&](u64 p, double s, bool w) {
+      const u64 row = A.row(p);
+      if(w)
+      {
+        const double d = dev::masked(mask, row) ? 0.0 : b[row] - s;
+        r[row] = d;
+        v += d * d;
+      }
+    });
+    dev::grid_reduce(v, partials, ticket, [&](double t) { set_initial_defect(st, hist, sqrt(t)); });
+  }
+
+  // r = def, ||r|| -> _set_initial_defect  (pcg.hpp:141-156 apply)
+  __global__ __launch_bounds__(256) void k_pcg_copy_norm(u64 n, const double* __restrict__ def, double* __restrict__ r,
+                                                         PcgState* st, double* hist, double* partials, unsigned* ticket)
+  {
+    double v = 0.0;
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+    {
+      const double d = def[i];
+      r[i] = d;
+      v += d * d;
+    }
+    dev::grid_reduce(v, partials, ticket, [&](double t) { set_initial_defect(st, hist, sqrt(t)); });
+  }
+
+  // p = z (correction converted back to FP64, convert_precond.hpp:115), gamma = <r, p>
+  template<typename DT_>
+  __global__ __launch_bounds__(256) void k_pcg_p_gamma(u64 n, const DT_* __restrict__ z, const double* __restrict__ r,
+                                                       double* __restrict__ p, PcgState* st, double* partials, unsigned* ticket)
+  {
+    if(st->status != kProgress) return;
+    double v = 0.0;
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+    {
+      const double zi = double(z[i]);
+      p[i] = zi;
+      v += r[i] * zi;
+    }
+    dev::grid_reduce(v, partials, ticket, [&](double t) { st->gamma = t; });
+  }
+
+  // q = filter(A p), alpha = gamma / <q, p>   (pcg.hpp:236-242)
+  template<typename Rows_>
+  __global__ __launch_bounds__(256) void k_pcg_spmv_dot(Rows_ A, u64 per, const u32* __restrict__ mask, const double* __restrict__ p,
+                                                        double* __restrict__ q, PcgState* st, double* partials, unsigned* ticket)
+  {
+    if(st->status != kProgress) return;
+    double v = 0.0;
+    for_rows(A, per, p, [&](u64 pos, double s, bool w) {
+      const u64 row = A.row(pos);
+      if(w)
+      {
+        const double qi = dev::masked(mask, row) ? 0.0 : s;
+        q[row] = qi;
+        v += qi * p[row];
+      }
+    });
+    dev::grid_reduce(v, partials, ticket, [&](double t) { st->alpha = st->gamma / t; });
+  }
+
+  // x += alpha p, r -= alpha q, ||r|| -> _set_new_defect   (pcg.hpp:244-259)
+  __global__ __launch_bounds__(256) void k_pcg_update_norm(u64 n, const double* __restrict__ p, const double* __restrict__ q,
+                                                           double* __restrict__ x, double* __restrict__ r, PcgState* st,
+                                                           double* hist, double* partials, unsigned* ticket)
+  {
+    if(st->status != kProgress) return;
+    const double alpha = st->alpha;
+    const bool upd = fabs(alpha) >= 2.220446049250313e-16; // axpy with |alpha| < eps is a copy
+    double v = 0.0;
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+    {
+      double ri = r[i];
+      if(upd)
+      {
+        x[i] = x[i] + alpha * p[i];
+        ri = ri + (-alpha) * q[i];
+        r[i] = ri;
+      }
+      v += ri * ri;
+    }
+    dev::grid_reduce(v, partials, ticket, [&](double t) { set_new_defect(st, hist, sqrt(t)); });
+  }
+
+  // gamma' = <r, z>, beta = gamma'/gamma   (pcg.hpp:270-279)
+  template<typename DT_>
+  __global__ __launch_bounds__(256) void k_pcg_gamma_beta(u64 n, const DT_* __restrict__ z, const double* __restrict__ r,
+                                                          PcgState* st, double* partials, unsigned* ticket)
+  {
+    if(st->status != kProgress) return;
+    double v = 0.0;
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      v += r[i] * double(z[i]);
+    dev::grid_reduce(v, partials, ticket, [&](double t) {
+      const double g2 = st->gamma;
+      st->gamma = t;
+      st->beta = t / g2;
+    });
+  }
+
+  // p = beta p + z  (axpy with r == x: r = r*a + y; |beta| < eps: copy)   (pcg.hpp:281-283)
+  template<typename DT_>
+  __global__ void k_pcg_update_p(u64 n, const DT_* __restrict__ z, double* __restrict__ p, const PcgState* st)
+  {
+    if(st->status != kProgress) return;
+    const double beta = st->beta;
+    const bool copy = fabs(beta) < 2.220446049250313e-16;
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      p[i] = copy ? double(z[i]) : p[i] * beta + double(z[i]);
+  }
+
+  // unpreconditioned CG: the "preconditioner" copies and filters (iterative.hpp:1070-1075)
+  __global__ void k_copy_filter(u64 n, const double* __restrict__ r, const u32* __restrict__ mask, double* __restrict__ z,
+                                const int* status)
+  {
+    if(stopped(status)) return;
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      z[i] = dev::masked(mask, i) ? 0.0 : r[i];
+  }
+} // namespace mg
+} // namespace feat3_amd

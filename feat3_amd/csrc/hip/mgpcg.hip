@@ -1,0 +1,939 @@
+// mgpcg.hip -- the device-resident PCG preconditioned by a multigrid V-cycle, with the
+// Solver lifecycle of the reference (solver/base.hpp:184-316: init_symbolic, init_numeric,
+// apply, correct, done) behind the feat3_mgpcg_* C ABI.
+//
+// Reference path restated (one GPU, no Global layer):
+//   PCG::_apply_intern          solver/pcg.hpp:188-288
+//   IterativeSolver criteria    solver/iterative.hpp:213-237,476-505,744-879
+//   MultiGrid V-cycle           solver/multigrid.hpp:1268-1351,1524-1587,1711-2000
+//   Richardson(Jacobi) smoother solver/richardson.hpp:187-228, jacobi_precond.hpp:137-165
+//   ConvertPrecond              solver/convert_precond.hpp:100-119 (FP64 outer / FP32 inner)
+//
+// Layout in HBM: per level CSR (u32 row_ptr/col_ind, values in the inner storage type),
+// optional SELL-C-sigma copy, Dirichlet row bitmask, dinv, and the four level vectors
+// rhs/x/t/def in the inner precision; the outer PCG holds the FP64 finest matrix and
+// x, r, p, q in FP64 plus a PcgState record of device scalars.  The whole PCG iteration
+// (SpMV+dot, update+norm, V-cycle, dot, p update) is captured once into a hipGraph and
+// replayed; every kernel returns immediately once the device status leaves "progress", so
+// the host only polls the status one iteration behind.
+
+#include "device_common.hpp"
+#include "mg_kernels.hpp"
+#include "runtime.hpp"
+#include "feat3_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using namespace feat3_amd;
+using feat3_amd::dev::half;
+typedef std::uint32_t u32;
+typedef std::uint64_t u64;
+
+namespace
+{
+  constexpr int kBlock = 256;
+
+#define F3_CHECK(...) do { if((__VA_ARGS__) != hipSuccess) return rt::kErrRuntime; } while(0)
+#define F3_RC(...) do { const int rc_ = (__VA_ARGS__); if(rc_) return rc_; } while(0)
+
+  struct DevMem
+  {
+    void* p = nullptr;
+    ~DevMem() { if(p) (void)hipFree(p); }
+    template<typename T> T* as() const { return static_cast<T*>(p); }
+    int alloc(u64 bytes)
+    {
+      if(p) { (void)hipFree(p); p = nullptr; }
+      if(bytes == 0) bytes = 16;
+      return hipMalloc(&p, bytes) == hipSuccess ? rt::kOk : rt::kErrRuntime;
+    }
+  };
+
+  template<typename VT> VT to_storage(double v) { return VT(v); }
+  template<> half to_storage<half>(double v) { return half(float(v)); }
+
+  // host copy of one CSR matrix (FP64 values), as pushed through the C ABI
+  struct HostCsr
+  {
+    u64 rows = 0, cols = 0;
+    std::vector<u32> rp, col;
+    std::vector<double> val;
+    u64 nnz() const { return rp.empty() ? 0 : rp[rows]; }
+  };
+
+  struct HostLevel
+  {
+    HostCsr A, P, R;
+    std::vector<u32> fidx;
+  };
+
+  // device CSR with an optional SELL-C-sigma copy (derived; the CSR arrays stay as given)
+  template<typename VT>
+  struct DevCsr
+  {
+    u64 rows = 0, cols = 0, nnz = 0;
+    int lanes = 8;
+    DevMem rp, col, val;
+    bool sell = false, sell_perm = false, sell_d16 = false;
+    u64 C = 64, sigma = 1, sell_size = 0;
+    DevMem cs, cl, perm, scol, sval;
+
+    // drop_zeros: upload a compact copy without the stored exact zeros (used for the
+    // transfer matrices, whose two-level pattern holds structural zeros; a zero entry adds
+    // nothing to a finite product, so the result is unchanged)
+    int upload(const HostCsr& h0, hipStream_t s, bool want_sell, u64 sell_C, u64 sell_sigma, int force_lanes,
+               bool drop_zeros = false, bool allow_d16 = true)
+    {
+      HostCsr compact;
+      const HostCsr* hp = &h0;
+      if(drop_zeros)
+      {
+        compact.rows = h0.rows; compact.cols = h0.cols;
+        compact.rp.assign(h0.rows + 1, 0);
+        for(u64 r = 0; r < h0.rows; ++r)
+        {
+          for(u32 k = h0.rp[r]; k < h0.rp[r + 1]; ++k)
+            if(h0.val[k] != 0.0) { compact.col.push_back(h0.col[k]); compact.val.push_back(h0.val[k]); }
+          compact.rp[r + 1] = u32(compact.col.size());
+        }
+        hp = &compact;
+      }
+      const HostCsr& h = *hp;
+      rows = h.rows; cols = h.cols; nnz = h.nnz();
+      F3_RC(rp.alloc(sizeof(u32) * (rows + 1)));
+      F3_RC(col.alloc(sizeof(u32) * std::max<u64>(nnz, 1)));
+      F3_RC(val.alloc(sizeof(VT) * std::max<u64>(nnz, 1)));
+      std::vector<VT> v(nnz);
+      for(u64 i = 0; i < nnz; ++i) v[i] = to_storage<VT>(h.val[i]);
+      F3_CHECK(hipMemcpyAsync(rp.p, h.rp.data(), sizeof(u32) * (rows + 1), hipMemcpyHostToDevice, s));
+      if(nnz)
+      {
+        F3_CHECK(hipMemcpyAsync(col.p, h.col.data(), sizeof(u32) * nnz, hipMemcpyHostToDevice, s));
+        F3_CHECK(hipMemcpyAsync(val.p, v.data(), sizeof(VT) * nnz, hipMemcpyHostToDevice, s));
+      }
+      F3_CHECK(hipStreamSynchronize(s)); // host staging vectors go out of scope
+      const double mean = rows ? double(nnz) / double(rows) : 1.0;
+      // about four entries per lane: enough independent loads in flight per lane while the
+      // group reduction stays short
+      lanes = 2;
+      while(lanes < 64 && double(lanes) * 4.0 < mean) lanes *= 2;
+      if(force_lanes > 0) lanes = force_lanes;
+      sell = want_sell;
+      if(sell) F3_RC(build_sell(h, s, sell_C, sell_sigma, allow_d16));
+      return rt::kOk;
+    }
+
+    // SELL-C-sigma from the host converter feat3_sell_build (no permutation is stored for
+    // sigma = 1); values converted to the storage type, columns as 16-bit row offsets when
+    // they all fit.
+    int build_sell(const HostCsr& h, hipStream_t s, u64 Cc, u64 sig, bool allow_d16)
+    {
+      C = Cc; sigma = std::max<u64>(1, sig);
+      sell_perm = sigma > 1;
+      u64 nch = 0;
+      if(feat3_sell_sizes(rows, h.rp.data(), C, sigma, &nch, &sell_size)) return rt::kErrArg;
+      std::vector<u32> hcs(nch + 1), hcl(std::max<u64>(nch, 1)), hpm(std::max<u64>(rows, 1)), hcol(std::max<u64>(sell_size, 1));
+      std::vector<double> hv(std::max<u64>(sell_size, 1));
+      std::vector<std::int16_t> hd16(allow_d16 ? std::max<u64>(sell_size, 1) : 0);
+      int d16 = 0;
+      if(feat3_sell_build(rows, cols, h.rp.data(), h.col.data(), h.val.data(), C, sigma, hcs.data(), hcl.data(),
+                          hpm.data(), hcol.data(), hv.data(), allow_d16 ? hd16.data() : nullptr, &d16))
+        return rt::kErrArg;
+      sell_d16 = d16 != 0;
+      std::vector<VT> hval(sell_size);
+      for(u64 i = 0; i < sell_size; ++i) hval[i] = to_storage<VT>(hv[i]);
+      F3_RC(cs.alloc(sizeof(u32) * (nch + 1)));
+      F3_RC(cl.alloc(sizeof(u32) * std::max<u64>(nch, 1)));
+      F3_RC(sval.alloc(sizeof(VT) * std::max<u64>(sell_size, 1)));
+      F3_CHECK(hipMemcpyAsync(cs.p, hcs.data(), sizeof(u32) * (nch + 1), hipMemcpyHostToDevice, s));
+      if(nch) F3_CHECK(hipMemcpyAsync(cl.p, hcl.data(), sizeof(u32) * nch, hipMemcpyHostToDevice, s));
+      if(sell_perm)
+      {
+        F3_RC(perm.alloc(sizeof(u32) * std::max<u64>(rows, 1)));
+        if(rows) F3_CHECK(hipMemcpyAsync(perm.p, hpm.data(), sizeof(u32) * rows, hipMemcpyHostToDevice, s));
+      }
+      if(sell_d16)
+      {
+        F3_RC(scol.alloc(sizeof(std::int16_t) * std::max<u64>(sell_size, 1)));
+        if(sell_size) F3_CHECK(hipMemcpyAsync(scol.p, hd16.data(), sizeof(std::int16_t) * sell_size, hipMemcpyHostToDevice, s));
+      }
+      else
+      {
+        F3_RC(scol.alloc(sizeof(u32) * std::max<u64>(sell_size, 1)));
+        if(sell_size) F3_CHECK(hipMemcpyAsync(scol.p, hcol.data(), sizeof(u32) * sell_size, hipMemcpyHostToDevice, s));
+      }
+      if(sell_size) F3_CHECK(hipMemcpyAsync(sval.p, hval.data(), sizeof(VT) * sell_size, hipMemcpyHostToDevice, s));
+      F3_CHECK(hipStreamSynchronize(s));
+      return rt::kOk;
+    }
+
+    // bytes of one row-evaluation pass over this matrix as stored (values + indices + row
+    // pointers / chunk descriptors, SELL padding included), excluding the vector traffic
+    u64 matrix_bytes() const
+    {
+      if(sell)
+      {
+        const u64 nch = (rows + C - 1) / C;
+        return sell_size * (sizeof(VT) + (sell_d16 ? 2 : 4)) + (nch + 1) * 4 + nch * 4 + (sell_perm ? rows * 4 : 0);
+      }
+      return nnz * (sizeof(VT) + 4) + (rows + 1) * 4;
+    }
+  };
+
+  // calls f(rows_evaluator, launch grid) for the matrix's format and lane count
+  template<typename DT, typename VT, typename F>
+  int with_rows(const DevCsr<VT>& m, F&& f)
+  {
+    if(m.sell)
+    {
+      const mg::RowGrid g = mg::row_grid(m.rows, 1);
+      const u32* cs = m.cs.template as<u32>();
+      const u32* cl = m.cl.template as<u32>();
+      const u32* pm = m.perm.template as<u32>();
+      const VT* v = m.sval.template as<VT>();
+      if(m.sell_d16)
+      {
+        const std::int16_t* c = m.scol.template as<std::int16_t>();
+        if(m.sell_perm) f(mg::SellRows<DT, VT, std::int16_t, true>{cs, cl, pm, c, v, m.rows, m.C}, g);
+        else f(mg::SellRows<DT, VT, std::int16_t, false>{cs, cl, pm, c, v, m.rows, m.C}, g);
+      }
+      else
+      {
+        const u32* c = m.scol.template as<u32>();
+        if(m.sell_perm) f(mg::SellRows<DT, VT, u32, true>{cs, cl, pm, c, v, m.rows, m.C}, g);
+        else f(mg::SellRows<DT, VT, u32, false>{cs, cl, pm, c, v, m.rows, m.C}, g);
+      }
+      return rt::kOk;
+    }
+    const u32* rp = m.rp.template as<u32>();
+    const u32* col = m.col.template as<u32>();
+    const VT* val = m.val.template as<VT>();
+    switch(m.lanes)
+    {
+      case 2: f(mg::CsrRows<DT, VT, 2>{rp, col, val, m.rows}, mg::row_grid(m.rows, 2)); break;
+      case 4: f(mg::CsrRows<DT, VT, 4>{rp, col, val, m.rows}, mg::row_grid(m.rows, 4)); break;
+      case 8: f(mg::CsrRows<DT, VT, 8>{rp, col, val, m.rows}, mg::row_grid(m.rows, 8)); break;
+      case 16: f(mg::CsrRows<DT, VT, 16>{rp, col, val, m.rows}, mg::row_grid(m.rows, 16)); break;
+      case 32: f(mg::CsrRows<DT, VT, 32>{rp, col, val, m.rows}, mg::row_grid(m.rows, 32)); break;
+      case 64: f(mg::CsrRows<DT, VT, 64>{rp, col, val, m.rows}, mg::row_grid(m.rows, 64)); break;
+      default: return rt::kErrArg;
+    }
+    return rt::kOk;
+  }
+
+  inline unsigned nblocks(u64 threads)
+  {
+    const u64 b = (threads + kBlock - 1) / kBlock;
+    return unsigned(std::max<u64>(b, 1));
+  }
+  inline unsigned stream_blocks(u64 n)
+  {
+    return unsigned(std::max<u64>(1, std::min<u64>((n + kBlock - 1) / kBlock, 4096)));
+  }
+
+  // Gauss-Jordan inverse with partial pivoting (FP64), for the coarse level
+  bool invert_dense(std::vector<double>& a, u64 n)
+  {
+    std::vector<double> inv(n * n, 0.0);
+    for(u64 i = 0; i < n; ++i) inv[i * n + i] = 1.0;
+    for(u64 k = 0; k < n; ++k)
+    {
+      u64 pr = k;
+      for(u64 i = k + 1; i < n; ++i)
+        if(std::fabs(a[i * n + k]) > std::fabs(a[pr * n + k])) pr = i;
+      if(a[pr * n + k] == 0.0) return false;
+      if(pr != k)
+        for(u64 j = 0; j < n; ++j) { std::swap(a[k * n + j], a[pr * n + j]); std::swap(inv[k * n + j], inv[pr * n + j]); }
+      const double d = a[k * n + k];
+      for(u64 j = 0; j < n; ++j) { a[k * n + j] /= d; inv[k * n + j] /= d; }
+      for(u64 i = 0; i < n; ++i)
+      {
+        if(i == k) continue;
+        const double f = a[i * n + k];
+        if(f == 0.0) continue;
+        for(u64 j = 0; j < n; ++j) { a[i * n + j] -= f * a[k * n + j]; inv[i * n + j] -= f * inv[k * n + j]; }
+      }
+    }
+    a.swap(inv);
+    return true;
+  }
+
+  std::vector<u32> make_mask(u64 n, const std::vector<u32>& fidx)
+  {
+    std::vector<u32> m((n + 31) / 32 + 1, 0u);
+    for(u32 i : fidx) m[i >> 5] |= (1u << (i & 31u));
+    return m;
+  }
+
+  template<typename T>
+  __global__ void k_to_double(u64 n, const T* z, double* out)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      out[i] = double(z[i]);
+  }
+
+  // -----------------------------------------------------------------------------------------
+  // the engine: type-erased interface + implementation per inner precision
+  // -----------------------------------------------------------------------------------------
+  struct Config
+  {
+    double omega_j = 0.5, omega_r = 1.0;
+    int pre = 4, post = 4;
+    int precond = 0;          // 0 MG V-cycle, 1 none (CG)
+    bool use_graph = true;
+    int format = 0;           // 0 CSR, 1 SELL-C-sigma (inner hierarchy)
+    u64 sell_C = 64, sell_sigma = 64;
+    int lanes = 0;            // forced lanes per CSR row (0 = by mean row length)
+    bool index16 = true;      // SELL columns as 16-bit row offsets where they fit
+    bool compact_transfer = true; // device P/R without the stored structural zeros
+    double tol_rel = 1e-8, tol_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16), tol_abs_low = 0.0;
+    double div_rel = 1.0 / 2.220446049250313e-16, div_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16);
+    double stag_rate = 0.95;
+    u64 min_iter = 0, max_iter = 1000, min_stag_iter = 0;
+  };
+
+  struct EngineBase
+  {
+    virtual ~EngineBase() {}
+    virtual int init(const std::vector<HostLevel>& lv, const Config& cfg, hipStream_t s) = 0;
+    virtual int vcycle(double* cor, const double* def) = 0;
+    virtual int solve(double* x, const double* b, bool correct, int* status, u64* iters) = 0;
+    virtual int history(double* out, u64 cap, u64* len) = 0;
+    virtual int time_kernel(int which, int reps, double* avg_ms, u64* bytes) = 0;
+    virtual int info(u64* out, int cap) = 0;
+    virtual bool set_tolerances(const Config& c) = 0;
+  };
+
+  template<typename DT, typename VT>
+  struct Level
+  {
+    u64 n = 0;
+    DevCsr<VT> A, P, R;
+    DevMem mask, dinv, rhs, x, t, def;
+    DT* xfinal = nullptr; // buffer holding this level's solution after its post-smoothing
+  };
+
+  template<typename DT, typename VT>
+  struct Engine : public EngineBase
+  {
+    Config cfg;
+    hipStream_t s = nullptr;
+    std::vector<std::unique_ptr<Level<DT, VT>>> L;
+    u64 ncoarse = 0;
+    DevMem ainv;
+    // outer FP64 PCG
+    DevCsr<double> A64own;
+    DevMem mask0, x64dummy, r, p, q, zcg, state, hist, partials, ticket;
+    const DevCsr<double>* A64 = nullptr;
+    u64 n0 = 0, hist_cap = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int* pinned = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+
+    ~Engine() override
+    {
+      if(gexec) (void)hipGraphExecDestroy(gexec);
+      if(graph) (void)hipGraphDestroy(graph);
+      if(pinned) (void)hipHostFree(pinned);
+      for(auto& e : ev) if(e) (void)hipEventDestroy(e);
+      if(t0) (void)hipEventDestroy(t0);
+      if(t1) (void)hipEventDestroy(t1);
+    }
+
+    const DevCsr<double>& outer_matrix() const { return *A64; }
+
+    // tolerance changes after init; false if the history buffer is too small
+    bool set_tolerances(const Config& c) override
+    {
+      if(c.max_iter + 2 > hist_cap) return false;
+      cfg.tol_rel = c.tol_rel; cfg.tol_abs = c.tol_abs; cfg.tol_abs_low = c.tol_abs_low;
+      cfg.min_iter = c.min_iter; cfg.max_iter = c.max_iter;
+      return true;
+    }
+
+    int init(const std::vector<HostLevel>& lv, const Config& c, hipStream_t stream) override
+    {
+      cfg = c;
+      s = stream;
+      if(lv.empty()) return rt::kErrState;
+      if(cfg.pre < 1 || cfg.post < 1) return rt::kErrArg;
+      const bool want_sell = cfg.format == 1;
+      L.clear();
+      for(std::size_t i = 0; i < lv.size(); ++i)
+      {
+        std::unique_ptr<Level<DT, VT>> l(new Level<DT, VT>());
+        const HostLevel& h = lv[i];
+        l->n = h.A.rows;
+        F3_RC(l->A.upload(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16));
+        if(i + 1 < lv.size())
+        {
+          if(h.P.rows != l->n || h.R.cols != l->n || h.P.cols != lv[i + 1].A.rows || h.R.rows != lv[i + 1].A.rows)
+            return rt::kErrArg;
+          F3_RC(l->P.upload(h.P, s, want_sell, cfg.sell_C, 1, cfg.lanes, cfg.compact_transfer, cfg.index16));
+          F3_RC(l->R.upload(h.R, s, want_sell, cfg.sell_C, 1, cfg.lanes, cfg.compact_transfer, cfg.index16));
+        }
+        const std::vector<u32> m = make_mask(l->n, h.fidx);
+        F3_RC(l->mask.alloc(sizeof(u32) * m.size()));
+        F3_CHECK(hipMemcpyAsync(l->mask.p, m.data(), sizeof(u32) * m.size(), hipMemcpyHostToDevice, s));
+        F3_RC(l->dinv.alloc(sizeof(DT) * l->n));
+        F3_RC(l->rhs.alloc(sizeof(DT) * l->n));
+        F3_RC(l->x.alloc(sizeof(DT) * l->n));
+        F3_RC(l->t.alloc(sizeof(DT) * l->n));
+        F3_RC(l->def.alloc(sizeof(DT) * l->n));
+        F3_CHECK(hipMemsetAsync(l->x.p, 0, sizeof(DT) * l->n, s));
+        F3_CHECK(hipMemsetAsync(l->t.p, 0, sizeof(DT) * l->n, s));
+        // Jacobi setup on the device matrix in the inner storage precision
+        if(l->n)
+        {
+          mg::k_make_dinv<DT, VT><<<nblocks(l->n), kBlock, 0, s>>>(
+            l->n, l->A.rp.template as<u32>(), l->A.col.template as<u32>(), l->A.val.template as<VT>(),
+            l->mask.template as<u32>(), DT(cfg.omega_j), DT(cfg.omega_r), l->dinv.template as<DT>());
+          F3_RC(rt::check_launch());
+        }
+        L.push_back(std::move(l));
+      }
+      // FP64 coarse inverse of the unit-filtered coarsest matrix
+      const HostLevel& hc = lv.back();
+      ncoarse = hc.A.rows;
+      if(ncoarse > 1024) return rt::kErrArg;
+      {
+        std::vector<double> a(ncoarse * ncoarse, 0.0);
+        for(u64 i = 0; i < ncoarse; ++i)
+          for(u32 k = hc.A.rp[i]; k < hc.A.rp[i + 1]; ++k) a[i * ncoarse + hc.A.col[k]] += hc.A.val[k];
+        for(u32 i : hc.fidx)
+        {
+          for(u64 j = 0; j < ncoarse; ++j) a[u64(i) * ncoarse + j] = 0.0;
+          a[u64(i) * ncoarse + i] = 1.0;
+        }
+        if(!invert_dense(a, ncoarse)) return rt::kErrNumeric;
+        F3_RC(ainv.alloc(sizeof(double) * ncoarse * ncoarse));
+        F3_CHECK(hipMemcpyAsync(ainv.p, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice, s));
+        F3_CHECK(hipStreamSynchronize(s));
+      }
+      // final-solution buffers: x, then sweeps alternate between x and t
+      for(std::size_t i = 0; i + 1 < L.size(); ++i)
+      {
+        // pre: entry/restrict writes x, (pre-1) sweeps; post: prolong in place, post sweeps
+        const int flips = (cfg.pre - 1) + cfg.post;
+        L[i]->xfinal = (flips % 2 == 0) ? L[i]->x.template as<DT>() : L[i]->t.template as<DT>();
+      }
+      L.back()->xfinal = L.back()->x.template as<DT>();
+
+      // outer PCG
+      const HostLevel& h0 = lv.front();
+      n0 = h0.A.rows;
+      bool shared = false;
+      if constexpr(std::is_same<VT, double>::value)
+      {
+        if(!want_sell) { A64 = &L.front()->A; shared = true; }
+      }
+      if(!shared)
+      {
+        F3_RC(A64own.upload(h0.A, s, false, 64, 1, cfg.lanes));
+        A64 = &A64own;
+      }
+      F3_RC(r.alloc(sizeof(double) * n0));
+      F3_RC(p.alloc(sizeof(double) * n0));
+      F3_RC(q.alloc(sizeof(double) * n0));
+      F3_RC(zcg.alloc(sizeof(double) * n0));
+      F3_RC(state.alloc(sizeof(mg::PcgState)));
+      hist_cap = cfg.max_iter + 2;
+      F3_RC(hist.alloc(sizeof(double) * hist_cap));
+      // one partial per block of the largest reducing launch: the row-group kernels on the
+      // outer matrix launch rows*lanes/256 blocks, the streaming ones at most 4096
+      const u64 max_blocks = std::max<u64>(4096, nblocks(A64->rows * u64(A64->lanes)));
+      F3_RC(partials.alloc(sizeof(double) * max_blocks));
+      F3_RC(ticket.alloc(64));
+      F3_CHECK(hipMemsetAsync(ticket.p, 0, 64, s));
+      if(!pinned) F3_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), 64, hipHostMallocDefault));
+      for(auto& e : ev) if(!e) F3_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      if(!t0) F3_CHECK(hipEventCreate(&t0));
+      if(!t1) F3_CHECK(hipEventCreate(&t1));
+      F3_CHECK(hipStreamSynchronize(s));
+      if(gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+      if(graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+      return rt::kOk;
+    }
+
+    mg::PcgState* st() const { return state.template as<mg::PcgState>(); }
+    const int* status_ptr() const { return reinterpret_cast<const int*>(&st()->status); }
+
+    // ---- V-cycle -------------------------------------------------------------------------
+    int enqueue_vcycle(const double* def, const int* status)
+    {
+      const std::size_t nl = L.size();
+      Level<DT, VT>& top = *L[0];
+      mg::k_mg_entry<DT><<<stream_blocks(top.n), kBlock, 0, s>>>(top.n, def, top.rhs.template as<DT>(),
+                                                                top.x.template as<DT>(), top.dinv.template as<DT>(), status);
+      std::vector<DT*> cur(nl, nullptr), oth(nl, nullptr);
+      for(std::size_t i = 0; i + 1 < nl; ++i)
+      {
+        Level<DT, VT>& f = *L[i];
+        Level<DT, VT>& c = *L[i + 1];
+        cur[i] = f.x.template as<DT>();
+        oth[i] = f.t.template as<DT>();
+        for(int k = 1; k < cfg.pre; ++k)
+        {
+          F3_RC(sweep(f, cur[i], oth[i], status));
+          std::swap(cur[i], oth[i]);
+        }
+        F3_RC(with_rows<DT, VT>(f.A, [&](auto A, mg::RowGrid g) {
+          mg::k_defect<<<g.blocks, kBlock, 0, s>>>(A, g.per,f.mask.template as<u32>(), f.rhs.template as<DT>(),
+                                                       (const DT*)cur[i], f.def.template as<DT>(), status);
+        }));
+        DT* xc = (i + 2 < nl) ? c.x.template as<DT>() : nullptr;
+        F3_RC(with_rows<DT, VT>(f.R, [&](auto R, mg::RowGrid g) {
+          mg::k_restrict<<<g.blocks, kBlock, 0, s>>>(R, g.per,c.mask.template as<u32>(), f.def.template as<DT>(),
+                                                         c.rhs.template as<DT>(), xc, c.dinv.template as<DT>(), status);
+        }));
+      }
+      Level<DT, VT>& cl = *L.back();
+      mg::k_coarse<DT><<<1, kBlock, 0, s>>>(cl.n, ainv.template as<double>(), cl.mask.template as<u32>(),
+                                            cl.rhs.template as<DT>(), cl.x.template as<DT>(), status);
+      for(std::size_t ii = nl - 1; ii > 0; --ii)
+      {
+        const std::size_t i = ii - 1;
+        Level<DT, VT>& f = *L[i];
+        Level<DT, VT>& c = *L[i + 1];
+        F3_RC(with_rows<DT, VT>(f.P, [&](auto P, mg::RowGrid g) {
+          mg::k_prolong<<<g.blocks, kBlock, 0, s>>>(P, g.per,f.mask.template as<u32>(), (const DT*)c.xfinal, cur[i], status);
+        }));
+        for(int k = 0; k < cfg.post; ++k)
+        {
+          F3_RC(sweep(f, cur[i], oth[i], status));
+          std::swap(cur[i], oth[i]);
+        }
+        if(cur[i] != f.xfinal) return rt::kErrState;
+      }
+      return rt::check_launch();
+    }
+
+    int sweep(Level<DT, VT>& f, DT* xin, DT* xout, const int* status)
+    {
+      return with_rows<DT, VT>(f.A, [&](auto A, mg::RowGrid g) {
+        mg::k_sweep<<<g.blocks, kBlock, 0, s>>>(A, g.per,f.dinv.template as<DT>(), f.rhs.template as<DT>(),
+                                                    (const DT*)xin, xout, status);
+      });
+    }
+
+    const DT* z_ptr() const { return L.front()->xfinal; }
+
+    int vcycle(double* cor, const double* def) override
+    {
+      F3_RC(enqueue_vcycle(def, nullptr));
+      // cor = (double) z
+      k_to_double<<<stream_blocks(n0), kBlock, 0, s>>>(n0, z_ptr(), cor);
+      F3_RC(rt::check_launch());
+      F3_CHECK(hipStreamSynchronize(s));
+      return rt::kOk;
+    }
+
+    // ---- PCG ------------------------------------------------------------------------------
+    int enqueue_precond(const int* status)
+    {
+      if(cfg.precond == 0)
+        return enqueue_vcycle(r.template as<double>(), status);
+      mg::k_copy_filter<<<stream_blocks(n0), kBlock, 0, s>>>(n0, r.template as<double>(), mask0_ptr(), zcg.template as<double>(),
+                                                            status);
+      return rt::check_launch();
+    }
+    const u32* mask0_ptr() const { return L.front()->mask.template as<u32>(); }
+
+    template<typename F>
+    int with_z(F&& f)
+    {
+      if(cfg.precond == 0) f(z_ptr());
+      else f((const double*)zcg.template as<double>());
+      return rt::kOk;
+    }
+
+    int enqueue_iteration()
+    {
+      mg::PcgState* S = st();
+      double* part = partials.template as<double>();
+      unsigned* tk = ticket.template as<unsigned>();
+      F3_RC(with_rows<double, double>(outer_matrix(), [&](auto A, mg::RowGrid g) {
+        mg::k_pcg_spmv_dot<<<g.blocks, kBlock, 0, s>>>(A, g.per,mask0_ptr(), p.template as<double>(), q.template as<double>(),
+                                                           S, part, tk);
+      }));
+      mg::k_pcg_update_norm<<<stream_blocks(n0), kBlock, 0, s>>>(n0, p.template as<double>(), q.template as<double>(),
+                                                                x_cur, r.template as<double>(), S, hist.template as<double>(),
+                                                                part, tk);
+      F3_RC(enqueue_precond(status_ptr()));
+      F3_RC(with_z([&](auto z) {
+        mg::k_pcg_gamma_beta<<<stream_blocks(n0), kBlock, 0, s>>>(n0, z, r.template as<double>(), S, part, tk);
+        mg::k_pcg_update_p<<<stream_blocks(n0), kBlock, 0, s>>>(n0, z, p.template as<double>(), S);
+      }));
+      return rt::check_launch();
+    }
+
+    double* x_cur = nullptr;   // the solution vector of the running solve
+    double* x_graph = nullptr; // the one the captured graph was built for
+
+    int solve(double* x, const double* b, bool correct, int* status, u64* iters) override
+    {
+      if(!x || !b) return rt::kErrArg;
+      mg::PcgState h{};
+      h.tol_rel = cfg.tol_rel; h.tol_abs = cfg.tol_abs; h.tol_abs_low = cfg.tol_abs_low;
+      h.div_rel = cfg.div_rel; h.div_abs = cfg.div_abs; h.stag_rate = cfg.stag_rate;
+      h.min_iter = cfg.min_iter; h.max_iter = cfg.max_iter; h.min_stag_iter = cfg.min_stag_iter;
+      h.hist_cap = hist_cap; h.status = mg::kUndefined;
+      F3_CHECK(hipMemcpyAsync(state.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+      mg::PcgState* S = st();
+      double* part = partials.template as<double>();
+      unsigned* tk = ticket.template as<unsigned>();
+      x_cur = x;
+      if(correct)
+      {
+        F3_RC(with_rows<double, double>(outer_matrix(), [&](auto A, mg::RowGrid g) {
+          mg::k_pcg_defect_norm<<<g.blocks, kBlock, 0, s>>>(A, g.per,mask0_ptr(), b, (const double*)x, r.template as<double>(),
+                                                                S, hist.template as<double>(), part, tk);
+        }));
+      }
+      else
+      {
+        // apply(cor, def): r = def, cor = 0
+        F3_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n0, s));
+        mg::k_pcg_copy_norm<<<stream_blocks(n0), kBlock, 0, s>>>(n0, b, r.template as<double>(), S, hist.template as<double>(),
+                                                                part, tk);
+      }
+      F3_RC(rt::check_launch());
+      F3_RC(enqueue_precond(status_ptr()));
+      F3_RC(with_z([&](auto z) {
+        mg::k_pcg_p_gamma<<<stream_blocks(n0), kBlock, 0, s>>>(n0, z, r.template as<double>(), p.template as<double>(), S, part, tk);
+      }));
+      F3_RC(rt::check_launch());
+
+      if(cfg.use_graph && (gexec == nullptr || x_graph != x))
+      {
+        if(gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+        if(graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+        F3_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_iteration();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if(rc) { if(g) (void)hipGraphDestroy(g); return rc; }
+        if(e != hipSuccess) return rt::kErrRuntime;
+        graph = g;
+        F3_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        x_graph = x;
+      }
+      // host loop: launch iteration k+1 before looking at the status of iteration k
+      const u64 max_launch = cfg.max_iter + 1;
+      u64 launched = 0;
+      int seen = mg::kProgress;
+      while(launched < max_launch)
+      {
+        if(cfg.use_graph) F3_CHECK(hipGraphLaunch(gexec, s));
+        else F3_RC(enqueue_iteration());
+        F3_CHECK(hipMemcpyAsync(&pinned[launched % 2], &S->status, sizeof(int), hipMemcpyDeviceToHost, s));
+        F3_CHECK(hipEventRecord(ev[launched % 2], s));
+        ++launched;
+        if(launched >= 2)
+        {
+          F3_CHECK(hipEventSynchronize(ev[(launched - 2) % 2]));
+          seen = pinned[(launched - 2) % 2];
+          if(seen != mg::kProgress) break;
+        }
+      }
+      F3_CHECK(hipStreamSynchronize(s));
+      mg::PcgState out{};
+      F3_CHECK(hipMemcpy(&out, state.p, sizeof(out), hipMemcpyDeviceToHost));
+      if(status) *status = out.status;
+      if(iters) *iters = out.num_iter;
+      last_hist_len = std::min<u64>(out.num_iter + 1, hist_cap);
+      return rt::kOk;
+    }
+
+    u64 last_hist_len = 0;
+
+    int history(double* out, u64 cap, u64* len) override
+    {
+      const u64 m = std::min(cap, last_hist_len);
+      if(m) F3_CHECK(hipMemcpy(out, hist.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+      if(len) *len = last_hist_len;
+      return rt::kOk;
+    }
+
+    // times `reps` launches of one kernel of the hot path between two events on the engine's
+    // stream: which = 0 fine-level smoothing sweep, 1 fine-level defect, 2 outer SpMV+dot,
+    // 3 one full V-cycle, 4 one full PCG iteration (graph)
+    int time_kernel(int which, int reps, double* avg_ms, u64* bytes) override
+    {
+      if(reps < 1) return rt::kErrArg;
+      Level<DT, VT>& f = *L.front();
+      // warm-up launch, then the timed region
+      for(int pass = 0; pass < 2; ++pass)
+      {
+        const int nrep = pass == 0 ? 1 : reps;
+        if(pass == 1) F3_CHECK(hipEventRecord(t0, s));
+        for(int k = 0; k < nrep; ++k)
+        {
+          switch(which)
+          {
+            case 0: F3_RC(sweep(f, f.x.template as<DT>(), f.t.template as<DT>(), nullptr)); break;
+            case 1:
+              F3_RC(with_rows<DT, VT>(f.A, [&](auto A, mg::RowGrid g) {
+                mg::k_defect<<<g.blocks, kBlock, 0, s>>>(A, g.per,f.mask.template as<u32>(), f.rhs.template as<DT>(),
+                                                             (const DT*)f.x.template as<DT>(), f.def.template as<DT>(), nullptr);
+              }));
+              break;
+            case 3: F3_RC(enqueue_vcycle(r.template as<double>(), nullptr)); break;
+            default: return rt::kErrArg;
+          }
+        }
+        if(pass == 1) F3_CHECK(hipEventRecord(t1, s));
+        F3_CHECK(hipStreamSynchronize(s));
+      }
+      float ms = 0.f;
+      F3_CHECK(hipEventElapsedTime(&ms, t0, t1));
+      *avg_ms = double(ms) / reps;
+      if(bytes)
+      {
+        const u64 vb = sizeof(DT);
+        // sweep: matrix + x gathered once + rhs + dinv + x_out; defect: matrix + x + rhs + def (+mask)
+        if(which == 0) *bytes = f.A.matrix_bytes() + f.n * vb * 4;
+        else if(which == 1) *bytes = f.A.matrix_bytes() + f.n * vb * 3 + (f.n + 31) / 32 * 4;
+        else *bytes = 0;
+      }
+      return rt::kOk;
+    }
+
+    // info: [0] levels, [1] n finest, [2] nnz finest, [3] lanes finest, [4] sell size finest,
+    // [5] kernels per V-cycle, [6] algorithmic bytes of one V-cycle, [7] bytes of one outer
+    // iteration excluding the V-cycle, [8] sizeof inner DT, [9] sizeof VT
+    int info(u64* out, int cap) override
+    {
+      std::vector<u64> v(10, 0);
+      v[0] = L.size();
+      v[1] = L.front()->n;
+      v[2] = L.front()->A.nnz;
+      v[3] = u64(L.front()->A.lanes);
+      v[4] = L.front()->A.sell_size;
+      u64 kern = 2, bytes = 0;
+      const u64 vb = sizeof(DT);
+      bytes += L.front()->n * (8 + vb * 3); // entry
+      for(std::size_t i = 0; i + 1 < L.size(); ++i)
+      {
+        const Level<DT, VT>& f = *L[i];
+        const u64 sweeps = u64(cfg.pre - 1 + cfg.post);
+        kern += sweeps + 3;
+        bytes += sweeps * (f.A.matrix_bytes() + f.n * vb * 4);
+        bytes += f.A.matrix_bytes() + f.n * vb * 3;                       // defect
+        bytes += f.R.matrix_bytes() + f.n * vb + L[i + 1]->n * vb * 4;     // restrict (+first step)
+        bytes += f.P.matrix_bytes() + L[i + 1]->n * vb + f.n * vb * 2;     // prolong
+      }
+      bytes += ncoarse * ncoarse * 8 + ncoarse * vb * 2;
+      v[5] = kern;
+      v[6] = bytes;
+      v[7] = outer_matrix().matrix_bytes() + n0 * 8 * 2 + n0 * 8 * 5 + n0 * (8 + vb) * 2 + n0 * (8 + vb * 2);
+      v[8] = sizeof(DT);
+      v[9] = sizeof(VT);
+      for(int i = 0; i < cap && i < 10; ++i) out[i] = v[i];
+      return rt::kOk;
+    }
+  };
+} // namespace
+
+// =========================================================================================
+// C ABI
+// =========================================================================================
+struct feat3_mgpcg
+{
+  int inner = 0;     // 0 FP64, 1 FP32, 2 FP16-stored values with FP32 arithmetic
+  Config cfg;
+  std::vector<HostLevel> levels;
+  std::unique_ptr<EngineBase> eng;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+};
+
+extern "C" {
+
+int feat3_mgpcg_create(feat3_mgpcg** h, int inner_precision, int format)
+{
+  if(!h || inner_precision < 0 || inner_precision > 2 || format < 0 || format > 1) return rt::kErrArg;
+  std::unique_ptr<feat3_mgpcg> o(new feat3_mgpcg());
+  o->inner = inner_precision;
+  o->cfg.format = format;
+  if(hipStreamCreateWithFlags(&o->stream, hipStreamNonBlocking) != hipSuccess) return rt::kErrRuntime;
+  o->own_stream = true;
+  *h = o.release();
+  return rt::kOk;
+}
+
+int feat3_mgpcg_destroy(feat3_mgpcg* h)
+{
+  if(!h) return rt::kOk;
+  if(h->stream) (void)hipStreamSynchronize(h->stream);
+  h->eng.reset();
+  if(h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return rt::kOk;
+}
+
+// push one level (finest first); host arrays are copied.  The transfer arrays describe the
+// prolongation P (n x nc) and restriction R (nc x n) to the next coarser level and are
+// ignored (may be NULL) for the coarsest level.
+int feat3_mgpcg_push_level(feat3_mgpcg* h, u64 n, const u32* row_ptr, const u32* col_ind, const double* val,
+                           u64 nfilt, const u32* filter_idx, u64 nc, const u32* p_row_ptr, const u32* p_col_ind,
+                           const double* p_val, const u32* r_row_ptr, const u32* r_col_ind, const double* r_val)
+{
+  if(!h || !row_ptr || (n > 0 && (!col_ind || !val))) return rt::kErrArg;
+  HostLevel L;
+  L.A.rows = n; L.A.cols = n;
+  L.A.rp.assign(row_ptr, row_ptr + n + 1);
+  const u64 nnz = row_ptr[n];
+  L.A.col.assign(col_ind, col_ind + nnz);
+  L.A.val.assign(val, val + nnz);
+  for(u64 k = 0; k < nnz; ++k) if(col_ind[k] >= n) return rt::kErrArg;
+  if(nfilt && !filter_idx) return rt::kErrArg;
+  L.fidx.assign(filter_idx, filter_idx + nfilt);
+  for(u32 i : L.fidx) if(i >= n) return rt::kErrArg;
+  if(nc > 0)
+  {
+    if(!p_row_ptr || !p_col_ind || !p_val || !r_row_ptr || !r_col_ind || !r_val) return rt::kErrArg;
+    L.P.rows = n; L.P.cols = nc;
+    L.P.rp.assign(p_row_ptr, p_row_ptr + n + 1);
+    L.P.col.assign(p_col_ind, p_col_ind + p_row_ptr[n]);
+    L.P.val.assign(p_val, p_val + p_row_ptr[n]);
+    L.R.rows = nc; L.R.cols = n;
+    L.R.rp.assign(r_row_ptr, r_row_ptr + nc + 1);
+    L.R.col.assign(r_col_ind, r_col_ind + r_row_ptr[nc]);
+    L.R.val.assign(r_val, r_val + r_row_ptr[nc]);
+    for(u32 c : L.P.col) if(c >= nc) return rt::kErrArg;
+    for(u32 c : L.R.col) if(c >= n) return rt::kErrArg;
+  }
+  h->levels.push_back(std::move(L));
+  h->eng.reset();
+  return rt::kOk;
+}
+
+int feat3_mgpcg_set_smoother(feat3_mgpcg* h, double omega_jacobi, double omega_richardson, int pre, int post)
+{
+  if(!h || pre < 1 || post < 1 || !(omega_jacobi > 0.0)) return rt::kErrArg;
+  h->cfg.omega_j = omega_jacobi; h->cfg.omega_r = omega_richardson; h->cfg.pre = pre; h->cfg.post = post;
+  h->eng.reset();
+  return rt::kOk;
+}
+
+int feat3_mgpcg_set_tolerances(feat3_mgpcg* h, double tol_rel, double tol_abs, double tol_abs_low, u64 min_iter, u64 max_iter)
+{
+  if(!h || max_iter == 0) return rt::kErrArg;
+  h->cfg.tol_rel = tol_rel;
+  if(tol_abs > 0) h->cfg.tol_abs = tol_abs;
+  h->cfg.tol_abs_low = tol_abs_low;
+  h->cfg.min_iter = min_iter;
+  h->cfg.max_iter = max_iter;
+  // an initialised solver keeps running unless its history buffer is too small
+  if(h->eng && !h->eng->set_tolerances(h->cfg)) h->eng.reset();
+  return rt::kOk;
+}
+
+// 0: MG V-cycle preconditioner, 1: none (plain CG)
+int feat3_mgpcg_set_precond(feat3_mgpcg* h, int kind)
+{
+  if(!h || kind < 0 || kind > 1) return rt::kErrArg;
+  h->cfg.precond = kind;
+  h->eng.reset();
+  return rt::kOk;
+}
+
+// graph: replay the captured iteration (1) or launch kernel by kernel (0);
+// lanes: lanes per CSR row for all levels (0 = by mean row length);
+// sell_C / sell_sigma: chunk height and sort window of the SELL format (format 1)
+int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, u64 sell_C, u64 sell_sigma)
+{
+  if(!h) return rt::kErrArg;
+  if(lanes != 0 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) return rt::kErrArg;
+  if(sell_C == 0 || sell_C > 1024 || sell_sigma == 0) return rt::kErrArg;
+  h->cfg.use_graph = graph != 0;
+  h->cfg.lanes = lanes;
+  h->cfg.sell_C = sell_C;
+  h->cfg.sell_sigma = sell_sigma;
+  h->eng.reset();
+  return rt::kOk;
+}
+
+// named options: "index16" (0/1), "compact_transfer" (0/1), "sell_C", "sell_sigma", "lanes"
+int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
+{
+  if(!h || !name) return rt::kErrArg;
+  const std::string k(name);
+  if(k == "index16") h->cfg.index16 = value != 0.0;
+  else if(k == "compact_transfer") h->cfg.compact_transfer = value != 0.0;
+  else if(k == "sell_C" && value >= 1 && value <= 1024) h->cfg.sell_C = u64(value);
+  else if(k == "sell_sigma" && value >= 1) h->cfg.sell_sigma = u64(value);
+  else if(k == "lanes" && (value == 0 || value == 2 || value == 4 || value == 8 || value == 16 || value == 32 || value == 64))
+    h->cfg.lanes = int(value);
+  else return rt::kErrArg;
+  h->eng.reset();
+  return rt::kOk;
+}
+
+// init_symbolic + init_numeric: uploads, converts to the inner precision, Jacobi setup,
+// coarse inverse (solver/base.hpp:227-260, multigrid.hpp:814-818)
+int feat3_mgpcg_init(feat3_mgpcg* h)
+{
+  if(!h || h->levels.empty()) return rt::kErrState;
+  std::unique_ptr<EngineBase> e;
+  if(h->inner == 0) e.reset(new Engine<double, double>());
+  else if(h->inner == 1) e.reset(new Engine<float, float>());
+  else e.reset(new Engine<float, half>());
+  const int rc = e->init(h->levels, h->cfg, h->stream);
+  if(rc) return rc;
+  h->eng = std::move(e);
+  return rt::kOk;
+}
+
+// Solver::solve / IterativeSolver::correct(x, b): x and b are FP64 device vectors
+int feat3_mgpcg_correct(feat3_mgpcg* h, void* caller_stream, double* x, const double* b, int* status, u64* iters)
+{
+  if(!h || !h->eng) return rt::kErrState;
+  if(caller_stream) { if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime; }
+  return h->eng->solve(x, b, true, status, iters);
+}
+
+// SolverBase::apply(cor, def): cor = PCG(def) from a zero start
+int feat3_mgpcg_apply(feat3_mgpcg* h, void* caller_stream, double* cor, const double* def, int* status, u64* iters)
+{
+  if(!h || !h->eng) return rt::kErrState;
+  if(caller_stream) { if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime; }
+  return h->eng->solve(cor, def, false, status, iters);
+}
+
+// the MultiGrid preconditioner alone (one V-cycle): cor = MG(def), FP64 in and out
+int feat3_mgpcg_vcycle(feat3_mgpcg* h, void* caller_stream, double* cor, const double* def)
+{
+  if(!h || !h->eng) return rt::kErrState;
+  if(caller_stream) { if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime; }
+  return h->eng->vcycle(cor, def);
+}
+
+int feat3_mgpcg_history(feat3_mgpcg* h, double* out, u64 cap, u64* len)
+{
+  if(!h || !h->eng) return rt::kErrState;
+  return h->eng->history(out, cap, len);
+}
+
+int feat3_mgpcg_time_kernel(feat3_mgpcg* h, int which, int reps, double* avg_ms, u64* bytes)
+{
+  if(!h || !h->eng || !avg_ms) return rt::kErrState;
+  return h->eng->time_kernel(which, reps, avg_ms, bytes);
+}
+
+int feat3_mgpcg_info(feat3_mgpcg* h, u64* out, int cap)
+{
+  if(!h || !h->eng || !out) return rt::kErrState;
+  return h->eng->info(out, cap);
+}
+
+} // extern "C"

@@ -1,0 +1,160 @@
+"""Host-side problem setup: the systems of the benchmark configurations as CSR hierarchies.
+
+This wraps the C setup routines (``feat3_setup_*``): Q1/Q2 Lagrange spaces on the refined unit
+square/cube, Laplace matrices, two-level prolongations with the reference's structural pattern,
+R = P^T, Dirichlet lists and load vectors.  Everything here runs once before the solve (the
+reference's assembly step, kernel/assembly + kernel/geometry) and returns numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import capi
+
+U32 = np.uint32
+F64 = np.float64
+
+
+def _p(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+@dataclass
+class Csr:
+    rows: int
+    cols: int
+    row_ptr: np.ndarray
+    col_ind: np.ndarray
+    val: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_ptr[self.rows])
+
+    def to_dense(self) -> np.ndarray:
+        d = np.zeros((self.rows, self.cols))
+        for i in range(self.rows):
+            s, e = self.row_ptr[i], self.row_ptr[i + 1]
+            np.add.at(d[i], self.col_ind[s:e], self.val[s:e])
+        return d
+
+
+@dataclass
+class Level:
+    A: Csr
+    bnd: np.ndarray
+    P: Optional[Csr] = None   # to the next coarser level (fine x coarse)
+    R: Optional[Csr] = None   # coarse x fine
+    kcell: Optional[np.ndarray] = None
+
+
+@dataclass
+class Hierarchy:
+    dim: int
+    degree: int
+    levels: List[Level] = field(default_factory=list)   # finest first
+    level_fine: int = 0
+    level_coarse: int = 0
+
+    @property
+    def n(self) -> int:
+        return self.levels[0].A.rows
+
+
+def laplace(dim: int, degree: int, level: int, kcell: Optional[np.ndarray] = None):
+    n, nnz, nb = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    capi.call("feat3_setup_laplace_sizes", dim, degree, level, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(nb))
+    rp = np.empty(n.value + 1, U32)
+    col = np.empty(nnz.value, U32)
+    val = np.empty(nnz.value, F64)
+    bnd = np.empty(nb.value, U32)
+    kc = None if kcell is None else np.ascontiguousarray(kcell, F64)
+    capi.call("feat3_setup_laplace", dim, degree, level, _p(kc), _p(rp), _p(col), _p(val), _p(bnd))
+    return Csr(n.value, n.value, rp, col, val), bnd
+
+
+def prolongation(dim: int, degree: int, level_fine: int) -> Csr:
+    nf, nc, nnz = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    capi.call("feat3_setup_prol_sizes", dim, degree, level_fine, ctypes.byref(nf), ctypes.byref(nc), ctypes.byref(nnz))
+    rp = np.empty(nf.value + 1, U32)
+    col = np.empty(nnz.value, U32)
+    val = np.empty(nnz.value, F64)
+    capi.call("feat3_setup_prol", dim, degree, level_fine, _p(rp), _p(col), _p(val))
+    return Csr(nf.value, nc.value, rp, col, val)
+
+
+def transpose(m: Csr) -> Csr:
+    rp = np.empty(m.cols + 1, U32)
+    col = np.empty(m.nnz, U32)
+    val = np.empty(m.nnz, F64)
+    capi.call("feat3_setup_transpose", m.rows, m.cols, _p(m.row_ptr), _p(m.col_ind), _p(m.val), _p(rp), _p(col), _p(val))
+    return Csr(m.cols, m.rows, rp, col, val)
+
+
+def load_vector(dim: int, degree: int, level: int, func: int = 0) -> np.ndarray:
+    """func: 0 sine bubble (f = dim*pi^2*prod sin(pi x_i)), 1 constant one, 2 exp bubble."""
+    n, nnz, nb = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    capi.call("feat3_setup_laplace_sizes", dim, degree, level, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(nb))
+    b = np.empty(n.value, F64)
+    capi.call("feat3_setup_rhs", dim, degree, level, func, _p(b))
+    return b
+
+
+def coarsen_cellfield(dim: int, level_fine: int, kf: np.ndarray) -> np.ndarray:
+    kc = np.empty(kf.size >> dim, F64)
+    capi.call("feat3_setup_coarsen_cellfield", dim, level_fine, _p(np.ascontiguousarray(kf, F64)), _p(kc))
+    return kc
+
+
+def ref_stiffness(dim: int, degree: int) -> np.ndarray:
+    nloc = (degree + 1) ** dim
+    K = np.empty(nloc * nloc, F64)
+    capi.call("feat3_setup_ref_stiffness", dim, degree, _p(K))
+    return K.reshape(nloc, nloc)
+
+
+def hierarchy(dim: int, degree: int, level_fine: int, level_coarse: int = 1,
+              kcell: Optional[np.ndarray] = None) -> Hierarchy:
+    """Levels level_fine (finest, index 0) down to level_coarse.  kcell: per-cell coefficient
+    on the finest level; coarser levels use the mean of the children."""
+    H = Hierarchy(dim, degree, [], level_fine, level_coarse)
+    k = kcell
+    for lvl in range(level_fine, level_coarse - 1, -1):
+        A, bnd = laplace(dim, degree, lvl, k)
+        L = Level(A, bnd, kcell=k)
+        if lvl > level_coarse:
+            L.P = prolongation(dim, degree, lvl)
+            L.R = transpose(L.P)
+            if k is not None:
+                k = coarsen_cellfield(dim, lvl, k)
+        H.levels.append(L)
+    return H
+
+
+def filtered_rhs(H: Hierarchy, func: int = 0) -> np.ndarray:
+    """Load vector on the finest level with homogeneous Dirichlet values (filter_rhs)."""
+    b = load_vector(H.dim, H.degree, H.level_fine, func)
+    b[H.levels[0].bnd] = 0.0
+    return b
+
+
+# -------------------------------------------------------------------------------------------
+# the benchmark configurations
+# -------------------------------------------------------------------------------------------
+CONFIGS = {
+    1: dict(dim=2, degree=1, level=6, name="2D Poisson Q1 L6 (4225 DOF)"),
+    2: dict(dim=2, degree=2, level=9, name="2D Poisson Q2 L9 (1,050,625 DOF)"),
+    3: dict(dim=3, degree=1, level=7, name="3D Poisson Q1 L7 (2,146,689 DOF)"),
+    4: dict(dim=2, degree=1, level=10, name="2D variable-coefficient Q1 L10 (1,050,625 DOF)"),
+    5: dict(dim=3, degree=1, level=6, name="3D vector Laplacian Q1 L6 (274,625 nodes x 3)"),
+}
+
+
+def config_hierarchy(cfg: int, level: Optional[int] = None) -> Hierarchy:
+    c = CONFIGS[cfg]
+    lvl = c["level"] if level is None else level
+    return hierarchy(c["dim"], c["degree"], lvl, 1)

@@ -1,0 +1,333 @@
+/*
+ * feat3_hip.h -- C ABI of the MI355X-native LAFEM + Solver hot path (libfeat3_hip.so).
+ *
+ * Conventions
+ *  - Every function returns int: 0 on success, otherwise one of FEAT3_HIP_ERR_*; the
+ *    message is feat3_hip_error_string(code).  A reference-side shim turns a non-zero code
+ *    into FEAT::InternalError (util/exception.hpp), as the reference's CUDA backend does for
+ *    cuBLAS/cuSPARSE errors (lafem/arch/apply.cu:104-105).
+ *  - `stream` is a hipStream_t passed as void* (NULL = the legacy default stream).  Kernels
+ *    are launched asynchronously on it; only the *_value reductions synchronise it, like the
+ *    reference's host-returning cuBLAS reductions (lafem/arch/dot_product.cu:22-41).
+ *  - All array arguments are device pointers unless stated otherwise; sizes are element
+ *    counts.  Suffixes: f64/f32 = value type, u32/u64 = index type (FEAT's Index is u64,
+ *    kernel/base_header.hpp:94-98), h16f32 = binary16 stored values with FP32 vectors.
+ *  - No function allocates device memory per call, except the first call on a device of a
+ *    reduction (dot/norm), which creates that device's reduction workspace once; the
+ *    solver object allocates in feat3_mgpcg_init only.
+ *  - Not thread-safe by contract (one host thread drives one device), as the reference.
+ *
+ * The first group mirrors LAFEM::Arch::<Op><Mem::CUDA> (lafem/arch/apply.hpp:267-303 and the
+ * sibling headers): a reference build adds Mem::HIP and forwards Arch::<Op><Mem::HIP> to these
+ * functions with the same raw-pointer arguments.  The second group mirrors
+ * MemoryPool<Mem::CUDA> (util/memory_pool.hpp:53-377, util/memory_pool.cu:56-320).  The third
+ * group is the fused device-resident solver with the Solver lifecycle of solver/base.hpp.
+ */
+#ifndef FEAT3_HIP_H
+#define FEAT3_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEAT3_HIP_OK 0
+#define FEAT3_HIP_ERR_ARG 1     /* invalid argument: null pointer, unsupported size/format */
+#define FEAT3_HIP_ERR_LAUNCH 2  /* kernel launch failed */
+#define FEAT3_HIP_ERR_RUNTIME 3 /* HIP runtime error (allocation, copy, synchronisation) */
+#define FEAT3_HIP_ERR_STATE 4   /* object used in the wrong lifecycle state */
+#define FEAT3_HIP_ERR_NUMERIC 5 /* numerical breakdown during setup (singular coarse matrix) */
+
+/* solver status, as FEAT::Solver::Status (solver/base.hpp:48-64) */
+#define FEAT3_STATUS_UNDEFINED 0
+#define FEAT3_STATUS_PROGRESS 1
+#define FEAT3_STATUS_SUCCESS 2
+#define FEAT3_STATUS_ABORTED 3
+#define FEAT3_STATUS_DIVERGED 4
+#define FEAT3_STATUS_MAX_ITER 5
+#define FEAT3_STATUS_STAGNATED 6
+
+const char* feat3_hip_error_string(int code);
+
+/* ===================================================================================== */
+/* Arch::Apply<Mem::HIP>                                                                  */
+/* ===================================================================================== */
+
+/* Apply::csr (apply.hpp:271-274, CUDA apply.cu:242-302): r = a*A*x + b*y; |b| < eps ignores
+ * y; r may alias y but not x; transposed != 0 computes r = a*A^T*x + b*y. */
+int lafem_hip_apply_csr_f64_u32(void* stream, double* r, double a, const double* x, double b, const double* y,
+                                const double* val, const uint32_t* col_ind, const uint32_t* row_ptr, uint64_t rows,
+                                uint64_t columns, uint64_t used_elements, int transposed);
+int lafem_hip_apply_csr_f64_u64(void* stream, double* r, double a, const double* x, double b, const double* y,
+                                const double* val, const uint64_t* col_ind, const uint64_t* row_ptr, uint64_t rows,
+                                uint64_t columns, uint64_t used_elements, int transposed);
+int lafem_hip_apply_csr_f32_u32(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                const float* val, const uint32_t* col_ind, const uint32_t* row_ptr, uint64_t rows,
+                                uint64_t columns, uint64_t used_elements, int transposed);
+int lafem_hip_apply_csr_f32_u64(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                const float* val, const uint64_t* col_ind, const uint64_t* row_ptr, uint64_t rows,
+                                uint64_t columns, uint64_t used_elements, int transposed);
+/* binary16 stored values, FP32 arithmetic (no reference counterpart: storage variant of
+ * Apply::csr for the mixed-precision smoother) */
+int lafem_hip_apply_csr_h16f32_u32(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                   const void* val, const uint32_t* col_ind, const uint32_t* row_ptr, uint64_t rows,
+                                   uint64_t columns, uint64_t used_elements);
+
+/* Apply::ell (apply.hpp:296, CUDA apply.cu:42-61,433-457): FEAT's SELL-C layout with
+ * sigma = 1: val/col_ind[cs[row/C] + row%C + k*C], cl = chunk lengths. */
+int lafem_hip_apply_ell_f64_u32(void* stream, double* r, double a, const double* x, double b, const double* y,
+                                const double* val, const uint32_t* col_ind, const uint32_t* cs, const uint32_t* cl,
+                                uint64_t C, uint64_t rows);
+int lafem_hip_apply_ell_f64_u64(void* stream, double* r, double a, const double* x, double b, const double* y,
+                                const double* val, const uint64_t* col_ind, const uint64_t* cs, const uint64_t* cl,
+                                uint64_t C, uint64_t rows);
+int lafem_hip_apply_ell_f32_u32(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                const float* val, const uint32_t* col_ind, const uint32_t* cs, const uint32_t* cl,
+                                uint64_t C, uint64_t rows);
+int lafem_hip_apply_ell_f32_u64(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                const float* val, const uint64_t* col_ind, const uint64_t* cs, const uint64_t* cl,
+                                uint64_t C, uint64_t rows);
+/* SELL-C-sigma (extension of Apply::ell, same chunk layout over rows permuted inside windows
+ * of sigma rows): position p holds row perm[p] with true length rl[p]. */
+int lafem_hip_apply_sell_f64_u32(void* stream, double* r, double a, const double* x, double b, const double* y,
+                                 const double* val, const uint32_t* col_ind, const uint32_t* cs, const uint32_t* cl,
+                                 const uint32_t* rl, const uint32_t* perm, uint64_t C, uint64_t rows);
+int lafem_hip_apply_sell_f64_u64(void* stream, double* r, double a, const double* x, double b, const double* y,
+                                 const double* val, const uint64_t* col_ind, const uint64_t* cs, const uint64_t* cl,
+                                 const uint64_t* rl, const uint64_t* perm, uint64_t C, uint64_t rows);
+int lafem_hip_apply_sell_f32_u32(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                 const float* val, const uint32_t* col_ind, const uint32_t* cs, const uint32_t* cl,
+                                 const uint32_t* rl, const uint32_t* perm, uint64_t C, uint64_t rows);
+int lafem_hip_apply_sell_f32_u64(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                 const float* val, const uint64_t* col_ind, const uint64_t* cs, const uint64_t* cl,
+                                 const uint64_t* rl, const uint64_t* perm, uint64_t C, uint64_t rows);
+int lafem_hip_apply_sell_h16f32_u32(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                    const void* val, const uint32_t* col_ind, const uint32_t* cs, const uint32_t* cl,
+                                    const uint32_t* rl, const uint32_t* perm, uint64_t C, uint64_t rows);
+
+/* Apply::csrb (apply.hpp:277, CUDA apply.cu:203-231): BCSR with row-major square blocks,
+ * block_height == block_width in {1,2,3,4}. */
+int lafem_hip_apply_csrb_f64_u32(void* stream, double* r, double a, const double* x, double b, const double* y,
+                                 const double* val, const uint32_t* col_ind, const uint32_t* row_ptr, uint64_t rows,
+                                 uint64_t columns, uint64_t used_elements, int block_height, int block_width);
+int lafem_hip_apply_csrb_f64_u64(void* stream, double* r, double a, const double* x, double b, const double* y,
+                                 const double* val, const uint64_t* col_ind, const uint64_t* row_ptr, uint64_t rows,
+                                 uint64_t columns, uint64_t used_elements, int block_height, int block_width);
+int lafem_hip_apply_csrb_f32_u32(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                 const float* val, const uint32_t* col_ind, const uint32_t* row_ptr, uint64_t rows,
+                                 uint64_t columns, uint64_t used_elements, int block_height, int block_width);
+int lafem_hip_apply_csrb_f32_u64(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                 const float* val, const uint64_t* col_ind, const uint64_t* row_ptr, uint64_t rows,
+                                 uint64_t columns, uint64_t used_elements, int block_height, int block_width);
+
+/* Apply::csrsb (apply.hpp:283, CUDA apply.cu:174-200): scalar CSR times a vector of
+ * block_size-blocks (block_size in {1,2,3}). */
+int lafem_hip_apply_csrsb_f64_u32(void* stream, double* r, double a, const double* x, double b, const double* y,
+                                  const double* val, const uint32_t* col_ind, const uint32_t* row_ptr, uint64_t rows,
+                                  uint64_t columns, uint64_t used_elements, int block_size);
+int lafem_hip_apply_csrsb_f64_u64(void* stream, double* r, double a, const double* x, double b, const double* y,
+                                  const double* val, const uint64_t* col_ind, const uint64_t* row_ptr, uint64_t rows,
+                                  uint64_t columns, uint64_t used_elements, int block_size);
+int lafem_hip_apply_csrsb_f32_u32(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                  const float* val, const uint32_t* col_ind, const uint32_t* row_ptr, uint64_t rows,
+                                  uint64_t columns, uint64_t used_elements, int block_size);
+int lafem_hip_apply_csrsb_f32_u64(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                  const float* val, const uint64_t* col_ind, const uint64_t* row_ptr, uint64_t rows,
+                                  uint64_t columns, uint64_t used_elements, int block_size);
+
+/* ===================================================================================== */
+/* BLAS-1: Axpy::dv (axpy.hpp), DotProduct/Norm2::value (dot_product.hpp, norm.hpp),      */
+/* ComponentProduct/ComponentInvert/Scale::value (component_*.hpp, scale.hpp)             */
+/* ===================================================================================== */
+
+/* r = a*x + y (r may alias x or y) */
+int lafem_hip_axpy_f64(void* stream, double* r, double a, const double* x, const double* y, uint64_t n);
+int lafem_hip_axpy_f32(void* stream, float* r, float a, const float* x, const float* y, uint64_t n);
+/* <x,y> into a device scalar (deterministic order), or returned to the host (synchronises) */
+int lafem_hip_dot_f64(void* stream, const double* x, const double* y, uint64_t n, double* result_dev);
+int lafem_hip_dot_f32(void* stream, const float* x, const float* y, uint64_t n, float* result_dev);
+int lafem_hip_dot_value_f64(void* stream, const double* x, const double* y, uint64_t n, double* result_host);
+int lafem_hip_dot_value_f32(void* stream, const float* x, const float* y, uint64_t n, float* result_host);
+/* ||x||_2 into a device scalar, or returned to the host (synchronises) */
+int lafem_hip_norm2_f64(void* stream, const double* x, uint64_t n, double* result_dev);
+int lafem_hip_norm2_f32(void* stream, const float* x, uint64_t n, float* result_dev);
+int lafem_hip_norm2_value_f64(void* stream, const double* x, uint64_t n, double* result_host);
+int lafem_hip_norm2_value_f32(void* stream, const float* x, uint64_t n, float* result_host);
+/* r = x .* y */
+int lafem_hip_component_product_f64(void* stream, double* r, const double* x, const double* y, uint64_t n);
+int lafem_hip_component_product_f32(void* stream, float* r, const float* x, const float* y, uint64_t n);
+/* r = s ./ x */
+int lafem_hip_component_invert_f64(void* stream, double* r, const double* x, double s, uint64_t n);
+int lafem_hip_component_invert_f32(void* stream, float* r, const float* x, float s, uint64_t n);
+/* r = s * x */
+int lafem_hip_scale_f64(void* stream, double* r, const double* x, double s, uint64_t n);
+int lafem_hip_scale_f32(void* stream, float* r, const float* x, float s, uint64_t n);
+
+/* ===================================================================================== */
+/* Diagonal::{csr,ell,csrb} (diagonal.hpp; CUDA diagonal.cu:20-161)                       */
+/* ===================================================================================== */
+int lafem_hip_diagonal_csr_f64_u32(void* stream, double* diag, const double* val, const uint32_t* col_ind,
+                                   const uint32_t* row_ptr, uint64_t rows);
+int lafem_hip_diagonal_csr_f64_u64(void* stream, double* diag, const double* val, const uint64_t* col_ind,
+                                   const uint64_t* row_ptr, uint64_t rows);
+int lafem_hip_diagonal_csr_f32_u32(void* stream, float* diag, const float* val, const uint32_t* col_ind,
+                                   const uint32_t* row_ptr, uint64_t rows);
+int lafem_hip_diagonal_csr_f32_u64(void* stream, float* diag, const float* val, const uint64_t* col_ind,
+                                   const uint64_t* row_ptr, uint64_t rows);
+int lafem_hip_diagonal_ell_f64_u32(void* stream, double* diag, const double* val, const uint32_t* col_ind,
+                                   const uint32_t* cs, const uint32_t* cl, uint64_t C, uint64_t rows);
+int lafem_hip_diagonal_ell_f64_u64(void* stream, double* diag, const double* val, const uint64_t* col_ind,
+                                   const uint64_t* cs, const uint64_t* cl, uint64_t C, uint64_t rows);
+int lafem_hip_diagonal_ell_f32_u32(void* stream, float* diag, const float* val, const uint32_t* col_ind,
+                                   const uint32_t* cs, const uint32_t* cl, uint64_t C, uint64_t rows);
+int lafem_hip_diagonal_ell_f32_u64(void* stream, float* diag, const float* val, const uint64_t* col_ind,
+                                   const uint64_t* cs, const uint64_t* cl, uint64_t C, uint64_t rows);
+int lafem_hip_diagonal_csrb_f64_u32(void* stream, double* diag, const double* val, const uint32_t* col_ind,
+                                    const uint32_t* row_ptr, uint64_t rows, int block_height, int block_width);
+int lafem_hip_diagonal_csrb_f64_u64(void* stream, double* diag, const double* val, const uint64_t* col_ind,
+                                    const uint64_t* row_ptr, uint64_t rows, int block_height, int block_width);
+int lafem_hip_diagonal_csrb_f32_u32(void* stream, float* diag, const float* val, const uint32_t* col_ind,
+                                    const uint32_t* row_ptr, uint64_t rows, int block_height, int block_width);
+int lafem_hip_diagonal_csrb_f32_u64(void* stream, float* diag, const float* val, const uint64_t* col_ind,
+                                    const uint64_t* row_ptr, uint64_t rows, int block_height, int block_width);
+
+/* ===================================================================================== */
+/* UnitFilter::{filter_rhs,filter_def} and the blocked variants (unit_filter.hpp,         */
+/* unit_filter_blocked.hpp; CUDA unit_filter.cu:21-78, unit_filter_blocked.cu:21-96);      */
+/* block_size 1 is the scalar filter.  filter_sol = filter_rhs, filter_cor = filter_def.  */
+/* ===================================================================================== */
+int lafem_hip_unit_filter_rhs_f64_u32(void* stream, double* v, const double* sv_elements, const uint32_t* sv_indices,
+                                      uint64_t ue, int block_size);
+int lafem_hip_unit_filter_rhs_f64_u64(void* stream, double* v, const double* sv_elements, const uint64_t* sv_indices,
+                                      uint64_t ue, int block_size);
+int lafem_hip_unit_filter_rhs_f32_u32(void* stream, float* v, const float* sv_elements, const uint32_t* sv_indices,
+                                      uint64_t ue, int block_size);
+int lafem_hip_unit_filter_rhs_f32_u64(void* stream, float* v, const float* sv_elements, const uint64_t* sv_indices,
+                                      uint64_t ue, int block_size);
+int lafem_hip_unit_filter_def_f64_u32(void* stream, double* v, const uint32_t* sv_indices, uint64_t ue, int block_size);
+int lafem_hip_unit_filter_def_f64_u64(void* stream, double* v, const uint64_t* sv_indices, uint64_t ue, int block_size);
+int lafem_hip_unit_filter_def_f32_u32(void* stream, float* v, const uint32_t* sv_indices, uint64_t ue, int block_size);
+int lafem_hip_unit_filter_def_f32_u64(void* stream, float* v, const uint64_t* sv_indices, uint64_t ue, int block_size);
+
+/* ===================================================================================== */
+/* MemoryPool<Mem::HIP>::set_memory / convert (memory_pool.cu:26-41,176-227)              */
+/* ===================================================================================== */
+int lafem_hip_set_memory_f64(void* stream, double* dst, double value, uint64_t n);
+int lafem_hip_set_memory_f32(void* stream, float* dst, float value, uint64_t n);
+int lafem_hip_set_memory_u32(void* stream, uint32_t* dst, uint32_t value, uint64_t n);
+int lafem_hip_set_memory_u64(void* stream, uint64_t* dst, uint64_t value, uint64_t n);
+int lafem_hip_convert_f32_f64(void* stream, float* dst, const double* src, uint64_t n);
+int lafem_hip_convert_f64_f32(void* stream, double* dst, const float* src, uint64_t n);
+int lafem_hip_convert_u32_u64(void* stream, uint32_t* dst, const uint64_t* src, uint64_t n);
+int lafem_hip_convert_u64_u32(void* stream, uint64_t* dst, const uint32_t* src, uint64_t n);
+/* binary16 <-> FP32 (round to nearest even) */
+int lafem_hip_convert_h16_f32(void* stream, void* dst, const float* src, uint64_t n);
+int lafem_hip_convert_f32_h16(void* stream, float* dst, const void* src, uint64_t n);
+
+/* ===================================================================================== */
+/* MemoryPool<Mem::HIP> / Runtime (memory_pool.hpp:53-377, runtime.cpp:33-144)             */
+/* ===================================================================================== */
+/* initialise: select the device and create its reduction workspace (memory_pool.cu:56-72) */
+int feat3_hip_initialise(int device);
+/* finalise: reports the number of allocations still held (memory_pool.cu:74-95) */
+int feat3_hip_finalise(uint64_t* leaked);
+int feat3_hip_device_count(int* n);
+/* allocate_memory / increase_memory / release_memory (refcounted, memory_pool.cu:97-149) */
+int feat3_hip_malloc(void** p, uint64_t bytes);
+int feat3_hip_increase(void* p);
+int feat3_hip_free(void* p);
+/* upload / download / copy (memory_pool.cu:158-175), asynchronous on stream */
+int feat3_hip_upload(void* stream, void* dst, const void* src, uint64_t bytes);
+int feat3_hip_download(void* stream, void* dst, const void* src, uint64_t bytes);
+int feat3_hip_copy(void* stream, void* dst, const void* src, uint64_t bytes);
+int feat3_hip_synchronise(void* stream);
+/* lanes per row of the CSR/BCSR apply kernels (0 = by mean row length); replaces the block
+ * size globals that lafem/tuning.hpp:72-163 sets */
+int feat3_hip_set_csr_lanes(int lanes);
+
+/* ===================================================================================== */
+/* Fused device solver: PCG( MultiGrid-V( Richardson(omega_R, Jacobi(omega_J)) ) ) with   */
+/* an exact FP64 coarse solve.  Lifecycle of SolverBase (solver/base.hpp:184-316):        */
+/*   create -> push_level (finest first) -> set_* -> init (init_symbolic+init_numeric)    */
+/*   -> correct / apply (any number) -> destroy (done).                                   */
+/* Reference objects replaced: PCG (solver/pcg.hpp:33-328), MultiGrid + MultiGridHierarchy */
+/* (solver/multigrid.hpp:419-2035), Richardson (richardson.hpp:32-230), JacobiPrecond      */
+/* (jacobi_precond.hpp:36-213), ConvertPrecond (convert_precond.hpp:41-237).               */
+/* ===================================================================================== */
+typedef struct feat3_mgpcg feat3_mgpcg;
+
+/* inner_precision: 0 FP64, 1 FP32 inside FP64 PCG, 2 binary16 matrix values with FP32
+ * arithmetic inside FP64 PCG; format: 0 CSR, 1 SELL-C-sigma for the level matrices */
+int feat3_mgpcg_create(feat3_mgpcg** h, int inner_precision, int format);
+int feat3_mgpcg_destroy(feat3_mgpcg* h);
+/* one level (finest first), host arrays, copied: CSR matrix (n x n, u32 indices, FP64
+ * values), Dirichlet index list, and for all but the coarsest level the prolongation
+ * P (n x nc) and restriction R = P^T (nc x n) to the next coarser level
+ * (MultiGridHierarchy::push_level, multigrid.hpp:612-700) */
+int feat3_mgpcg_push_level(feat3_mgpcg* h, uint64_t n, const uint32_t* row_ptr, const uint32_t* col_ind,
+                           const double* val, uint64_t nfilt, const uint32_t* filter_idx, uint64_t nc,
+                           const uint32_t* p_row_ptr, const uint32_t* p_col_ind, const double* p_val,
+                           const uint32_t* r_row_ptr, const uint32_t* r_col_ind, const double* r_val);
+/* Jacobi damping, Richardson damping, pre/post smoothing steps (>= 1) */
+int feat3_mgpcg_set_smoother(feat3_mgpcg* h, double omega_jacobi, double omega_richardson, int pre, int post);
+/* IterativeSolver::set_tol_rel/set_tol_abs/set_min_iter/set_max_iter (iterative.hpp);
+ * tol_abs <= 0 keeps the default 1/eps^2 */
+int feat3_mgpcg_set_tolerances(feat3_mgpcg* h, double tol_rel, double tol_abs, double tol_abs_low, uint64_t min_iter,
+                               uint64_t max_iter);
+/* 0: MG V-cycle preconditioner, 1: none (plain CG) */
+int feat3_mgpcg_set_precond(feat3_mgpcg* h, int kind);
+/* graph replay on/off, lanes per CSR row (0 auto), SELL chunk height and sort window */
+int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C, uint64_t sell_sigma);
+/* named launch/format options (replaces the globals of lafem/tuning.hpp:72-163 that the
+ * reference's block-size tuner sets): "index16" 0/1 (SELL columns as 16-bit row offsets where
+ * they fit), "compact_transfer" 0/1 (device P/R without stored structural zeros), "sell_C",
+ * "sell_sigma", "lanes"; takes effect at the next init */
+int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value);
+/* init_symbolic + init_numeric (solver/base.hpp:227-260) */
+int feat3_mgpcg_init(feat3_mgpcg* h);
+/* Solver::solve / IterativeSolver::correct (iterative.hpp:600,926-935): x, b FP64 device
+ * vectors; caller_stream is synchronised first; blocks until the solve has ended */
+int feat3_mgpcg_correct(feat3_mgpcg* h, void* caller_stream, double* x, const double* b, int* status, uint64_t* iters);
+/* SolverBase::apply(cor, def) (pcg.hpp:141-156) */
+int feat3_mgpcg_apply(feat3_mgpcg* h, void* caller_stream, double* cor, const double* def, int* status, uint64_t* iters);
+/* the MultiGrid preconditioner alone: cor = one V-cycle applied to def (multigrid.hpp:1268-1329) */
+int feat3_mgpcg_vcycle(feat3_mgpcg* h, void* caller_stream, double* cor, const double* def);
+/* defect norms of the last solve: [0] initial, [k] after iteration k */
+int feat3_mgpcg_history(feat3_mgpcg* h, double* out_host, uint64_t cap, uint64_t* len);
+/* average duration of `reps` launches of one hot-path kernel on the solver's stream
+ * (which: 0 finest smoothing sweep, 1 finest defect, 3 full V-cycle) and its algorithmic bytes */
+int feat3_mgpcg_time_kernel(feat3_mgpcg* h, int which, int reps, double* avg_ms, uint64_t* bytes);
+/* [levels, n, nnz, lanes, sell size, kernels per V-cycle, V-cycle bytes, outer bytes,
+ *  sizeof inner, sizeof stored value] */
+int feat3_mgpcg_info(feat3_mgpcg* h, uint64_t* out, int cap);
+
+/* ===================================================================================== */
+/* Host-side problem setup (the step before the hot path: assembly, geometry)         */
+/* ===================================================================================== */
+int feat3_setup_laplace_sizes(int dim, int degree, int level, uint64_t* n, uint64_t* nnz, uint64_t* nbnd);
+int feat3_setup_laplace(int dim, int degree, int level, const double* kcell, uint32_t* row_ptr, uint32_t* col_ind,
+                        double* val, uint32_t* bnd);
+int feat3_setup_rhs(int dim, int degree, int level, int func, double* rhs);
+int feat3_setup_prol_sizes(int dim, int degree, int level_fine, uint64_t* nf, uint64_t* nc, uint64_t* nnz);
+int feat3_setup_prol(int dim, int degree, int level_fine, uint32_t* row_ptr, uint32_t* col_ind, double* val);
+int feat3_setup_transpose(uint64_t rows, uint64_t cols, const uint32_t* row_ptr, const uint32_t* col_ind,
+                          const double* val, uint32_t* t_row_ptr, uint32_t* t_col_ind, double* t_val);
+int feat3_setup_coarsen_cellfield(int dim, int level_fine, const double* kf, double* kc);
+
+/* CSR -> SELL-C-sigma on the host (generalises SparseMatrixELL::convert,
+ * lafem/sparse_matrix_ell.hpp:865-960): rows sorted by length inside windows of sigma rows,
+ * perm[p] = row at position p, padding slots repeat a column of their row with value 0, and
+ * optional 16-bit offsets d16 = col - row (*used_d16 = 1 when all fit).  Host arrays. */
+int feat3_sell_sizes(uint64_t rows, const uint32_t* row_ptr, uint64_t C, uint64_t sigma, uint64_t* nchunks,
+                     uint64_t* val_size);
+int feat3_sell_build(uint64_t rows, uint64_t cols, const uint32_t* row_ptr, const uint32_t* col_ind, const double* val,
+                     uint64_t C, uint64_t sigma, uint32_t* cs, uint32_t* cl, uint32_t* perm, uint32_t* col, double* sval,
+                     int16_t* d16, int* used_d16);
+int feat3_setup_ref_stiffness(int dim, int degree, double* K);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEAT3_HIP_H */

@@ -1,0 +1,163 @@
+"""Parity of the device MG-PCG solver with the oracle's restatement of FEAT's PCG / MultiGrid /
+Richardson / Jacobi / ConvertPrecond on the same hierarchies.
+
+Bars (stated per test): FP64 path: identical iteration count and status, relative solution
+difference <= 1e-10, defect history within 1e-9 relative; mixed FP32 / FP16-stored paths:
+iteration count within +-1 of the oracle's identical mixed algorithm and relative solution
+difference to the FP64 oracle solution <= 1e-8 (both solves stop at relative residual 1e-8).
+"""
+import numpy as np
+import pytest
+
+from feat3_amd import problems as pb
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def torch(torch_cuda):
+    return torch_cuda
+
+
+def solve_gpu(torch, H, b, x0=None, **kw):
+    from feat3_amd.solver import MGPCG
+    s = MGPCG(H, **kw).init()
+    x = torch.zeros(H.n, dtype=torch.float64, device="cuda") if x0 is None else torch.from_numpy(x0.copy()).cuda()
+    db = torch.from_numpy(b).cuda()
+    st, it = s.correct(x, db)
+    out = x.cpu().numpy()
+    hist = s.history()
+    s.done()
+    return st, it, out, hist
+
+
+def solve_orc(H, b, **kw):
+    S = po.System.from_hierarchy(H)
+    x = np.zeros(H.n)
+    st, it, hist = S.solve(x, b, po.params(**kw))
+    return st, it, x, hist
+
+
+CASES = [
+    ("2D Q1 L6 (config 1)", 2, 1, 6),
+    ("2D Q2 L5", 2, 2, 5),
+    ("3D Q1 L5", 3, 1, 5),
+]
+
+
+@pytest.mark.parametrize("name,dim,deg,lvl", CASES)
+def test_fp64_solve_matches_oracle(torch, name, dim, deg, lvl):
+    print(f"case {name}", flush=True)
+    H = pb.hierarchy(dim, deg, lvl, 1)
+    b = pb.filtered_rhs(H, 0)
+    so, io, xo, ho = solve_orc(H, b)
+    sg, ig, xg, hg = solve_gpu(torch, H, b)
+    assert so == 2 and sg == 2
+    assert ig == io
+    assert rel(xg, xo) <= 1e-10
+    assert np.allclose(hg[: io + 1], ho[: io + 1], rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("inner,code", [("f32", 1), ("f16", 2)])
+@pytest.mark.parametrize("name,dim,deg,lvl", CASES)
+def test_mixed_solve_matches_oracle(torch, inner, code, name, dim, deg, lvl):
+    print(f"case {name} inner {inner}", flush=True)
+    H = pb.hierarchy(dim, deg, lvl, 1)
+    b = pb.filtered_rhs(H, 0)
+    so, io, xo, _ = solve_orc(H, b, inner=code)
+    s64, i64, x64, _ = solve_orc(H, b)
+    sg, ig, xg, hg = solve_gpu(torch, H, b, inner=inner)
+    assert sg == 2
+    assert abs(ig - io) <= 1
+    assert hg[ig] <= 1e-8 * hg[0]
+    assert rel(xg, x64) <= 1e-8
+
+
+def test_vcycle_matches_oracle(torch):
+    from feat3_amd.solver import MGPCG
+    H = pb.hierarchy(2, 1, 6, 1)
+    rng = np.random.default_rng(1)
+    d = rng.uniform(-1, 1, H.n)
+    d[H.levels[0].bnd] = 0.0
+    for inner, code, tol in [("f64", 0, 1e-12), ("f32", 1, 1e-5), ("f16", 2, 1e-5)]:
+        S = po.System.from_hierarchy(H)
+        cor = np.zeros(H.n)
+        S.vcycle(cor, d, po.params(inner=code))
+        s = MGPCG(H, inner=inner).init()
+        dd = torch.from_numpy(d).cuda()
+        dc = torch.zeros_like(dd)
+        s.vcycle(dc, dd)
+        assert rel(dc.cpu().numpy(), cor) <= tol, inner
+        s.done()
+
+
+def test_sell_graph_and_lanes_variants_agree(torch):
+    H = pb.hierarchy(3, 1, 4, 1)
+    b = pb.filtered_rhs(H, 0)
+    ref = solve_gpu(torch, H, b)
+    for kw in [dict(fmt="sell"), dict(graph=False), dict(lanes=16), dict(fmt="sell", sell_C=32, sell_sigma=1)]:
+        st, it, x, _ = solve_gpu(torch, H, b, **kw)
+        assert st == 2 and it == ref[1], kw
+        assert rel(x, ref[2]) <= 1e-12, kw
+    # graph replay is bitwise identical to kernel-by-kernel launch (deterministic reductions)
+    st, it, x, _ = solve_gpu(torch, H, b, graph=False)
+    assert np.array_equal(x, ref[2])
+
+
+def test_cg_pointstar_kat(torch):
+    """basic_solver-test.cpp:136-140: plain CG on PointstarFactoryFD(17, 2), 28 +- 2 iterations."""
+    m = 17
+    rp, col, val = po.pointstar_fd(m, 2)
+    n = m * m
+    ref = po.pointstar_q2_bubble(m, 2)
+    b = np.zeros(n)
+    po.csr_apply(b, 1.0, ref, 0.0, b, val, col, rp, n, n)
+    H = pb.Hierarchy(2, 1, [pb.Level(pb.Csr(n, n, rp, col, val), np.zeros(0, np.uint32))], 0, 0)
+    st, it, x, _ = solve_gpu(torch, H, b, precond="none", tol_rel=np.sqrt(np.finfo(float).eps), max_iter=100)
+    assert st == 2
+    assert abs(it - 28) <= 2
+    assert np.sum((x - ref) ** 2) <= np.sqrt(np.finfo(float).eps)
+
+
+def test_status_semantics(torch):
+    """iterative.hpp:744-832: zero rhs -> success after 0 iterations; NaN -> aborted;
+    max_iter reached -> max-iter."""
+    H = pb.hierarchy(2, 1, 4, 1)
+    z = np.zeros(H.n)
+    st, it, x, _ = solve_gpu(torch, H, z)
+    assert st == 2 and it == 0 and not np.any(x)
+    bn = pb.filtered_rhs(H, 0)
+    bn[H.n // 2] = np.nan
+    st, it, _, _ = solve_gpu(torch, H, bn)
+    assert st == 3
+    b = pb.filtered_rhs(H, 0)
+    st, it, _, _ = solve_gpu(torch, H, b, max_iter=1)
+    assert st == 5 and it == 1
+    so, io, _, _ = solve_orc(H, b, max_iter=1)
+    assert so == 5 and io == 1
+
+
+def test_apply_and_restart(torch):
+    """apply(cor, def) starts from zero (pcg.hpp:141-156); correct(x, b) starts from the given
+    x and measures the relative tolerance against its own initial defect."""
+    from feat3_amd.solver import MGPCG
+    H = pb.hierarchy(2, 2, 4, 1)
+    b = pb.filtered_rhs(H, 0)
+    s = MGPCG(H).init()
+    db = torch.from_numpy(b).cuda()
+    cor = torch.full_like(db, 7.0)
+    st, it = s.apply(cor, db)
+    assert st == 2
+    so, io, xo, _ = solve_orc(H, b, use_def=0)
+    assert it == io and rel(cor.cpu().numpy(), xo) <= 1e-10
+    x = cor.clone()
+    s.set_tolerances(1e-8)
+    st, it2 = s.correct(x, db)
+    assert st == 2 and it2 >= 1
+    assert rel(x.cpu().numpy(), xo) <= 1e-9
+    s.done()
