@@ -290,14 +290,21 @@ namespace
     return unsigned(std::max<u64>(1, std::min<u64>(b, 8192)));
   }
 
-  // dot / squared norm: deterministic grid reduction into a device scalar
-  template<typename DT, bool SQRT>
-  __global__ __launch_bounds__(kBlock) void k_dot(const DT* x, const DT* y, u64 n, DT* out, DT* partials, unsigned* ticket)
+  // dot / norm: deterministic two-kernel reduction into a device scalar (block partials,
+  // then one block sums them in block order)
+  template<typename DT>
+  __global__ __launch_bounds__(kBlock) void k_dot(const DT* x, const DT* y, u64 n, DT* partials)
   {
     DT s = DT(0);
     for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
       s += x[i] * y[i];
-    dev::grid_reduce(s, partials, ticket, [&](DT t) { *out = SQRT ? DT(sqrt(t)) : t; });
+    dev::block_partial(s, partials);
+  }
+  template<typename DT, bool SQRT>
+  __global__ __launch_bounds__(kBlock) void k_dot_fin(const DT* partials, unsigned nb, DT* out)
+  {
+    const DT t = dev::sum_partials(partials, nb);
+    if(threadIdx.x == 0) *out = SQRT ? DT(sqrt(t)) : t;
   }
 
   template<typename DT, bool SQRT>
@@ -306,7 +313,9 @@ namespace
     rt::Workspace* ws = rt::workspace();
     if(!ws) return rt::kErrRuntime;
     const unsigned nb = unsigned(std::min<u64>(stream_blocks(n), rt::Workspace::kMaxPartials));
-    k_dot<DT, SQRT><<<nb, kBlock, 0, s>>>(x, y, n, out, reinterpret_cast<DT*>(ws->partials), ws->ticket);
+    DT* part = reinterpret_cast<DT*>(ws->partials);
+    k_dot<DT><<<nb, kBlock, 0, s>>>(x, y, n, part);
+    k_dot_fin<DT, SQRT><<<1, kBlock, 0, s>>>(part, nb, out);
     return rt::check_launch();
   }
 

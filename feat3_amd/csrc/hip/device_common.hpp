@@ -62,35 +62,26 @@ namespace dev
     return s;
   }
 
-  // Deterministic grid reduction: every block stores its partial, the last block to arrive
-  // (ticket) sums the partials in block order and calls fin(total) in thread 0.  The ticket
-  // is reset by the last block, so the counter is reusable across launches and graph
-  // replays without a memset.  Returns true in the thread that ran fin().
-  template<typename T, typename Fin>
-  __device__ __forceinline__ void grid_reduce(T v, T* partials, unsigned int* ticket, Fin fin)
+  // Deterministic two-kernel grid reduction.  Stage 1 (in the producing kernel): every block
+  // stores its block sum to partials[blockIdx.x].  Stage 2 (a one-block kernel launched
+  // after it on the same stream): sum_partials adds them in block order.  The kernel boundary
+  // makes the partials visible across the XCDs' L2s; a per-block fence + ticket atomic instead
+  // forces an L2 write-back in every block, which costs more than the reduced data.
+  template<typename T>
+  __device__ __forceinline__ void block_partial(T v, T* partials)
   {
-    __shared__ bool last;
     const T bs = block_sum(v);
-    if(threadIdx.x == 0)
-    {
-      partials[blockIdx.x] = bs;
-      __threadfence();
-      const unsigned int t = atomicAdd(ticket, 1u);
-      last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if(!last) return;
-    __threadfence();
-    // the last block sums all partials in fixed order
+    if(threadIdx.x == 0) partials[blockIdx.x] = bs;
+  }
+
+  // sum of partials[0, n) in a fixed order; valid in thread 0 (one block)
+  template<typename T>
+  __device__ __forceinline__ T sum_partials(const T* partials, unsigned int n)
+  {
     T s = T(0);
-    for(unsigned int i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-      s += __builtin_nontemporal_load(&partials[i]);
-    s = block_sum(s);
-    if(threadIdx.x == 0)
-    {
-      fin(s);
-      *ticket = 0u;
-    }
+    for(unsigned int i = threadIdx.x; i < n; i += blockDim.x)
+      s += partials[i];
+    return block_sum(s);
   }
 
   // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8), each XCD with

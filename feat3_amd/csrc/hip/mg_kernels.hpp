@@ -222,6 +222,121 @@ namespace mg
     }
   }
 
+  // ---- coarse tail: all small levels in one workgroup ----------------------------------------
+  //
+  // The levels below a size threshold cost one launch each per step (~4-5 us) while holding a
+  // few thousand rows.  k_coarse_tail runs the same V-cycle steps for levels tail[0] (the
+  // finest small level, whose first smoothing step was written by the restriction above it)
+  // down to the coarsest, with block barriers between the steps, in one 1024-thread
+  // workgroup.  Matrices are read in CSR (4 lanes per row).
+  constexpr int kTailMax = 12;
+  constexpr int kTailThreads = 1024;
+  constexpr int kTailG = 4;
+
+  template<typename DT_, typename VT_>
+  struct TailLevel
+  {
+    u64 n;
+    const u32* arp; const u32* acol; const VT_* aval;
+    const u32* prp; const u32* pcol; const VT_* pval;   // to the next coarser tail level
+    const u32* rrp; const u32* rcol; const VT_* rval;
+    const u32* mask;
+    const DT_* dinv;
+    DT_* rhs; DT_* x; DT_* t; DT_* def;
+  };
+
+  template<typename DT_, typename VT_>
+  struct TailDesc
+  {
+    int nlev, pre, post;
+    u64 ncoarse;
+    const double* ainv;
+    TailLevel<DT_, VT_> lv[kTailMax];
+  };
+
+  // calls f(row, row_sum) for every row of a CSR matrix, by the whole block
+  template<typename DT_, typename VT_, typename F>
+  __device__ __forceinline__ void tail_rows(u64 rows, const u32* rp, const u32* col, const VT_* val,
+                                            const DT_* __restrict__ x, F&& f)
+  {
+    const CsrRows<DT_, VT_, kTailG> A{rp, col, val, rows};
+    const int lane = int(threadIdx.x % kTailG);
+    for(u64 p = threadIdx.x / kTailG; p < rows; p += kTailThreads / kTailG)
+    {
+      const DT_ s = A.row_sum(p, lane, x);
+      if(lane == 0) f(p, s);
+    }
+  }
+
+  template<typename DT_, typename VT_>
+  __global__ __launch_bounds__(1024) void k_coarse_tail(TailDesc<DT_, VT_> d, const int* status)
+  {
+    if(stopped(status)) return;
+    DT_* cur[kTailMax];
+    DT_* oth[kTailMax];
+    for(int i = 0; i + 1 < d.nlev; ++i)
+    {
+      const TailLevel<DT_, VT_>& f = d.lv[i];
+      const TailLevel<DT_, VT_>& c = d.lv[i + 1];
+      cur[i] = f.x;
+      oth[i] = f.t;
+      for(int k = 1; k < d.pre; ++k)
+      {
+        DT_* xi = cur[i];
+        DT_* xo = oth[i];
+        tail_rows<DT_, VT_>(f.n, f.arp, f.acol, f.aval, xi, [&](u64 r, DT_ s) { xo[r] = xi[r] + f.dinv[r] * (f.rhs[r] - s); });
+        __syncthreads();
+        cur[i] = xo;
+        oth[i] = xi;
+      }
+      tail_rows<DT_, VT_>(f.n, f.arp, f.acol, f.aval, cur[i], [&](u64 r, DT_ s) {
+        f.def[r] = dev::masked(f.mask, r) ? DT_(0) : f.rhs[r] - s;
+      });
+      __syncthreads();
+      const bool first_step = i + 2 < d.nlev;
+      tail_rows<DT_, VT_>(c.n, f.rrp, f.rcol, f.rval, f.def, [&](u64 r, DT_ s) {
+        const DT_ v = dev::masked(c.mask, r) ? DT_(0) : s;
+        c.rhs[r] = v;
+        if(first_step) c.x[r] = c.dinv[r] * v;
+      });
+      __syncthreads();
+    }
+    {
+      // exact coarse solve with the FP64 inverse
+      const TailLevel<DT_, VT_>& c = d.lv[d.nlev - 1];
+      __shared__ double sr[1024];
+      for(u64 j = threadIdx.x; j < d.ncoarse; j += blockDim.x) sr[j] = double(c.rhs[j]);
+      __syncthreads();
+      for(u64 i = threadIdx.x; i < d.ncoarse; i += blockDim.x)
+      {
+        double s = 0.0;
+        for(u64 j = 0; j < d.ncoarse; ++j) s += d.ainv[i * d.ncoarse + j] * sr[j];
+        c.x[i] = dev::masked(c.mask, i) ? DT_(0) : DT_(s);
+      }
+      __syncthreads();
+    }
+    for(int i = d.nlev - 2; i >= 0; --i)
+    {
+      const TailLevel<DT_, VT_>& f = d.lv[i];
+      const TailLevel<DT_, VT_>& c = d.lv[i + 1];
+      const DT_* xc = (i + 1 == d.nlev - 1) ? c.x : cur[i + 1];
+      DT_* xf = cur[i];
+      tail_rows<DT_, VT_>(f.n, f.prp, f.pcol, f.pval, xc, [&](u64 r, DT_ s) {
+        if(!dev::masked(f.mask, r)) xf[r] = xf[r] + s;
+      });
+      __syncthreads();
+      for(int k = 0; k < d.post; ++k)
+      {
+        DT_* xi = cur[i];
+        DT_* xo = oth[i];
+        tail_rows<DT_, VT_>(f.n, f.arp, f.acol, f.aval, xi, [&](u64 r, DT_ s) { xo[r] = xi[r] + f.dinv[r] * (f.rhs[r] - s); });
+        __syncthreads();
+        cur[i] = xo;
+        oth[i] = xi;
+      }
+    }
+  }
+
   // ---- setup kernels -----------------------------------------------------------------------
 
   // dinv = omega_R * (omega_J / diag), 0 on Dirichlet rows (diagonal_generic.hpp:20-37,
@@ -292,11 +407,13 @@ namespace mg
     st->status = s;
   }
 
-  // r = filter(b - A x)  (pcg.hpp:159-173 correct) fused with ||r|| -> _set_initial_defect
+  // The reducing PCG kernels store one partial per block; k_pcg_fin (one block, launched next
+  // on the stream) sums them in block order and updates the device scalars.
+
+  // r = filter(b - A x)  (pcg.hpp:159-173 correct), partial sums of r.r
   template<typename Rows_>
   __global__ __launch_bounds__(256) void k_pcg_defect_norm(Rows_ A, u64 per, const u32* __restrict__ mask, const double* __restrict__ b,
-                                                           const double* __restrict__ x, double* __restrict__ r,
-                                                           PcgState* st, double* hist, double* partials, unsigned* ticket)
+                                                           const double* __restrict__ x, double* __restrict__ r, double* partials)
   {
     double v = 0.0;
     for_rows(A, per, x, [&](u64 p, double s, bool w) {
@@ -308,12 +425,12 @@ namespace mg
         v += d * d;
       }
     });
-    dev::grid_reduce(v, partials, ticket, [&](double t) { set_initial_defect(st, hist, sqrt(t)); });
+    dev::block_partial(v, partials);
   }
 
-  // r = def, ||r|| -> _set_initial_defect  (pcg.hpp:141-156 apply)
+  // r = def  (pcg.hpp:141-156 apply), partial sums of r.r
   __global__ __launch_bounds__(256) void k_pcg_copy_norm(u64 n, const double* __restrict__ def, double* __restrict__ r,
-                                                         PcgState* st, double* hist, double* partials, unsigned* ticket)
+                                                         double* partials)
   {
     double v = 0.0;
     for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
@@ -322,13 +439,13 @@ namespace mg
       r[i] = d;
       v += d * d;
     }
-    dev::grid_reduce(v, partials, ticket, [&](double t) { set_initial_defect(st, hist, sqrt(t)); });
+    dev::block_partial(v, partials);
   }
 
-  // p = z (correction converted back to FP64, convert_precond.hpp:115), gamma = <r, p>
+  // p = z (correction converted back to FP64, convert_precond.hpp:115), partial sums of r.p
   template<typename DT_>
   __global__ __launch_bounds__(256) void k_pcg_p_gamma(u64 n, const DT_* __restrict__ z, const double* __restrict__ r,
-                                                       double* __restrict__ p, PcgState* st, double* partials, unsigned* ticket)
+                                                       double* __restrict__ p, const PcgState* st, double* partials)
   {
     if(st->status != kProgress) return;
     double v = 0.0;
@@ -338,13 +455,13 @@ namespace mg
       p[i] = zi;
       v += r[i] * zi;
     }
-    dev::grid_reduce(v, partials, ticket, [&](double t) { st->gamma = t; });
+    dev::block_partial(v, partials);
   }
 
-  // q = filter(A p), alpha = gamma / <q, p>   (pcg.hpp:236-242)
+  // q = filter(A p), partial sums of q.p   (pcg.hpp:236-242)
   template<typename Rows_>
   __global__ __launch_bounds__(256) void k_pcg_spmv_dot(Rows_ A, u64 per, const u32* __restrict__ mask, const double* __restrict__ p,
-                                                        double* __restrict__ q, PcgState* st, double* partials, unsigned* ticket)
+                                                        double* __restrict__ q, const PcgState* st, double* partials)
   {
     if(st->status != kProgress) return;
     double v = 0.0;
@@ -357,13 +474,13 @@ namespace mg
         v += qi * p[row];
       }
     });
-    dev::grid_reduce(v, partials, ticket, [&](double t) { st->alpha = st->gamma / t; });
+    dev::block_partial(v, partials);
   }
 
-  // x += alpha p, r -= alpha q, ||r|| -> _set_new_defect   (pcg.hpp:244-259)
+  // x += alpha p, r -= alpha q, partial sums of r.r   (pcg.hpp:244-259)
   __global__ __launch_bounds__(256) void k_pcg_update_norm(u64 n, const double* __restrict__ p, const double* __restrict__ q,
-                                                           double* __restrict__ x, double* __restrict__ r, PcgState* st,
-                                                           double* hist, double* partials, unsigned* ticket)
+                                                           double* __restrict__ x, double* __restrict__ r, const PcgState* st,
+                                                           double* partials)
   {
     if(st->status != kProgress) return;
     const double alpha = st->alpha;
@@ -380,23 +497,35 @@ namespace mg
       }
       v += ri * ri;
     }
-    dev::grid_reduce(v, partials, ticket, [&](double t) { set_new_defect(st, hist, sqrt(t)); });
+    dev::block_partial(v, partials);
   }
 
-  // gamma' = <r, z>, beta = gamma'/gamma   (pcg.hpp:270-279)
+  // partial sums of r.z   (pcg.hpp:270-279)
   template<typename DT_>
   __global__ __launch_bounds__(256) void k_pcg_gamma_beta(u64 n, const DT_* __restrict__ z, const double* __restrict__ r,
-                                                          PcgState* st, double* partials, unsigned* ticket)
+                                                          const PcgState* st, double* partials)
   {
     if(st->status != kProgress) return;
     double v = 0.0;
     for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
       v += r[i] * double(z[i]);
-    dev::grid_reduce(v, partials, ticket, [&](double t) {
-      const double g2 = st->gamma;
-      st->gamma = t;
-      st->beta = t / g2;
-    });
+    dev::block_partial(v, partials);
+  }
+
+  enum : int { kFinInitDefect = 0, kFinAlpha = 1, kFinNewDefect = 2, kFinGamma = 3, kFinBeta = 4 };
+
+  // finalisation of one PCG reduction (one block of 256 threads)
+  template<int OP>
+  __global__ __launch_bounds__(256) void k_pcg_fin(const double* __restrict__ partials, unsigned nb, PcgState* st, double* hist)
+  {
+    if(OP != kFinInitDefect && st->status != kProgress) return;
+    const double t = dev::sum_partials(partials, nb);
+    if(threadIdx.x != 0) return;
+    if(OP == kFinInitDefect) set_initial_defect(st, hist, sqrt(t));        // iterative.hpp:744-766
+    else if(OP == kFinAlpha) st->alpha = st->gamma / t;                     // pcg.hpp:240
+    else if(OP == kFinNewDefect) set_new_defect(st, hist, sqrt(t));         // iterative.hpp:848-879
+    else if(OP == kFinGamma) st->gamma = t;                                 // pcg.hpp:229
+    else { const double g2 = st->gamma; st->gamma = t; st->beta = t / g2; } // pcg.hpp:273-279
   }
 
   // p = beta p + z  (axpy with r == x: r = r*a + y; |beta| < eps: copy)   (pcg.hpp:281-283)

@@ -292,6 +292,10 @@ namespace
     int lanes = 0;            // forced lanes per CSR row (0 = by mean row length)
     bool index16 = true;      // SELL columns as 16-bit row offsets where they fit
     bool compact_transfer = true; // device P/R without the stored structural zeros
+    // levels with at most this many rows run in the single-workgroup coarse-tail kernel; off
+    // by default: on MI355X one CU stays latency-bound even at 27..729 rows and is slower
+    // than one full-chip launch per step
+    u64 tail_max_n = 0;
     double tol_rel = 1e-8, tol_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16), tol_abs_low = 0.0;
     double div_rel = 1.0 / 2.220446049250313e-16, div_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16);
     double stag_rate = 0.95;
@@ -329,7 +333,7 @@ namespace
     DevMem ainv;
     // outer FP64 PCG
     DevCsr<double> A64own;
-    DevMem mask0, x64dummy, r, p, q, zcg, state, hist, partials, ticket;
+    DevMem r, p, q, zcg, state, hist, partials;
     const DevCsr<double>* A64 = nullptr;
     u64 n0 = 0, hist_cap = 0;
     hipGraphExec_t gexec = nullptr;
@@ -426,18 +430,22 @@ namespace
         L[i]->xfinal = (flips % 2 == 0) ? L[i]->x.template as<DT>() : L[i]->t.template as<DT>();
       }
       L.back()->xfinal = L.back()->x.template as<DT>();
+      setup_tail();
 
       // outer PCG
       const HostLevel& h0 = lv.front();
       n0 = h0.A.rows;
+      // the outer FP64 matrix uses the hierarchy's format; with an FP64 hierarchy it is the
+      // finest level's matrix itself
       bool shared = false;
       if constexpr(std::is_same<VT, double>::value)
       {
-        if(!want_sell) { A64 = &L.front()->A; shared = true; }
+        A64 = &L.front()->A;
+        shared = true;
       }
       if(!shared)
       {
-        F3_RC(A64own.upload(h0.A, s, false, 64, 1, cfg.lanes));
+        F3_RC(A64own.upload(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16));
         A64 = &A64own;
       }
       F3_RC(r.alloc(sizeof(double) * n0));
@@ -447,12 +455,10 @@ namespace
       F3_RC(state.alloc(sizeof(mg::PcgState)));
       hist_cap = cfg.max_iter + 2;
       F3_RC(hist.alloc(sizeof(double) * hist_cap));
-      // one partial per block of the largest reducing launch: the row-group kernels on the
-      // outer matrix launch rows*lanes/256 blocks, the streaming ones at most 4096
-      const u64 max_blocks = std::max<u64>(4096, nblocks(A64->rows * u64(A64->lanes)));
+      // one partial per block of the largest reducing launch: row kernels launch at most
+      // 8192 blocks (mg::row_grid), the streaming ones at most 4096
+      const u64 max_blocks = 8192;
       F3_RC(partials.alloc(sizeof(double) * max_blocks));
-      F3_RC(ticket.alloc(64));
-      F3_CHECK(hipMemsetAsync(ticket.p, 0, 64, s));
       if(!pinned) F3_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), 64, hipHostMallocDefault));
       for(auto& e : ev) if(!e) F3_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       if(!t0) F3_CHECK(hipEventCreate(&t0));
@@ -467,6 +473,40 @@ namespace
     const int* status_ptr() const { return reinterpret_cast<const int*>(&st()->status); }
 
     // ---- V-cycle -------------------------------------------------------------------------
+    // first level of the coarse tail (levels t0..nl-1 run in one workgroup); nl - 1 when
+    // only the coarse solve is left to a single kernel
+    std::size_t tail0 = 0;
+    mg::TailDesc<DT, VT> tail{};
+
+    void setup_tail()
+    {
+      const std::size_t nl = L.size();
+      tail0 = nl - 1;
+      while(tail0 > 0 && L[tail0 - 1]->n <= cfg.tail_max_n && nl - (tail0 - 1) <= std::size_t(mg::kTailMax))
+        --tail0;
+      tail = mg::TailDesc<DT, VT>{};
+      tail.nlev = int(nl - tail0);
+      tail.pre = cfg.pre;
+      tail.post = cfg.post;
+      tail.ncoarse = ncoarse;
+      tail.ainv = ainv.template as<double>();
+      for(std::size_t i = tail0; i < nl; ++i)
+      {
+        Level<DT, VT>& l = *L[i];
+        mg::TailLevel<DT, VT>& t = tail.lv[i - tail0];
+        t.n = l.n;
+        t.arp = l.A.rp.template as<u32>(); t.acol = l.A.col.template as<u32>(); t.aval = l.A.val.template as<VT>();
+        if(i + 1 < nl)
+        {
+          t.prp = l.P.rp.template as<u32>(); t.pcol = l.P.col.template as<u32>(); t.pval = l.P.val.template as<VT>();
+          t.rrp = l.R.rp.template as<u32>(); t.rcol = l.R.col.template as<u32>(); t.rval = l.R.val.template as<VT>();
+        }
+        t.mask = l.mask.template as<u32>();
+        t.dinv = l.dinv.template as<DT>();
+        t.rhs = l.rhs.template as<DT>(); t.x = l.x.template as<DT>(); t.t = l.t.template as<DT>(); t.def = l.def.template as<DT>();
+      }
+    }
+
     int enqueue_vcycle(const double* def, const int* status)
     {
       const std::size_t nl = L.size();
@@ -474,7 +514,7 @@ namespace
       mg::k_mg_entry<DT><<<stream_blocks(top.n), kBlock, 0, s>>>(top.n, def, top.rhs.template as<DT>(),
                                                                 top.x.template as<DT>(), top.dinv.template as<DT>(), status);
       std::vector<DT*> cur(nl, nullptr), oth(nl, nullptr);
-      for(std::size_t i = 0; i + 1 < nl; ++i)
+      for(std::size_t i = 0; i < tail0; ++i)
       {
         Level<DT, VT>& f = *L[i];
         Level<DT, VT>& c = *L[i + 1];
@@ -486,25 +526,30 @@ namespace
           std::swap(cur[i], oth[i]);
         }
         F3_RC(with_rows<DT, VT>(f.A, [&](auto A, mg::RowGrid g) {
-          mg::k_defect<<<g.blocks, kBlock, 0, s>>>(A, g.per,f.mask.template as<u32>(), f.rhs.template as<DT>(),
-                                                       (const DT*)cur[i], f.def.template as<DT>(), status);
+          mg::k_defect<<<g.blocks, kBlock, 0, s>>>(A, g.per, f.mask.template as<u32>(), f.rhs.template as<DT>(),
+                                                   (const DT*)cur[i], f.def.template as<DT>(), status);
         }));
         DT* xc = (i + 2 < nl) ? c.x.template as<DT>() : nullptr;
         F3_RC(with_rows<DT, VT>(f.R, [&](auto R, mg::RowGrid g) {
-          mg::k_restrict<<<g.blocks, kBlock, 0, s>>>(R, g.per,c.mask.template as<u32>(), f.def.template as<DT>(),
-                                                         c.rhs.template as<DT>(), xc, c.dinv.template as<DT>(), status);
+          mg::k_restrict<<<g.blocks, kBlock, 0, s>>>(R, g.per, c.mask.template as<u32>(), f.def.template as<DT>(),
+                                                     c.rhs.template as<DT>(), xc, c.dinv.template as<DT>(), status);
         }));
       }
-      Level<DT, VT>& cl = *L.back();
-      mg::k_coarse<DT><<<1, kBlock, 0, s>>>(cl.n, ainv.template as<double>(), cl.mask.template as<u32>(),
-                                            cl.rhs.template as<DT>(), cl.x.template as<DT>(), status);
-      for(std::size_t ii = nl - 1; ii > 0; --ii)
+      if(tail0 + 1 < nl)
+        mg::k_coarse_tail<DT, VT><<<1, mg::kTailThreads, 0, s>>>(tail, status);
+      else
+      {
+        Level<DT, VT>& cl = *L.back();
+        mg::k_coarse<DT><<<1, kBlock, 0, s>>>(cl.n, ainv.template as<double>(), cl.mask.template as<u32>(),
+                                              cl.rhs.template as<DT>(), cl.x.template as<DT>(), status);
+      }
+      for(std::size_t ii = tail0; ii > 0; --ii)
       {
         const std::size_t i = ii - 1;
         Level<DT, VT>& f = *L[i];
         Level<DT, VT>& c = *L[i + 1];
         F3_RC(with_rows<DT, VT>(f.P, [&](auto P, mg::RowGrid g) {
-          mg::k_prolong<<<g.blocks, kBlock, 0, s>>>(P, g.per,f.mask.template as<u32>(), (const DT*)c.xfinal, cur[i], status);
+          mg::k_prolong<<<g.blocks, kBlock, 0, s>>>(P, g.per, f.mask.template as<u32>(), (const DT*)c.xfinal, cur[i], status);
         }));
         for(int k = 0; k < cfg.post; ++k)
         {
@@ -555,21 +600,31 @@ namespace
       return rt::kOk;
     }
 
+    // one-block finalisation of a reduction whose producing kernel ran with nb blocks
+    template<int OP>
+    void fin(unsigned nb)
+    {
+      mg::k_pcg_fin<OP><<<1, kBlock, 0, s>>>(partials.template as<double>(), nb, st(), hist.template as<double>());
+    }
+
     int enqueue_iteration()
     {
       mg::PcgState* S = st();
       double* part = partials.template as<double>();
-      unsigned* tk = ticket.template as<unsigned>();
+      unsigned nb = 0;
       F3_RC(with_rows<double, double>(outer_matrix(), [&](auto A, mg::RowGrid g) {
-        mg::k_pcg_spmv_dot<<<g.blocks, kBlock, 0, s>>>(A, g.per,mask0_ptr(), p.template as<double>(), q.template as<double>(),
-                                                           S, part, tk);
+        mg::k_pcg_spmv_dot<<<g.blocks, kBlock, 0, s>>>(A, g.per, mask0_ptr(), p.template as<double>(), q.template as<double>(),
+                                                       S, part);
+        nb = g.blocks;
       }));
+      fin<mg::kFinAlpha>(nb);
       mg::k_pcg_update_norm<<<stream_blocks(n0), kBlock, 0, s>>>(n0, p.template as<double>(), q.template as<double>(),
-                                                                x_cur, r.template as<double>(), S, hist.template as<double>(),
-                                                                part, tk);
+                                                                x_cur, r.template as<double>(), S, part);
+      fin<mg::kFinNewDefect>(stream_blocks(n0));
       F3_RC(enqueue_precond(status_ptr()));
       F3_RC(with_z([&](auto z) {
-        mg::k_pcg_gamma_beta<<<stream_blocks(n0), kBlock, 0, s>>>(n0, z, r.template as<double>(), S, part, tk);
+        mg::k_pcg_gamma_beta<<<stream_blocks(n0), kBlock, 0, s>>>(n0, z, r.template as<double>(), S, part);
+        fin<mg::kFinBeta>(stream_blocks(n0));
         mg::k_pcg_update_p<<<stream_blocks(n0), kBlock, 0, s>>>(n0, z, p.template as<double>(), S);
       }));
       return rt::check_launch();
@@ -589,26 +644,28 @@ namespace
       F3_CHECK(hipMemcpyAsync(state.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
       mg::PcgState* S = st();
       double* part = partials.template as<double>();
-      unsigned* tk = ticket.template as<unsigned>();
       x_cur = x;
       if(correct)
       {
+        unsigned nb = 0;
         F3_RC(with_rows<double, double>(outer_matrix(), [&](auto A, mg::RowGrid g) {
-          mg::k_pcg_defect_norm<<<g.blocks, kBlock, 0, s>>>(A, g.per,mask0_ptr(), b, (const double*)x, r.template as<double>(),
-                                                                S, hist.template as<double>(), part, tk);
+          mg::k_pcg_defect_norm<<<g.blocks, kBlock, 0, s>>>(A, g.per, mask0_ptr(), b, (const double*)x, r.template as<double>(), part);
+          nb = g.blocks;
         }));
+        fin<mg::kFinInitDefect>(nb);
       }
       else
       {
         // apply(cor, def): r = def, cor = 0
         F3_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n0, s));
-        mg::k_pcg_copy_norm<<<stream_blocks(n0), kBlock, 0, s>>>(n0, b, r.template as<double>(), S, hist.template as<double>(),
-                                                                part, tk);
+        mg::k_pcg_copy_norm<<<stream_blocks(n0), kBlock, 0, s>>>(n0, b, r.template as<double>(), part);
+        fin<mg::kFinInitDefect>(stream_blocks(n0));
       }
       F3_RC(rt::check_launch());
       F3_RC(enqueue_precond(status_ptr()));
       F3_RC(with_z([&](auto z) {
-        mg::k_pcg_p_gamma<<<stream_blocks(n0), kBlock, 0, s>>>(n0, z, r.template as<double>(), p.template as<double>(), S, part, tk);
+        mg::k_pcg_p_gamma<<<stream_blocks(n0), kBlock, 0, s>>>(n0, z, r.template as<double>(), p.template as<double>(), S, part);
+        fin<mg::kFinGamma>(stream_blocks(n0));
       }));
       F3_RC(rt::check_launch());
 
@@ -871,6 +928,7 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
   if(k == "index16") h->cfg.index16 = value != 0.0;
   else if(k == "compact_transfer") h->cfg.compact_transfer = value != 0.0;
   else if(k == "sell_C" && value >= 1 && value <= 1024) h->cfg.sell_C = u64(value);
+  else if(k == "tail_max_n" && value >= 0) h->cfg.tail_max_n = u64(value);
   else if(k == "sell_sigma" && value >= 1) h->cfg.sell_sigma = u64(value);
   else if(k == "lanes" && (value == 0 || value == 2 || value == 4 || value == 8 || value == 16 || value == 32 || value == 64))
     h->cfg.lanes = int(value);

@@ -33,9 +33,7 @@ namespace rt
     if(it != g_ws.end()) return &it->second;
     Workspace w{};
     if(hipMalloc(&w.partials, sizeof(double) * Workspace::kMaxPartials) != hipSuccess) return nullptr;
-    if(hipMalloc(reinterpret_cast<void**>(&w.ticket), 64) != hipSuccess) return nullptr;
     if(hipMalloc(&w.scalar, 64) != hipSuccess) return nullptr;
-    if(hipMemset(w.ticket, 0, 64) != hipSuccess) return nullptr;
     if(hipDeviceSynchronize() != hipSuccess) return nullptr;
     return &(g_ws[dev] = w);
   }

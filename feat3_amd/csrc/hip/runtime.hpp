@@ -25,7 +25,6 @@ namespace rt
   {
     static constexpr unsigned kMaxPartials = 8192;
     void* partials;        // kMaxPartials doubles
-    unsigned int* ticket;  // arrival counter, left at zero by every reduction
     void* scalar;          // one double, target of the host-returning reductions
   };
 

@@ -100,7 +100,9 @@ def test_sell_graph_and_lanes_variants_agree(torch):
     H = pb.hierarchy(3, 1, 4, 1)
     b = pb.filtered_rhs(H, 0)
     ref = solve_gpu(torch, H, b)
-    for kw in [dict(fmt="sell"), dict(graph=False), dict(lanes=16), dict(fmt="sell", sell_C=32, sell_sigma=1)]:
+    for kw in [dict(fmt="sell"), dict(graph=False), dict(lanes=16), dict(fmt="sell", sell_C=32, sell_sigma=1),
+               dict(fmt="sell", sell_sigma=64), dict(options=dict(tail_max_n=0)), dict(options=dict(tail_max_n=10 ** 6)),
+               dict(fmt="sell", options=dict(index16=0, compact_transfer=0))]:
         st, it, x, _ = solve_gpu(torch, H, b, **kw)
         assert st == 2 and it == ref[1], kw
         assert rel(x, ref[2]) <= 1e-12, kw
