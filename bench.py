@@ -2,13 +2,15 @@
 """Benchmark of the MI355X MG-PCG hot path.
 
 Workload (one "step" = one full solve to relative residual 1e-8 from x = 0, inputs resident in
-HBM): 3D Poisson, Q1 hexahedra on the unit cube, level 7 (128^3 cells, 2,146,689 DOF),
-f = 3 pi^2 prod sin(pi x_i), zero Dirichlet; PCG (FP64 outer) preconditioned by an MG V(4,4)
-cycle with damped Jacobi (omega 0.5) and an exact 2x2x2-cell coarse solve.  The timed solver
-runs its V-cycle in FP32 inside the FP64 PCG (ConvertPrecond semantics); the FP64 path is timed
-beside it.  Synthetic data: the system is assembled by the package's own setup.
+HBM, setup excluded): 3D Poisson, Q1 hexahedra on the unit cube, level 7 (128^3 cells,
+2,146,689 DOF, 57,066,625 nnz), f = 3 pi^2 prod sin(pi x_i), zero Dirichlet; PCG (FP64 outer:
+residual, update, dot products) preconditioned by an MG V(4,4) cycle with Richardson(1.0,
+Jacobi 0.5) smoothing and an exact FP64 solve on the 2x2x2-cell coarse grid.  The headline
+(value) runs the V-cycle in FP32 inside the FP64 PCG (ConvertPrecond semantics); the FP64 path
+and the FP16-stored-values/FP32-arithmetic V-cycle are timed beside it on the same system.
+Synthetic data: the system is assembled by the package's own setup.
 
-Prints ONE JSON line (rank 0).  --gpus N with N > 1 exits: this build runs on one GPU.
+Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
 import argparse
 import json
@@ -19,28 +21,38 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def sweep_algorithmic_bytes(info, n, nnz, nchunks):
+    """Fused damped-Jacobi sweep x_out = x + dinv (b - A x) with SELL-C storage: the CSR defect
+    formula over the unpadded nnz with the stored value and index sizes, the SELL chunk
+    descriptors (cs, cl) in place of row_ptr, x counted once, plus dinv, b and x_out."""
+    s_v = info["sizeof_value"]
+    s_i = 2 if info.get("index16", 1) else 4
+    s_x = info["sizeof_inner"]
+    return nnz * (s_v + s_i) + 8 * nchunks + n * s_x * 4
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--level", type=int, default=7)
     ap.add_argument("--inner", default="f32", choices=["f64", "f32", "f16"])
-    ap.add_argument("--fmt", default="csr", choices=["csr", "sell"])
+    ap.add_argument("--fmt", default="sell", choices=["csr", "sell"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
+    ap.add_argument("--no-variants", action="store_true", help="skip the FP64 / FP16 comparison solves")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+              "(DESIGN.md describes how the solve would shard)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -55,8 +67,8 @@ def main():
     t = time.time()
     H = pb.hierarchy(3, 1, args.level, 1)
     b = pb.filtered_rhs(H, 0)
-    n = H.n
-    log(f"setup: {len(H.levels)} levels, n={n}, nnz={H.levels[0].A.nnz} ({time.time() - t:.1f} s)")
+    n, nnz = H.n, H.levels[0].A.nnz
+    log(f"setup: {len(H.levels)} levels, n={n}, nnz={nnz} ({time.time() - t:.1f} s)")
 
     db = torch.from_numpy(b).to(dev)
     x = torch.zeros(n, dtype=torch.float64, device=dev)
@@ -65,7 +77,6 @@ def main():
         for _ in range(warmup):
             x.zero_()
             solver.correct(x, db)
-        torch.cuda.synchronize()
         times, iters = [], []
         for _ in range(steps):
             x.zero_()
@@ -78,57 +89,62 @@ def main():
             assert st == 2, f"solve did not converge (status {st})"
         return times, iters
 
-    t = time.time()
-    s = MGPCG(H, inner=args.inner, fmt=args.fmt).init()
-    log(f"init {args.inner}/{args.fmt}: {time.time() - t:.1f} s, info {s.info()}")
-    times, iters = run(s, args.steps, args.warmup)
-    total = sum(times)
-    ms_per_step = 1e3 * total / args.steps
-    it_mean = float(np.mean(iters))
-    value = n * sum(iters) / total
-    hist = s.history()
-    log(f"{args.inner}: {ms_per_step:.3f} ms/solve, iterations {iters}, final rel. residual {hist[-1] / hist[0]:.3e}")
-    xm = x.cpu().numpy().copy()
+    def variant(inner, steps, warmup):
+        t0 = time.time()
+        s = MGPCG(H, inner=inner, fmt=args.fmt).init()
+        log(f"init {inner}/{args.fmt}: {time.time() - t0:.1f} s")
+        times, iters = run(s, steps, warmup)
+        hist = s.history()
+        sol = x.cpu().numpy().copy()
+        sweep_ms, _ = s.time_kernel(0, 50)
+        vc_ms, _ = s.time_kernel(3, 10)
+        info = s.info()
+        s.done()
+        ms = 1e3 * sum(times) / len(times)
+        sb = sweep_algorithmic_bytes(info, n, nnz, info["sell_chunks"])
+        r = dict(ms_per_solve=ms, ms_min=1e3 * min(times), iterations=iters,
+                 dof_iter_per_s=n * sum(iters) / sum(times), final_rel_residual=float(hist[-1] / hist[0]),
+                 sweep_us=sweep_ms * 1e3, sweep_bytes=sb, sweep_GBs=sb / (sweep_ms * 1e-3) / 1e9,
+                 vcycle_ms=vc_ms, kernels_per_vcycle=info["kernels_per_vcycle"])
+        log(f"{inner}: {ms:.3f} ms/solve, it {iters}, rel.res {r['final_rel_residual']:.2e}, "
+            f"sweep {r['sweep_us']:.1f} us {r['sweep_GBs']:.0f} GB/s, V-cycle {vc_ms:.3f} ms")
+        return r, sol, times, iters
 
-    # dominant kernel: the finest-level damped-Jacobi sweep, timed on the solver's own stream
-    sweep_ms, sweep_bytes = s.time_kernel(0, 50)
-    defect_ms, defect_bytes = s.time_kernel(1, 50)
-    vc_ms, _ = s.time_kernel(3, 10)
-    info = s.info()
-    achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
-    log(f"sweep {sweep_ms * 1e3:.1f} us, {sweep_bytes / 1e6:.1f} MB -> {achieved:.0f} GB/s; "
-        f"defect {defect_ms * 1e3:.1f} us -> {defect_bytes / defect_ms / 1e6:.0f} GB/s; V-cycle {vc_ms:.3f} ms")
+    head, xm, times, iters = variant(args.inner, args.steps, args.warmup)
+    ms_per_step = head["ms_per_solve"]
+    value = n * sum(iters) / sum(times)
 
-    # FP64 path beside it
-    fp64 = None
-    if not args.no_fp64 and args.inner != "f64":
-        s64 = MGPCG(H, inner="f64", fmt=args.fmt).init()
-        t64, i64 = run(s64, max(1, args.steps // 2), 1)
-        x64 = x.cpu().numpy().copy()
-        sw64_ms, sw64_bytes = s64.time_kernel(0, 30)
-        fp64 = dict(ms_per_solve=1e3 * sum(t64) / len(t64), iterations=i64,
-                    dof_iter_per_s=n * sum(i64) / sum(t64),
-                    sweep_GBs=sw64_bytes / (sw64_ms * 1e-3) / 1e9,
-                    rel_diff_mixed_vs_fp64=float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)))
-        s64.done()
-        log(f"fp64: {fp64}")
-    s.done()
+    variants = {}
+    if not args.no_variants:
+        for inner in ("f64", "f32", "f16"):
+            if inner == args.inner:
+                continue
+            r, xv, _, _ = variant(inner, max(2, args.steps // 2), 1)
+            r["speedup_of_headline"] = r["ms_per_solve"] / ms_per_step
+            variants[inner] = (r, xv)
+    fp64 = variants.get("f64")
+    if fp64:
+        x64 = fp64[1]
+        head["rel_diff_vs_fp64_solution"] = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
 
-    # CPU baseline: the oracle's FP64 PCG-MG on one host core, one full solve of the same
-    # system (bounded sample: one solve, ~10-30 s)
+    # CPU baseline: the oracle's FP64 PCG-MG (FEAT's generic loops restated) on one host core,
+    # one full solve of the same system, solve phase only
     cpu = None
     if not args.no_cpu:
         from oracle import pyoracle as po
         S = po.System.from_hierarchy(H)
         xo = np.zeros(n)
-        t0 = time.perf_counter()
         st, it, _ = S.solve(xo, b, po.params())
-        dt = time.perf_counter() - t0
+        dt = po.lib().orc_last_solve_seconds()
         cpu = {"value": n * it / dt, "unit": "DOF*iter/s", "cores": 1, "kind": "port",
-               "sample": f"one full FP64 PCG-MG solve of the same system ({it} iterations, {dt:.1f} s, "
-                         "single-threaded oracle restating FEAT's generic CPU loops)"}
+               "sample": f"one full FP64 PCG-MG solve of the same 3D Q1 L{args.level} system on one core "
+                         f"({it} iterations, {dt:.2f} s solve phase), oracle restating FEAT's generic CPU loops",
+               "time_to_tol_ms": dt * 1e3}
+        if fp64:
+            cpu["rel_diff_gpu_fp64_vs_cpu"] = float(np.linalg.norm(fp64[1] - xo) / np.linalg.norm(xo))
         log(f"cpu baseline: {cpu}")
 
+    achieved = head["sweep_GBs"]
     out = {
         "metric": "time to rel. residual 1e-8 & DOF·iter/s, FP32-inner vs FP64; SpMV GB/s",
         "value": value,
@@ -140,21 +156,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.inner == "f32" else ("f64" if args.inner == "f64" else "f16-values/f32"),
+        "dtype": {"f32": "f32", "f64": "f64", "f16": "f16-values/f32"}[args.inner],
         "data": "synthetic (assembled 3D Q1 Poisson system, sine-bubble load)",
-        "config": {"workload": f"3D Poisson Q1 L{args.level}, {n} DOF, PCG(FP64)+MG V(4,4) Jacobi 0.5, "
-                               f"{args.inner} V-cycle, {args.fmt}",
-                   "dofs": n, "levels": len(H.levels), "tol_rel": 1e-8},
+        "config": {"workload": f"3D Poisson Q1 level {args.level} ({n} DOF, {nnz} nnz), PCG(FP64) + MG V(4,4) "
+                               f"Richardson/Jacobi 0.5, exact coarse 2x2x2 cells, {args.inner} V-cycle, "
+                               f"{args.fmt} format, rel. tol 1e-8",
+                   "dofs": n, "nnz": nnz, "levels": len(H.levels)},
         "time_to_tol_ms": ms_per_step,
-        "iterations": it_mean,
+        "iterations": float(np.mean(iters)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sweep (finest level)", "bytes_per_launch": sweep_bytes,
-                     "avg_launch_us": sweep_ms * 1e3},
-        "vcycle_ms": vc_ms,
-        "vcycle_kernels": info["kernels_per_vcycle"],
-        "fp64": fp64,
-        "speedup_vs_fp64": (fp64["ms_per_solve"] / ms_per_step) if fp64 else None,
+                     "kernel": "k_sweep, finest level (fused damped-Jacobi step)",
+                     "bytes_per_launch": head["sweep_bytes"], "avg_launch_us": head["sweep_us"]},
+        "headline": head,
+        "variants": {k: v[0] for k, v in variants.items()},
+        "speedup_vs_fp64": (fp64[0]["ms_per_solve"] / ms_per_step) if fp64 else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

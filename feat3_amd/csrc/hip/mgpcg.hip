@@ -769,16 +769,16 @@ namespace
     // iteration excluding the V-cycle, [8] sizeof inner DT, [9] sizeof VT
     int info(u64* out, int cap) override
     {
-      std::vector<u64> v(10, 0);
+      std::vector<u64> v(12, 0);
       v[0] = L.size();
       v[1] = L.front()->n;
       v[2] = L.front()->A.nnz;
       v[3] = u64(L.front()->A.lanes);
       v[4] = L.front()->A.sell_size;
-      u64 kern = 2, bytes = 0;
+      u64 kern = 1, bytes = 0;
       const u64 vb = sizeof(DT);
       bytes += L.front()->n * (8 + vb * 3); // entry
-      for(std::size_t i = 0; i + 1 < L.size(); ++i)
+      for(std::size_t i = 0; i < tail0; ++i)
       {
         const Level<DT, VT>& f = *L[i];
         const u64 sweeps = u64(cfg.pre - 1 + cfg.post);
@@ -788,13 +788,16 @@ namespace
         bytes += f.R.matrix_bytes() + f.n * vb + L[i + 1]->n * vb * 4;     // restrict (+first step)
         bytes += f.P.matrix_bytes() + L[i + 1]->n * vb + f.n * vb * 2;     // prolong
       }
+      kern += 1; // coarse solve or coarse tail
       bytes += ncoarse * ncoarse * 8 + ncoarse * vb * 2;
       v[5] = kern;
       v[6] = bytes;
       v[7] = outer_matrix().matrix_bytes() + n0 * 8 * 2 + n0 * 8 * 5 + n0 * (8 + vb) * 2 + n0 * (8 + vb * 2);
       v[8] = sizeof(DT);
       v[9] = sizeof(VT);
-      for(int i = 0; i < cap && i < 10; ++i) out[i] = v[i];
+      v[10] = L.front()->A.sell_d16 ? 1 : 0;
+      v[11] = L.front()->A.sell ? (L.front()->n + L.front()->A.C - 1) / L.front()->A.C : 0;
+      for(int i = 0; i < cap && i < 12; ++i) out[i] = v[i];
       return rt::kOk;
     }
   };

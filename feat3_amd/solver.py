@@ -121,8 +121,8 @@ class MGPCG:
         return ms.value, nb.value
 
     def info(self) -> dict:
-        out = np.zeros(10, np.uint64)
-        capi.call("feat3_mgpcg_info", self._h, _ptr(out), 10)
+        out = np.zeros(12, np.uint64)
+        capi.call("feat3_mgpcg_info", self._h, _ptr(out), 12)
         keys = ["levels", "n", "nnz", "lanes", "sell_size", "kernels_per_vcycle", "vcycle_bytes",
-                "outer_bytes", "sizeof_inner", "sizeof_value"]
+                "outer_bytes", "sizeof_inner", "sizeof_value", "index16", "sell_chunks"]
         return {k: int(v) for k, v in zip(keys, out)}

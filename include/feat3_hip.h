@@ -299,7 +299,8 @@ int feat3_mgpcg_history(feat3_mgpcg* h, double* out_host, uint64_t cap, uint64_t
  * (which: 0 finest smoothing sweep, 1 finest defect, 3 full V-cycle) and its algorithmic bytes */
 int feat3_mgpcg_time_kernel(feat3_mgpcg* h, int which, int reps, double* avg_ms, uint64_t* bytes);
 /* [levels, n, nnz, lanes, sell size, kernels per V-cycle, V-cycle bytes, outer bytes,
- *  sizeof inner, sizeof stored value] */
+ *  sizeof inner, sizeof stored value, 16-bit SELL columns on the finest level, SELL chunks
+ *  of the finest level] */
 int feat3_mgpcg_info(feat3_mgpcg* h, uint64_t* out, int cap);
 
 /* ===================================================================================== */

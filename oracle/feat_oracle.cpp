@@ -24,6 +24,7 @@
 // See tests/test_oracle_golden.py.
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1238,6 +1239,11 @@ namespace
 
 extern "C" {
 
+// wall time of the solve phase of the last orc_system_solve (matrix copies, hierarchy
+// construction and solver init excluded)
+static double g_last_solve_seconds = 0.0;
+double orc_last_solve_seconds() { return g_last_solve_seconds; }
+
 void* orc_system_create() { return new OrcSystem(); }
 void orc_system_destroy(void* h) { delete static_cast<OrcSystem*>(h); }
 
@@ -1312,8 +1318,10 @@ u64 orc_system_solve(void* h, const double* praw, double* x, const double* b, in
     if(q.tol_abs > 0) sol.tol_abs = q.tol_abs;
     sol.history = &history;
     sol.init();
+    const auto t0 = std::chrono::steady_clock::now();
     if(q.use_def) st = sol.correct(x, b);
     else st = sol.apply(x, b);
+    g_last_solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     iters = sol.num_iter;
   }
   else
@@ -1324,8 +1332,10 @@ u64 orc_system_solve(void* h, const double* praw, double* x, const double* b, in
     if(q.tol_abs > 0) sol.tol_abs = q.tol_abs;
     sol.history = &history;
     sol.init();
+    const auto t0 = std::chrono::steady_clock::now();
     if(q.use_def) st = sol.correct(x, b);
     else st = sol.apply(x, b);
+    g_last_solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     iters = sol.num_iter;
   }
   *status = int(st);

@@ -55,6 +55,7 @@ _PROTOS = {
                                    c_vp, c_vp, c_vp]),
     "orc_system_solve": (c_u64, [c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_i), c_vp, c_u64, ctypes.POINTER(c_u64)]),
     "orc_system_vcycle": (c_i, [c_vp, c_vp, c_vp, c_vp]),
+    "orc_last_solve_seconds": (c_d, []),
 }
 
 
