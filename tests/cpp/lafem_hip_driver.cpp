@@ -1,0 +1,175 @@
+// lafem_hip_driver.cpp -- C++ application-level use of the Mem::HIP containers and solvers,
+// written the way applications/poisson_dirichlet.cpp:131-200 builds its solver:
+//   PCG( MultiGrid-V( Richardson(1.0, Jacobi(omega)) x4 pre/post, coarse = the smoother ) )
+// on 2D Q1 Poisson (levels `fine` .. `coarse`, ExpBubble load).  Runs
+//   generic64  -- op-by-op solver classes, FP64 containers
+//   mixed32    -- FP64 PCG + ConvertPrecond + FP32 multigrid
+//   fused64    -- the fused device solver (exact coarse solve)
+// and writes "<name> <status> <iterations>" lines to stdout and each solution as raw FP64 to
+// <outdir>/<name>.bin for the test to compare against the oracle.
+#include "feat3_amd/lafem_hip.hpp"
+#include "feat3_amd/solver_hip.hpp"
+
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <string>
+
+using namespace feat3_amd;
+typedef std::uint32_t u32;
+
+struct HostLevel
+{
+  Index n = 0, nc = 0;
+  std::vector<u32> rp, col, bnd, prp, pcol, rrp, rcol;
+  std::vector<double> val, pval, rval;
+};
+
+static HostLevel make_level(int level, bool with_transfer)
+{
+  HostLevel L;
+  std::uint64_t n, nnz, nb;
+  hip_check(feat3_setup_laplace_sizes(2, 1, level, &n, &nnz, &nb), "sizes");
+  L.n = n;
+  L.rp.resize(n + 1); L.col.resize(nnz); L.val.resize(nnz); L.bnd.resize(nb);
+  hip_check(feat3_setup_laplace(2, 1, level, nullptr, L.rp.data(), L.col.data(), L.val.data(), L.bnd.data()), "laplace");
+  if(with_transfer)
+  {
+    std::uint64_t nf, nc, pn;
+    hip_check(feat3_setup_prol_sizes(2, 1, level, &nf, &nc, &pn), "prol sizes");
+    L.nc = nc;
+    L.prp.resize(nf + 1); L.pcol.resize(pn); L.pval.resize(pn);
+    hip_check(feat3_setup_prol(2, 1, level, L.prp.data(), L.pcol.data(), L.pval.data()), "prol");
+    L.rrp.resize(nc + 1); L.rcol.resize(pn); L.rval.resize(pn);
+    hip_check(feat3_setup_transpose(nf, nc, L.prp.data(), L.pcol.data(), L.pval.data(), L.rrp.data(), L.rcol.data(),
+                                    L.rval.data()), "transpose");
+  }
+  return L;
+}
+
+template<typename DT_>
+struct DeviceLevel
+{
+  typedef LAFEM::SparseMatrixCSR<Mem::HIP, DT_, u32> Matrix;
+  typedef LAFEM::UnitFilter<Mem::HIP, DT_, u32> Filter;
+  typedef LAFEM::Transfer<Matrix> Transfer;
+  Matrix A;
+  Filter F;
+  std::unique_ptr<Transfer> T;
+  explicit DeviceLevel(const HostLevel& h)
+    : A(h.n, h.n, h.rp, h.col, h.val), F(h.bnd)
+  {
+    if(h.nc)
+      T.reset(new Transfer(Matrix(h.n, h.nc, h.prp, h.pcol, h.pval), Matrix(h.nc, h.n, h.rrp, h.rcol, h.rval)));
+  }
+};
+
+template<typename DT_>
+std::shared_ptr<Solver::MultiGrid<typename DeviceLevel<DT_>::Matrix, typename DeviceLevel<DT_>::Filter,
+                                  typename DeviceLevel<DT_>::Transfer>>
+make_mg(std::vector<std::unique_ptr<DeviceLevel<DT_>>>& lv, double omega,
+        std::shared_ptr<Solver::MultiGridHierarchy<typename DeviceLevel<DT_>::Matrix, typename DeviceLevel<DT_>::Filter,
+                                                   typename DeviceLevel<DT_>::Transfer>>& hier)
+{
+  typedef typename DeviceLevel<DT_>::Matrix M;
+  typedef typename DeviceLevel<DT_>::Filter F;
+  typedef typename DeviceLevel<DT_>::Transfer T;
+  hier = std::make_shared<Solver::MultiGridHierarchy<M, F, T>>();
+  for(std::size_t i = 0; i < lv.size(); ++i)
+  {
+    auto jac = std::make_shared<Solver::JacobiPrecond<M, F>>(lv[i]->A, lv[i]->F, DT_(omega));
+    auto sm = std::make_shared<Solver::Richardson<M, F>>(lv[i]->A, lv[i]->F, DT_(1), jac);
+    sm->set_min_iter(4);
+    sm->set_max_iter(4);
+    if(i + 1 < lv.size()) hier->push_level(lv[i]->A, lv[i]->F, *lv[i]->T, sm, sm, sm);
+    else hier->push_level(lv[i]->A, lv[i]->F, sm);
+  }
+  return std::make_shared<Solver::MultiGrid<M, F, T>>(hier);
+}
+
+static void write(const std::string& path, const std::vector<double>& v)
+{
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(v.data()), std::streamsize(v.size() * sizeof(double)));
+}
+
+int main(int argc, char** argv)
+{
+  if(argc < 5)
+  {
+    std::fprintf(stderr, "usage: %s <fine level> <coarse level> <jacobi omega> <outdir>\n", argv[0]);
+    return 2;
+  }
+  const int fine = std::atoi(argv[1]), coarse = std::atoi(argv[2]);
+  const double omega = std::atof(argv[3]);
+  const std::string outdir = argv[4];
+  try
+  {
+    MemoryPool<Mem::HIP>::initialise(0);
+    std::vector<HostLevel> host;
+    for(int l = fine; l >= coarse; --l) host.push_back(make_level(l, l > coarse));
+    const HostLevel& h0 = host.front();
+    std::vector<double> rhs(h0.n);
+    hip_check(feat3_setup_rhs(2, 1, fine, 2, rhs.data()), "rhs");   // ExpBubble
+    for(u32 i : h0.bnd) rhs[i] = 0.0;
+
+    {
+      // generic FP64
+      std::vector<std::unique_ptr<DeviceLevel<double>>> lv;
+      for(const HostLevel& h : host) lv.emplace_back(new DeviceLevel<double>(h));
+      std::shared_ptr<Solver::MultiGridHierarchy<DeviceLevel<double>::Matrix, DeviceLevel<double>::Filter,
+                                                 DeviceLevel<double>::Transfer>> hier;
+      auto mg = make_mg<double>(lv, omega, hier);
+      Solver::PCG<DeviceLevel<double>::Matrix, DeviceLevel<double>::Filter> pcg(lv[0]->A, lv[0]->F, mg);
+      pcg.set_tol_rel(1e-8);
+      pcg.set_max_iter(1000);
+      hier->init();
+      pcg.init();
+      LAFEM::DenseVector<Mem::HIP, double, u32> x(h0.n, 0.0), b(rhs);
+      lv[0]->F.filter_sol(x);
+      const Solver::Status st = Solver::solve(pcg, x, b, lv[0]->A, lv[0]->F);
+      std::cout << "generic64 " << int(st) << " " << pcg.get_num_iter() << std::endl;
+      write(outdir + "/generic64.bin", x.to_host());
+
+      // FP64 PCG with an FP32 multigrid through ConvertPrecond
+      std::vector<std::unique_ptr<DeviceLevel<float>>> lv32;
+      for(const HostLevel& h : host) lv32.emplace_back(new DeviceLevel<float>(h));
+      std::shared_ptr<Solver::MultiGridHierarchy<DeviceLevel<float>::Matrix, DeviceLevel<float>::Filter,
+                                                 DeviceLevel<float>::Transfer>> hier32;
+      auto mg32 = make_mg<float>(lv32, omega, hier32);
+      typedef LAFEM::DenseVector<Mem::HIP, double, u32> V64;
+      typedef LAFEM::DenseVector<Mem::HIP, float, u32> V32;
+      auto conv = std::make_shared<Solver::ConvertPrecond<V64, V32>>(mg32);
+      Solver::PCG<DeviceLevel<double>::Matrix, DeviceLevel<double>::Filter> pcg2(lv[0]->A, lv[0]->F, conv);
+      pcg2.set_tol_rel(1e-8);
+      pcg2.set_max_iter(1000);
+      hier32->init();
+      pcg2.init();
+      V64 x2(h0.n, 0.0);
+      const Solver::Status st2 = Solver::solve(pcg2, x2, b, lv[0]->A, lv[0]->F);
+      std::cout << "mixed32 " << int(st2) << " " << pcg2.get_num_iter() << std::endl;
+      write(outdir + "/mixed32.bin", x2.to_host());
+    }
+    {
+      // the fused device solver (exact FP64 coarse solve on the coarsest level)
+      Solver::FusedMGPCG fused(Solver::FusedMGPCG::fp64, Solver::FusedMGPCG::sell);
+      for(const HostLevel& h : host)
+        fused.push_level(h.n, h.rp, h.col, h.val, h.bnd, h.nc, h.prp, h.pcol, h.pval, h.rrp, h.rcol, h.rval);
+      fused.set_smoother(omega, 1.0, 4, 4);
+      fused.set_tol_rel(1e-8);
+      fused.set_max_iter(1000);
+      fused.init();
+      LAFEM::DenseVector<Mem::HIP, double, u32> x(h0.n, 0.0), b(rhs);
+      const Solver::Status st = fused.correct(x, b);
+      std::cout << "fused64 " << int(st) << " " << fused.get_num_iter() << std::endl;
+      write(outdir + "/fused64.bin", x.to_host());
+    }
+    std::cout << "leaked " << MemoryPool<Mem::HIP>::finalise() << std::endl;
+  }
+  catch(const std::exception& e)
+  {
+    std::cerr << "error: " << e.what() << std::endl;
+    return 1;
+  }
+  return 0;
+}
