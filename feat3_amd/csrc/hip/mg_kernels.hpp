@@ -429,7 +429,7 @@ namespace mg
   }
 
   // r = def  (pcg.hpp:141-156 apply), partial sums of r.r
-  __global__ __launch_bounds__(256) void k_pcg_copy_norm(u64 n, const double* __restrict__ def, double* __restrict__ r,
+  static __global__ __launch_bounds__(256) void k_pcg_copy_norm(u64 n, const double* __restrict__ def, double* __restrict__ r,
                                                          double* partials)
   {
     double v = 0.0;
@@ -478,7 +478,7 @@ namespace mg
   }
 
   // x += alpha p, r -= alpha q, partial sums of r.r   (pcg.hpp:244-259)
-  __global__ __launch_bounds__(256) void k_pcg_update_norm(u64 n, const double* __restrict__ p, const double* __restrict__ q,
+  static __global__ __launch_bounds__(256) void k_pcg_update_norm(u64 n, const double* __restrict__ p, const double* __restrict__ q,
                                                            double* __restrict__ x, double* __restrict__ r, const PcgState* st,
                                                            double* partials)
   {
@@ -540,7 +540,7 @@ namespace mg
   }
 
   // unpreconditioned CG: the "preconditioner" copies and filters (iterative.hpp:1070-1075)
-  __global__ void k_copy_filter(u64 n, const double* __restrict__ r, const u32* __restrict__ mask, double* __restrict__ z,
+  static __global__ void k_copy_filter(u64 n, const double* __restrict__ r, const u32* __restrict__ mask, double* __restrict__ z,
                                 const int* status)
   {
     if(stopped(status)) return;

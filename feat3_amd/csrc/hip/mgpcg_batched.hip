@@ -1,0 +1,629 @@
+// mgpcg_batched.hip -- the batched device MG-PCG behind the feat3_batch_* C ABI: K systems
+// that share one sparsity graph solved in lockstep (same algorithm as mgpcg.hip, per-system
+// scalars and statuses; reference: solver/pcg.hpp:188-288, multigrid.hpp:1268-2000,
+// richardson.hpp:187-228, jacobi_precond.hpp:137-165, convert_precond.hpp:100-119).
+//
+//  * K right-hand sides of one matrix (values shared), e.g. 32 RHS of the 3x3-block vector
+//    Laplacian (SparseMatrixBCSR<3,3>, sparse_matrix_bcsr.hpp:1295-1600; Jacobi on the
+//    diagonal of the diagonal blocks, diagonal_generic.hpp:39-58);
+//  * K coefficient fields (one value set per system), e.g. 16 variable-coefficient diffusion
+//    matrices on one mesh.
+// Layout in HBM: graph (u32 row_ptr/col_ind) once; values [entry][h][w] or [entry][h][w][k];
+// vectors [node][h][k]; transfers as scalar graphs applied per component (compact, no stored
+// zeros); node Dirichlet bitmask; per-system PCG scalars (SysState[K]).
+
+#include "batch_kernels.hpp"
+#include "device_common.hpp"
+#include "mg_kernels.hpp"
+#include "runtime.hpp"
+#include "feat3_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+using namespace feat3_amd;
+using feat3_amd::dev::half;
+typedef std::uint32_t u32;
+typedef std::uint64_t u64;
+
+namespace
+{
+  constexpr int kBlock = 256;
+#define F3_CHECK(...) do { if((__VA_ARGS__) != hipSuccess) return rt::kErrRuntime; } while(0)
+#define F3_RC(...) do { const int rc_ = (__VA_ARGS__); if(rc_) return rc_; } while(0)
+
+  struct DevMem
+  {
+    void* p = nullptr;
+    ~DevMem() { if(p) (void)hipFree(p); }
+    template<typename T> T* as() const { return static_cast<T*>(p); }
+    int alloc(u64 bytes)
+    {
+      if(p) { (void)hipFree(p); p = nullptr; }
+      return hipMalloc(&p, bytes ? bytes : 16) == hipSuccess ? rt::kOk : rt::kErrRuntime;
+    }
+    template<typename T> int put(const std::vector<T>& h, hipStream_t s)
+    {
+      F3_RC(alloc(sizeof(T) * h.size()));
+      if(!h.empty()) F3_CHECK(hipMemcpyAsync(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+      return rt::kOk;
+    }
+  };
+
+  template<typename VT> VT to_storage(double v) { return VT(v); }
+  template<> half to_storage<half>(double v) { return half(float(v)); }
+
+  struct HostGraph
+  {
+    u64 rows = 0, cols = 0;
+    std::vector<u32> rp, col;
+    std::vector<double> val;   // nnz * vals_per_entry
+  };
+
+  struct HostLevel
+  {
+    u64 nodes = 0;
+    HostGraph A, P, R;
+    std::vector<u32> fidx;
+  };
+
+  struct Config
+  {
+    double omega_j = 0.5, omega_r = 1.0;
+    int pre = 4, post = 4;
+    double tol_rel = 1e-8, tol_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16), tol_abs_low = 0.0;
+    u64 min_iter = 0, max_iter = 1000;
+    bool use_graph = true;
+  };
+
+  // drop stored exact zeros of a scalar graph (the transfers' structural zeros)
+  HostGraph compact(const HostGraph& h)
+  {
+    HostGraph c;
+    c.rows = h.rows; c.cols = h.cols;
+    c.rp.assign(h.rows + 1, 0);
+    for(u64 r = 0; r < h.rows; ++r)
+    {
+      for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
+        if(h.val[k] != 0.0) { c.col.push_back(h.col[k]); c.val.push_back(h.val[k]); }
+      c.rp[r + 1] = u32(c.col.size());
+    }
+    return c;
+  }
+
+  bool invert_dense(std::vector<double>& a, u64 n)
+  {
+    std::vector<double> inv(n * n, 0.0);
+    for(u64 i = 0; i < n; ++i) inv[i * n + i] = 1.0;
+    for(u64 k = 0; k < n; ++k)
+    {
+      u64 pr = k;
+      for(u64 i = k + 1; i < n; ++i) if(std::fabs(a[i * n + k]) > std::fabs(a[pr * n + k])) pr = i;
+      if(a[pr * n + k] == 0.0) return false;
+      if(pr != k) for(u64 j = 0; j < n; ++j) { std::swap(a[k * n + j], a[pr * n + j]); std::swap(inv[k * n + j], inv[pr * n + j]); }
+      const double d = a[k * n + k];
+      for(u64 j = 0; j < n; ++j) { a[k * n + j] /= d; inv[k * n + j] /= d; }
+      for(u64 i = 0; i < n; ++i)
+      {
+        if(i == k) continue;
+        const double f = a[i * n + k];
+        if(f == 0.0) continue;
+        for(u64 j = 0; j < n; ++j) { a[i * n + j] -= f * a[k * n + j]; inv[i * n + j] -= f * inv[k * n + j]; }
+      }
+    }
+    a.swap(inv);
+    return true;
+  }
+
+  inline unsigned stream_blocks(u64 items, int G)
+  {
+    const u64 per_block = 256 / u64(G);
+    return unsigned(std::max<u64>(1, std::min<u64>((items + per_block - 1) / per_block, 4096)));
+  }
+
+  struct BatchBase
+  {
+    virtual ~BatchBase() {}
+    virtual int init(const std::vector<HostLevel>& lv, const Config& c, hipStream_t s) = 0;
+    virtual int solve(double* x, const double* b, int* status, u64* iters) = 0;
+    virtual int history(int k, double* out, u64 cap, u64* len) = 0;
+    virtual int time_kernel(int which, int reps, double* ms, u64* bytes) = 0;
+    virtual bool set_tolerances(const Config& c) = 0;
+  };
+
+  template<typename DT, typename VT, int G, int B, bool SHARED>
+  struct Engine : public BatchBase
+  {
+    struct Level
+    {
+      u64 nodes = 0, nnz = 0;
+      DevMem arp, acol, aval, prp, pcol, pval, rrp, rcol, rval, mask, dinv, rhs, x, t, def;
+      u64 nc = 0;
+      DT* xfinal = nullptr;
+    };
+    Config cfg;
+    hipStream_t s = nullptr;
+    int K = 1;
+    std::vector<std::unique_ptr<Level>> L;
+    DevMem ainv, a64, r, p, q, st, cr, hist, partials, any;
+    u64 m0 = 0, hist_cap = 0, ncoarse_rows = 0;
+    int* pinned = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr}, t0 = nullptr, t1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double* x_cur = nullptr;
+    double* x_graph = nullptr;
+
+    explicit Engine(int k) : K(k) {}
+    ~Engine() override
+    {
+      if(gexec) (void)hipGraphExecDestroy(gexec);
+      if(graph) (void)hipGraphDestroy(graph);
+      if(pinned) (void)hipHostFree(pinned);
+      for(auto& e : ev) if(e) (void)hipEventDestroy(e);
+      if(t0) (void)hipEventDestroy(t0);
+      if(t1) (void)hipEventDestroy(t1);
+    }
+
+    static constexpr u64 vpe() { return u64(B) * u64(B); }   // values per entry and system
+    u64 vals_per_entry() const { return vpe() * (SHARED ? 1 : u64(K)); }
+
+    template<typename T> std::vector<T> conv(const std::vector<double>& v) const
+    {
+      std::vector<T> o(v.size());
+      for(std::size_t i = 0; i < v.size(); ++i) o[i] = to_storage<T>(v[i]);
+      return o;
+    }
+
+    batch::BlockRows<DT, VT, B, SHARED> rowsA(const Level& l) const
+    {
+      return {l.arp.template as<u32>(), l.acol.template as<u32>(), l.aval.template as<VT>(), l.nodes, K};
+    }
+    batch::CompRows<DT, VT, B> rowsP(const Level& l) const
+    {
+      return {l.prp.template as<u32>(), l.pcol.template as<u32>(), l.pval.template as<VT>(), l.nodes, K};
+    }
+    batch::CompRows<DT, VT, B> rowsR(const Level& l) const
+    {
+      return {l.rrp.template as<u32>(), l.rcol.template as<u32>(), l.rval.template as<VT>(), l.nc, K};
+    }
+    batch::BlockRows<double, double, B, SHARED> rows64() const
+    {
+      return {L[0]->arp.template as<u32>(), L[0]->acol.template as<u32>(), a64.template as<double>(), L[0]->nodes, K};
+    }
+
+    int init(const std::vector<HostLevel>& lv, const Config& c, hipStream_t stream) override
+    {
+      cfg = c;
+      s = stream;
+      if(lv.empty() || cfg.pre < 1 || cfg.post < 1) return rt::kErrArg;
+      L.clear();
+      for(std::size_t i = 0; i < lv.size(); ++i)
+      {
+        const HostLevel& h = lv[i];
+        std::unique_ptr<Level> l(new Level());
+        l->nodes = h.nodes;
+        l->nnz = h.A.rp[h.nodes];
+        F3_RC(l->arp.put(h.A.rp, s));
+        F3_RC(l->acol.put(h.A.col, s));
+        F3_RC(l->aval.put(conv<VT>(h.A.val), s));
+        if(i + 1 < lv.size())
+        {
+          const HostGraph P = compact(h.P), R = compact(h.R);
+          l->nc = lv[i + 1].nodes;
+          F3_RC(l->prp.put(P.rp, s)); F3_RC(l->pcol.put(P.col, s)); F3_RC(l->pval.put(conv<VT>(P.val), s));
+          F3_RC(l->rrp.put(R.rp, s)); F3_RC(l->rcol.put(R.col, s)); F3_RC(l->rval.put(conv<VT>(R.val), s));
+        }
+        std::vector<u32> m((h.nodes + 31) / 32 + 1, 0u);
+        for(u32 f : h.fidx) m[f >> 5] |= 1u << (f & 31u);
+        F3_RC(l->mask.put(m, s));
+        // Jacobi: dinv = omega_R * omega_J / diag (diagonal of the diagonal block), 0 at Dirichlet
+        const u64 nd = h.nodes * B * (SHARED ? 1 : u64(K));
+        std::vector<DT> dinv(nd, DT(0));
+        const u64 kv = SHARED ? 1 : u64(K);
+        for(u64 node = 0; node < h.nodes; ++node)
+        {
+          const bool dir = (m[node >> 5] >> (node & 31u)) & 1u;
+          for(u32 e = h.A.rp[node]; e < h.A.rp[node + 1]; ++e)
+            if(h.A.col[e] == node)
+              for(int hh = 0; hh < B; ++hh)
+                for(u64 k = 0; k < kv; ++k)
+                {
+                  // the diagonal in the storage precision, as the reference's Jacobi extracts it
+                  const DT d = DT(to_storage<VT>(h.A.val[(u64(e) * vpe() + u64(hh) * B + u64(hh)) * kv + k]));
+                  dinv[(node * B + u64(hh)) * kv + k] = dir ? DT(0) : DT(cfg.omega_r) * (DT(cfg.omega_j) / d);
+                }
+        }
+        F3_RC(l->dinv.put(dinv, s));
+        const u64 nv = h.nodes * B * u64(K);
+        F3_RC(l->rhs.alloc(sizeof(DT) * nv)); F3_RC(l->x.alloc(sizeof(DT) * nv));
+        F3_RC(l->t.alloc(sizeof(DT) * nv)); F3_RC(l->def.alloc(sizeof(DT) * nv));
+        L.push_back(std::move(l));
+      }
+      // FP64 inverses of the unit-filtered coarse matrices
+      const HostLevel& hc = lv.back();
+      ncoarse_rows = hc.nodes * B;
+      const u64 mc = ncoarse_rows;
+      if(mc > 4096) return rt::kErrArg;
+      const u64 ninv = SHARED ? 1 : u64(K);
+      std::vector<double> inv_all(ninv * mc * mc);
+      for(u64 k = 0; k < ninv; ++k)
+      {
+        std::vector<double> a(mc * mc, 0.0);
+        for(u64 node = 0; node < hc.nodes; ++node)
+          for(u32 e = hc.A.rp[node]; e < hc.A.rp[node + 1]; ++e)
+            for(int hh = 0; hh < B; ++hh)
+              for(int ww = 0; ww < B; ++ww)
+                a[(node * B + hh) * mc + u64(hc.A.col[e]) * B + ww] +=
+                  hc.A.val[(u64(e) * vpe() + u64(hh) * B + u64(ww)) * ninv + k];
+        for(u32 f : hc.fidx)
+          for(int hh = 0; hh < B; ++hh)
+          {
+            const u64 row = u64(f) * B + hh;
+            for(u64 j = 0; j < mc; ++j) a[row * mc + j] = 0.0;
+            a[row * mc + row] = 1.0;
+          }
+        if(!invert_dense(a, mc)) return rt::kErrNumeric;
+        std::copy(a.begin(), a.end(), inv_all.begin() + k * mc * mc);
+      }
+      F3_RC(ainv.put(inv_all, s));
+      for(std::size_t i = 0; i + 1 < L.size(); ++i)
+        L[i]->xfinal = ((cfg.pre - 1 + cfg.post) % 2 == 0) ? L[i]->x.template as<DT>() : L[i]->t.template as<DT>();
+      L.back()->xfinal = L.back()->x.template as<DT>();
+      // outer FP64
+      m0 = lv[0].nodes * B;
+      F3_RC(a64.put(lv[0].A.val, s));
+      const u64 nv = m0 * u64(K);
+      F3_RC(r.alloc(sizeof(double) * nv)); F3_RC(p.alloc(sizeof(double) * nv)); F3_RC(q.alloc(sizeof(double) * nv));
+      F3_RC(st.alloc(sizeof(batch::SysState) * K));
+      F3_RC(cr.alloc(sizeof(batch::Criteria)));
+      hist_cap = cfg.max_iter + 2;
+      F3_RC(hist.alloc(sizeof(double) * hist_cap * K));
+      F3_RC(partials.alloc(sizeof(double) * 8192 * u64(K)));
+      F3_RC(any.alloc(64));
+      if(!pinned) F3_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), 64, hipHostMallocDefault));
+      for(auto& e : ev) if(!e) F3_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      if(!t0) F3_CHECK(hipEventCreate(&t0));
+      if(!t1) F3_CHECK(hipEventCreate(&t1));
+      F3_CHECK(hipStreamSynchronize(s));
+      if(gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+      if(graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+      return rt::kOk;
+    }
+
+    bool set_tolerances(const Config& c) override
+    {
+      if(c.max_iter + 2 > hist_cap) return false;
+      cfg.tol_rel = c.tol_rel; cfg.tol_abs = c.tol_abs; cfg.tol_abs_low = c.tol_abs_low;
+      cfg.min_iter = c.min_iter; cfg.max_iter = c.max_iter;
+      return true;
+    }
+
+    const int* anyp() const { return any.template as<int>(); }
+
+    int sweep(Level& f, DT* xin, DT* xout, const int* a)
+    {
+      const mg::RowGrid g = mg::row_grid(f.nodes, G);
+      batch::k_sweep<G, B, SHARED><<<g.blocks, kBlock, 0, s>>>(rowsA(f), g.per, f.dinv.template as<DT>(), f.rhs.template as<DT>(),
+                                                               (const DT*)xin, xout, a);
+      return rt::kOk;
+    }
+
+    int enqueue_vcycle(const double* def, const int* a)
+    {
+      const std::size_t nl = L.size();
+      Level& top = *L[0];
+      batch::k_entry<DT, B, SHARED><<<4096, kBlock, 0, s>>>(top.nodes, K, def, top.rhs.template as<DT>(), top.x.template as<DT>(),
+                                                           top.dinv.template as<DT>(), a);
+      std::vector<DT*> cur(nl, nullptr), oth(nl, nullptr);
+      for(std::size_t i = 0; i + 1 < nl; ++i)
+      {
+        Level& f = *L[i];
+        Level& c = *L[i + 1];
+        cur[i] = f.x.template as<DT>();
+        oth[i] = f.t.template as<DT>();
+        for(int k = 1; k < cfg.pre; ++k) { F3_RC(sweep(f, cur[i], oth[i], a)); std::swap(cur[i], oth[i]); }
+        const mg::RowGrid g = mg::row_grid(f.nodes, G);
+        batch::k_defect<G, B><<<g.blocks, kBlock, 0, s>>>(rowsA(f), g.per, f.mask.template as<u32>(), f.rhs.template as<DT>(),
+                                                          (const DT*)cur[i], f.def.template as<DT>(), a);
+        const mg::RowGrid gr = mg::row_grid(f.nc, G);
+        DT* xc = (i + 2 < nl) ? c.x.template as<DT>() : nullptr;
+        batch::k_restrict<G, B, SHARED><<<gr.blocks, kBlock, 0, s>>>(rowsR(f), gr.per, c.mask.template as<u32>(),
+                                                                     f.def.template as<DT>(), c.rhs.template as<DT>(), xc,
+                                                                     c.dinv.template as<DT>(), a);
+      }
+      Level& cl = *L.back();
+      const unsigned cb = unsigned(std::max<u64>(1, (ncoarse_rows * u64(K) + kBlock - 1) / kBlock));
+      batch::k_coarse<DT><<<cb, kBlock, 0, s>>>(ncoarse_rows, K, SHARED ? 0 : 1, ainv.template as<double>(),
+                                               cl.mask.template as<u32>(), B, cl.rhs.template as<DT>(), cl.x.template as<DT>(), a);
+      for(std::size_t ii = nl - 1; ii > 0; --ii)
+      {
+        const std::size_t i = ii - 1;
+        Level& f = *L[i];
+        Level& c = *L[i + 1];
+        const mg::RowGrid g = mg::row_grid(f.nodes, G);
+        batch::k_prolong<G, B><<<g.blocks, kBlock, 0, s>>>(rowsP(f), g.per, f.mask.template as<u32>(), (const DT*)c.xfinal, cur[i], a);
+        for(int k = 0; k < cfg.post; ++k) { F3_RC(sweep(f, cur[i], oth[i], a)); std::swap(cur[i], oth[i]); }
+        if(cur[i] != f.xfinal) return rt::kErrState;
+      }
+      return rt::check_launch();
+    }
+
+    template<int OP>
+    void fin(unsigned nb)
+    {
+      batch::k_fin<OP><<<1, kBlock, 0, s>>>(partials.template as<double>(), nb, K, st.template as<batch::SysState>(),
+                                             cr.template as<batch::Criteria>(), hist.template as<double>(), any.template as<int>());
+    }
+
+    int enqueue_iteration()
+    {
+      double* part = partials.template as<double>();
+      batch::SysState* S = st.template as<batch::SysState>();
+      const mg::RowGrid g = mg::row_grid(L[0]->nodes, G);
+      batch::k_pcg_spmv_dot<G, B><<<g.blocks, kBlock, 0, s>>>(rows64(), g.per, L[0]->mask.template as<u32>(), p.template as<double>(),
+                                                              q.template as<double>(), S, anyp(), part);
+      fin<mg::kFinAlpha>(g.blocks);
+      const unsigned sb = stream_blocks(m0, G);
+      batch::k_pcg_update<G, B><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, p.template as<double>(), q.template as<double>(), x_cur,
+                                                      r.template as<double>(), S, anyp(), part);
+      fin<mg::kFinNewDefect>(sb);
+      F3_RC(enqueue_vcycle(r.template as<double>(), anyp()));
+      const DT* z = L[0]->xfinal;
+      batch::k_pcg_rz<G, B, DT><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, z, r.template as<double>(), S, anyp(), part);
+      fin<mg::kFinBeta>(sb);
+      batch::k_pcg_update_p<G, B, DT><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, z, p.template as<double>(), S, anyp());
+      return rt::check_launch();
+    }
+
+    int solve(double* x, const double* b, int* status, u64* iters) override
+    {
+      if(!x || !b) return rt::kErrArg;
+      batch::Criteria c{};
+      c.tol_rel = cfg.tol_rel; c.tol_abs = cfg.tol_abs; c.tol_abs_low = cfg.tol_abs_low;
+      c.div_rel = 1.0 / 2.220446049250313e-16; c.div_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16);
+      c.stag_rate = 0.95; c.min_iter = cfg.min_iter; c.max_iter = cfg.max_iter; c.min_stag_iter = 0; c.hist_cap = hist_cap;
+      F3_CHECK(hipMemcpyAsync(cr.p, &c, sizeof(c), hipMemcpyHostToDevice, s));
+      x_cur = x;
+      double* part = partials.template as<double>();
+      batch::SysState* S = st.template as<batch::SysState>();
+      const mg::RowGrid g = mg::row_grid(L[0]->nodes, G);
+      batch::k_pcg_defect<G, B><<<g.blocks, kBlock, 0, s>>>(rows64(), g.per, L[0]->mask.template as<u32>(), b, (const double*)x,
+                                                            r.template as<double>(), part);
+      fin<mg::kFinInitDefect>(g.blocks);
+      F3_RC(enqueue_vcycle(r.template as<double>(), anyp()));
+      const unsigned sb = stream_blocks(m0, G);
+      batch::k_pcg_p_gamma<G, B, DT><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, L[0]->xfinal, r.template as<double>(),
+                                                           p.template as<double>(), S, part);
+      fin<mg::kFinGamma>(sb);
+      F3_RC(rt::check_launch());
+      if(cfg.use_graph && (gexec == nullptr || x_graph != x))
+      {
+        if(gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+        if(graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+        F3_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_iteration();
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &gr);
+        if(rc) { if(gr) (void)hipGraphDestroy(gr); return rc; }
+        if(e != hipSuccess) return rt::kErrRuntime;
+        graph = gr;
+        F3_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        x_graph = x;
+      }
+      u64 launched = 0;
+      while(launched < cfg.max_iter + 1)
+      {
+        if(cfg.use_graph) F3_CHECK(hipGraphLaunch(gexec, s));
+        else F3_RC(enqueue_iteration());
+        F3_CHECK(hipMemcpyAsync(&pinned[launched % 2], any.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        F3_CHECK(hipEventRecord(ev[launched % 2], s));
+        ++launched;
+        if(launched >= 2)
+        {
+          F3_CHECK(hipEventSynchronize(ev[(launched - 2) % 2]));
+          if(pinned[(launched - 2) % 2] == 0) break;
+        }
+      }
+      F3_CHECK(hipStreamSynchronize(s));
+      std::vector<batch::SysState> out(K);
+      F3_CHECK(hipMemcpy(out.data(), st.p, sizeof(batch::SysState) * K, hipMemcpyDeviceToHost));
+      for(int k = 0; k < K; ++k)
+      {
+        if(status) status[k] = out[k].status;
+        if(iters) iters[k] = out[k].num_iter;
+      }
+      last_iters.assign(K, 0);
+      for(int k = 0; k < K; ++k) last_iters[k] = out[k].num_iter;
+      return rt::kOk;
+    }
+    std::vector<u64> last_iters;
+
+    int history(int k, double* out, u64 cap, u64* len) override
+    {
+      if(k < 0 || k >= K || last_iters.empty()) return rt::kErrArg;
+      const u64 n = std::min<u64>(last_iters[k] + 1, hist_cap);
+      const u64 m = std::min(cap, n);
+      if(m) F3_CHECK(hipMemcpy(out, hist.template as<double>() + u64(k) * hist_cap, sizeof(double) * m, hipMemcpyDeviceToHost));
+      if(len) *len = n;
+      return rt::kOk;
+    }
+
+    // which: 0 finest smoothing sweep (all systems), 3 one V-cycle
+    int time_kernel(int which, int reps, double* ms, u64* bytes) override
+    {
+      if(reps < 1) return rt::kErrArg;
+      Level& f = *L[0];
+      for(int pass = 0; pass < 2; ++pass)
+      {
+        const int n = pass == 0 ? 1 : reps;
+        if(pass == 1) F3_CHECK(hipEventRecord(t0, s));
+        for(int k = 0; k < n; ++k)
+        {
+          if(which == 0) F3_RC(sweep(f, f.x.template as<DT>(), f.t.template as<DT>(), nullptr));
+          else if(which == 3) F3_RC(enqueue_vcycle(r.template as<double>(), nullptr));
+          else return rt::kErrArg;
+        }
+        if(pass == 1) F3_CHECK(hipEventRecord(t1, s));
+        F3_CHECK(hipStreamSynchronize(s));
+      }
+      float e = 0.f;
+      F3_CHECK(hipEventElapsedTime(&e, t0, t1));
+      *ms = double(e) / reps;
+      if(bytes)
+      {
+        // algorithmic: values + column indices + row pointers, vectors x (once), rhs, x_out, dinv
+        const u64 vb = sizeof(DT);
+        *bytes = f.nnz * (vals_per_entry() * sizeof(VT) + 4) + (f.nodes + 1) * 4 + f.nodes * B * u64(K) * vb * 3 +
+                 f.nodes * B * (SHARED ? 1 : u64(K)) * vb;
+      }
+      return rt::kOk;
+    }
+  };
+
+  template<typename DT, typename VT>
+  BatchBase* make_engine(int block, int K, bool shared)
+  {
+    const int G = K <= 16 ? 16 : (K <= 32 ? 32 : 64);
+#define F3_MK(GG, BB, SS) if(G == GG && block == BB && shared == SS) return new Engine<DT, VT, GG, BB, SS>(K);
+    F3_MK(16, 1, false) F3_MK(32, 1, false) F3_MK(64, 1, false)
+    F3_MK(16, 1, true) F3_MK(32, 1, true) F3_MK(64, 1, true)
+    F3_MK(16, 3, true) F3_MK(32, 3, true) F3_MK(64, 3, true)
+#undef F3_MK
+    return nullptr;
+  }
+} // namespace
+
+struct feat3_batch
+{
+  int inner = 0, block = 1, K = 1;
+  bool shared = true;
+  Config cfg;
+  std::vector<HostLevel> levels;
+  std::unique_ptr<BatchBase> eng;
+  hipStream_t stream = nullptr;
+};
+
+extern "C" {
+
+int feat3_batch_create(feat3_batch** h, int inner_precision, int block, int nsys, int shared_values)
+{
+  if(!h || inner_precision < 0 || inner_precision > 2 || nsys < 1 || nsys > 64) return rt::kErrArg;
+  if(!((block == 3 && shared_values) || block == 1)) return rt::kErrArg;
+  std::unique_ptr<feat3_batch> o(new feat3_batch());
+  o->inner = inner_precision; o->block = block; o->K = nsys; o->shared = shared_values != 0;
+  if(hipStreamCreateWithFlags(&o->stream, hipStreamNonBlocking) != hipSuccess) return rt::kErrRuntime;
+  *h = o.release();
+  return rt::kOk;
+}
+
+int feat3_batch_destroy(feat3_batch* h)
+{
+  if(!h) return rt::kOk;
+  if(h->stream) (void)hipStreamSynchronize(h->stream);
+  h->eng.reset();
+  if(h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return rt::kOk;
+}
+
+int feat3_batch_push_level(feat3_batch* h, u64 nodes, const u32* row_ptr, const u32* col_ind, const double* val,
+                           u64 nfilt, const u32* filter_nodes, u64 nc, const u32* p_row_ptr, const u32* p_col_ind,
+                           const double* p_val, const u32* r_row_ptr, const u32* r_col_ind, const double* r_val)
+{
+  if(!h || !row_ptr || (row_ptr[nodes] > 0 && (!col_ind || !val)) || (nfilt && !filter_nodes)) return rt::kErrArg;
+  HostLevel L;
+  L.nodes = nodes;
+  L.A.rows = L.A.cols = nodes;
+  L.A.rp.assign(row_ptr, row_ptr + nodes + 1);
+  const u64 nnz = row_ptr[nodes];
+  L.A.col.assign(col_ind, col_ind + nnz);
+  for(u32 c : L.A.col) if(c >= nodes) return rt::kErrArg;
+  const u64 vpe = u64(h->block) * u64(h->block) * (h->shared ? 1 : u64(h->K));
+  L.A.val.assign(val, val + nnz * vpe);
+  L.fidx.assign(filter_nodes, filter_nodes + nfilt);
+  for(u32 f : L.fidx) if(f >= nodes) return rt::kErrArg;
+  if(nc > 0)
+  {
+    if(!p_row_ptr || !p_col_ind || !p_val || !r_row_ptr || !r_col_ind || !r_val) return rt::kErrArg;
+    L.P.rows = nodes; L.P.cols = nc;
+    L.P.rp.assign(p_row_ptr, p_row_ptr + nodes + 1);
+    L.P.col.assign(p_col_ind, p_col_ind + p_row_ptr[nodes]);
+    L.P.val.assign(p_val, p_val + p_row_ptr[nodes]);
+    L.R.rows = nc; L.R.cols = nodes;
+    L.R.rp.assign(r_row_ptr, r_row_ptr + nc + 1);
+    L.R.col.assign(r_col_ind, r_col_ind + r_row_ptr[nc]);
+    L.R.val.assign(r_val, r_val + r_row_ptr[nc]);
+    for(u32 c : L.P.col) if(c >= nc) return rt::kErrArg;
+    for(u32 c : L.R.col) if(c >= nodes) return rt::kErrArg;
+  }
+  if(!h->levels.empty() && h->levels.back().P.cols != nodes) return rt::kErrArg;
+  h->levels.push_back(std::move(L));
+  h->eng.reset();
+  return rt::kOk;
+}
+
+int feat3_batch_set_smoother(feat3_batch* h, double omega_jacobi, double omega_richardson, int pre, int post)
+{
+  if(!h || pre < 1 || post < 1 || !(omega_jacobi > 0.0)) return rt::kErrArg;
+  h->cfg.omega_j = omega_jacobi; h->cfg.omega_r = omega_richardson; h->cfg.pre = pre; h->cfg.post = post;
+  h->eng.reset();
+  return rt::kOk;
+}
+
+int feat3_batch_set_tolerances(feat3_batch* h, double tol_rel, double tol_abs, double tol_abs_low, u64 min_iter, u64 max_iter)
+{
+  if(!h || max_iter == 0) return rt::kErrArg;
+  h->cfg.tol_rel = tol_rel;
+  if(tol_abs > 0) h->cfg.tol_abs = tol_abs;
+  h->cfg.tol_abs_low = tol_abs_low;
+  h->cfg.min_iter = min_iter;
+  h->cfg.max_iter = max_iter;
+  if(h->eng && !h->eng->set_tolerances(h->cfg)) h->eng.reset();
+  return rt::kOk;
+}
+
+int feat3_batch_set_graph(feat3_batch* h, int graph)
+{
+  if(!h) return rt::kErrArg;
+  h->cfg.use_graph = graph != 0;
+  h->eng.reset();
+  return rt::kOk;
+}
+
+int feat3_batch_init(feat3_batch* h)
+{
+  if(!h || h->levels.empty()) return rt::kErrState;
+  std::unique_ptr<BatchBase> e;
+  if(h->inner == 0) e.reset(make_engine<double, double>(h->block, h->K, h->shared));
+  else if(h->inner == 1) e.reset(make_engine<float, float>(h->block, h->K, h->shared));
+  else e.reset(make_engine<float, half>(h->block, h->K, h->shared));
+  if(!e) return rt::kErrArg;
+  const int rc = e->init(h->levels, h->cfg, h->stream);
+  if(rc) return rc;
+  h->eng = std::move(e);
+  return rt::kOk;
+}
+
+int feat3_batch_correct(feat3_batch* h, void* caller_stream, double* x, const double* b, int* status, u64* iters)
+{
+  if(!h || !h->eng) return rt::kErrState;
+  if(caller_stream && hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime;
+  return h->eng->solve(x, b, status, iters);
+}
+
+int feat3_batch_history(feat3_batch* h, int system, double* out, u64 cap, u64* len)
+{
+  if(!h || !h->eng) return rt::kErrState;
+  return h->eng->history(system, out, cap, len);
+}
+
+int feat3_batch_time_kernel(feat3_batch* h, int which, int reps, double* avg_ms, u64* bytes)
+{
+  if(!h || !h->eng || !avg_ms) return rt::kErrState;
+  return h->eng->time_kernel(which, reps, avg_ms, bytes);
+}
+
+} // extern "C"

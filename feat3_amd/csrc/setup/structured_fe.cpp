@@ -477,4 +477,24 @@ int feat3_setup_ref_stiffness(int dim, int degree, double* K)
   return 0;
 }
 
+// n doubles uniform in [a, b] from the reference's xorshift64* stream (util/random.hpp:62-110:
+// shifts 12/25/27, multiplier 2685821657736338717, a zero seed replaced by 28054777172512;
+// doubles (next & 0xFFFFFFFF) / 0xFFFFFFFF mapped to a + x (b - a), random.hpp:259-280)
+int feat3_setup_random_uniform(std::uint64_t seed, std::uint64_t n, double a, double b, double* out)
+{
+  if(n && !out) return 1;
+  std::uint64_t x = seed != 0 ? seed : std::uint64_t(28054777172512ull);
+  const double d = 1.0 / double(std::uint64_t(0xFFFFFFFFull));
+  const bool ranged = a + 2.220446049250313e-16 < b;
+  for(std::uint64_t i = 0; i < n; ++i)
+  {
+    x ^= x >> 12;
+    x ^= x << 25;
+    x ^= x >> 27;
+    const double u = double((x * std::uint64_t(2685821657736338717ull)) & std::uint64_t(0xFFFFFFFFull)) * d;
+    out[i] = ranged ? a + u * (b - a) : a;
+  }
+  return 0;
+}
+
 } // extern "C"

@@ -158,3 +158,93 @@ def config_hierarchy(cfg: int, level: Optional[int] = None) -> Hierarchy:
     c = CONFIGS[cfg]
     lvl = c["level"] if level is None else level
     return hierarchy(c["dim"], c["degree"], lvl, 1)
+
+
+# -------------------------------------------------------------------------------------------
+# batched systems: vector Laplacian blocks, coefficient fields, seeded inputs
+# -------------------------------------------------------------------------------------------
+def block_identity_values(A: Csr, block: int = 3) -> np.ndarray:
+    """Values [entry][h][w] of the blocked Laplacian: every block is a_ij * I (the reference's
+    LaplaceOperatorBlocked, assembly/common_operators.hpp:172-181), stored as full row-major
+    blocks as in SparseMatrixBCSR<block,block>."""
+    v = np.zeros((A.nnz, block, block), F64)
+    for h in range(block):
+        v[:, h, h] = A.val
+    return v
+
+
+def stacked_values(mats: List[Csr]) -> np.ndarray:
+    """Values [entry][system] of K matrices that share one graph."""
+    A0 = mats[0]
+    for M in mats[1:]:
+        if not (np.array_equal(M.row_ptr, A0.row_ptr) and np.array_equal(M.col_ind, A0.col_ind)):
+            raise ValueError("matrices of a batch must share the sparsity graph")
+    return np.ascontiguousarray(np.stack([M.val for M in mats], axis=1))
+
+
+class XorShift64Star:
+    """The reference's random number generator (util/random.hpp:62-110): xorshift64* with
+    multiplier 2685821657736338717; doubles are (next() & 0xFFFFFFFF) / 0xFFFFFFFF and ranged
+    doubles a + x (b - a) (random.hpp:259-280)."""
+
+    MULT = np.uint64(2685821657736338717)
+    DEF_SEED = 28054777172512
+
+    def __init__(self, seed: int = DEF_SEED):
+        self.x = np.uint64(seed if seed != 0 else self.DEF_SEED)
+
+    def next_block(self, n: int) -> np.ndarray:
+        out = np.empty(n, np.uint64)
+        x = self.x
+        with np.errstate(over="ignore"):
+            for i in range(n):
+                x ^= x >> np.uint64(12)
+                x ^= x << np.uint64(25)
+                x ^= x >> np.uint64(27)
+                out[i] = x * self.MULT
+        self.x = x
+        return out
+
+    def uniform(self, n: int, a: float, b: float) -> np.ndarray:
+        u = (self.next_block(n) & np.uint64(0xFFFFFFFF)).astype(F64) * (1.0 / float(0xFFFFFFFF))
+        return a + u * (b - a) if a + np.finfo(F64).eps < b else np.full(n, a)
+
+
+def random_rhs(nodes: int, block: int, nsys: int, bnd: np.ndarray, seed: int) -> np.ndarray:
+    """nsys right-hand sides uniform in [-1, 1] (system fastest: [node][component][system]),
+    zero on Dirichlet nodes (filter_rhs)."""
+    b = np.empty(nodes * block * nsys, F64)
+    capi.call("feat3_setup_random_uniform", seed, b.size, -1.0, 1.0, _p(b))
+    b = b.reshape(nodes, block, nsys)
+    b[bnd] = 0.0
+    return np.ascontiguousarray(b)
+
+
+def gaussian_random_field(dim: int, level: int, seed: int, sigma: float = 1.0, length: float = 0.1) -> np.ndarray:
+    """A stationary Gaussian field with covariance sigma^2 exp(-|x-y|^2 / (2 length^2)) on the cell
+    centres of the level's mesh (x fastest), sampled by circulant embedding: the covariance of
+    the periodic grid of twice the extent is diagonalised by the FFT; the few negative
+    eigenvalues of the embedding (of order 1e-16 for length 0.1) are set to zero."""
+    m = 1 << level
+    h = 1.0 / m
+    M = 2 * m
+    d = np.minimum(np.arange(M), M - np.arange(M)) * h
+    r2 = np.zeros((M,) * dim)
+    for ax in range(dim):
+        shape = [1] * dim
+        shape[ax] = M
+        r2 = r2 + (d ** 2).reshape(shape)
+    c = sigma ** 2 * np.exp(-r2 / (2.0 * length ** 2))
+    lam = np.fft.fftn(c).real
+    lam = np.maximum(lam, 0.0)
+    rng = np.random.default_rng(seed)
+    z = rng.standard_normal((M,) * dim) + 1j * rng.standard_normal((M,) * dim)
+    f = np.fft.fftn(np.sqrt(lam / lam.size) * z).real
+    g = f[tuple(slice(0, m) for _ in range(dim))]
+    # array axes are (x, y[, z]); cells are numbered x fastest
+    return np.ascontiguousarray(g.transpose(tuple(range(dim - 1, -1, -1))).reshape(-1))
+
+
+def field_hierarchies(dim: int, level_fine: int, kfields: np.ndarray, level_coarse: int = 1) -> List[Hierarchy]:
+    """One hierarchy per coefficient field (rows of kfields: per-cell values on the finest level)."""
+    return [hierarchy(dim, 1, level_fine, level_coarse, kcell=k) for k in np.atleast_2d(kfields)]

@@ -304,6 +304,44 @@ int feat3_mgpcg_time_kernel(feat3_mgpcg* h, int which, int reps, double* avg_ms,
 int feat3_mgpcg_info(feat3_mgpcg* h, uint64_t* out, int cap);
 
 /* ===================================================================================== */
+/* Batched fused solver: K systems that share one sparsity graph, solved in lockstep by   */
+/* the same PCG(MG-V(Richardson(Jacobi))) as feat3_mgpcg; every system keeps its own PCG  */
+/* scalars, status and history and performs exactly the iterations of its own solve.      */
+/*  - shared values, block 1 or 3: K right-hand sides of one matrix (block 3 = the 3x3     */
+/*    BCSR vector Laplacian, SparseMatrixBCSR<3,3>, sparse_matrix_bcsr.hpp:1295-1600; the */
+/*    reference applies it once per right-hand side, apply.hpp:71-117)                     */
+/*  - per-system values, block 1: K coefficient fields on one mesh                         */
+/* The reference has no multi-system solver; the per-system semantics are those of        */
+/* PCG (pcg.hpp:188-288) with MultiGrid (multigrid.hpp:1268-2000) and UnitFilterBlocked.   */
+/* Vectors are [node][component][system] (system fastest), FP64, on the device.           */
+/* ===================================================================================== */
+typedef struct feat3_batch feat3_batch;
+
+/* inner_precision as feat3_mgpcg_create; block 1 or 3 (3 needs shared_values);
+ * 1 <= nsys <= 64 */
+int feat3_batch_create(feat3_batch** h, int inner_precision, int block, int nsys, int shared_values);
+int feat3_batch_destroy(feat3_batch* h);
+/* one level (finest first), host arrays, copied: node graph (u32 CSR), values
+ * [entry][bh][bw] (shared) or [entry][system] (per-system, block 1); Dirichlet node list
+ * (all components filtered); scalar P (nodes x nc) and R = P^T applied per component */
+int feat3_batch_push_level(feat3_batch* h, uint64_t nodes, const uint32_t* row_ptr, const uint32_t* col_ind,
+                           const double* val, uint64_t nfilt, const uint32_t* filter_nodes, uint64_t nc,
+                           const uint32_t* p_row_ptr, const uint32_t* p_col_ind, const double* p_val,
+                           const uint32_t* r_row_ptr, const uint32_t* r_col_ind, const double* r_val);
+int feat3_batch_set_smoother(feat3_batch* h, double omega_jacobi, double omega_richardson, int pre, int post);
+int feat3_batch_set_tolerances(feat3_batch* h, double tol_rel, double tol_abs, double tol_abs_low, uint64_t min_iter,
+                               uint64_t max_iter);
+/* graph replay of one PCG iteration on/off */
+int feat3_batch_set_graph(feat3_batch* h, int graph);
+int feat3_batch_init(feat3_batch* h);
+/* x, b: FP64 device arrays [node][component][system]; status, iters: host arrays of nsys */
+int feat3_batch_correct(feat3_batch* h, void* caller_stream, double* x, const double* b, int* status, uint64_t* iters);
+/* defect norms of system `system` in the last solve */
+int feat3_batch_history(feat3_batch* h, int system, double* out_host, uint64_t cap, uint64_t* len);
+/* which: 0 finest smoothing sweep (all systems), 3 one V-cycle; algorithmic bytes of one launch */
+int feat3_batch_time_kernel(feat3_batch* h, int which, int reps, double* avg_ms, uint64_t* bytes);
+
+/* ===================================================================================== */
 /* Host-side problem setup (the step before the hot path: assembly, geometry)         */
 /* ===================================================================================== */
 int feat3_setup_laplace_sizes(int dim, int degree, int level, uint64_t* n, uint64_t* nnz, uint64_t* nbnd);
@@ -326,6 +364,9 @@ int feat3_sell_build(uint64_t rows, uint64_t cols, const uint32_t* row_ptr, cons
                      uint64_t C, uint64_t sigma, uint32_t* cs, uint32_t* cl, uint32_t* perm, uint32_t* col, double* sval,
                      int16_t* d16, int* used_d16);
 int feat3_setup_ref_stiffness(int dim, int degree, double* K);
+/* n doubles uniform in [a, b] from the xorshift64* stream of Random (util/random.hpp:62-110,
+ * 259-280); seed 0 selects the reference's default seed */
+int feat3_setup_random_uniform(uint64_t seed, uint64_t n, double a, double b, double* out);
 
 #ifdef __cplusplus
 }

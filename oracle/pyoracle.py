@@ -148,10 +148,13 @@ class System:
         self.nlev += 1
 
     @classmethod
-    def from_hierarchy(cls, H, omp: bool = False, block: int = 1):
+    def from_hierarchy(cls, H, omp: bool = False, block: int = 1, values=None):
+        """values: optional per-level value arrays replacing L.A.val (block x block blocks,
+        row-major, for block > 1)."""
         s = cls(omp)
-        for L in H.levels:
-            s.add_level(L.A.rows, L.A.row_ptr, L.A.col_ind, L.A.val, L.bnd, block, L.P, L.R)
+        for i, L in enumerate(H.levels):
+            v = L.A.val if values is None else np.ascontiguousarray(values[i], np.float64)
+            s.add_level(L.A.rows, L.A.row_ptr, L.A.col_ind, v, L.bnd, block, L.P, L.R)
         return s
 
     def solve(self, x, b, prm):

@@ -350,3 +350,26 @@ def test_mixed_precision_oracle_converges_like_fp64(inner):
     st, it, hist = S.solve(x, b, po.params(inner=inner))
     assert st == 2 and abs(it - i64) <= 1
     assert np.linalg.norm(x - x64) <= 1e-8 * np.linalg.norm(x64)
+
+
+@pytest.mark.parametrize("inner", [0, 1])
+@pytest.mark.parametrize("comp", [0, 2])
+def test_blocked_vector_laplacian_reduces_to_scalar(inner, comp):
+    """The 3x3-block Laplacian has blocks a_ij * I: with a load in one component only, the
+    blocked solve (BCSR apply, blocked Jacobi diagonal, csrsb transfers, UnitFilterBlocked)
+    must reproduce the scalar solve in that component and leave the others zero."""
+    H = pb.hierarchy(3, 1, 4, 1)
+    vals = [pb.block_identity_values(L.A, 3) for L in H.levels]
+    b = pb.filtered_rhs(H, 0)
+    x = np.zeros(H.n)
+    st, it, hist = po.System.from_hierarchy(H).solve(x, b, po.params(inner=inner))
+    B = np.zeros((H.n, 3))
+    B[:, comp] = b
+    X = np.zeros(3 * H.n)
+    stb, itb, histb = po.System.from_hierarchy(H, block=3, values=vals).solve(X, B.reshape(-1).copy(),
+                                                                              po.params(inner=inner))
+    X = X.reshape(H.n, 3)
+    assert (st, it) == (stb, itb) == (2, it)
+    assert np.max(np.abs(X[:, comp] - x)) <= 1e-13 * np.max(np.abs(x))
+    assert np.all(np.delete(X, comp, axis=1) == 0.0)
+    assert np.allclose(hist, histb, rtol=1e-12)
