@@ -245,6 +245,7 @@ def gaussian_random_field(dim: int, level: int, seed: int, sigma: float = 1.0, l
     return np.ascontiguousarray(g.transpose(tuple(range(dim - 1, -1, -1))).reshape(-1))
 
 
-def field_hierarchies(dim: int, level_fine: int, kfields: np.ndarray, level_coarse: int = 1) -> List[Hierarchy]:
+def field_hierarchies(dim: int, level_fine: int, kfields: np.ndarray, level_coarse: int = 1,
+                      degree: int = 1) -> List[Hierarchy]:
     """One hierarchy per coefficient field (rows of kfields: per-cell values on the finest level)."""
-    return [hierarchy(dim, 1, level_fine, level_coarse, kcell=k) for k in np.atleast_2d(kfields)]
+    return [hierarchy(dim, degree, level_fine, level_coarse, kcell=k) for k in np.atleast_2d(kfields)]

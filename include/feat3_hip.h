@@ -342,6 +342,18 @@ int feat3_batch_history(feat3_batch* h, int system, double* out_host, uint64_t c
 int feat3_batch_time_kernel(feat3_batch* h, int which, int reps, double* avg_ms, uint64_t* bytes);
 
 /* ===================================================================================== */
+/* Adjoint gradient of the variable-coefficient diffusion solve (no counterpart in the    */
+/* reference, which has neither a coefficient operator nor an adjoint): for A(k) =        */
+/* sum_e k_e E (E = h^(d-2) K_ref, the element stiffness of one cell), A u = b and          */
+/* A lambda = dL/du, grad[e] = dL/dk_e = -lambda_e^T E u_e for nsys systems at once.        */
+/* ===================================================================================== */
+/* dim 2 (degree 1, 2) or 3 (degree 1); level: 2^level cells per direction; elem_matrix:   */
+/* host array, nloc x nloc row-major, lexicographic local nodes; u, lam: device FP64       */
+/* [node][system]; grad: device FP64 [cell][system], cells x fastest; enqueued on stream   */
+int feat3_adjoint_cell_gradient(void* stream, int dim, int degree, int level, int nsys, const double* elem_matrix,
+                                const double* u, const double* lam, double* grad);
+
+/* ===================================================================================== */
 /* Host-side problem setup (the step before the hot path: assembly, geometry)         */
 /* ===================================================================================== */
 int feat3_setup_laplace_sizes(int dim, int degree, int level, uint64_t* n, uint64_t* nnz, uint64_t* nbnd);

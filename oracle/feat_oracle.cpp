@@ -21,7 +21,10 @@
 //                                at level 5) and exact two-grid/coarse-solve identities
 //   transpose                 -- R = P^T identity on random patterns
 //   half conversion           -- exhaustive round trip of all 65536 binary16 patterns
-// See tests/test_oracle_golden.py.
+//   cell_gradient (adjoint)   -- no reference counterpart; pinned by central finite differences
+//                                of the loss through two oracle solves and by the identity
+//                                sum_e k_e dL/dk_e = -lambda^T b (A is linear in k)
+// See tests/test_oracle_golden.py and tests/test_oracle_adjoint.py.
 
 #include <algorithm>
 #include <chrono>
@@ -1371,6 +1374,42 @@ int orc_system_vcycle(void* h, const double* praw, double* cor, const double* de
     st = cp.apply(cor, def);
   }
   return int(st);
+}
+
+// dL/dk_e = -lambda_e^T E u_e for one system: E = element stiffness (nloc x nloc, lexicographic
+// local nodes), u and lambda nodal vectors of the level-`level` Q`degree` space on the unit
+// square/cube (lexicographic nodes, x fastest), grad per cell (x fastest).
+int orc_cell_gradient(int dim, int degree, int level, const double* E, const double* u, const double* lam, double* grad)
+{
+  if(dim < 2 || dim > 3 || degree < 1 || degree > 2) return 1;
+  const u64 m = u64(1) << level;
+  const int q = degree + 1;
+  const int nloc = dim == 2 ? q * q : q * q * q;
+  const u64 np1 = u64(degree) * m + 1;
+  const u64 ncell = dim == 2 ? m * m : m * m * m;
+  std::vector<u64> nodes(nloc);
+  for(u64 c = 0; c < ncell; ++c)
+  {
+    u64 e[3] = {c % m, (c / m) % m, c / (m * m)};
+    for(int a = 0; a < nloc; ++a)
+    {
+      int s = a;
+      u64 node = 0, stride = 1;
+      for(int d = 0; d < dim; ++d)
+      {
+        node += (u64(degree) * e[d] + u64(s % q)) * stride;
+        s /= q;
+        stride *= np1;
+      }
+      nodes[a] = node;
+    }
+    double acc = 0.0;
+    for(int a = 0; a < nloc; ++a)
+      for(int b = 0; b < nloc; ++b)
+        acc += lam[nodes[a]] * E[a * nloc + b] * u[nodes[b]];
+    grad[c] = -acc;
+  }
+  return 0;
 }
 
 } // extern "C"

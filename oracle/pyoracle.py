@@ -56,6 +56,7 @@ _PROTOS = {
     "orc_system_solve": (c_u64, [c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_i), c_vp, c_u64, ctypes.POINTER(c_u64)]),
     "orc_system_vcycle": (c_i, [c_vp, c_vp, c_vp, c_vp]),
     "orc_last_solve_seconds": (c_d, []),
+    "orc_cell_gradient": (c_i, [c_i, c_i, c_i, c_vp, c_vp, c_vp, c_vp]),
 }
 
 
@@ -172,3 +173,13 @@ class System:
             self.L.orc_system_destroy(self.h)
         except Exception:
             pass
+
+
+def cell_gradient(dim, degree, level, E, u, lam):
+    """dL/dk_e = -lambda_e^T E u_e for every cell (one system)."""
+    m = 1 << level
+    g = np.empty(m ** dim)
+    E = np.ascontiguousarray(E, np.float64)
+    rc = lib().orc_cell_gradient(dim, degree, level, P(E), P(np.ascontiguousarray(u)), P(np.ascontiguousarray(lam)), P(g))
+    assert rc == 0
+    return g
