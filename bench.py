@@ -38,6 +38,191 @@ def sweep_algorithmic_bytes(info, n, nnz, nchunks):
     return nnz * (s_v + s_i) + 8 * nchunks + n * s_x * 4
 
 
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    times, outs = [], []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        o = fn()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        outs.append(o)
+    return times, outs
+
+
+def _common(args, metric_value, ms, dtype, data, config, roof, extra, cpu):
+    return {
+        "metric": "time to rel. residual 1e-8 & DOF·iter/s, FP32-inner vs FP64; SpMV GB/s",
+        "value": metric_value, "unit": "DOF*iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+        "data": data, "config": config, "roofline": roof, **extra, "cpu_baseline": cpu,
+    }
+
+
+def bench_vector32(args):
+    """32 right-hand sides (xorshift64* uniform[-1,1], seed 5) of the 3D Q1 level-6 vector Laplacian,
+    3x3-block BCSR (274,625 block rows), solved in one batched PCG-MG launch sequence.  One step
+    = one batched solve of all 32 systems to relative residual 1e-8 from x = 0."""
+    import numpy as np
+    import torch
+    from feat3_amd import problems as pb
+    from feat3_amd.batch import BatchMGPCG
+
+    K, lvl = 32, args.level if args.level != 7 else 6
+    t = time.time()
+    H = pb.hierarchy(3, 1, lvl, 1)
+    vals = [pb.block_identity_values(L.A, 3) for L in H.levels]
+    B = pb.random_rhs(H.n, 3, K, H.levels[0].bnd, seed=5)
+    nodes, nnzb = H.n, H.levels[0].A.nnz
+    log(f"setup vector32: {len(H.levels)} levels, {nodes} block rows, {nnzb} blocks ({time.time() - t:.1f} s)")
+    dB = torch.from_numpy(B).cuda()
+    X = torch.zeros_like(dB)
+    res = {}
+    for inner in ([args.inner] + ([] if args.no_variants else [v for v in ("f64", "f32") if v != args.inner])):
+        s = BatchMGPCG(H, K, values=vals, block=3, inner=inner).init()
+
+        def step():
+            X.zero_()
+            st, it = s.correct(X, dB)
+            assert all(int(v) == 2 for v in st), st
+            return it
+        times, its = _timed(step, args.steps if inner == args.inner else max(2, args.steps // 2), args.warmup)
+        sweep_ms, sweep_b = s.time_kernel(0, 20)
+        vc_ms, _ = s.time_kernel(3, 5)
+        it = its[-1]
+        r = dict(ms_per_solve=1e3 * float(np.mean(times)), iterations=[int(v) for v in it],
+                 dof_iter_per_s=3 * nodes * float(np.sum(it)) / float(np.mean(times)),
+                 sweep_us=sweep_ms * 1e3, sweep_bytes=sweep_b, sweep_GBs=sweep_b / (sweep_ms * 1e-3) / 1e9,
+                 vcycle_ms=vc_ms, solution=X.cpu().numpy().copy())
+        log(f"vector32 {inner}: {r['ms_per_solve']:.2f} ms/solve, it {min(it)}..{max(it)}, sweep {r['sweep_us']:.1f} us "
+            f"{r['sweep_GBs']:.0f} GB/s, V-cycle {vc_ms:.3f} ms")
+        res[inner] = r
+        s.done()
+    head = res[args.inner]
+    if "f64" in res and args.inner != "f64":
+        x64 = res["f64"]["solution"]
+        head["rel_diff_vs_fp64_solution"] = float(np.linalg.norm(head["solution"] - x64) / np.linalg.norm(x64))
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+        S = po.System.from_hierarchy(H, block=3, values=vals)
+        xo = np.zeros(3 * nodes)
+        st, it, _ = S.solve(xo, B[:, :, 0].reshape(-1).copy(), po.params())
+        dt = po.lib().orc_last_solve_seconds()
+        cpu = {"value": 3 * nodes * it / dt, "unit": "DOF*iter/s", "cores": 1, "kind": "port",
+               "sample": f"one of the 32 right-hand sides solved by the oracle's FP64 blocked PCG-MG on one core "
+                         f"({it} iterations, {dt:.2f} s solve phase); the reference solves the 32 one after another",
+               "time_to_tol_ms_32_rhs": 32 * dt * 1e3}
+        if "f64" in res:
+            x64 = res["f64"]["solution"][:, :, 0].reshape(-1)
+            cpu["rel_diff_gpu_fp64_vs_cpu_rhs0"] = float(np.linalg.norm(x64 - xo) / np.linalg.norm(xo))
+    for r in res.values():
+        r.pop("solution")
+    roof = {"bound": "hbm", "achieved": head["sweep_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": head["sweep_GBs"] / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "batch k_sweep<B=3>, finest level (fused damped-Jacobi step, 32 systems)",
+            "bytes_per_launch": head["sweep_bytes"], "avg_launch_us": head["sweep_us"]}
+    cfg = {"workload": f"3D vector Laplacian Q1 level {lvl}, 3x3 BCSR ({nodes} block rows, {nnzb} blocks), "
+                       f"32 right-hand sides in one batched PCG(FP64) + MG V(4,4), {args.inner} V-cycle, rel. tol 1e-8",
+           "block_rows": nodes, "blocks": nnzb, "rhs": K}
+    return _common(args, head["dof_iter_per_s"], head["ms_per_solve"], args.inner,
+                   "synthetic (assembled vector Laplacian, xorshift64* right-hand sides)", cfg, roof,
+                   {"headline": head, "variants": {k: v for k, v in res.items() if k != args.inner}}, cpu)
+
+
+def bench_coeff16(args):
+    """16 coefficient fields k = exp(g), g Gaussian random fields (sigma 1, squared-exponential,
+    length 0.1, seeds 0..15), on the 2D Q1 level-10 mesh; u_obs from the field of seed 1000.  One
+    step = forward batched solve + adjoint batched solve + dL/dk kernel (the solver built from k
+    beforehand; host assembly of the 16 hierarchies reported separately)."""
+    import numpy as np
+    import torch
+    from feat3_amd import problems as pb
+    from feat3_amd.autograd import DiffusionProblem, cell_gradient
+    from feat3_amd.batch import BatchMGPCG  # noqa: F401
+
+    K, lvl = 16, args.level if args.level != 7 else 10
+    prob = DiffusionProblem(dim=2, level=lvl, inner=args.inner)
+    t = time.time()
+    kf = np.exp(np.stack([pb.gaussian_random_field(2, lvl, seed=s) for s in range(K)]))
+    kobs = np.exp(pb.gaussian_random_field(2, lvl, seed=1000))
+    t_grf = time.time() - t
+    t = time.time()
+    s = prob.make_solver(kf)
+    t_setup = time.time() - t
+    log(f"setup coeff16: GRF {t_grf:.1f} s, 16 hierarchies + solver {t_setup:.1f} s, n={prob.nodes}")
+    so = prob.make_solver(kobs[None, :])
+    b1 = torch.from_numpy(prob.b[:, None].copy()).cuda()
+    uobs = torch.zeros_like(b1)
+    so.correct(uobs, b1)
+    so.done()
+    rhs = torch.from_numpy(np.repeat(prob.b[:, None], K, axis=1)).cuda()
+    U = torch.zeros_like(rhs)
+    Lam = torch.zeros_like(rhs)
+    bnd = torch.from_numpy(prob.bnd.astype(np.int64)).cuda()
+
+    def step():
+        U.zero_()
+        st1, it1 = s.correct(U, rhs)
+        g_u = U - uobs
+        g_u[bnd] = 0.0
+        Lam.zero_()
+        st2, it2 = s.correct(Lam, g_u)
+        g = cell_gradient(prob, U, Lam)
+        assert all(int(v) == 2 for v in st1) and all(int(v) == 2 for v in st2)
+        return it1, it2, g
+    times, outs = _timed(step, args.steps, args.warmup)
+    it1, it2, g = outs[-1]
+    iters = float(np.sum(it1) + np.sum(it2))
+    ms = 1e3 * float(np.mean(times))
+    sweep_ms, sweep_b = s.time_kernel(0, 20)
+    Ug, gg = U.cpu().numpy(), g.cpu().numpy()
+    uo = uobs.cpu().numpy()[:, 0]
+    ident = max(abs(kf[j] @ gg[:, j] + np.dot(Ug[:, j] - uo, Ug[:, j])) / abs(np.dot(Ug[:, j] - uo, Ug[:, j]))
+                for j in range(K))
+    # the gradient kernel alone
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        cell_gradient(prob, U, Lam)
+    e1.record()
+    torch.cuda.synchronize()
+    grad_ms = e0.elapsed_time(e1) / 20
+    grad_bytes = 2 * prob.nodes * K * 8 + prob.cells * K * 8
+    head = dict(ms_per_step=ms, forward_iterations=[int(v) for v in it1], adjoint_iterations=[int(v) for v in it2],
+                dof_iter_per_s=prob.nodes * iters / (ms * 1e-3), sweep_us=sweep_ms * 1e3, sweep_bytes=sweep_b,
+                sweep_GBs=sweep_b / (sweep_ms * 1e-3) / 1e9, grad_kernel_us=grad_ms * 1e3,
+                grad_kernel_GBs=grad_bytes / (grad_ms * 1e-3) / 1e9, max_rel_identity_error=ident,
+                host_setup_s=t_setup, grf_s=t_grf)
+    log(f"coeff16: {ms:.2f} ms/step, fwd it {min(it1)}..{max(it1)}, adj it {min(it2)}..{max(it2)}, "
+        f"sweep {head['sweep_us']:.1f} us {head['sweep_GBs']:.0f} GB/s, dL/dk {grad_ms * 1e3:.1f} us, identity {ident:.1e}")
+    s.done()
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+        H0 = pb.hierarchy(2, 1, lvl, 1, kcell=kf[0])
+        S = po.System.from_hierarchy(H0)
+        xo = np.zeros(prob.nodes)
+        st, it, _ = S.solve(xo, prob.b.copy(), po.params())
+        dt = po.lib().orc_last_solve_seconds()
+        cpu = {"value": prob.nodes * it / dt, "unit": "DOF*iter/s", "cores": 1, "kind": "port",
+               "sample": f"the forward solve of field 0 by the oracle's FP64 PCG-MG on one core ({it} iterations, "
+                         f"{dt:.2f} s solve phase)"}
+    roof = {"bound": "hbm", "achieved": head["sweep_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": head["sweep_GBs"] / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "batch k_sweep<B=1, per-system values>, finest level (16 fields)",
+            "bytes_per_launch": head["sweep_bytes"], "avg_launch_us": head["sweep_us"]}
+    cfg = {"workload": f"2D variable-coefficient diffusion Q1 level {lvl} ({prob.nodes} DOF), 16 coefficient fields: "
+                       f"forward + adjoint batched PCG(FP64) + MG V(4,4), {args.inner} V-cycle, rel. tol 1e-8, dL/dk kernel",
+           "dofs": prob.nodes, "fields": K}
+    return _common(args, head["dof_iter_per_s"], ms, args.inner,
+                   "synthetic (seeded Gaussian random fields, f = 1)", cfg, roof, {"headline": head}, cpu)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,6 +233,10 @@ def main():
     ap.add_argument("--fmt", default="sell", choices=["csr", "sell"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP64 / FP16 comparison solves")
+    ap.add_argument("--workload", default="poisson3d", choices=["poisson3d", "vector32", "coeff16"],
+                    help="poisson3d: the headline 3D Q1 L7 solve; vector32: 32 right-hand sides of the 3x3-block "
+                         "vector Laplacian (3D Q1 L6) in one batched solve; coeff16: forward + adjoint solves and "
+                         "dL/dk of 16 coefficient fields (2D Q1 L10)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -63,6 +252,10 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
+    if args.workload != "poisson3d":
+        out = (bench_vector32 if args.workload == "vector32" else bench_coeff16)(args)
+        print(json.dumps(out), flush=True)
+        return
 
     t = time.time()
     H = pb.hierarchy(3, 1, args.level, 1)

@@ -30,7 +30,7 @@ class BatchMGPCG:
                  block: int = 1, shared: bool = True, inner: str = "f64",
                  omega_jacobi: float = 0.5, omega_richardson: float = 1.0, pre: int = 4, post: int = 4,
                  tol_rel: float = 1e-8, tol_abs: float = 0.0, tol_abs_low: float = 0.0,
-                 min_iter: int = 0, max_iter: int = 1000, graph: bool = True):
+                 min_iter: int = 0, max_iter: int = 1000, graph: bool = True, options: Optional[dict] = None):
         self.hierarchy = hierarchy
         self.nsys, self.block, self.shared = nsys, block, shared
         self.max_iter = max_iter
@@ -56,6 +56,8 @@ class BatchMGPCG:
         capi.call("feat3_batch_set_smoother", h, omega_jacobi, omega_richardson, pre, post)
         capi.call("feat3_batch_set_tolerances", h, tol_rel, tol_abs, tol_abs_low, min_iter, max_iter)
         capi.call("feat3_batch_set_graph", h, int(graph))
+        for k, v in (options or {}).items():
+            capi.call("feat3_batch_set_option", h, k.encode(), float(v))
         self.status = np.zeros(nsys, np.int32)
         self.iters = np.zeros(nsys, np.uint64)
 

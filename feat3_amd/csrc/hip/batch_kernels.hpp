@@ -3,11 +3,12 @@
 // Laplacian of SparseMatrixBCSR<3,3>, sparse_matrix_bcsr.hpp:1295-1600), and values either
 // shared by all K systems (K right-hand sides) or one set per system (K coefficient fields).
 //
-// Vectors are stored [node][component][system] (system fastest): lane k of a G-lane group
-// handles system k of one block row, so a block's values are read once for all systems and the
-// gathers x[col][w][0..K) are contiguous.  Every system keeps its own PCG scalars and status;
-// a converged system's vectors are frozen, so each system performs exactly the iterations of its
-// independent solve (pcg.hpp:188-288, iterative.hpp:744-879).
+// Vectors are stored [node][component][system] (system fastest).  The row kernels give every
+// lane S consecutive systems (one 16-byte vector access: 4 floats or 2 doubles) and a row
+// G = K/S lanes, so a wave keeps 64/G rows and their x gathers in flight; a block's values are
+// read once per lane group for all of its S systems per lane.  Every system keeps its own PCG
+// scalars and status; a converged system's vectors are frozen, so each system performs
+// exactly the iterations of its independent solve (pcg.hpp:188-288, iterative.hpp:744-879).
 #pragma once
 
 #include "device_common.hpp"
@@ -21,45 +22,126 @@ namespace batch
   typedef std::uint64_t u64;
   using mg::RowGrid;
 
-  // values of graph entry e, block position (h, w), system k
-  template<typename DT_, typename VT_, int B_, bool SHARED_>
-  __device__ __forceinline__ DT_ bval(const VT_* val, u64 e, int h, int w, int k, int K)
+  // ---- S-wide accesses ---------------------------------------------------------------------
+  template<typename T, int S_>
+  __device__ __forceinline__ void ldv(const T* p, T (&o)[S_])
   {
-    const u64 q = (e * B_ + u64(h)) * B_ + u64(w);
-    return SHARED_ ? dev::ld<DT_>(val, q) : dev::ld<DT_>(val, q * u64(K) + u64(k));
+    if constexpr(S_ == 1)
+      o[0] = p[0];
+    else
+    {
+      typedef T V __attribute__((ext_vector_type(S_)));
+      const V v = *reinterpret_cast<const V*>(p);
+#pragma unroll
+      for(int i = 0; i < S_; ++i) o[i] = v[i];
+    }
   }
 
-  // block row sums s[h] = sum_e sum_w A_e[h][w] x[col_e][w][k]
-  template<typename DT_, typename VT_, int B_, bool SHARED_>
+  template<typename T, int S_>
+  __device__ __forceinline__ void stv(T* p, const T (&o)[S_])
+  {
+    if constexpr(S_ == 1)
+      p[0] = o[0];
+    else
+    {
+      typedef T V __attribute__((ext_vector_type(S_)));
+      V v;
+#pragma unroll
+      for(int i = 0; i < S_; ++i) v[i] = o[i];
+      *reinterpret_cast<V*>(p) = v;
+    }
+  }
+
+  template<typename DT_, typename VT_>
+  __device__ __forceinline__ DT_ cvt(VT_ v) { return DT_(v); }
+  template<>
+  __device__ __forceinline__ float cvt<float, dev::half>(dev::half v) { return float(v); }
+
+  template<typename DT_, typename VT_, int S_>
+  __device__ __forceinline__ void ldvc(const VT_* p, DT_ (&o)[S_])
+  {
+    VT_ t[S_];
+    ldv<VT_, S_>(p, t);
+#pragma unroll
+    for(int i = 0; i < S_; ++i) o[i] = cvt<DT_, VT_>(t[i]);
+  }
+
+  // ---- row evaluators: s[h][i] = sum_e sum_w A_e[h][w] x[col_e][w][k0 + i] -------------------
+  // Entries are taken in batches of kU: the column loads of a batch, then its x gathers, are in
+  // flight together instead of one dependent col -> x round trip per entry.
+#ifndef F3_BATCH_UNROLL
+#define F3_BATCH_UNROLL 4
+#endif
+  constexpr int kU = F3_BATCH_UNROLL;
+
+  template<typename DT_, typename VT_, int B_, bool SHARED_, int S_>
   struct BlockRows
   {
     const u32* rp;
     const u32* col;
-    const VT_* val;
+    const VT_* val;   // [entry][h][w] (shared) or [entry][k] (per system, B = 1)
     u64 rows;
     int K;
-    __device__ __forceinline__ void row_sum(u64 p, int k, const DT_* __restrict__ x, DT_ (&s)[B_]) const
+    const u32* ord;   // processing order of the rows (null: ascending)
+
+    __device__ __forceinline__ void acc(u32 e, int k0, const DT_ (&xv)[B_][S_], DT_ (&s)[B_][S_]) const
     {
-#pragma unroll
-      for(int h = 0; h < B_; ++h) s[h] = DT_(0);
-      const u32 end = rp[p + 1];
-      for(u32 e = rp[p]; e < end; ++e)
+      if constexpr(SHARED_)
       {
-        const u64 j = col[e];
-        DT_ xv[B_];
-#pragma unroll
-        for(int w = 0; w < B_; ++w) xv[w] = x[(j * B_ + u64(w)) * u64(K) + u64(k)];
 #pragma unroll
         for(int h = 0; h < B_; ++h)
 #pragma unroll
           for(int w = 0; w < B_; ++w)
-            s[h] += bval<DT_, VT_, B_, SHARED_>(val, e, h, w, k, K) * xv[w];
+          {
+            const DT_ a = dev::ld<DT_>(val, (u64(e) * B_ + u64(h)) * B_ + u64(w));
+#pragma unroll
+            for(int i = 0; i < S_; ++i) s[h][i] += a * xv[w][i];
+          }
+      }
+      else
+      {
+        static_assert(B_ == 1, "per-system values are scalar");
+        DT_ a[S_];
+        ldvc<DT_, VT_, S_>(val + u64(e) * u64(K) + u64(k0), a);
+#pragma unroll
+        for(int i = 0; i < S_; ++i) s[0][i] += a[i] * xv[0][i];
+      }
+    }
+
+    __device__ __forceinline__ void row_sum(u64 p, int k0, const DT_* __restrict__ x, DT_ (&s)[B_][S_]) const
+    {
+#pragma unroll
+      for(int h = 0; h < B_; ++h)
+#pragma unroll
+        for(int i = 0; i < S_; ++i) s[h][i] = DT_(0);
+      const u32 end = rp[p + 1];
+      u32 e = rp[p];
+      for(; e + kU <= end; e += kU)
+      {
+        u64 j[kU];
+#pragma unroll
+        for(int u = 0; u < kU; ++u) j[u] = col[e + u];
+        DT_ xv[kU][B_][S_];
+#pragma unroll
+        for(int u = 0; u < kU; ++u)
+#pragma unroll
+          for(int w = 0; w < B_; ++w) ldv<DT_, S_>(x + (j[u] * B_ + u64(w)) * u64(K) + u64(k0), xv[u][w]);
+#pragma unroll
+        for(int u = 0; u < kU; ++u) acc(e + u, k0, xv[u], s);
+      }
+      for(; e < end; ++e)
+      {
+        const u64 j = col[e];
+        DT_ xv[B_][S_];
+#pragma unroll
+        for(int w = 0; w < B_; ++w) ldv<DT_, S_>(x + (j * B_ + u64(w)) * u64(K) + u64(k0), xv[w]);
+        acc(e, k0, xv, s);
       }
     }
   };
 
   // scalar graph applied per component (the transfers: csrsb, apply_generic.hpp:156-181)
-  template<typename DT_, typename VT_, int B_>
+  template<typename DT_, typename VT_, int B_, int S_>
   struct CompRows
   {
     const u32* rp;
@@ -67,49 +149,98 @@ namespace batch
     const VT_* val;
     u64 rows;
     int K;
-    __device__ __forceinline__ void row_sum(u64 p, int k, const DT_* __restrict__ x, DT_ (&s)[B_]) const
+    const u32* ord;   // processing order of the rows (null: ascending)
+
+    __device__ __forceinline__ void row_sum(u64 p, int k0, const DT_* __restrict__ x, DT_ (&s)[B_][S_]) const
     {
 #pragma unroll
-      for(int h = 0; h < B_; ++h) s[h] = DT_(0);
+      for(int h = 0; h < B_; ++h)
+#pragma unroll
+        for(int i = 0; i < S_; ++i) s[h][i] = DT_(0);
       const u32 end = rp[p + 1];
-      for(u32 e = rp[p]; e < end; ++e)
+      u32 e = rp[p];
+      for(; e + kU <= end; e += kU)
+      {
+        u64 j[kU];
+        DT_ v[kU];
+#pragma unroll
+        for(int u = 0; u < kU; ++u) { j[u] = col[e + u]; v[u] = dev::ld<DT_>(val, e + u); }
+        DT_ xv[kU][B_][S_];
+#pragma unroll
+        for(int u = 0; u < kU; ++u)
+#pragma unroll
+          for(int h = 0; h < B_; ++h) ldv<DT_, S_>(x + (j[u] * B_ + u64(h)) * u64(K) + u64(k0), xv[u][h]);
+#pragma unroll
+        for(int u = 0; u < kU; ++u)
+#pragma unroll
+          for(int h = 0; h < B_; ++h)
+#pragma unroll
+            for(int i = 0; i < S_; ++i) s[h][i] += v[u] * xv[u][h][i];
+      }
+      for(; e < end; ++e)
       {
         const u64 j = col[e];
         const DT_ v = dev::ld<DT_>(val, e);
 #pragma unroll
-        for(int h = 0; h < B_; ++h) s[h] += v * x[(j * B_ + u64(h)) * u64(K) + u64(k)];
+        for(int h = 0; h < B_; ++h)
+        {
+          DT_ xv[S_];
+          ldv<DT_, S_>(x + (j * B_ + u64(h)) * u64(K) + u64(k0), xv);
+#pragma unroll
+          for(int i = 0; i < S_; ++i) s[h][i] += v * xv[i];
+        }
       }
     }
   };
 
-  // calls f(row, k, s[B]) for the rows of this block; lanes k >= K of a group are idle
-  template<int G_, int B_, typename Rows_, typename DT_, typename F>
+  // calls f(row, k0, s[B][S]) for the rows of this block: G lanes per row, lane l handles the
+  // systems k0 = l*S .. k0+S-1; lanes with k0 >= K are idle.  Block b takes positions
+  // [b*per, (b+1)*per) of the processing order (XCD-remapped, so one XCD works through one
+  // contiguous part of it).
+  template<int G_, int S_, int B_, typename Rows_, typename DT_, typename F>
   __device__ __forceinline__ void for_block_rows(const Rows_& A, u64 per, const DT_* __restrict__ x, F&& f)
   {
     const u64 b = dev::xcd_remap(blockIdx.x, gridDim.x);
     const u64 beg = b * per;
     const u64 end = beg + per < A.rows ? beg + per : A.rows;
-    const int k = int(threadIdx.x % G_);
-    if(k >= A.K) return;
-    for(u64 p = beg + threadIdx.x / G_; p < end; p += blockDim.x / G_)
+    const int k0 = int(threadIdx.x % G_) * S_;
+    if(k0 >= A.K) return;
+    for(u64 q = beg + threadIdx.x / G_; q < end; q += blockDim.x / G_)
     {
-      DT_ s[B_];
-      A.row_sum(p, k, x, s);
-      f(p, k, s);
+      const u64 p = A.ord ? u64(A.ord[q]) : q;
+      DT_ s[B_][S_];
+      A.row_sum(p, k0, x, s);
+      f(p, k0, s);
     }
   }
 
   __device__ __forceinline__ bool masked(const u32* mask, u64 node) { return dev::masked(mask, node); }
   __device__ __forceinline__ bool idle(const int* any) { return any != nullptr && *any == 0; }
 
-  // ---- V-cycle -------------------------------------------------------------------------------
   // dinv layout: [node][h] (shared values) or [node][h][k] (one matrix per system)
-  template<int B_, bool SHARED_>
-  __device__ __forceinline__ u64 dinv_index(u64 node, int h, int k, int K)
+  template<int B_, bool SHARED_, int S_, typename DT_>
+  __device__ __forceinline__ void ld_dinv(const DT_* dinv, u64 node, int h, int k0, int K, DT_ (&d)[S_])
   {
-    return SHARED_ ? node * B_ + u64(h) : (node * B_ + u64(h)) * u64(K) + u64(k);
+    if constexpr(SHARED_)
+    {
+      const DT_ v = dinv[node * B_ + u64(h)];
+#pragma unroll
+      for(int i = 0; i < S_; ++i) d[i] = v;
+    }
+    else
+      ldv<DT_, S_>(dinv + (node * B_ + u64(h)) * u64(K) + u64(k0), d);
   }
 
+  // Row kernels keep kU entries' gathers in flight per lane.  Left to itself the compiler keeps
+  // 8 waves per SIMD by issuing those loads one after another (fewer live registers); the
+  // bound below lets it spend registers on memory-level parallelism instead.
+#ifndef F3_BATCH_WAVES
+#define F3_BATCH_WAVES 4
+#endif
+#define F3_ROW_KERNEL __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, F3_BATCH_WAVES)))
+
+  // ---- V-cycle -------------------------------------------------------------------------------
+  // rhs = (inner) def, x = dinv * rhs (the first smoothing step from zero)
   template<typename DT_, int B_, bool SHARED_>
   __global__ void k_entry(u64 nodes, int K, const double* __restrict__ def, DT_* __restrict__ rhs, DT_* __restrict__ x,
                           const DT_* __restrict__ dinv, const int* any)
@@ -118,80 +249,107 @@ namespace batch
     const u64 n = nodes * B_ * u64(K);
     for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
     {
-      const int k = int(i % u64(K));
+      const u64 k = i % u64(K);
       const u64 nh = i / u64(K);
       const DT_ r = DT_(def[i]);
       rhs[i] = r;
-      x[i] = dinv[dinv_index<B_, SHARED_>(nh / B_, int(nh % B_), k, K)] * r;
+      x[i] = dinv[SHARED_ ? nh : nh * u64(K) + k] * r;
     }
   }
 
-  template<int G_, int B_, bool SHARED_, typename Rows_, typename DT_>
-  __global__ __launch_bounds__(256) void k_sweep(Rows_ A, u64 per, const DT_* __restrict__ dinv, const DT_* __restrict__ rhs,
+  // x_out = x_in + dinv (rhs - A x_in)
+  template<int G_, int S_, int B_, bool SHARED_, typename Rows_, typename DT_>
+  F3_ROW_KERNEL void k_sweep(Rows_ A, u64 per, const DT_* __restrict__ dinv, const DT_* __restrict__ rhs,
                                                  const DT_* __restrict__ xin, DT_* __restrict__ xout, const int* any)
   {
     if(idle(any)) return;
     const int K = A.K;
-    for_block_rows<G_, B_>(A, per, xin, [&](u64 p, int k, const DT_ (&s)[B_]) {
+    for_block_rows<G_, S_, B_>(A, per, xin, [&](u64 p, int k0, const DT_ (&s)[B_][S_]) {
 #pragma unroll
       for(int h = 0; h < B_; ++h)
       {
-        const u64 i = (p * B_ + u64(h)) * u64(K) + u64(k);
-        xout[i] = xin[i] + dinv[dinv_index<B_, SHARED_>(p, h, k, K)] * (rhs[i] - s[h]);
+        const u64 i0 = (p * B_ + u64(h)) * u64(K) + u64(k0);
+        DT_ xi[S_], ri[S_], di[S_], xo[S_];
+        ldv<DT_, S_>(xin + i0, xi);
+        ldv<DT_, S_>(rhs + i0, ri);
+        ld_dinv<B_, SHARED_, S_>(dinv, p, h, k0, K, di);
+#pragma unroll
+        for(int i = 0; i < S_; ++i) xo[i] = xi[i] + di[i] * (ri[i] - s[h][i]);
+        stv<DT_, S_>(xout + i0, xo);
       }
     });
   }
 
-  template<int G_, int B_, typename Rows_, typename DT_>
-  __global__ __launch_bounds__(256) void k_defect(Rows_ A, u64 per, const u32* __restrict__ mask, const DT_* __restrict__ rhs,
+  // def = filter(rhs - A x)
+  template<int G_, int S_, int B_, typename Rows_, typename DT_>
+  F3_ROW_KERNEL void k_defect(Rows_ A, u64 per, const u32* __restrict__ mask, const DT_* __restrict__ rhs,
                                                   const DT_* __restrict__ x, DT_* __restrict__ def, const int* any)
   {
     if(idle(any)) return;
     const int K = A.K;
-    for_block_rows<G_, B_>(A, per, x, [&](u64 p, int k, const DT_ (&s)[B_]) {
+    for_block_rows<G_, S_, B_>(A, per, x, [&](u64 p, int k0, const DT_ (&s)[B_][S_]) {
       const bool m = masked(mask, p);
 #pragma unroll
       for(int h = 0; h < B_; ++h)
       {
-        const u64 i = (p * B_ + u64(h)) * u64(K) + u64(k);
-        def[i] = m ? DT_(0) : rhs[i] - s[h];
+        const u64 i0 = (p * B_ + u64(h)) * u64(K) + u64(k0);
+        DT_ ri[S_], d[S_];
+        ldv<DT_, S_>(rhs + i0, ri);
+#pragma unroll
+        for(int i = 0; i < S_; ++i) d[i] = m ? DT_(0) : ri[i] - s[h][i];
+        stv<DT_, S_>(def + i0, d);
       }
     });
   }
 
-  template<int G_, int B_, bool SHARED_, typename Rows_, typename DT_>
-  __global__ __launch_bounds__(256) void k_restrict(Rows_ R, u64 per, const u32* __restrict__ mask_c, const DT_* __restrict__ def_f,
+  // rhs_c = filter(R def_f); x_c = dinv_c rhs_c (first coarse smoothing step) unless x_c is null
+  template<int G_, int S_, int B_, bool SHARED_, typename Rows_, typename DT_>
+  F3_ROW_KERNEL void k_restrict(Rows_ R, u64 per, const u32* __restrict__ mask_c, const DT_* __restrict__ def_f,
                                                     DT_* __restrict__ rhs_c, DT_* __restrict__ x_c, const DT_* __restrict__ dinv_c,
                                                     const int* any)
   {
     if(idle(any)) return;
     const int K = R.K;
-    for_block_rows<G_, B_>(R, per, def_f, [&](u64 p, int k, const DT_ (&s)[B_]) {
+    for_block_rows<G_, S_, B_>(R, per, def_f, [&](u64 p, int k0, const DT_ (&s)[B_][S_]) {
       const bool m = masked(mask_c, p);
 #pragma unroll
       for(int h = 0; h < B_; ++h)
       {
-        const u64 i = (p * B_ + u64(h)) * u64(K) + u64(k);
-        const DT_ r = m ? DT_(0) : s[h];
-        rhs_c[i] = r;
-        if(x_c) x_c[i] = dinv_c[dinv_index<B_, SHARED_>(p, h, k, K)] * r;
+        const u64 i0 = (p * B_ + u64(h)) * u64(K) + u64(k0);
+        DT_ r[S_];
+#pragma unroll
+        for(int i = 0; i < S_; ++i) r[i] = m ? DT_(0) : s[h][i];
+        stv<DT_, S_>(rhs_c + i0, r);
+        if(x_c)
+        {
+          DT_ d[S_];
+          ld_dinv<B_, SHARED_, S_>(dinv_c, p, h, k0, K, d);
+#pragma unroll
+          for(int i = 0; i < S_; ++i) d[i] *= r[i];
+          stv<DT_, S_>(x_c + i0, d);
+        }
       }
     });
   }
 
-  template<int G_, int B_, typename Rows_, typename DT_>
-  __global__ __launch_bounds__(256) void k_prolong(Rows_ P, u64 per, const u32* __restrict__ mask_f, const DT_* __restrict__ x_c,
+  // x_f += filter(P x_c)
+  template<int G_, int S_, int B_, typename Rows_, typename DT_>
+  F3_ROW_KERNEL void k_prolong(Rows_ P, u64 per, const u32* __restrict__ mask_f, const DT_* __restrict__ x_c,
                                                    DT_* __restrict__ x_f, const int* any)
   {
     if(idle(any)) return;
     const int K = P.K;
-    for_block_rows<G_, B_>(P, per, x_c, [&](u64 p, int k, const DT_ (&s)[B_]) {
+    for_block_rows<G_, S_, B_>(P, per, x_c, [&](u64 p, int k0, const DT_ (&s)[B_][S_]) {
       if(masked(mask_f, p)) return;
 #pragma unroll
       for(int h = 0; h < B_; ++h)
       {
-        const u64 i = (p * B_ + u64(h)) * u64(K) + u64(k);
-        x_f[i] = x_f[i] + s[h];
+        const u64 i0 = (p * B_ + u64(h)) * u64(K) + u64(k0);
+        DT_ xf[S_];
+        ldv<DT_, S_>(x_f + i0, xf);
+#pragma unroll
+        for(int i = 0; i < S_; ++i) xf[i] = xf[i] + s[h][i];
+        stv<DT_, S_>(x_f + i0, xf);
       }
     });
   }
@@ -229,17 +387,22 @@ namespace batch
     u64 min_iter, max_iter, min_stag_iter, hist_cap;
   };
 
-  // per-system partial sums of a block: partials[block][k]
-  template<int G_>
-  __device__ __forceinline__ void block_partials_k(double v, double* partials, int K)
+  // per-system partial sums of a block, partials[block][k]: lane groups of G lanes with S
+  // systems per lane (S = 1 for the streaming kernels); K <= G*S, at most 256*S values
+  template<int G_, int S_>
+  __device__ __forceinline__ void block_partials(const double (&v)[S_], double* partials, int K)
   {
-    __shared__ double red[256];
-    red[threadIdx.x] = v;
+    __shared__ double red[256 * S_];
+    const int k0 = int(threadIdx.x % G_) * S_;
+    const unsigned g = threadIdx.x / G_;
+    if(k0 < K)
+#pragma unroll
+      for(int i = 0; i < S_; ++i) red[g * unsigned(K) + unsigned(k0 + i)] = v[i];
     __syncthreads();
     if(int(threadIdx.x) < K)
     {
       double s = 0.0;
-      for(unsigned g = 0; g < blockDim.x / G_; ++g) s += red[g * G_ + threadIdx.x];
+      for(unsigned gg = 0; gg < blockDim.x / G_; ++gg) s += red[gg * unsigned(K) + threadIdx.x];
       partials[u64(blockIdx.x) * u64(K) + threadIdx.x] = s;
     }
   }
@@ -247,24 +410,62 @@ namespace batch
   __device__ __forceinline__ bool sys_active(const SysState* st, int k) { return st[k].status == mg::kProgress; }
 
   // r = filter(b - A x); partial sums of r.r per system
-  template<int G_, int B_, typename Rows_>
-  __global__ __launch_bounds__(256) void k_pcg_defect(Rows_ A, u64 per, const u32* __restrict__ mask, const double* __restrict__ b,
+  template<int G_, int S_, int B_, typename Rows_>
+  F3_ROW_KERNEL void k_pcg_defect(Rows_ A, u64 per, const u32* __restrict__ mask, const double* __restrict__ b,
                                                       const double* __restrict__ x, double* __restrict__ r, double* partials)
   {
     const int K = A.K;
-    double v = 0.0;
-    for_block_rows<G_, B_>(A, per, x, [&](u64 p, int k, const double (&s)[B_]) {
+    double v[S_];
+#pragma unroll
+    for(int i = 0; i < S_; ++i) v[i] = 0.0;
+    for_block_rows<G_, S_, B_>(A, per, x, [&](u64 p, int k0, const double (&s)[B_][S_]) {
       const bool m = masked(mask, p);
 #pragma unroll
       for(int h = 0; h < B_; ++h)
       {
-        const u64 i = (p * B_ + u64(h)) * u64(K) + u64(k);
-        const double d = m ? 0.0 : b[i] - s[h];
-        r[i] = d;
-        v += d * d;
+        const u64 i0 = (p * B_ + u64(h)) * u64(K) + u64(k0);
+        double bi[S_], d[S_];
+        ldv<double, S_>(b + i0, bi);
+#pragma unroll
+        for(int i = 0; i < S_; ++i)
+        {
+          d[i] = m ? 0.0 : bi[i] - s[h][i];
+          v[i] += d[i] * d[i];
+        }
+        stv<double, S_>(r + i0, d);
       }
     });
-    block_partials_k<G_>(v, partials, K);
+    block_partials<G_, S_>(v, partials, K);
+  }
+
+  // q = filter(A p); partial sums of q.p for the systems in progress
+  template<int G_, int S_, int B_, typename Rows_>
+  F3_ROW_KERNEL void k_pcg_spmv_dot(Rows_ A, u64 per, const u32* __restrict__ mask, const double* __restrict__ p,
+                                                        double* __restrict__ q, const SysState* st, const int* any, double* partials)
+  {
+    if(idle(any)) return;
+    const int K = A.K;
+    double v[S_];
+#pragma unroll
+    for(int i = 0; i < S_; ++i) v[i] = 0.0;
+    for_block_rows<G_, S_, B_>(A, per, p, [&](u64 row, int k0, const double (&s)[B_][S_]) {
+      const bool m = masked(mask, row);
+#pragma unroll
+      for(int h = 0; h < B_; ++h)
+      {
+        const u64 i0 = (row * B_ + u64(h)) * u64(K) + u64(k0);
+        double pi[S_], qi[S_];
+        ldv<double, S_>(p + i0, pi);
+#pragma unroll
+        for(int i = 0; i < S_; ++i)
+        {
+          qi[i] = m ? 0.0 : s[h][i];
+          if(sys_active(st, k0 + i)) v[i] += qi[i] * pi[i];
+        }
+        stv<double, S_>(q + i0, qi);
+      }
+    });
+    block_partials<G_, S_>(v, partials, K);
   }
 
   // p = z; partial sums of r.p
@@ -272,42 +473,19 @@ namespace batch
   __global__ __launch_bounds__(256) void k_pcg_p_gamma(u64 nodes, int K, const DT_* __restrict__ z, const double* __restrict__ r,
                                                        double* __restrict__ p, const SysState* st, double* partials)
   {
-    double v = 0.0;
+    double v[1] = {0.0};
     const int k = int(threadIdx.x % G_);
     const u64 ngrp = blockDim.x / G_;
     const u64 m = nodes * B_;
     if(k < K && sys_active(st, k))
       for(u64 i = u64(blockIdx.x) * ngrp + threadIdx.x / G_; i < m; i += u64(gridDim.x) * ngrp)
       {
-        const u64 q = i * u64(K) + u64(k);
-        const double zi = double(z[q]);
-        p[q] = zi;
-        v += r[q] * zi;
+        const u64 t = i * u64(K) + u64(k);
+        const double zi = double(z[t]);
+        p[t] = zi;
+        v[0] += r[t] * zi;
       }
-    block_partials_k<G_>(v, partials, K);
-  }
-
-  // q = filter(A p); partial sums of q.p
-  template<int G_, int B_, typename Rows_>
-  __global__ __launch_bounds__(256) void k_pcg_spmv_dot(Rows_ A, u64 per, const u32* __restrict__ mask, const double* __restrict__ p,
-                                                        double* __restrict__ q, const SysState* st, const int* any, double* partials)
-  {
-    if(idle(any)) return;
-    const int K = A.K;
-    double v = 0.0;
-    for_block_rows<G_, B_>(A, per, p, [&](u64 row, int k, const double (&s)[B_]) {
-      if(!sys_active(st, k)) return;
-      const bool m = masked(mask, row);
-#pragma unroll
-      for(int h = 0; h < B_; ++h)
-      {
-        const u64 i = (row * B_ + u64(h)) * u64(K) + u64(k);
-        const double qi = m ? 0.0 : s[h];
-        q[i] = qi;
-        v += qi * p[i];
-      }
-    });
-    block_partials_k<G_>(v, partials, K);
+    block_partials<G_, 1>(v, partials, K);
   }
 
   // x += alpha p, r -= alpha q (active systems); partial sums of r.r
@@ -317,7 +495,7 @@ namespace batch
                                                       const int* any, double* partials)
   {
     if(idle(any)) return;
-    double v = 0.0;
+    double v[1] = {0.0};
     const int k = int(threadIdx.x % G_);
     const u64 ngrp = blockDim.x / G_;
     const u64 m = nodes * B_;
@@ -335,10 +513,10 @@ namespace batch
           ri = ri + (-alpha) * q[t];
           r[t] = ri;
         }
-        v += ri * ri;
+        v[0] += ri * ri;
       }
     }
-    block_partials_k<G_>(v, partials, K);
+    block_partials<G_, 1>(v, partials, K);
   }
 
   // partial sums of r.z
@@ -347,7 +525,7 @@ namespace batch
                                                   const SysState* st, const int* any, double* partials)
   {
     if(idle(any)) return;
-    double v = 0.0;
+    double v[1] = {0.0};
     const int k = int(threadIdx.x % G_);
     const u64 ngrp = blockDim.x / G_;
     const u64 m = nodes * B_;
@@ -355,9 +533,9 @@ namespace batch
       for(u64 i = u64(blockIdx.x) * ngrp + threadIdx.x / G_; i < m; i += u64(gridDim.x) * ngrp)
       {
         const u64 t = i * u64(K) + u64(k);
-        v += r[t] * double(z[t]);
+        v[0] += r[t] * double(z[t]);
       }
-    block_partials_k<G_>(v, partials, K);
+    block_partials<G_, 1>(v, partials, K);
   }
 
   // p = beta p + z (active systems)

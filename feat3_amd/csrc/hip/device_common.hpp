@@ -22,6 +22,18 @@ namespace dev
   template<>
   __device__ __forceinline__ double ld<double, half>(const half* p, std::uint64_t i) { return double(float(p[i])); }
 
+  // read-only views in the constant address space: a load through one whose address is
+  // wave-uniform is issued as a scalar (SMEM) load through the scalar cache, so one load serves
+  // the whole wave instead of 64 lanes requesting the same address
+  template<typename T>
+  using cptr = const __attribute__((address_space(4))) T*;
+  template<typename T>
+  __device__ __forceinline__ cptr<T> to_const(const T* p) { return (cptr<T>)(p); }
+  template<typename DT_, typename VT_>
+  __device__ __forceinline__ DT_ ldc(cptr<VT_> p, std::uint64_t i) { return DT_(p[i]); }
+  template<>
+  __device__ __forceinline__ float ldc<float, half>(cptr<half> p, std::uint64_t i) { return float(p[i]); }
+
   // nontemporal streaming loads for matrix arrays that are read exactly once per sweep
   template<typename T>
   __device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }

@@ -21,6 +21,8 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace feat3_amd;
@@ -76,6 +78,7 @@ namespace
     double tol_rel = 1e-8, tol_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16), tol_abs_low = 0.0;
     u64 min_iter = 0, max_iter = 1000;
     bool use_graph = true;
+    u64 tile_rows = 512;   // rows per locality tile of the row kernels (0/1: ascending order)
   };
 
   // drop stored exact zeros of a scalar graph (the transfers' structural zeros)
@@ -133,19 +136,63 @@ namespace
     virtual bool set_tolerances(const Config& c) = 0;
   };
 
-  template<typename DT, typename VT, int G, int B, bool SHARED>
+  template<int N>
+  using ic = std::integral_constant<int, N>;
+
+  // Lane layouts.  Streaming kernels: one system per lane, G = 16/32/64 lanes per node
+  // component.  Row kernels over vectors of type T: S = 16 / sizeof(T) systems per lane (one
+  // 16-byte access) and GV = K / S lanes per row when K is a multiple of S and K / S a power of
+  // two; otherwise S = 1 and GV = G.
+  inline int stream_lanes(int K) { return K <= 16 ? 16 : (K <= 32 ? 32 : 64); }
+
+  template<typename T>
+  void row_layout(int K, int& GV, int& S)
+  {
+    const int sv = int(16 / sizeof(T));
+    const int q = K / sv;
+    if(K % sv == 0 && q >= 1 && (q & (q - 1)) == 0) { GV = q; S = sv; }
+    else { GV = stream_lanes(K); S = 1; }
+  }
+
+  template<typename T, typename F>
+  int with_layout(int GV, int S, F&& f)
+  {
+    constexpr int SV = int(16 / sizeof(T));
+    bool done = false;
+    auto one = [&](auto gv, auto sv) {
+      if(!done && GV == decltype(gv)::value && S == decltype(sv)::value) { f(gv, sv); done = true; }
+    };
+    one(ic<16>(), ic<1>()); one(ic<32>(), ic<1>()); one(ic<64>(), ic<1>());
+    one(ic<1>(), ic<SV>()); one(ic<2>(), ic<SV>()); one(ic<4>(), ic<SV>()); one(ic<8>(), ic<SV>()); one(ic<16>(), ic<SV>());
+    if constexpr(SV == 2) one(ic<32>(), ic<SV>());
+    return done ? rt::kOk : rt::kErrArg;
+  }
+
+  template<typename F>
+  int with_stream(int G, F&& f)
+  {
+    if(G == 16) f(ic<16>());
+    else if(G == 32) f(ic<32>());
+    else if(G == 64) f(ic<64>());
+    else return rt::kErrArg;
+    return rt::kOk;
+  }
+
+  template<typename DT, typename VT, int B, bool SHARED>
   struct Engine : public BatchBase
   {
     struct Level
     {
       u64 nodes = 0, nnz = 0;
-      DevMem arp, acol, aval, prp, pcol, pval, rrp, rcol, rval, mask, dinv, rhs, x, t, def;
+      DevMem arp, acol, aval, prp, pcol, pval, rrp, rcol, rval, mask, dinv, rhs, x, t, def, ord;
       u64 nc = 0;
       DT* xfinal = nullptr;
+      const u32* order() const { return ord.p ? ord.template as<u32>() : nullptr; }
     };
     Config cfg;
     hipStream_t s = nullptr;
     int K = 1;
+    int G = 16, GV = 16, S = 1, GO = 16, SO = 1;   // streaming, V-cycle rows, outer FP64 rows
     std::vector<std::unique_ptr<Level>> L;
     DevMem ainv, a64, r, p, q, st, cr, hist, partials, any;
     u64 m0 = 0, hist_cap = 0, ncoarse_rows = 0;
@@ -155,8 +202,14 @@ namespace
     hipGraphExec_t gexec = nullptr;
     double* x_cur = nullptr;
     double* x_graph = nullptr;
+    std::vector<u64> last_iters;
 
-    explicit Engine(int k) : K(k) {}
+    explicit Engine(int k) : K(k)
+    {
+      G = stream_lanes(K);
+      row_layout<DT>(K, GV, S);
+      row_layout<double>(K, GO, SO);
+    }
     ~Engine() override
     {
       if(gexec) (void)hipGraphExecDestroy(gexec);
@@ -177,21 +230,23 @@ namespace
       return o;
     }
 
-    batch::BlockRows<DT, VT, B, SHARED> rowsA(const Level& l) const
+    template<int S_> batch::BlockRows<DT, VT, B, SHARED, S_> rowsA(const Level& l) const
     {
-      return {l.arp.template as<u32>(), l.acol.template as<u32>(), l.aval.template as<VT>(), l.nodes, K};
+      return {l.arp.template as<u32>(), l.acol.template as<u32>(), l.aval.template as<VT>(), l.nodes, K, l.order()};
     }
-    batch::CompRows<DT, VT, B> rowsP(const Level& l) const
+    // P: rows are the fine nodes of level l; R: rows are the coarse nodes (level c)
+    template<int S_> batch::CompRows<DT, VT, B, S_> rowsP(const Level& l) const
     {
-      return {l.prp.template as<u32>(), l.pcol.template as<u32>(), l.pval.template as<VT>(), l.nodes, K};
+      return {l.prp.template as<u32>(), l.pcol.template as<u32>(), l.pval.template as<VT>(), l.nodes, K, l.order()};
     }
-    batch::CompRows<DT, VT, B> rowsR(const Level& l) const
+    template<int S_> batch::CompRows<DT, VT, B, S_> rowsR(const Level& l, const Level& c) const
     {
-      return {l.rrp.template as<u32>(), l.rcol.template as<u32>(), l.rval.template as<VT>(), l.nc, K};
+      return {l.rrp.template as<u32>(), l.rcol.template as<u32>(), l.rval.template as<VT>(), l.nc, K, c.order()};
     }
-    batch::BlockRows<double, double, B, SHARED> rows64() const
+    template<int S_> batch::BlockRows<double, double, B, SHARED, S_> rows64() const
     {
-      return {L[0]->arp.template as<u32>(), L[0]->acol.template as<u32>(), a64.template as<double>(), L[0]->nodes, K};
+      return {L[0]->arp.template as<u32>(), L[0]->acol.template as<u32>(), a64.template as<double>(), L[0]->nodes, K,
+              L[0]->order()};
     }
 
     int init(const std::vector<HostLevel>& lv, const Config& c, hipStream_t stream) override
@@ -219,10 +274,15 @@ namespace
         std::vector<u32> m((h.nodes + 31) / 32 + 1, 0u);
         for(u32 f : h.fidx) m[f >> 5] |= 1u << (f & 31u);
         F3_RC(l->mask.put(m, s));
+        if(cfg.tile_rows > 1 && h.nodes > cfg.tile_rows)
+        {
+          std::vector<u32> ord(h.nodes);
+          if(feat3_setup_tile_order(h.nodes, h.A.rp.data(), h.A.col.data(), cfg.tile_rows, ord.data())) return rt::kErrArg;
+          F3_RC(l->ord.put(ord, s));
+        }
         // Jacobi: dinv = omega_R * omega_J / diag (diagonal of the diagonal block), 0 at Dirichlet
-        const u64 nd = h.nodes * B * (SHARED ? 1 : u64(K));
-        std::vector<DT> dinv(nd, DT(0));
         const u64 kv = SHARED ? 1 : u64(K);
+        std::vector<DT> dinv(h.nodes * B * kv, DT(0));
         for(u64 node = 0; node < h.nodes; ++node)
         {
           const bool dir = (m[node >> 5] >> (node & 31u)) & 1u;
@@ -305,10 +365,42 @@ namespace
 
     int sweep(Level& f, DT* xin, DT* xout, const int* a)
     {
-      const mg::RowGrid g = mg::row_grid(f.nodes, G);
-      batch::k_sweep<G, B, SHARED><<<g.blocks, kBlock, 0, s>>>(rowsA(f), g.per, f.dinv.template as<DT>(), f.rhs.template as<DT>(),
-                                                               (const DT*)xin, xout, a);
-      return rt::kOk;
+      return with_layout<DT>(GV, S, [&](auto gv, auto sv) {
+        constexpr int Gc = decltype(gv)::value, Sc = decltype(sv)::value;
+        const mg::RowGrid g = mg::row_grid(f.nodes, Gc);
+        batch::k_sweep<Gc, Sc, B, SHARED><<<g.blocks, kBlock, 0, s>>>(rowsA<Sc>(f), g.per, f.dinv.template as<DT>(),
+                                                                      f.rhs.template as<DT>(), (const DT*)xin, xout, a);
+      });
+    }
+
+    int defect(Level& f, const DT* x, const int* a)
+    {
+      return with_layout<DT>(GV, S, [&](auto gv, auto sv) {
+        constexpr int Gc = decltype(gv)::value, Sc = decltype(sv)::value;
+        const mg::RowGrid g = mg::row_grid(f.nodes, Gc);
+        batch::k_defect<Gc, Sc, B><<<g.blocks, kBlock, 0, s>>>(rowsA<Sc>(f), g.per, f.mask.template as<u32>(),
+                                                              f.rhs.template as<DT>(), x, f.def.template as<DT>(), a);
+      });
+    }
+
+    int restrict_to(Level& f, Level& c, DT* xc, const int* a)
+    {
+      return with_layout<DT>(GV, S, [&](auto gv, auto sv) {
+        constexpr int Gc = decltype(gv)::value, Sc = decltype(sv)::value;
+        const mg::RowGrid g = mg::row_grid(f.nc, Gc);
+        batch::k_restrict<Gc, Sc, B, SHARED><<<g.blocks, kBlock, 0, s>>>(rowsR<Sc>(f, c), g.per, c.mask.template as<u32>(),
+                                                                         f.def.template as<DT>(), c.rhs.template as<DT>(), xc,
+                                                                         c.dinv.template as<DT>(), a);
+      });
+    }
+
+    int prolong(Level& f, const DT* xc, DT* xf, const int* a)
+    {
+      return with_layout<DT>(GV, S, [&](auto gv, auto sv) {
+        constexpr int Gc = decltype(gv)::value, Sc = decltype(sv)::value;
+        const mg::RowGrid g = mg::row_grid(f.nodes, Gc);
+        batch::k_prolong<Gc, Sc, B><<<g.blocks, kBlock, 0, s>>>(rowsP<Sc>(f), g.per, f.mask.template as<u32>(), xc, xf, a);
+      });
     }
 
     int enqueue_vcycle(const double* def, const int* a)
@@ -325,14 +417,8 @@ namespace
         cur[i] = f.x.template as<DT>();
         oth[i] = f.t.template as<DT>();
         for(int k = 1; k < cfg.pre; ++k) { F3_RC(sweep(f, cur[i], oth[i], a)); std::swap(cur[i], oth[i]); }
-        const mg::RowGrid g = mg::row_grid(f.nodes, G);
-        batch::k_defect<G, B><<<g.blocks, kBlock, 0, s>>>(rowsA(f), g.per, f.mask.template as<u32>(), f.rhs.template as<DT>(),
-                                                          (const DT*)cur[i], f.def.template as<DT>(), a);
-        const mg::RowGrid gr = mg::row_grid(f.nc, G);
-        DT* xc = (i + 2 < nl) ? c.x.template as<DT>() : nullptr;
-        batch::k_restrict<G, B, SHARED><<<gr.blocks, kBlock, 0, s>>>(rowsR(f), gr.per, c.mask.template as<u32>(),
-                                                                     f.def.template as<DT>(), c.rhs.template as<DT>(), xc,
-                                                                     c.dinv.template as<DT>(), a);
+        F3_RC(defect(f, cur[i], a));
+        F3_RC(restrict_to(f, c, (i + 2 < nl) ? c.x.template as<DT>() : nullptr, a));
       }
       Level& cl = *L.back();
       const unsigned cb = unsigned(std::max<u64>(1, (ncoarse_rows * u64(K) + kBlock - 1) / kBlock));
@@ -343,8 +429,7 @@ namespace
         const std::size_t i = ii - 1;
         Level& f = *L[i];
         Level& c = *L[i + 1];
-        const mg::RowGrid g = mg::row_grid(f.nodes, G);
-        batch::k_prolong<G, B><<<g.blocks, kBlock, 0, s>>>(rowsP(f), g.per, f.mask.template as<u32>(), (const DT*)c.xfinal, cur[i], a);
+        F3_RC(prolong(f, c.xfinal, cur[i], a));
         for(int k = 0; k < cfg.post; ++k) { F3_RC(sweep(f, cur[i], oth[i], a)); std::swap(cur[i], oth[i]); }
         if(cur[i] != f.xfinal) return rt::kErrState;
       }
@@ -358,29 +443,64 @@ namespace
                                              cr.template as<batch::Criteria>(), hist.template as<double>(), any.template as<int>());
     }
 
+    // outer q = filter(A p) with partials of q.p; returns the number of blocks (the partial count)
+    int outer_spmv_dot(unsigned& nb)
+    {
+      return with_layout<double>(GO, SO, [&](auto gv, auto sv) {
+        constexpr int Gc = decltype(gv)::value, Sc = decltype(sv)::value;
+        const mg::RowGrid g = mg::row_grid(L[0]->nodes, Gc);
+        nb = g.blocks;
+        batch::k_pcg_spmv_dot<Gc, Sc, B><<<g.blocks, kBlock, 0, s>>>(rows64<Sc>(), g.per, L[0]->mask.template as<u32>(),
+                                                                    p.template as<double>(), q.template as<double>(),
+                                                                    st.template as<batch::SysState>(), anyp(),
+                                                                    partials.template as<double>());
+      });
+    }
+
+    int outer_defect(const double* b, const double* x, unsigned& nb)
+    {
+      return with_layout<double>(GO, SO, [&](auto gv, auto sv) {
+        constexpr int Gc = decltype(gv)::value, Sc = decltype(sv)::value;
+        const mg::RowGrid g = mg::row_grid(L[0]->nodes, Gc);
+        nb = g.blocks;
+        batch::k_pcg_defect<Gc, Sc, B><<<g.blocks, kBlock, 0, s>>>(rows64<Sc>(), g.per, L[0]->mask.template as<u32>(), b, x,
+                                                                  r.template as<double>(), partials.template as<double>());
+      });
+    }
+
     int enqueue_iteration()
     {
       double* part = partials.template as<double>();
-      batch::SysState* S = st.template as<batch::SysState>();
-      const mg::RowGrid g = mg::row_grid(L[0]->nodes, G);
-      batch::k_pcg_spmv_dot<G, B><<<g.blocks, kBlock, 0, s>>>(rows64(), g.per, L[0]->mask.template as<u32>(), p.template as<double>(),
-                                                              q.template as<double>(), S, anyp(), part);
-      fin<mg::kFinAlpha>(g.blocks);
+      batch::SysState* Sst = st.template as<batch::SysState>();
+      unsigned nb = 0;
+      F3_RC(outer_spmv_dot(nb));
+      fin<mg::kFinAlpha>(nb);
       const unsigned sb = stream_blocks(m0, G);
-      batch::k_pcg_update<G, B><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, p.template as<double>(), q.template as<double>(), x_cur,
-                                                      r.template as<double>(), S, anyp(), part);
+      const DT* z = L[0]->xfinal;
+      F3_RC(with_stream(G, [&](auto gg) {
+        constexpr int Gc = decltype(gg)::value;
+        batch::k_pcg_update<Gc, B><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, p.template as<double>(), q.template as<double>(), x_cur,
+                                                         r.template as<double>(), Sst, anyp(), part);
+      }));
       fin<mg::kFinNewDefect>(sb);
       F3_RC(enqueue_vcycle(r.template as<double>(), anyp()));
-      const DT* z = L[0]->xfinal;
-      batch::k_pcg_rz<G, B, DT><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, z, r.template as<double>(), S, anyp(), part);
+      F3_RC(with_stream(G, [&](auto gg) {
+        constexpr int Gc = decltype(gg)::value;
+        batch::k_pcg_rz<Gc, B, DT><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, z, r.template as<double>(), Sst, anyp(), part);
+      }));
       fin<mg::kFinBeta>(sb);
-      batch::k_pcg_update_p<G, B, DT><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, z, p.template as<double>(), S, anyp());
+      F3_RC(with_stream(G, [&](auto gg) {
+        constexpr int Gc = decltype(gg)::value;
+        batch::k_pcg_update_p<Gc, B, DT><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, z, p.template as<double>(), Sst, anyp());
+      }));
       return rt::check_launch();
     }
 
     int solve(double* x, const double* b, int* status, u64* iters) override
     {
       if(!x || !b) return rt::kErrArg;
+      // the row kernels access S consecutive systems as one 16-byte vector
+      if((reinterpret_cast<std::uintptr_t>(x) | reinterpret_cast<std::uintptr_t>(b)) % 16 != 0) return rt::kErrArg;
       batch::Criteria c{};
       c.tol_rel = cfg.tol_rel; c.tol_abs = cfg.tol_abs; c.tol_abs_low = cfg.tol_abs_low;
       c.div_rel = 1.0 / 2.220446049250313e-16; c.div_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16);
@@ -388,15 +508,17 @@ namespace
       F3_CHECK(hipMemcpyAsync(cr.p, &c, sizeof(c), hipMemcpyHostToDevice, s));
       x_cur = x;
       double* part = partials.template as<double>();
-      batch::SysState* S = st.template as<batch::SysState>();
-      const mg::RowGrid g = mg::row_grid(L[0]->nodes, G);
-      batch::k_pcg_defect<G, B><<<g.blocks, kBlock, 0, s>>>(rows64(), g.per, L[0]->mask.template as<u32>(), b, (const double*)x,
-                                                            r.template as<double>(), part);
-      fin<mg::kFinInitDefect>(g.blocks);
+      batch::SysState* Sst = st.template as<batch::SysState>();
+      unsigned nb = 0;
+      F3_RC(outer_defect(b, x, nb));
+      fin<mg::kFinInitDefect>(nb);
       F3_RC(enqueue_vcycle(r.template as<double>(), anyp()));
       const unsigned sb = stream_blocks(m0, G);
-      batch::k_pcg_p_gamma<G, B, DT><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, L[0]->xfinal, r.template as<double>(),
-                                                           p.template as<double>(), S, part);
+      F3_RC(with_stream(G, [&](auto gg) {
+        constexpr int Gc = decltype(gg)::value;
+        batch::k_pcg_p_gamma<Gc, B, DT><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, L[0]->xfinal, r.template as<double>(),
+                                                              p.template as<double>(), Sst, part);
+      }));
       fin<mg::kFinGamma>(sb);
       F3_RC(rt::check_launch());
       if(cfg.use_graph && (gexec == nullptr || x_graph != x))
@@ -430,16 +552,15 @@ namespace
       F3_CHECK(hipStreamSynchronize(s));
       std::vector<batch::SysState> out(K);
       F3_CHECK(hipMemcpy(out.data(), st.p, sizeof(batch::SysState) * K, hipMemcpyDeviceToHost));
+      last_iters.assign(K, 0);
       for(int k = 0; k < K; ++k)
       {
         if(status) status[k] = out[k].status;
         if(iters) iters[k] = out[k].num_iter;
+        last_iters[k] = out[k].num_iter;
       }
-      last_iters.assign(K, 0);
-      for(int k = 0; k < K; ++k) last_iters[k] = out[k].num_iter;
       return rt::kOk;
     }
-    std::vector<u64> last_iters;
 
     int history(int k, double* out, u64 cap, u64* len) override
     {
@@ -451,20 +572,31 @@ namespace
       return rt::kOk;
     }
 
-    // which: 0 finest smoothing sweep (all systems), 3 one V-cycle
+    // which: 0 finest smoothing sweep (all systems), 2 outer FP64 q = A p with dot, 3 one V-cycle
     int time_kernel(int which, int reps, double* ms, u64* bytes) override
     {
-      if(reps < 1) return rt::kErrArg;
+      if(reps < 1 || (which != 0 && which != 2 && which != 3)) return rt::kErrArg;
       Level& f = *L[0];
+      if(which == 2)
+      {
+        // a state where every system is in progress, so that the kernel does its full work
+        std::vector<batch::SysState> sp(K);
+        for(auto& v : sp) { v = batch::SysState{}; v.status = mg::kProgress; }
+        const int one = 1;
+        F3_CHECK(hipStreamSynchronize(s));
+        F3_CHECK(hipMemcpy(st.p, sp.data(), sizeof(batch::SysState) * K, hipMemcpyHostToDevice));
+        F3_CHECK(hipMemcpy(any.p, &one, sizeof(int), hipMemcpyHostToDevice));
+      }
       for(int pass = 0; pass < 2; ++pass)
       {
         const int n = pass == 0 ? 1 : reps;
         if(pass == 1) F3_CHECK(hipEventRecord(t0, s));
         for(int k = 0; k < n; ++k)
         {
+          unsigned nb = 0;
           if(which == 0) F3_RC(sweep(f, f.x.template as<DT>(), f.t.template as<DT>(), nullptr));
-          else if(which == 3) F3_RC(enqueue_vcycle(r.template as<double>(), nullptr));
-          else return rt::kErrArg;
+          else if(which == 2) F3_RC(outer_spmv_dot(nb));
+          else F3_RC(enqueue_vcycle(r.template as<double>(), nullptr));
         }
         if(pass == 1) F3_CHECK(hipEventRecord(t1, s));
         F3_CHECK(hipStreamSynchronize(s));
@@ -474,10 +606,14 @@ namespace
       *ms = double(e) / reps;
       if(bytes)
       {
-        // algorithmic: values + column indices + row pointers, vectors x (once), rhs, x_out, dinv
-        const u64 vb = sizeof(DT);
-        *bytes = f.nnz * (vals_per_entry() * sizeof(VT) + 4) + (f.nodes + 1) * 4 + f.nodes * B * u64(K) * vb * 3 +
-                 f.nodes * B * (SHARED ? 1 : u64(K)) * vb;
+        // algorithmic: values + column indices + row pointers, x once, and the row-wise vectors
+        // (sweep: rhs, x_out, dinv; outer: p read for the dot is the x read, q written)
+        const u64 nvk = f.nodes * B * u64(K);
+        if(which == 2)
+          *bytes = f.nnz * (vals_per_entry() * 8 + 4) + (f.nodes + 1) * 4 + nvk * 8 * 2;
+        else
+          *bytes = f.nnz * (vals_per_entry() * sizeof(VT) + 4) + (f.nodes + 1) * 4 + nvk * sizeof(DT) * 3 +
+                   f.nodes * B * (SHARED ? 1 : u64(K)) * sizeof(DT);
       }
       return rt::kOk;
     }
@@ -486,12 +622,9 @@ namespace
   template<typename DT, typename VT>
   BatchBase* make_engine(int block, int K, bool shared)
   {
-    const int G = K <= 16 ? 16 : (K <= 32 ? 32 : 64);
-#define F3_MK(GG, BB, SS) if(G == GG && block == BB && shared == SS) return new Engine<DT, VT, GG, BB, SS>(K);
-    F3_MK(16, 1, false) F3_MK(32, 1, false) F3_MK(64, 1, false)
-    F3_MK(16, 1, true) F3_MK(32, 1, true) F3_MK(64, 1, true)
-    F3_MK(16, 3, true) F3_MK(32, 3, true) F3_MK(64, 3, true)
-#undef F3_MK
+    if(block == 1 && !shared) return new Engine<DT, VT, 1, false>(K);
+    if(block == 1 && shared) return new Engine<DT, VT, 1, true>(K);
+    if(block == 3 && shared) return new Engine<DT, VT, 3, true>(K);
     return nullptr;
   }
 } // namespace
@@ -589,6 +722,16 @@ int feat3_batch_set_graph(feat3_batch* h, int graph)
 {
   if(!h) return rt::kErrArg;
   h->cfg.use_graph = graph != 0;
+  h->eng.reset();
+  return rt::kOk;
+}
+
+int feat3_batch_set_option(feat3_batch* h, const char* name, double value)
+{
+  if(!h || !name) return rt::kErrArg;
+  const std::string n(name);
+  if(n == "tile_rows" && value >= 0) h->cfg.tile_rows = u64(value);
+  else return rt::kErrArg;
   h->eng.reset();
   return rt::kOk;
 }

@@ -220,26 +220,36 @@ def random_rhs(nodes: int, block: int, nsys: int, bnd: np.ndarray, seed: int) ->
     return np.ascontiguousarray(b)
 
 
+_GRF_SPECTRA = {}
+
+
+def _grf_spectrum(dim: int, level: int, sigma: float, length: float) -> np.ndarray:
+    key = (dim, level, sigma, length)
+    if key not in _GRF_SPECTRA:
+        m = 1 << level
+        h = 1.0 / m
+        M = 2 * m
+        d = np.minimum(np.arange(M), M - np.arange(M)) * h
+        r2 = np.zeros((M,) * dim)
+        for ax in range(dim):
+            shape = [1] * dim
+            shape[ax] = M
+            r2 = r2 + (d ** 2).reshape(shape)
+        lam = np.fft.fftn(sigma ** 2 * np.exp(-r2 / (2.0 * length ** 2))).real
+        _GRF_SPECTRA[key] = np.sqrt(np.maximum(lam, 0.0) / lam.size)
+    return _GRF_SPECTRA[key]
+
+
 def gaussian_random_field(dim: int, level: int, seed: int, sigma: float = 1.0, length: float = 0.1) -> np.ndarray:
     """A stationary Gaussian field with covariance sigma^2 exp(-|x-y|^2 / (2 length^2)) on the cell
     centres of the level's mesh (x fastest), sampled by circulant embedding: the covariance of
     the periodic grid of twice the extent is diagonalised by the FFT; the few negative
     eigenvalues of the embedding (of order 1e-16 for length 0.1) are set to zero."""
     m = 1 << level
-    h = 1.0 / m
     M = 2 * m
-    d = np.minimum(np.arange(M), M - np.arange(M)) * h
-    r2 = np.zeros((M,) * dim)
-    for ax in range(dim):
-        shape = [1] * dim
-        shape[ax] = M
-        r2 = r2 + (d ** 2).reshape(shape)
-    c = sigma ** 2 * np.exp(-r2 / (2.0 * length ** 2))
-    lam = np.fft.fftn(c).real
-    lam = np.maximum(lam, 0.0)
     rng = np.random.default_rng(seed)
     z = rng.standard_normal((M,) * dim) + 1j * rng.standard_normal((M,) * dim)
-    f = np.fft.fftn(np.sqrt(lam / lam.size) * z).real
+    f = np.fft.fftn(_grf_spectrum(dim, level, sigma, length) * z).real
     g = f[tuple(slice(0, m) for _ in range(dim))]
     # array axes are (x, y[, z]); cells are numbered x fastest
     return np.ascontiguousarray(g.transpose(tuple(range(dim - 1, -1, -1))).reshape(-1))
@@ -247,5 +257,19 @@ def gaussian_random_field(dim: int, level: int, seed: int, sigma: float = 1.0, l
 
 def field_hierarchies(dim: int, level_fine: int, kfields: np.ndarray, level_coarse: int = 1,
                       degree: int = 1) -> List[Hierarchy]:
-    """One hierarchy per coefficient field (rows of kfields: per-cell values on the finest level)."""
-    return [hierarchy(dim, degree, level_fine, level_coarse, kcell=k) for k in np.atleast_2d(kfields)]
+    """One hierarchy per coefficient field (rows of kfields: per-cell values on the finest level).
+    The graph, Dirichlet lists and transfers do not depend on the field and are shared."""
+    kf = np.atleast_2d(kfields)
+    base = hierarchy(dim, degree, level_fine, level_coarse, kcell=kf[0])
+    out = [base]
+    for k in kf[1:]:
+        H = Hierarchy(dim, degree, [], level_fine, level_coarse)
+        kl = k
+        for i, lvl in enumerate(range(level_fine, level_coarse - 1, -1)):
+            A, _ = laplace(dim, degree, lvl, kl)
+            B = base.levels[i]
+            H.levels.append(Level(A, B.bnd, B.P, B.R, kcell=kl))
+            if lvl > level_coarse:
+                kl = coarsen_cellfield(dim, lvl, kl)
+        out.append(H)
+    return out

@@ -333,12 +333,17 @@ int feat3_batch_set_tolerances(feat3_batch* h, double tol_rel, double tol_abs, d
                                uint64_t max_iter);
 /* graph replay of one PCG iteration on/off */
 int feat3_batch_set_graph(feat3_batch* h, int graph);
+/* named options, applied at the next init: "tile_rows" (rows per locality tile of the row
+ * kernels, feat3_setup_tile_order; default 512, 0 = ascending row order) */
+int feat3_batch_set_option(feat3_batch* h, const char* name, double value);
 int feat3_batch_init(feat3_batch* h);
-/* x, b: FP64 device arrays [node][component][system]; status, iters: host arrays of nsys */
+/* x, b: FP64 device arrays [node][component][system], 16-byte aligned; status, iters: host
+ * arrays of nsys */
 int feat3_batch_correct(feat3_batch* h, void* caller_stream, double* x, const double* b, int* status, uint64_t* iters);
 /* defect norms of system `system` in the last solve */
 int feat3_batch_history(feat3_batch* h, int system, double* out_host, uint64_t cap, uint64_t* len);
-/* which: 0 finest smoothing sweep (all systems), 3 one V-cycle; algorithmic bytes of one launch */
+/* which: 0 finest smoothing sweep (all systems), 2 outer FP64 q = A p with q.p partials,
+ * 3 one V-cycle; algorithmic bytes of one launch (0, 2) */
 int feat3_batch_time_kernel(feat3_batch* h, int which, int reps, double* avg_ms, uint64_t* bytes);
 
 /* ===================================================================================== */
@@ -376,6 +381,11 @@ int feat3_sell_build(uint64_t rows, uint64_t cols, const uint32_t* row_ptr, cons
                      uint64_t C, uint64_t sigma, uint32_t* cs, uint32_t* cl, uint32_t* perm, uint32_t* col, double* sval,
                      int16_t* d16, int* used_d16);
 int feat3_setup_ref_stiffness(int dim, int degree, double* K);
+/* a locality order of the rows of a graph: compact tiles of about `tile` rows grown by
+ * breadth-first search from the lowest unvisited row, ascending inside a tile; tile <= 1 gives
+ * the identity (no reference counterpart: a processing order for the batched row kernels) */
+int feat3_setup_tile_order(uint64_t rows, const uint32_t* row_ptr, const uint32_t* col_ind, uint64_t tile,
+                           uint32_t* order);
 /* n doubles uniform in [a, b] from the xorshift64* stream of Random (util/random.hpp:62-110,
  * 259-280); seed 0 selects the reference's default seed */
 int feat3_setup_random_uniform(uint64_t seed, uint64_t n, double a, double b, double* out);

@@ -74,3 +74,24 @@ def test_stacked_values_requires_shared_graph():
     assert s.shape == (Hs[0].levels[0].A.nnz, 3)
     for j in range(3):
         assert np.array_equal(s[:, j], Hs[j].levels[0].A.val)
+
+
+@pytest.mark.parametrize("dim,lvl,tile", [(3, 4, 64), (2, 6, 100), (3, 3, 1), (2, 2, 1000)])
+def test_tile_order_is_compact_permutation(dim, lvl, tile):
+    from feat3_amd import capi
+    A = pb.hierarchy(dim, 1, lvl, lvl).levels[0].A
+    order = np.empty(A.rows, np.uint32)
+    capi.call("feat3_setup_tile_order", A.rows, pb._p(A.row_ptr), pb._p(A.col_ind), tile, pb._p(order))
+    assert np.array_equal(np.sort(order), np.arange(A.rows, dtype=np.uint32))
+    if tile <= 1:
+        assert np.array_equal(order, np.arange(A.rows, dtype=np.uint32))
+        return
+    # tiles are grown from a seed by breadth-first search: the first tile is the graph ball of
+    # node 0, a cube of side about tile^(1/dim) on the Q1 grid, where `tile` consecutive rows
+    # would span whole grid lines
+    m = (1 << lvl) + 1
+    first = np.sort(order[:tile])
+    coords = np.stack([(first // m ** d) % m for d in range(dim)], axis=1)
+    ext = coords.max(axis=0) - coords.min(axis=0) + 1
+    assert np.prod(ext) <= 2 * tile
+    assert np.all(ext <= int(np.ceil(tile ** (1.0 / dim))) + 1)

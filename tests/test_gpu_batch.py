@@ -58,7 +58,10 @@ def check_system(k, g, o, inner):
         assert rel(xg, xo) <= 1e-8
 
 
-@pytest.mark.parametrize("inner,nsys", [("f64", 1), ("f64", 3), ("f32", 5), ("f64", 32), ("f32", 40)])
+# K covers every lane layout: one system per lane (1, 3, 5, 40), S = 4 floats per lane with
+# K/4 lanes per row (f32: 4, 16, 64), S = 2 doubles per lane (f64: 16, 32, 64)
+@pytest.mark.parametrize("inner,nsys", [("f64", 1), ("f64", 3), ("f32", 4), ("f32", 5), ("f32", 16), ("f64", 16),
+                                        ("f64", 32), ("f32", 40), ("f32", 64), ("f64", 64)])
 def test_shared_scalar_rhs(torch, inner, nsys):
     print(f"case shared scalar 2D Q1 L5 inner {inner} K {nsys}", flush=True)
     H = pb.hierarchy(2, 1, 5, 1)
@@ -76,9 +79,8 @@ def test_shared_scalar_rhs(torch, inner, nsys):
             check_system(k, (sg[k], ig[k], X[:, 0, k], hg[k]), (ref[0], ref[1], o64[2], ref[3]), inner)
 
 
-@pytest.mark.parametrize("inner", ["f64", "f32"])
-def test_vector_laplacian_bcsr(torch, inner):
-    nsys = 8
+@pytest.mark.parametrize("inner,nsys", [("f64", 8), ("f32", 8), ("f32", 32), ("f64", 3)])
+def test_vector_laplacian_bcsr(torch, inner, nsys):
     print(f"case 3x3 BCSR vector Laplacian 3D Q1 L4 inner {inner} K {nsys}", flush=True)
     H = pb.hierarchy(3, 1, 4, 1)
     vals = [pb.block_identity_values(L.A, 3) for L in H.levels]
@@ -91,9 +93,8 @@ def test_vector_laplacian_bcsr(torch, inner):
         check_system(k, (sg[k], ig[k], X[:, :, k].reshape(-1), hg[k]), (ref[0], ref[1], o64[2], ref[3]), inner)
 
 
-@pytest.mark.parametrize("inner", ["f64", "f32"])
-def test_coefficient_fields(torch, inner):
-    nsys = 6
+@pytest.mark.parametrize("inner,nsys", [("f64", 6), ("f32", 6), ("f64", 16), ("f32", 16)])
+def test_coefficient_fields(torch, inner, nsys):
     lvl = 5
     print(f"case {nsys} coefficient fields 2D Q1 L{lvl} inner {inner}", flush=True)
     K = np.exp(np.stack([pb.gaussian_random_field(2, lvl, seed=100 + k) for k in range(nsys)]))
@@ -144,3 +145,16 @@ def test_max_iter_status(torch):
         check_system(k, (sg[k], ig[k], X[:, 0, k], hg[k]),
                      orc_solve(H, B[:, 0, k].copy(), max_iter=2, tol_rel=1e-14), "f64")
         assert sg[k] == 5 and ig[k] == 2
+
+
+def test_tile_order_does_not_change_results(torch):
+    """The locality order changes which rows run together, not a row's arithmetic: iteration
+    counts agree and solutions differ only by the reductions' summation order."""
+    print("case tile order 3D Q1 L4 vector K 8", flush=True)
+    H = pb.hierarchy(3, 1, 4, 1)
+    vals = [pb.block_identity_values(L.A, 3) for L in H.levels]
+    B = pb.random_rhs(H.n, 3, 8, H.levels[0].bnd, seed=21)
+    a = gpu_batch(torch, H, B, 8, values=vals, block=3, inner="f64", options={"tile_rows": 0})
+    b = gpu_batch(torch, H, B, 8, values=vals, block=3, inner="f64", options={"tile_rows": 64})
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert rel(b[2], a[2]) <= 1e-13
