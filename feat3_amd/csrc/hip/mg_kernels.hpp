@@ -114,6 +114,19 @@ namespace mg
     return g;
   }
 
+  // a fixed number of row positions per block (rounded up to a multiple of 64, and raised where
+  // the grid would exceed max_blocks, the size of the reduction partials)
+  inline RowGrid row_grid_fixed(u64 rows, u64 per, unsigned max_blocks = 8192)
+  {
+    per = (per < 64 ? 64 : (per + 63) / 64 * 64);
+    if((rows + per - 1) / per > max_blocks) per = ((rows + max_blocks - 1) / max_blocks + 63) / 64 * 64;
+    RowGrid g;
+    g.per = per;
+    g.blocks = unsigned((rows + per - 1) / per);
+    if(g.blocks == 0) g.blocks = 1;
+    return g;
+  }
+
   // calls f(position, row_sum, is_group_leader) for every row position of this block
   template<typename Rows_, typename DT_, typename F>
   __device__ __forceinline__ void for_rows(const Rows_& A, u64 per, const DT_* __restrict__ x, F&& f)

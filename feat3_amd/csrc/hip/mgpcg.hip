@@ -79,6 +79,7 @@ namespace
   {
     u64 rows = 0, cols = 0, nnz = 0;
     int lanes = 8;
+    u64 rows_per_block = 0;   // row positions per block of the row kernels (0: by row_grid)
     DevMem rp, col, val;
     bool sell = false, sell_perm = false, sell_d16 = false;
     u64 C = 64, sigma = 1, sell_size = 0;
@@ -190,9 +191,10 @@ namespace
   template<typename DT, typename VT, typename F>
   int with_rows(const DevCsr<VT>& m, F&& f)
   {
+    auto grid = [&](int G) { return m.rows_per_block ? mg::row_grid_fixed(m.rows, m.rows_per_block) : mg::row_grid(m.rows, G); };
     if(m.sell)
     {
-      const mg::RowGrid g = mg::row_grid(m.rows, 1);
+      const mg::RowGrid g = grid(1);
       const u32* cs = m.cs.template as<u32>();
       const u32* cl = m.cl.template as<u32>();
       const u32* pm = m.perm.template as<u32>();
@@ -216,12 +218,12 @@ namespace
     const VT* val = m.val.template as<VT>();
     switch(m.lanes)
     {
-      case 2: f(mg::CsrRows<DT, VT, 2>{rp, col, val, m.rows}, mg::row_grid(m.rows, 2)); break;
-      case 4: f(mg::CsrRows<DT, VT, 4>{rp, col, val, m.rows}, mg::row_grid(m.rows, 4)); break;
-      case 8: f(mg::CsrRows<DT, VT, 8>{rp, col, val, m.rows}, mg::row_grid(m.rows, 8)); break;
-      case 16: f(mg::CsrRows<DT, VT, 16>{rp, col, val, m.rows}, mg::row_grid(m.rows, 16)); break;
-      case 32: f(mg::CsrRows<DT, VT, 32>{rp, col, val, m.rows}, mg::row_grid(m.rows, 32)); break;
-      case 64: f(mg::CsrRows<DT, VT, 64>{rp, col, val, m.rows}, mg::row_grid(m.rows, 64)); break;
+      case 2: f(mg::CsrRows<DT, VT, 2>{rp, col, val, m.rows}, grid(2)); break;
+      case 4: f(mg::CsrRows<DT, VT, 4>{rp, col, val, m.rows}, grid(4)); break;
+      case 8: f(mg::CsrRows<DT, VT, 8>{rp, col, val, m.rows}, grid(8)); break;
+      case 16: f(mg::CsrRows<DT, VT, 16>{rp, col, val, m.rows}, grid(16)); break;
+      case 32: f(mg::CsrRows<DT, VT, 32>{rp, col, val, m.rows}, grid(32)); break;
+      case 64: f(mg::CsrRows<DT, VT, 64>{rp, col, val, m.rows}, grid(64)); break;
       default: return rt::kErrArg;
     }
     return rt::kOk;
@@ -296,6 +298,7 @@ namespace
     // by default: on MI355X one CU stays latency-bound even at 27..729 rows and is slower
     // than one full-chip launch per step
     u64 tail_max_n = 0;
+    u64 rows_per_block = 0;   // row positions per block of the row kernels (0: by row_grid)
     double tol_rel = 1e-8, tol_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16), tol_abs_low = 0.0;
     double div_rel = 1.0 / 2.220446049250313e-16, div_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16);
     double stag_rate = 0.95;
@@ -377,6 +380,7 @@ namespace
         const HostLevel& h = lv[i];
         l->n = h.A.rows;
         F3_RC(l->A.upload(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16));
+        l->A.rows_per_block = cfg.rows_per_block;
         if(i + 1 < lv.size())
         {
           if(h.P.rows != l->n || h.R.cols != l->n || h.P.cols != lv[i + 1].A.rows || h.R.rows != lv[i + 1].A.rows)
@@ -923,7 +927,8 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, u64 sell_C, u64
   return rt::kOk;
 }
 
-// named options: "index16" (0/1), "compact_transfer" (0/1), "sell_C", "sell_sigma", "lanes"
+// named options: "index16" (0/1), "compact_transfer" (0/1), "sell_C", "sell_sigma", "lanes",
+// "tail_max_n", "rows_per_block" (level matrices; 0 = derived from the row count)
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
 {
   if(!h || !name) return rt::kErrArg;
@@ -932,6 +937,7 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
   else if(k == "compact_transfer") h->cfg.compact_transfer = value != 0.0;
   else if(k == "sell_C" && value >= 1 && value <= 1024) h->cfg.sell_C = u64(value);
   else if(k == "tail_max_n" && value >= 0) h->cfg.tail_max_n = u64(value);
+  else if(k == "rows_per_block" && value >= 0) h->cfg.rows_per_block = u64(value);
   else if(k == "sell_sigma" && value >= 1) h->cfg.sell_sigma = u64(value);
   else if(k == "lanes" && (value == 0 || value == 2 || value == 4 || value == 8 || value == 16 || value == 32 || value == 64))
     h->cfg.lanes = int(value);

@@ -78,7 +78,7 @@ namespace
     double tol_rel = 1e-8, tol_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16), tol_abs_low = 0.0;
     u64 min_iter = 0, max_iter = 1000;
     bool use_graph = true;
-    u64 tile_rows = 512;   // rows per locality tile of the row kernels (0/1: ascending order)
+    u64 tile_rows = 0;     // rows per locality tile of the row kernels (0/1: ascending order)
   };
 
   // drop stored exact zeros of a scalar graph (the transfers' structural zeros)

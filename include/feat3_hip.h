@@ -282,7 +282,9 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
 /* named launch/format options (replaces the globals of lafem/tuning.hpp:72-163 that the
  * reference's block-size tuner sets): "index16" 0/1 (SELL columns as 16-bit row offsets where
  * they fit), "compact_transfer" 0/1 (device P/R without stored structural zeros), "sell_C",
- * "sell_sigma", "lanes"; takes effect at the next init */
+ * "sell_sigma", "lanes", "tail_max_n", "rows_per_block" (row positions per workgroup of the
+ * level-matrix kernels, a multiple of 64; 0 = derived from the row count); takes effect at the
+ * next init */
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value);
 /* init_symbolic + init_numeric (solver/base.hpp:227-260) */
 int feat3_mgpcg_init(feat3_mgpcg* h);
@@ -334,7 +336,7 @@ int feat3_batch_set_tolerances(feat3_batch* h, double tol_rel, double tol_abs, d
 /* graph replay of one PCG iteration on/off */
 int feat3_batch_set_graph(feat3_batch* h, int graph);
 /* named options, applied at the next init: "tile_rows" (rows per locality tile of the row
- * kernels, feat3_setup_tile_order; default 512, 0 = ascending row order) */
+ * kernels, feat3_setup_tile_order; default 0 = ascending row order) */
 int feat3_batch_set_option(feat3_batch* h, const char* name, double value);
 int feat3_batch_init(feat3_batch* h);
 /* x, b: FP64 device arrays [node][component][system], 16-byte aligned; status, iters: host
