@@ -23,19 +23,32 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 
+# HBM bytes per launch of the finest-level sweep, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (separate runs) and scaled by the same counters on a plain 512 MiB device copy
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); profiles/k_sweep_3d_q1_l7_f32_sell_pmc.md has the
+# commands and raw values.  Keyed by (workload, level, inner, format); absent -> null.
+MEASURED_SWEEP_TRAFFIC = {
+    ("poisson3d", 7, "f32", "sell"): (2 * 183468.5 + 8385.5) * 1024,
+}
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def sweep_algorithmic_bytes(info, n, nnz, nchunks):
-    """Fused damped-Jacobi sweep x_out = x + dinv (b - A x) with SELL-C storage: the CSR defect
-    formula over the unpadded nnz with the stored value and index sizes, the SELL chunk
-    descriptors (cs, cl) in place of row_ptr, x counted once, plus dinv, b and x_out."""
+    """Fused damped-Jacobi sweep x_out = x + dinv (b - A x): the CSR defect formula over the
+    unpadded nnz with the stored value and index sizes -- for SELL-C the chunk descriptors (cs, cl)
+    replace row_ptr, the banded format stores no indices at all (its offsets, 4 bytes per
+    diagonal, replace them) -- x counted once, plus dinv, b and x_out."""
     s_v = info["sizeof_value"]
-    s_i = 2 if info.get("index16", 1) else 4
     s_x = info["sizeof_inner"]
-    return nnz * (s_v + s_i) + 8 * nchunks + n * s_x * 4
+    if info.get("banded"):
+        return nnz * s_v + 4 * info["ndiag"] + n * s_x * 4
+    if info.get("sell_chunks"):
+        s_i = 2 if info.get("index16", 1) else 4
+        return nnz * (s_v + s_i) + 8 * nchunks + n * s_x * 4
+    return nnz * (s_v + 4) + 4 * (n + 1) + n * s_x * 4
 
 
 def _timed(fn, steps, warmup):
@@ -228,16 +241,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--level", type=int, default=7)
+    ap.add_argument("--level", type=int, default=None,
+                    help="finest level (default 7 for poisson3d, 9 for poisson2d, 6 for vector32, 10 for coeff16)")
     ap.add_argument("--inner", default="f32", choices=["f64", "f32", "f16"])
-    ap.add_argument("--fmt", default="sell", choices=["csr", "sell"])
+    ap.add_argument("--fmt", default="sell", choices=["csr", "sell", "banded"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP64 / FP16 comparison solves")
-    ap.add_argument("--workload", default="poisson3d", choices=["poisson3d", "vector32", "coeff16"],
-                    help="poisson3d: the headline 3D Q1 L7 solve; vector32: 32 right-hand sides of the 3x3-block "
-                         "vector Laplacian (3D Q1 L6) in one batched solve; coeff16: forward + adjoint solves and "
-                         "dL/dk of 16 coefficient fields (2D Q1 L10)")
+    ap.add_argument("--workload", default="poisson3d", choices=["poisson3d", "poisson2d", "vector32", "coeff16"],
+                    help="poisson3d: the headline 3D Q1 L7 solve; poisson2d: 2D Q2 L9 (1,050,625 DOF); vector32: 32 "
+                         "right-hand sides of the 3x3-block vector Laplacian (3D Q1 L6) in one batched solve; coeff16: "
+                         "forward + adjoint solves and dL/dk of 16 coefficient fields (2D Q1 L10)")
     args = ap.parse_args()
+    if args.level is None:
+        args.level = {"poisson3d": 7, "poisson2d": 9, "vector32": 6, "coeff16": 10}[args.workload]
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -252,13 +268,15 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
-    if args.workload != "poisson3d":
+    if args.workload in ("vector32", "coeff16"):
         out = (bench_vector32 if args.workload == "vector32" else bench_coeff16)(args)
         print(json.dumps(out), flush=True)
         return
+    dim, deg = (3, 1) if args.workload == "poisson3d" else (2, 2)
+    shape = f"{dim}D Poisson Q{deg} level {args.level}"
 
     t = time.time()
-    H = pb.hierarchy(3, 1, args.level, 1)
+    H = pb.hierarchy(dim, deg, args.level, 1)
     b = pb.filtered_rhs(H, 0)
     n, nnz = H.n, H.levels[0].A.nnz
     log(f"setup: {len(H.levels)} levels, n={n}, nnz={nnz} ({time.time() - t:.1f} s)")
@@ -282,10 +300,11 @@ def main():
             assert st == 2, f"solve did not converge (status {st})"
         return times, iters
 
-    def variant(inner, steps, warmup):
+    def variant(inner, steps, warmup, fmt=None):
+        fmt = fmt or args.fmt
         t0 = time.time()
-        s = MGPCG(H, inner=inner, fmt=args.fmt).init()
-        log(f"init {inner}/{args.fmt}: {time.time() - t0:.1f} s")
+        s = MGPCG(H, inner=inner, fmt=fmt).init()
+        log(f"init {inner}/{fmt}: {time.time() - t0:.1f} s")
         times, iters = run(s, steps, warmup)
         hist = s.history()
         sol = x.cpu().numpy().copy()
@@ -299,7 +318,7 @@ def main():
                  dof_iter_per_s=n * sum(iters) / sum(times), final_rel_residual=float(hist[-1] / hist[0]),
                  sweep_us=sweep_ms * 1e3, sweep_bytes=sb, sweep_GBs=sb / (sweep_ms * 1e-3) / 1e9,
                  vcycle_ms=vc_ms, kernels_per_vcycle=info["kernels_per_vcycle"])
-        log(f"{inner}: {ms:.3f} ms/solve, it {iters}, rel.res {r['final_rel_residual']:.2e}, "
+        log(f"{inner}/{fmt}: {ms:.3f} ms/solve, it {iters}, rel.res {r['final_rel_residual']:.2e}, "
             f"sweep {r['sweep_us']:.1f} us {r['sweep_GBs']:.0f} GB/s, V-cycle {vc_ms:.3f} ms")
         return r, sol, times, iters
 
@@ -315,6 +334,12 @@ def main():
             r, xv, _, _ = variant(inner, max(2, args.steps // 2), 1)
             r["speedup_of_headline"] = r["ms_per_solve"] / ms_per_step
             variants[inner] = (r, xv)
+        # the level-matrix formats at the headline precision (CSR vs SELL-C-sigma vs banded)
+        for fmt in ("csr", "sell", "banded"):
+            if fmt == args.fmt:
+                continue
+            r, _, _, _ = variant(args.inner, max(2, args.steps // 2), 1, fmt=fmt)
+            variants[f"{args.inner}/{fmt}"] = (r, None)
     fp64 = variants.get("f64")
     if fp64:
         x64 = fp64[1]
@@ -330,7 +355,7 @@ def main():
         st, it, _ = S.solve(xo, b, po.params())
         dt = po.lib().orc_last_solve_seconds()
         cpu = {"value": n * it / dt, "unit": "DOF*iter/s", "cores": 1, "kind": "port",
-               "sample": f"one full FP64 PCG-MG solve of the same 3D Q1 L{args.level} system on one core "
+               "sample": f"one full FP64 PCG-MG solve of the same {shape} system on one core "
                          f"({it} iterations, {dt:.2f} s solve phase), oracle restating FEAT's generic CPU loops",
                "time_to_tol_ms": dt * 1e3}
         if fp64:
@@ -338,6 +363,7 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     achieved = head["sweep_GBs"]
+    traffic = MEASURED_SWEEP_TRAFFIC.get((args.workload, args.level, args.inner, args.fmt))
     out = {
         "metric": "time to rel. residual 1e-8 & DOF·iter/s, FP32-inner vs FP64; SpMV GB/s",
         "value": value,
@@ -350,15 +376,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"f32": "f32", "f64": "f64", "f16": "f16-values/f32"}[args.inner],
-        "data": "synthetic (assembled 3D Q1 Poisson system, sine-bubble load)",
-        "config": {"workload": f"3D Poisson Q1 level {args.level} ({n} DOF, {nnz} nnz), PCG(FP64) + MG V(4,4) "
-                               f"Richardson/Jacobi 0.5, exact coarse 2x2x2 cells, {args.inner} V-cycle, "
+        "data": f"synthetic (assembled {dim}D Q{deg} Poisson system, sine-bubble load)",
+        "config": {"workload": f"{shape} ({n} DOF, {nnz} nnz), PCG(FP64) + MG V(4,4) "
+                               f"Richardson/Jacobi 0.5, exact coarse {'2x2x2' if dim == 3 else '2x2'} cells, {args.inner} V-cycle, "
                                f"{args.fmt} format, rel. tol 1e-8",
                    "dofs": n, "nnz": nnz, "levels": len(H.levels)},
         "time_to_tol_ms": ms_per_step,
         "iterations": float(np.mean(iters)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_sweep, finest level (fused damped-Jacobi step)",
                      "bytes_per_launch": head["sweep_bytes"], "avg_launch_us": head["sweep_us"]},
         "headline": head,

@@ -28,6 +28,7 @@
 #include <string>
 #include <type_traits>
 #include <utility>
+#include <algorithm>
 #include <vector>
 
 namespace feat3_amd
@@ -122,6 +123,9 @@ namespace feat3_amd
         static int ell(DT* r, DT a, const DT* x, DT b, const DT* y, const DT* v, const IT* c, const IT* cs,       \
                        const IT* cl, Index C, Index rows)                                                          \
         { return lafem_hip_apply_ell_##D##_##I(nullptr, r, a, x, b, y, v, c, cs, cl, C, rows); }                  \
+        static int banded(DT* r, DT a, const DT* x, DT b, const DT* y, const DT* v, const IT* offs, Index noo,    \
+                          Index rows, Index cols)                                                                  \
+        { return lafem_hip_apply_banded_##D##_##I(nullptr, r, a, x, b, y, v, offs, noo, rows, cols); }            \
         static int csrb(DT* r, DT a, const DT* x, DT b, const DT* y, const DT* v, const IT* c, const IT* rp,      \
                         Index rows, Index cols, Index nnz, int bh, int bw)                                         \
         { return lafem_hip_apply_csrb_##D##_##I(nullptr, r, a, x, b, y, v, c, rp, rows, cols, nnz, bh, bw); }     \
@@ -383,6 +387,71 @@ namespace feat3_amd
       {
         hip_check(Arch::Ops<DT_, IT_>::diag_ell(d.elements(), _val.data(), _col.data(), _cs.data(), _cl.data(), _C, _rows),
                   "SparseMatrixELL::extract_diag");
+      }
+    };
+
+    // SparseMatrixBanded<Mem::HIP, DT_, IT_> (sparse_matrix_banded.hpp): sorted diagonal offsets
+    // stored as offset + rows - 1, values val[d * rows + row]
+    template<typename Mem_, typename DT_, typename IT_ = Index> class SparseMatrixBanded;
+
+    template<typename DT_, typename IT_>
+    class SparseMatrixBanded<Mem::HIP, DT_, IT_>
+    {
+      Index _rows = 0, _cols = 0, _noo = 0;
+      DeviceArray<IT_> _offsets;
+      DeviceArray<DT_> _val;
+    public:
+      typedef DenseVector<Mem::HIP, DT_, IT_> VectorTypeL;
+      typedef DenseVector<Mem::HIP, DT_, IT_> VectorTypeR;
+
+      // from CSR arrays (throws if a row has more distinct diagonals than max_diag in total)
+      template<typename DTH_, typename ITH_>
+      SparseMatrixBanded(Index rows, Index cols, const std::vector<ITH_>& rp, const std::vector<ITH_>& ci,
+                         const std::vector<DTH_>& v, Index max_diag = 64)
+        : _rows(rows), _cols(cols)
+      {
+        std::vector<long long> offs;
+        for(Index i = 0; i < rows; ++i)
+          for(Index j = rp[i]; j < Index(rp[i + 1]); ++j)
+          {
+            const long long o = (long long)(ci[j]) - (long long)(i);
+            auto it = std::lower_bound(offs.begin(), offs.end(), o);
+            if(it == offs.end() || *it != o)
+            {
+              if(offs.size() >= max_diag) throw InternalError("SparseMatrixBanded: more than max_diag diagonals");
+              offs.insert(it, o);
+            }
+          }
+        _noo = offs.size();
+        std::vector<IT_> ho(_noo);
+        for(Index d = 0; d < _noo; ++d) ho[d] = IT_(offs[d] + (long long)(rows) - 1);
+        std::vector<DT_> val(_noo * rows, DT_(0));
+        for(Index i = 0; i < rows; ++i)
+          for(Index j = rp[i]; j < Index(rp[i + 1]); ++j)
+          {
+            const long long o = (long long)(ci[j]) - (long long)(i);
+            const Index d = Index(std::lower_bound(offs.begin(), offs.end(), o) - offs.begin());
+            val[d * rows + i] = DT_(v[j]);
+          }
+        _offsets = DeviceArray<IT_>(_noo); _offsets.upload(ho.data());
+        _val = DeviceArray<DT_>(val.size()); _val.upload(val.data());
+      }
+      Index rows() const { return _rows; }
+      Index columns() const { return _cols; }
+      Index num_of_offsets() const { return _noo; }
+      VectorTypeL create_vector_l() const { return VectorTypeL(_rows); }
+      VectorTypeR create_vector_r() const { return VectorTypeR(_cols); }
+      // sparse_matrix_banded.hpp apply -> Arch::Apply::banded (apply.hpp:162)
+      void apply(VectorTypeL& r, const VectorTypeR& x) const
+      {
+        hip_check(Arch::Ops<DT_, IT_>::banded(r.elements(), DT_(1), x.elements(), DT_(0), r.elements(), _val.data(),
+                                              _offsets.data(), _noo, _rows, _cols), "SparseMatrixBanded::apply");
+      }
+      void apply(VectorTypeL& r, const VectorTypeR& x, const VectorTypeL& y, DT_ alpha = DT_(1)) const
+      {
+        if(Math::abs(alpha) < Math::eps<DT_>()) { r.copy(y); return; }
+        hip_check(Arch::Ops<DT_, IT_>::banded(r.elements(), alpha, x.elements(), DT_(1), y.elements(), _val.data(),
+                                              _offsets.data(), _noo, _rows, _cols), "SparseMatrixBanded::apply");
       }
     };
 

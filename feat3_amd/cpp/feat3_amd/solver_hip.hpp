@@ -393,7 +393,7 @@ namespace feat3_amd
     public:
       typedef LAFEM::DenseVector<Mem::HIP, double, std::uint32_t> VectorType;
       enum Inner { fp64 = 0, fp32 = 1, fp16 = 2 };
-      enum Format { csr = 0, sell = 1 };
+      enum Format { csr = 0, sell = 1, banded = 2 };
 
       FusedMGPCG(Inner inner = fp64, Format fmt = sell) { hip_check(feat3_mgpcg_create(&_h, int(inner), int(fmt)), "FusedMGPCG"); }
       ~FusedMGPCG() override { if(_h) feat3_mgpcg_destroy(_h); }

@@ -140,6 +140,29 @@ namespace
   }
 
   // -------------------------------------------------------------------------------------
+  // Banded (FEAT's SparseMatrixBanded layout, apply_generic.hpp:519-598): num_of_offsets
+  // diagonals, val[d*rows + row] on diagonal d with column row + offsets[d] + 1 - rows; the
+  // diagonals whose column lies outside [0, columns) do not contribute.  One lane per row: a
+  // wave reads 64 consecutive values of each diagonal and 64 consecutive x, no indices.
+  // -------------------------------------------------------------------------------------
+  template<typename DT, typename VT, typename IT>
+  __global__ __launch_bounds__(kBlock) void k_banded(DT* __restrict__ r, DT a, const DT* __restrict__ x, DT b,
+                                                     const DT* y, const VT* __restrict__ val, const IT* __restrict__ offsets,
+                                                     u64 ndiag, u64 rows, u64 cols, int use_y)
+  {
+    const unsigned lb = dev::xcd_remap(blockIdx.x, gridDim.x);
+    const u64 row = u64(lb) * kBlock + threadIdx.x;
+    if(row >= rows) return;
+    DT sum = DT(0);
+    for(u64 d = 0; d < ndiag; ++d)
+    {
+      const u64 c = row + u64(offsets[d]) + 1 - rows;   // wraps past 2^64 for columns below 0
+      if(c < cols) sum += dev::ld<DT>(val, d * rows + row) * x[c];
+    }
+    r[row] = use_y ? (b * y[row]) + (a * sum) : (a * sum);
+  }
+
+  // -------------------------------------------------------------------------------------
   // BCSR with BH x BW row-major blocks (apply_generic.hpp:121-154): G lanes per block row
   // -------------------------------------------------------------------------------------
   template<int G, int BH, int BW, typename DT, typename IT>
@@ -456,6 +479,25 @@ int lafem_hip_apply_sell_h16f32_u32(void* stream, float* r, float a, const float
                                                                 cs, cl, rl, perm, C, rows, use_y);
   return rt::check_launch();
 }
+
+// ---- Apply<Mem::HIP>::banded  (apply.hpp:162,299) ----------------------------------------
+#define F3_DEF_BANDED(SUF, DT, IT)                                                                           \
+  int lafem_hip_apply_banded_##SUF(void* stream, DT* r, DT a, const DT* x, DT b, const DT* y, const DT* val,   \
+                                   const IT* offsets, u64 num_of_offsets, u64 rows, u64 columns)              \
+  {                                                                                                           \
+    if(!r || (num_of_offsets > 0 && (!x || !val || !offsets))) return rt::kErrArg;                            \
+    if(rows == 0) return 0;                                                                                   \
+    const int use_y = std::abs(double(b)) < double(F3_EPS(DT)) ? 0 : 1;                                       \
+    if(use_y && !y) return rt::kErrArg;                                                                       \
+    k_banded<DT, DT, IT><<<blocks_for(rows), kBlock, 0, S_>>>(r, a, x, b, y, val, offsets, num_of_offsets,    \
+                                                              rows, columns, use_y);                          \
+    return rt::check_launch();                                                                                \
+  }
+F3_DEF_BANDED(f64_u32, double, u32)
+F3_DEF_BANDED(f64_u64, double, u64)
+F3_DEF_BANDED(f32_u32, float, u32)
+F3_DEF_BANDED(f32_u64, float, u64)
+#undef F3_DEF_BANDED
 
 // ---- Apply<Mem::HIP>::csrb  (apply.hpp:277) --------------------------------------------
 #define F3_DEF_CSRB(SUF, DT, IT)                                                                             \

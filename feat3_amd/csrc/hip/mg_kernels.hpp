@@ -90,6 +90,33 @@ namespace mg
     }
   };
 
+  // Banded storage (FEAT's SparseMatrixBanded, apply_generic.hpp:519-598): diagonal d holds
+  // val[d*rows + p] at column p + off[d] + 1 - rows; a column outside [0, cols) does not
+  // contribute.  One lane per row: a wave reads 64 consecutive values of each diagonal and 64
+  // consecutive x, with no column indices; the offsets are wave-uniform (scalar loads).
+  template<typename DT_, typename VT_>
+  struct BandedRows
+  {
+    static constexpr int G = 1;
+    const u32* off;
+    const VT_* val;
+    u64 rows, cols;
+    u32 ndiag;
+    __device__ __forceinline__ u64 row(u64 p) const { return p; }
+    __device__ __forceinline__ DT_ row_sum(u64 p, int, const DT_* __restrict__ x) const
+    {
+      const dev::cptr<u32> co = dev::to_const(off);
+      DT_ s = DT_(0);
+#pragma unroll 9
+      for(u32 d = 0; d < ndiag; ++d)
+      {
+        const u64 c = p + u64(co[d]) + 1 - rows;   // wraps past 2^64 for columns below 0
+        if(c < cols) s += dev::ld<DT_>(val, u64(d) * rows + p) * x[c];
+      }
+      return s;
+    }
+  };
+
   // Launch shape of the row kernels: every block owns a contiguous range of `per` row
   // positions (a multiple of 64), its 256/G lane groups stride through that range, and the
   // blocks are XCD-remapped so that neighbouring ranges share an L2.

@@ -84,12 +84,15 @@ namespace
     bool sell = false, sell_perm = false, sell_d16 = false;
     u64 C = 64, sigma = 1, sell_size = 0;
     DevMem cs, cl, perm, scol, sval;
+    bool banded = false;
+    u32 ndiag = 0;
+    DevMem boff, bval;
 
     // drop_zeros: upload a compact copy without the stored exact zeros (used for the
     // transfer matrices, whose two-level pattern holds structural zeros; a zero entry adds
     // nothing to a finite product, so the result is unchanged)
     int upload(const HostCsr& h0, hipStream_t s, bool want_sell, u64 sell_C, u64 sell_sigma, int force_lanes,
-               bool drop_zeros = false, bool allow_d16 = true)
+               bool drop_zeros = false, bool allow_d16 = true, bool want_banded = false)
     {
       HostCsr compact;
       const HostCsr* hp = &h0;
@@ -125,8 +128,54 @@ namespace
       lanes = 2;
       while(lanes < 64 && double(lanes) * 4.0 < mean) lanes *= 2;
       if(force_lanes > 0) lanes = force_lanes;
-      sell = want_sell;
+      if(want_banded) F3_RC(build_banded(h, s));
+      sell = want_sell && !banded;
       if(sell) F3_RC(build_sell(h, s, sell_C, sell_sigma, allow_d16));
+      return rt::kOk;
+    }
+
+    // Banded storage (SparseMatrixBanded layout) when the matrix has few distinct diagonals
+    // (col - row) and the zero fill stays small: at most kMaxDiag diagonals and
+    // ndiag * rows <= 1.25 * nnz.  Otherwise the matrix keeps its other format.
+    static constexpr u32 kMaxDiag = 64;
+    int build_banded(const HostCsr& h, hipStream_t s)
+    {
+      banded = false;
+      if(rows != cols || rows == 0 || rows >= (u64(1) << 31)) return rt::kOk;
+      std::vector<std::int64_t> offs;
+      auto find = [&](std::int64_t o) { return std::lower_bound(offs.begin(), offs.end(), o); };
+      for(u64 r = 0; r < rows; ++r)
+        for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
+        {
+          const std::int64_t o = std::int64_t(h.col[k]) - std::int64_t(r);
+          auto it = find(o);
+          if(it == offs.end() || *it != o)
+          {
+            if(offs.size() >= kMaxDiag) return rt::kOk;
+            offs.insert(it, o);
+          }
+        }
+      if(double(offs.size()) * double(rows) > 1.25 * double(std::max<u64>(nnz, 1))) return rt::kOk;
+      const u64 nd = offs.size();
+      std::vector<VT> hv(nd * rows, to_storage<VT>(0.0));
+      std::vector<char> used(nd * rows, 0);
+      for(u64 r = 0; r < rows; ++r)
+        for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
+        {
+          const u64 d = u64(find(std::int64_t(h.col[k]) - std::int64_t(r)) - offs.begin());
+          if(used[d * rows + r]) return rt::kOk;   // duplicate entry: keep the general format
+          used[d * rows + r] = 1;
+          hv[d * rows + r] = to_storage<VT>(h.val[k]);
+        }
+      std::vector<u32> ho(nd);
+      for(u64 d = 0; d < nd; ++d) ho[d] = u32(offs[d] + std::int64_t(rows) - 1);
+      F3_RC(boff.alloc(sizeof(u32) * nd));
+      F3_RC(bval.alloc(sizeof(VT) * nd * rows));
+      F3_CHECK(hipMemcpyAsync(boff.p, ho.data(), sizeof(u32) * nd, hipMemcpyHostToDevice, s));
+      F3_CHECK(hipMemcpyAsync(bval.p, hv.data(), sizeof(VT) * nd * rows, hipMemcpyHostToDevice, s));
+      F3_CHECK(hipStreamSynchronize(s));
+      ndiag = u32(nd);
+      banded = true;
       return rt::kOk;
     }
 
@@ -178,6 +227,7 @@ namespace
     // pointers / chunk descriptors, SELL padding included), excluding the vector traffic
     u64 matrix_bytes() const
     {
+      if(banded) return u64(ndiag) * rows * sizeof(VT) + u64(ndiag) * 4;
       if(sell)
       {
         const u64 nch = (rows + C - 1) / C;
@@ -192,6 +242,11 @@ namespace
   int with_rows(const DevCsr<VT>& m, F&& f)
   {
     auto grid = [&](int G) { return m.rows_per_block ? mg::row_grid_fixed(m.rows, m.rows_per_block) : mg::row_grid(m.rows, G); };
+    if(m.banded)
+    {
+      f(mg::BandedRows<DT, VT>{m.boff.template as<u32>(), m.bval.template as<VT>(), m.rows, m.cols, m.ndiag}, grid(1));
+      return rt::kOk;
+    }
     if(m.sell)
     {
       const mg::RowGrid g = grid(1);
@@ -289,7 +344,7 @@ namespace
     int pre = 4, post = 4;
     int precond = 0;          // 0 MG V-cycle, 1 none (CG)
     bool use_graph = true;
-    int format = 0;           // 0 CSR, 1 SELL-C-sigma (inner hierarchy)
+    int format = 0;           // 0 CSR, 1 SELL-C-sigma, 2 banded where it fits (else SELL)
     u64 sell_C = 64, sell_sigma = 64;
     int lanes = 0;            // forced lanes per CSR row (0 = by mean row length)
     bool index16 = true;      // SELL columns as 16-bit row offsets where they fit
@@ -372,14 +427,15 @@ namespace
       s = stream;
       if(lv.empty()) return rt::kErrState;
       if(cfg.pre < 1 || cfg.post < 1) return rt::kErrArg;
-      const bool want_sell = cfg.format == 1;
+      const bool want_sell = cfg.format >= 1;
+      const bool want_banded = cfg.format == 2;
       L.clear();
       for(std::size_t i = 0; i < lv.size(); ++i)
       {
         std::unique_ptr<Level<DT, VT>> l(new Level<DT, VT>());
         const HostLevel& h = lv[i];
         l->n = h.A.rows;
-        F3_RC(l->A.upload(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16));
+        F3_RC(l->A.upload(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
         l->A.rows_per_block = cfg.rows_per_block;
         if(i + 1 < lv.size())
         {
@@ -449,7 +505,7 @@ namespace
       }
       if(!shared)
       {
-        F3_RC(A64own.upload(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16));
+        F3_RC(A64own.upload(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
         A64 = &A64own;
       }
       F3_RC(r.alloc(sizeof(double) * n0));
@@ -773,7 +829,7 @@ namespace
     // iteration excluding the V-cycle, [8] sizeof inner DT, [9] sizeof VT
     int info(u64* out, int cap) override
     {
-      std::vector<u64> v(12, 0);
+      std::vector<u64> v(14, 0);
       v[0] = L.size();
       v[1] = L.front()->n;
       v[2] = L.front()->A.nnz;
@@ -801,7 +857,9 @@ namespace
       v[9] = sizeof(VT);
       v[10] = L.front()->A.sell_d16 ? 1 : 0;
       v[11] = L.front()->A.sell ? (L.front()->n + L.front()->A.C - 1) / L.front()->A.C : 0;
-      for(int i = 0; i < cap && i < 12; ++i) out[i] = v[i];
+      v[12] = L.front()->A.banded ? 1 : 0;
+      v[13] = L.front()->A.ndiag;
+      for(int i = 0; i < cap && i < 14; ++i) out[i] = v[i];
       return rt::kOk;
     }
   };
@@ -824,7 +882,7 @@ extern "C" {
 
 int feat3_mgpcg_create(feat3_mgpcg** h, int inner_precision, int format)
 {
-  if(!h || inner_precision < 0 || inner_precision > 2 || format < 0 || format > 1) return rt::kErrArg;
+  if(!h || inner_precision < 0 || inner_precision > 2 || format < 0 || format > 2) return rt::kErrArg;
   std::unique_ptr<feat3_mgpcg> o(new feat3_mgpcg());
   o->inner = inner_precision;
   o->cfg.format = format;

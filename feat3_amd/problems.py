@@ -106,7 +106,8 @@ def load_vector(dim: int, degree: int, level: int, func: int = 0) -> np.ndarray:
 
 def coarsen_cellfield(dim: int, level_fine: int, kf: np.ndarray) -> np.ndarray:
     kc = np.empty(kf.size >> dim, F64)
-    capi.call("feat3_setup_coarsen_cellfield", dim, level_fine, _p(np.ascontiguousarray(kf, F64)), _p(kc))
+    kfc = np.ascontiguousarray(kf, F64)   # held while the call reads it
+    capi.call("feat3_setup_coarsen_cellfield", dim, level_fine, _p(kfc), _p(kc))
     return kc
 
 
@@ -273,3 +274,19 @@ def field_hierarchies(dim: int, level_fine: int, kfields: np.ndarray, level_coar
                 kl = coarsen_cellfield(dim, lvl, kl)
         out.append(H)
     return out
+
+
+def to_banded(A: Csr, max_diag: int = 64):
+    """SparseMatrixBanded form of a CSR matrix (sparse_matrix_banded.hpp): sorted offsets stored as
+    offset + rows - 1 (u32) and val[d * rows + row] with zeros where the matrix has no entry.
+    Returns None when the matrix has more than max_diag distinct diagonals."""
+    rows = A.rows
+    r = np.repeat(np.arange(rows, dtype=np.int64), np.diff(A.row_ptr.astype(np.int64)))
+    off = A.col_ind.astype(np.int64) - r
+    offs = np.unique(off)
+    if offs.size > max_diag:
+        return None
+    d = np.searchsorted(offs, off)
+    val = np.zeros(offs.size * rows)
+    val[d * rows + r] = A.val
+    return (offs + rows - 1).astype(np.uint32), val

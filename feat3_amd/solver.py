@@ -18,7 +18,7 @@ from .problems import Hierarchy
 
 STATUS = {0: "undefined", 1: "progress", 2: "success", 3: "aborted", 4: "diverged", 5: "max-iter", 6: "stagnated"}
 INNER = {"f64": 0, "f32": 1, "f16": 2}
-FORMAT = {"csr": 0, "sell": 1}
+FORMAT = {"csr": 0, "sell": 1, "banded": 2}
 
 
 def status_success(status: int) -> bool:
@@ -121,8 +121,8 @@ class MGPCG:
         return ms.value, nb.value
 
     def info(self) -> dict:
-        out = np.zeros(12, np.uint64)
-        capi.call("feat3_mgpcg_info", self._h, _ptr(out), 12)
+        out = np.zeros(14, np.uint64)
+        capi.call("feat3_mgpcg_info", self._h, _ptr(out), 14)
         keys = ["levels", "n", "nnz", "lanes", "sell_size", "kernels_per_vcycle", "vcycle_bytes",
-                "outer_bytes", "sizeof_inner", "sizeof_value", "index16", "sell_chunks"]
+                "outer_bytes", "sizeof_inner", "sizeof_value", "index16", "sell_chunks", "banded", "ndiag"]
         return {k: int(v) for k, v in zip(keys, out)}

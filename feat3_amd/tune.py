@@ -3,8 +3,9 @@ lafem/tuning.hpp:72-163, which times Arch kernels for block sizes and stores the
 
 For one hierarchy and inner precision it times candidate configurations on the GPU and returns
 the fastest:
-  * the level-matrix format: CSR with 4/8/16 lanes per row, or SELL-C-sigma with chunk height
-    C in {32, 64, 128, 256} and sort window sigma in {1, 64};
+  * the level-matrix format: banded (where the matrix has few diagonals), CSR with 4/8/16 lanes
+    per row, or SELL-C-sigma with chunk height C in {32, 64, 128, 256} and sort window sigma in
+    {1, 64};
   * rows per workgroup of the row kernels (0 = derived from the row count, or 128..2048);
   * for the batched solver, the locality tile of the row order.
 The objective is the measured time of one full V-cycle (feat3_*_time_kernel(3)), which covers
@@ -21,7 +22,8 @@ from typing import Dict, List, Optional
 from .problems import Hierarchy
 
 FORMATS: List[dict] = (
-    [dict(fmt="csr", lanes=L) for L in (4, 8, 16)]
+    [dict(fmt="banded")]
+    + [dict(fmt="csr", lanes=L) for L in (4, 8, 16)]
     + [dict(fmt="sell", sell_C=C, sell_sigma=s) for C in (32, 64, 128, 256) for s in (1, 64)]
 )
 ROWS_PER_BLOCK = (0, 128, 256, 512, 1024, 2048)

@@ -106,6 +106,22 @@ int lafem_hip_apply_sell_h16f32_u32(void* stream, float* r, float a, const float
                                     const void* val, const uint32_t* col_ind, const uint32_t* cs, const uint32_t* cl,
                                     const uint32_t* rl, const uint32_t* perm, uint64_t C, uint64_t rows);
 
+/* Apply::banded (apply.hpp:162,299; generic apply_generic.hpp:519-598): SparseMatrixBanded
+ * layout, val[d*rows + row] on diagonal d whose column is row + offsets[d] + 1 - rows;
+ * diagonals with a column outside [0, columns) do not contribute; r = b*y + a*A*x. */
+int lafem_hip_apply_banded_f64_u32(void* stream, double* r, double a, const double* x, double b, const double* y,
+                                   const double* val, const uint32_t* offsets, uint64_t num_of_offsets, uint64_t rows,
+                                   uint64_t columns);
+int lafem_hip_apply_banded_f64_u64(void* stream, double* r, double a, const double* x, double b, const double* y,
+                                   const double* val, const uint64_t* offsets, uint64_t num_of_offsets, uint64_t rows,
+                                   uint64_t columns);
+int lafem_hip_apply_banded_f32_u32(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                   const float* val, const uint32_t* offsets, uint64_t num_of_offsets, uint64_t rows,
+                                   uint64_t columns);
+int lafem_hip_apply_banded_f32_u64(void* stream, float* r, float a, const float* x, float b, const float* y,
+                                   const float* val, const uint64_t* offsets, uint64_t num_of_offsets, uint64_t rows,
+                                   uint64_t columns);
+
 /* Apply::csrb (apply.hpp:277, CUDA apply.cu:203-231): BCSR with row-major square blocks,
  * block_height == block_width in {1,2,3,4}. */
 int lafem_hip_apply_csrb_f64_u32(void* stream, double* r, double a, const double* x, double b, const double* y,
@@ -258,7 +274,9 @@ int feat3_hip_set_csr_lanes(int lanes);
 typedef struct feat3_mgpcg feat3_mgpcg;
 
 /* inner_precision: 0 FP64, 1 FP32 inside FP64 PCG, 2 binary16 matrix values with FP32
- * arithmetic inside FP64 PCG; format: 0 CSR, 1 SELL-C-sigma for the level matrices */
+ * arithmetic inside FP64 PCG; format: 0 CSR, 1 SELL-C-sigma for the level matrices, 2 banded
+ * (SparseMatrixBanded layout, no column indices) for every square matrix with at most 64
+ * distinct diagonals and at most 25% zero fill, SELL-C-sigma for the others */
 int feat3_mgpcg_create(feat3_mgpcg** h, int inner_precision, int format);
 int feat3_mgpcg_destroy(feat3_mgpcg* h);
 /* one level (finest first), host arrays, copied: CSR matrix (n x n, u32 indices, FP64
@@ -302,7 +320,7 @@ int feat3_mgpcg_history(feat3_mgpcg* h, double* out_host, uint64_t cap, uint64_t
 int feat3_mgpcg_time_kernel(feat3_mgpcg* h, int which, int reps, double* avg_ms, uint64_t* bytes);
 /* [levels, n, nnz, lanes, sell size, kernels per V-cycle, V-cycle bytes, outer bytes,
  *  sizeof inner, sizeof stored value, 16-bit SELL columns on the finest level, SELL chunks
- *  of the finest level] */
+ *  of the finest level, finest level banded (0/1), its number of diagonals] */
 int feat3_mgpcg_info(feat3_mgpcg* h, uint64_t* out, int cap);
 
 /* ===================================================================================== */

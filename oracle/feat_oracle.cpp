@@ -12,6 +12,8 @@
 // the multi-core CPU baseline; its reductions are then no longer in sequential order.
 //
 // Pinning (what fixes each function independently of this text):
+//   banded apply              -- dense brute force (square/rectangular, diagonals leaving the
+//                                matrix), bit-identity with the CSR apply on the Q1 matrix
 //   csr/ell/csrb/csrsb apply  -- dense brute force on tiny matrices, closed-form eigenpairs of
 //                                the pointstar FD matrix (pointstar_factory.hpp:388-411)
 //   pointstar_fd              -- nnz formula, symmetric 5/7-point pattern, eigenpair KAT
@@ -281,6 +283,32 @@ namespace orc
           sum += val[Index(cs[c]) + j * C + k] * x[col[Index(cs[c]) + j * C + k]];
         r[c * C + k] = sum * a + b * r[c * C + k];
       }
+    }
+  }
+
+  // apply_generic.hpp:519-598 (SparseMatrixBanded): offsets[d] = diagonal offset + rows - 1,
+  // val[d * rows + l] on diagonal d.  The reference splits the rows into segments on which the
+  // same contiguous range [i, j) of (ascending) diagonals has its column inside [0, columns)
+  // (start_offset/end_offset, :415-448) and sums a = i .. j-1 in order; per row that is the sum
+  // over the diagonals with a column in range, in storage order, restated here row by row.
+  template<typename DT_, typename IT_>
+  void banded_apply(DT_* r, DT_ a, const DT_* x, DT_ b, const DT_* y, const DT_* val, const IT_* offsets,
+                    Index noo, Index rows, Index columns)
+  {
+    if(std::abs(b) < eps<DT_>())
+      std::fill(r, r + rows, DT_(0));
+    else if(r != y)
+      std::copy(y, y + rows, r);
+    for(Index l(0); l < rows; ++l)
+    {
+      DT_ s(0);
+      for(Index d(0); d < noo; ++d)
+      {
+        const Index c = l + Index(offsets[d]) + 1 - rows;   // wraps for columns below 0
+        if(c < columns)
+          s += val[d * rows + l] * x[c];
+      }
+      r[l] = b * r[l] + a * s;
     }
   }
 
@@ -1071,6 +1099,18 @@ u64 orc_ell_val_size_u32(u64 C, u64 rows, const u32* rp) { return orc::ell_val_s
 void orc_ell_from_csr_f64_u32(u64 C, u64 rows, const u32* rp, const u32* col, const double* val,
                               u32* cs, u32* cl, u32* rl, double* ev, u32* ec)
 { orc::ell_from_csr<double, u32>(C, rows, rp, col, val, cs, cl, rl, ev, ec); }
+void orc_banded_apply_f64_u32(double* r, double a, const double* x, double b, const double* y, const double* val,
+                              const u32* offsets, u64 noo, u64 rows, u64 columns)
+{
+  orc::banded_apply(r, a, x, b, y, val, offsets, noo, rows, columns);
+}
+
+void orc_banded_apply_f32_u32(float* r, float a, const float* x, float b, const float* y, const float* val,
+                              const u32* offsets, u64 noo, u64 rows, u64 columns)
+{
+  orc::banded_apply(r, a, x, b, y, val, offsets, noo, rows, columns);
+}
+
 void orc_ell_apply_f64_u32(double* r, double a, const double* x, double b, const double* y, const double* val,
                            const u32* col, const u32* cs, const u32* cl, u64 C, u64 rows)
 { orc::ell_apply<double, u32>(r, a, x, b, y, val, col, cs, cl, C, rows); }

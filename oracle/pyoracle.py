@@ -27,6 +27,8 @@ _PROTOS = {
     "orc_csrb_apply_f64_u32": (None, [c_vp, c_d, c_vp, c_d, c_vp, c_vp, c_vp, c_vp, c_u64, c_i, c_i]),
     "orc_csrb_apply_f32_u32": (None, [c_vp, c_f, c_vp, c_f, c_vp, c_vp, c_vp, c_vp, c_u64, c_i, c_i]),
     "orc_csrsb_apply_f64_u32": (None, [c_vp, c_d, c_vp, c_d, c_vp, c_vp, c_vp, c_vp, c_u64, c_i]),
+    "orc_banded_apply_f64_u32": (None, [c_vp, c_d, c_vp, c_d, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64]),
+    "orc_banded_apply_f32_u32": (None, [c_vp, c_f, c_vp, c_f, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64]),
     "orc_ell_val_size_u32": (c_u64, [c_u64, c_u64, c_vp]),
     "orc_ell_from_csr_f64_u32": (None, [c_u64, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "orc_ell_apply_f64_u32": (None, [c_vp, c_d, c_vp, c_d, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64]),
@@ -180,6 +182,8 @@ def cell_gradient(dim, degree, level, E, u, lam):
     m = 1 << level
     g = np.empty(m ** dim)
     E = np.ascontiguousarray(E, np.float64)
-    rc = lib().orc_cell_gradient(dim, degree, level, P(E), P(np.ascontiguousarray(u)), P(np.ascontiguousarray(lam)), P(g))
+    u = np.ascontiguousarray(u, np.float64)      # held while the call reads them
+    lam = np.ascontiguousarray(lam, np.float64)
+    rc = lib().orc_cell_gradient(dim, degree, level, P(E), P(u), P(lam), P(g))
     assert rc == 0
     return g

@@ -185,6 +185,39 @@ def test_ell_and_sell_apply(T, C):
     assert np.array_equal(host(dd), dref)
 
 
+@pytest.mark.parametrize("shape", [(1031, 1031), (700, 1100), (900, 400)])
+def test_banded_apply(T, shape):
+    """Apply::banded (SparseMatrixBanded): FP64 u32/u64 offsets within 1e-13 and FP32 within 1e-5 of
+    the oracle; diagonals leaving the matrix at both ends; r aliasing y."""
+    torch, capi = T
+    rows, cols = shape
+    rng = np.random.default_rng(rows)
+    offs = sorted(set(int(k) for k in rng.integers(-rows + 1, cols, size=9)) | {0, 1, -1})
+    noo = len(offs)
+    o = np.array([k + rows - 1 for k in offs], np.uint32)
+    val = rng.uniform(-1, 1, noo * rows)
+    x = rng.uniform(-1, 1, cols)
+    y = rng.uniform(-1, 1, rows)
+    L = po.lib()
+    for a, b in [(1.0, 0.0), (-0.75, 1.5)]:
+        ref = y.copy()
+        L.orc_banded_apply_f64_u32(po.P(ref), a, po.P(x), b, po.P(ref), po.P(val), po.P(o), noo, rows, cols)
+        for name, oo in [("lafem_hip_apply_banded_f64_u32", o), ("lafem_hip_apply_banded_f64_u64", o.astype(np.uint64))]:
+            dr = dev(torch, y.copy())
+            capi.call(name, None, vp(dr), a, vp(dev(torch, x)), b, vp(dr), vp(dev(torch, val)), vp(dev(torch, oo.view(np.int64) if oo.dtype == np.uint64 else oo.astype(np.int32))), noo, rows, cols)
+            torch.cuda.synchronize()
+            err = rel(host(dr), ref)
+            assert err <= 1e-13, (name, err)
+        ref32 = y.astype(np.float32)
+        v32, x32 = val.astype(np.float32), x.astype(np.float32)
+        L.orc_banded_apply_f32_u32(po.P(ref32), a, po.P(x32), b, po.P(ref32), po.P(v32), po.P(o), noo, rows, cols)
+        dr = dev(torch, y.astype(np.float32))
+        capi.call("lafem_hip_apply_banded_f32_u32", None, vp(dr), a, vp(dev(torch, x32)), b, vp(dr), vp(dev(torch, v32)),
+                  vp(dev(torch, o.astype(np.int32))), noo, rows, cols)
+        torch.cuda.synchronize()
+        assert rel(host(dr), ref32) <= 1e-5
+
+
 @pytest.mark.parametrize("bs", [1, 2, 3, 4])
 def test_csrb_and_csrsb_apply(T, bs):
     torch, capi = T

@@ -163,3 +163,27 @@ def test_apply_and_restart(torch):
     assert st == 2 and it2 >= 1
     assert rel(x.cpu().numpy(), xo) <= 1e-9
     s.done()
+
+
+@pytest.mark.parametrize("dim,deg,lvl", [(3, 1, 5), (2, 2, 5), (2, 1, 6)])
+@pytest.mark.parametrize("inner", ["f64", "f32", "f16"])
+def test_banded_format_matches_oracle(torch, dim, deg, lvl, inner):
+    """The banded level-matrix format (lexicographic numbering: 27, 25 and 9 diagonals) keeps the
+    parity bars: FP64 identical iterations and <= 1e-10; mixed paths iterations +-1 of the
+    oracle's mixed algorithm and <= 1e-8 against the FP64 oracle solution."""
+    print(f"case banded {dim}D Q{deg} L{lvl} {inner}", flush=True)
+    from feat3_amd.solver import MGPCG
+    H = pb.hierarchy(dim, deg, lvl, 1)
+    b = pb.filtered_rhs(H, 0)
+    so, io, xo, _ = solve_orc(H, b)
+    s = MGPCG(H, inner=inner, fmt="banded").init()
+    info = s.info()
+    assert info["banded"] == 1 and info["ndiag"] == (27 if dim == 3 else (25 if deg == 2 else 9))
+    s.done()
+    sg, ig, xg, _ = solve_gpu(torch, H, b, inner=inner, fmt="banded")
+    assert so == 2 and sg == 2
+    if inner == "f64":
+        assert ig == io and rel(xg, xo) <= 1e-10
+    else:
+        _, im, _, _ = solve_orc(H, b, inner={"f32": 1, "f16": 2}[inner])
+        assert abs(ig - im) <= 1 and rel(xg, xo) <= 1e-8

@@ -75,11 +75,12 @@ def test_bcsr_csrsb_ell_brute_force():
     x = rng.uniform(-1, 1, nb * bs)
     r = np.zeros(nb * bs)
     L = po.lib()
-    L.orc_csrb_apply_f64_u32(po.P(r), 1.0, po.P(x), 0.0, po.P(r), po.P(blocks.ravel().copy()), po.P(col), po.P(rp), nb, bs, bs)
+    bflat = blocks.ravel().copy()     # kept alive across the call (a raw pointer does not hold it)
+    L.orc_csrb_apply_f64_u32(po.P(r), 1.0, po.P(x), 0.0, po.P(r), po.P(bflat), po.P(col), po.P(rp), nb, bs, bs)
     assert np.allclose(r, D @ x, rtol=1e-14, atol=1e-14)
     # diagonal of the diagonal blocks
     dg = np.zeros(nb * bs)
-    L.orc_diag_csrb_f64_u32(po.P(dg), po.P(blocks.ravel().copy()), po.P(col), po.P(rp), nb, bs, bs)
+    L.orc_diag_csrb_f64_u32(po.P(dg), po.P(bflat), po.P(col), po.P(rp), nb, bs, bs)
     assert np.array_equal(dg, np.diag(D))
     # csrsb: scalar matrix times blocked vector == kron(A, I)
     sval = rng.uniform(-1, 1, col.size)
@@ -373,3 +374,39 @@ def test_blocked_vector_laplacian_reduces_to_scalar(inner, comp):
     assert np.max(np.abs(X[:, comp] - x)) <= 1e-13 * np.max(np.abs(x))
     assert np.all(np.delete(X, comp, axis=1) == 0.0)
     assert np.allclose(hist, histb, rtol=1e-12)
+
+
+def test_banded_apply_brute_force_and_csr_identity():
+    """Banded apply (apply_generic.hpp:519-598): dense brute force on random banded matrices
+    (square and rectangular, diagonals leaving the matrix at both ends), beta/alias semantics,
+    and bit-identity with the CSR apply on the Q1 matrix (same entries, same summation order,
+    the fill adds exact zeros)."""
+    L = po.lib()
+    rng = np.random.default_rng(7)
+    for rows, cols, offs in [(17, 17, [-5, -1, 0, 2, 9]), (12, 20, [-3, 0, 4, 15]), (20, 9, [-19, -4, 0, 8])]:
+        noo = len(offs)
+        o = np.array([k + rows - 1 for k in offs], np.uint32)
+        val = rng.standard_normal(noo * rows)
+        x = rng.standard_normal(cols)
+        y = rng.standard_normal(rows)
+        dense = np.zeros((rows, cols))
+        for d, k in enumerate(offs):
+            for l in range(rows):
+                if 0 <= l + k < cols:
+                    dense[l, l + k] = val[d * rows + l]
+        for a, b in [(1.0, 0.0), (-0.5, 2.0)]:
+            r = np.empty(rows)
+            L.orc_banded_apply_f64_u32(po.P(r), a, po.P(x), b, po.P(y), po.P(val), po.P(o), noo, rows, cols)
+            assert np.allclose(r, a * dense @ x + b * y, rtol=1e-14, atol=1e-14)
+        r = y.copy()                       # r aliases y
+        L.orc_banded_apply_f64_u32(po.P(r), 1.5, po.P(x), -1.0, po.P(r), po.P(val), po.P(o), noo, rows, cols)
+        assert np.allclose(r, 1.5 * dense @ x - y, rtol=1e-14, atol=1e-14)
+    A = pb.hierarchy(3, 1, 3, 3).levels[0].A
+    o, val = pb.to_banded(A)
+    assert o.size == 27
+    x = rng.standard_normal(A.rows)
+    rb = np.empty(A.rows)
+    rc = np.empty(A.rows)
+    L.orc_banded_apply_f64_u32(po.P(rb), 1.0, po.P(x), 0.0, po.P(x), po.P(val), po.P(o), o.size, A.rows, A.rows)
+    po.csr_apply(rc, 1.0, x, 0.0, x, A.val, A.col_ind, A.row_ptr, A.rows, A.rows)
+    assert np.array_equal(rb, rc)
