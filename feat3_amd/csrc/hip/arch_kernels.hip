@@ -156,8 +156,12 @@ namespace
     DT sum = DT(0);
     for(u64 d = 0; d < ndiag; ++d)
     {
-      const u64 c = row + u64(offsets[d]) + 1 - rows;   // wraps past 2^64 for columns below 0
-      if(c < cols) sum += dev::ld<DT>(val, d * rows + row) * x[c];
+      // column (wraps past 2^64 below 0); an unconditional load from a stand-in column and a
+      // select keep the loads of consecutive diagonals in flight together
+      const u64 c = row + u64(offsets[d]) + 1 - rows;
+      const bool in = c < cols;
+      const DT t = dev::ld<DT>(val, d * rows + row) * x[in ? c : 0];
+      sum += in ? t : DT(0);
     }
     r[row] = use_y ? (b * y[row]) + (a * sum) : (a * sum);
   }
@@ -489,8 +493,10 @@ int lafem_hip_apply_sell_h16f32_u32(void* stream, float* r, float a, const float
     if(rows == 0) return 0;                                                                                   \
     const int use_y = std::abs(double(b)) < double(F3_EPS(DT)) ? 0 : 1;                                       \
     if(use_y && !y) return rt::kErrArg;                                                                       \
-    k_banded<DT, DT, IT><<<blocks_for(rows), kBlock, 0, S_>>>(r, a, x, b, y, val, offsets, num_of_offsets,    \
-                                                              rows, columns, use_y);                          \
+    /* no column: no diagonal contributes (and x holds nothing to stand in) */                                \
+    k_banded<DT, DT, IT><<<blocks_for(rows), kBlock, 0, S_>>>(r, a, x, b, y, val, offsets,                    \
+                                                              columns ? num_of_offsets : 0, rows, columns,    \
+                                                              use_y);                                         \
     return rt::check_launch();                                                                                \
   }
 F3_DEF_BANDED(f64_u32, double, u32)

@@ -90,10 +90,14 @@ namespace mg
     }
   };
 
-  // Banded storage (FEAT's SparseMatrixBanded, apply_generic.hpp:519-598): diagonal d holds
-  // val[d*rows + p] at column p + off[d] + 1 - rows; a column outside [0, cols) does not
-  // contribute.  One lane per row: a wave reads 64 consecutive values of each diagonal and 64
-  // consecutive x, with no column indices; the offsets are wave-uniform (scalar loads).
+  // Banded storage (the diagonals of FEAT's SparseMatrixBanded, apply_generic.hpp:519-598) of a
+  // square matrix:
+  // diagonal d of row p holds the entry at column p + off[d] + 1 - rows; a column outside
+  // [0, cols) does not contribute.  Values are chunk-major, val[(p/64)*ndiag*64 + d*64 + p%64],
+  // so one wave (64 consecutive rows) reads one contiguous region -- FEAT's diagonal-major
+  // order would make every wave read ndiag streams one matrix-length apart.  One lane per row,
+  // 64 consecutive x per diagonal, no column indices; the offsets are wave-uniform (scalar
+  // loads).
   template<typename DT_, typename VT_>
   struct BandedRows
   {
@@ -106,12 +110,17 @@ namespace mg
     __device__ __forceinline__ DT_ row_sum(u64 p, int, const DT_* __restrict__ x) const
     {
       const dev::cptr<u32> co = dev::to_const(off);
+      const u64 base = (p >> 6) * u64(ndiag) * 64 + (p & 63);
       DT_ s = DT_(0);
 #pragma unroll 9
       for(u32 d = 0; d < ndiag; ++d)
       {
-        const u64 c = p + u64(co[d]) + 1 - rows;   // wraps past 2^64 for columns below 0
-        if(c < cols) s += dev::ld<DT_>(val, u64(d) * rows + p) * x[c];
+        // column; wraps past 2^64 below 0.  The load is unconditional (an in-range stand-in
+        // column) and the product selected away, so that no branch splits the batch of loads
+        const u64 c = p + u64(co[d]) + 1 - rows;
+        const bool in = c < cols;
+        const DT_ t = dev::ld<DT_>(val, base + u64(d) * 64) * x[in ? c : p];
+        s += in ? t : DT_(0);
       }
       return s;
     }

@@ -134,7 +134,7 @@ namespace
       return rt::kOk;
     }
 
-    // Banded storage (SparseMatrixBanded layout) when the matrix has few distinct diagonals
+    // Banded storage (the diagonals of SparseMatrixBanded, chunk-major) when the matrix has few distinct diagonals
     // (col - row) and the zero fill stays small: at most kMaxDiag diagonals and
     // ndiag * rows <= 1.25 * nnz.  Otherwise the matrix keeps its other format.
     static constexpr u32 kMaxDiag = 64;
@@ -157,22 +157,26 @@ namespace
         }
       if(double(offs.size()) * double(rows) > 1.25 * double(std::max<u64>(nnz, 1))) return rt::kOk;
       const u64 nd = offs.size();
-      std::vector<VT> hv(nd * rows, to_storage<VT>(0.0));
-      std::vector<char> used(nd * rows, 0);
+      // chunk-major: the nd diagonals of 64 consecutive rows are contiguous, so that a wave
+      // reads one contiguous region (val[(p / 64) * nd * 64 + d * 64 + p % 64])
+      const u64 padded = (rows + 63) / 64 * 64;
+      std::vector<VT> hv(nd * padded, to_storage<VT>(0.0));
+      std::vector<char> used(nd * padded, 0);
       for(u64 r = 0; r < rows; ++r)
         for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
         {
           const u64 d = u64(find(std::int64_t(h.col[k]) - std::int64_t(r)) - offs.begin());
-          if(used[d * rows + r]) return rt::kOk;   // duplicate entry: keep the general format
-          used[d * rows + r] = 1;
-          hv[d * rows + r] = to_storage<VT>(h.val[k]);
+          const u64 at = (r / 64) * nd * 64 + d * 64 + (r % 64);
+          if(used[at]) return rt::kOk;   // duplicate entry: keep the general format
+          used[at] = 1;
+          hv[at] = to_storage<VT>(h.val[k]);
         }
       std::vector<u32> ho(nd);
       for(u64 d = 0; d < nd; ++d) ho[d] = u32(offs[d] + std::int64_t(rows) - 1);
       F3_RC(boff.alloc(sizeof(u32) * nd));
-      F3_RC(bval.alloc(sizeof(VT) * nd * rows));
+      F3_RC(bval.alloc(sizeof(VT) * nd * padded));
       F3_CHECK(hipMemcpyAsync(boff.p, ho.data(), sizeof(u32) * nd, hipMemcpyHostToDevice, s));
-      F3_CHECK(hipMemcpyAsync(bval.p, hv.data(), sizeof(VT) * nd * rows, hipMemcpyHostToDevice, s));
+      F3_CHECK(hipMemcpyAsync(bval.p, hv.data(), sizeof(VT) * nd * padded, hipMemcpyHostToDevice, s));
       F3_CHECK(hipStreamSynchronize(s));
       ndiag = u32(nd);
       banded = true;

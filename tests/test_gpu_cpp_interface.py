@@ -48,7 +48,7 @@ def test_cpp_application_level_solvers(tmp_path, torch_cuda):
     assert rel(load("fused64")) <= 1e-10
     assert res["fusedbanded64"] == [2, io]
     assert rel(load("fusedbanded64")) <= 1e-10
-    assert res["banded_vs_csr_mismatches"] == [0, 9]
+    assert res["banded_vs_csr"][0] <= 1000 and res["banded_vs_csr"][1] == 9   # within 1e-13 of max |entry|
     assert res["mixed32"][0] == 2 and abs(res["mixed32"][1] - io) <= 1
     assert rel(load("mixed32")) <= 1e-8
     assert res["leaked"] == [0]
