@@ -24,11 +24,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 
 # HBM bytes per launch of the finest-level sweep, measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (separate runs) and scaled by the same counters on a plain 512 MiB device copy
-# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); profiles/k_sweep_3d_q1_l7_f32_sell_pmc.md has the
-# commands and raw values.  Keyed by (workload, level, inner, format); absent -> null.
+# WRITE_SIZE (separate runs of this script) and scaled by the same counters on a plain 512 MiB
+# device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); profiles/*_pmc.md hold the commands and raw
+# values.  Keyed by (workload, level, inner, format); absent -> null.
 MEASURED_SWEEP_TRAFFIC = {
-    ("poisson3d", 7, "f32", "sell"): (2 * 183468.5 + 8385.5) * 1024,
+    ("poisson3d", 7, "f32", "banded"): (2 * 126989.4 + 8385.5) * 1024,
+    ("vector32", 6, "f32", "banded"): (2 * 449339.9 + 102984.4) * 1024,
 }
 
 
@@ -135,7 +136,8 @@ def bench_vector32(args):
     for r in res.values():
         r.pop("solution")
     roof = {"bound": "hbm", "achieved": head["sweep_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": head["sweep_GBs"] / HBM_PEAK_GBS, "traffic": None,
+            "frac": head["sweep_GBs"] / HBM_PEAK_GBS,
+            "traffic": MEASURED_SWEEP_TRAFFIC.get(("vector32", lvl, args.inner, "banded")),
             "kernel": "batch k_sweep<B=3>, finest level (fused damped-Jacobi step, 32 systems)",
             "bytes_per_launch": head["sweep_bytes"], "avg_launch_us": head["sweep_us"]}
     cfg = {"workload": f"3D vector Laplacian Q1 level {lvl}, 3x3 BCSR ({nodes} block rows, {nnzb} blocks), "
@@ -244,7 +246,8 @@ def main():
     ap.add_argument("--level", type=int, default=None,
                     help="finest level (default 7 for poisson3d, 9 for poisson2d, 6 for vector32, 10 for coeff16)")
     ap.add_argument("--inner", default="f32", choices=["f64", "f32", "f16"])
-    ap.add_argument("--fmt", default="sell", choices=["csr", "sell", "banded"])
+    ap.add_argument("--fmt", default="banded", choices=["csr", "sell", "banded"],
+                    help="level-matrix format of the V-cycle (banded falls back to SELL-C-sigma where a matrix is not banded)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP64 / FP16 comparison solves")
     ap.add_argument("--workload", default="poisson3d", choices=["poisson3d", "poisson2d", "vector32", "coeff16"],

@@ -301,8 +301,8 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
  * reference's block-size tuner sets): "index16" 0/1 (SELL columns as 16-bit row offsets where
  * they fit), "compact_transfer" 0/1 (device P/R without stored structural zeros), "sell_C",
  * "sell_sigma", "lanes", "tail_max_n", "rows_per_block" (row positions per workgroup of the
- * level-matrix kernels, a multiple of 64; 0 = derived from the row count); takes effect at the
- * next init */
+ * level-matrix kernels, a multiple of 64; 0 = derived from the row count), "transfer_sigma" (SELL
+ * sort window of the transfer matrices, default 1); takes effect at the next init */
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value);
 /* init_symbolic + init_numeric (solver/base.hpp:227-260) */
 int feat3_mgpcg_init(feat3_mgpcg* h);
