@@ -14,6 +14,8 @@
 #include "device_common.hpp"
 #include "mg_kernels.hpp"
 
+#include <type_traits>
+
 namespace feat3_amd
 {
 namespace batch
@@ -83,6 +85,7 @@ namespace batch
     u64 rows;
     int K;
     const u32* ord;   // processing order of the rows (null: ascending)
+    static constexpr bool kShared = SHARED_;
 
     __device__ __forceinline__ void acc(u32 e, int k0, const DT_ (&xv)[B_][S_], DT_ (&s)[B_][S_]) const
     {
@@ -108,7 +111,23 @@ namespace batch
       }
     }
 
-    __device__ __forceinline__ void row_sum(u64 p, int k0, const DT_* __restrict__ x, DT_ (&s)[B_][S_]) const
+    // shared values of entry e from a block-local copy (entries e0.. staged in LDS)
+    __device__ __forceinline__ void acc_lds(u32 e, u32 e0, const VT_* lds, const DT_ (&xv)[B_][S_], DT_ (&s)[B_][S_]) const
+    {
+#pragma unroll
+      for(int h = 0; h < B_; ++h)
+#pragma unroll
+        for(int w = 0; w < B_; ++w)
+        {
+          const DT_ a = cvt<DT_, VT_>(lds[(e - e0) * u32(B_ * B_) + u32(h * B_ + w)]);
+#pragma unroll
+          for(int i = 0; i < S_; ++i) s[h][i] += a * xv[w][i];
+        }
+    }
+
+    // LDS != nullptr: the values of entries [e0, ...) are staged there (shared values only)
+    __device__ __forceinline__ void row_sum(u64 p, int k0, const DT_* __restrict__ x, DT_ (&s)[B_][S_],
+                                            const VT_* lds = nullptr, u32 e0 = 0) const
     {
 #pragma unroll
       for(int h = 0; h < B_; ++h)
@@ -126,6 +145,15 @@ namespace batch
         for(int u = 0; u < kU; ++u)
 #pragma unroll
           for(int w = 0; w < B_; ++w) ldv<DT_, S_>(x + (j[u] * B_ + u64(w)) * u64(K) + u64(k0), xv[u][w]);
+        if constexpr(SHARED_)
+        {
+          if(lds)
+          {
+#pragma unroll
+            for(int u = 0; u < kU; ++u) acc_lds(e + u, e0, lds, xv[u], s);
+            continue;
+          }
+        }
 #pragma unroll
         for(int u = 0; u < kU; ++u) acc(e + u, k0, xv[u], s);
       }
@@ -135,6 +163,10 @@ namespace batch
         DT_ xv[B_][S_];
 #pragma unroll
         for(int w = 0; w < B_; ++w) ldv<DT_, S_>(x + (j * B_ + u64(w)) * u64(K) + u64(k0), xv[w]);
+        if constexpr(SHARED_)
+        {
+          if(lds) { acc_lds(e, e0, lds, xv, s); continue; }
+        }
         acc(e, k0, xv, s);
       }
     }
@@ -150,6 +182,7 @@ namespace batch
     u64 rows;
     int K;
     const u32* ord;   // processing order of the rows (null: ascending)
+    static constexpr bool kShared = false;
 
     __device__ __forceinline__ void row_sum(u64 p, int k0, const DT_* __restrict__ x, DT_ (&s)[B_][S_]) const
     {
@@ -214,6 +247,41 @@ namespace batch
     }
   }
 
+  // The same loop with the shared block values of each round of rows staged in LDS: a block
+  // runs 256/G rows at a time; their entries [rp[q0], rp[q1]) are contiguous, so the block copies
+  // their values with coalesced loads (each value once) instead of every lane of a row's group
+  // loading all B*B values of every entry.  A round with more than CAP entries, a permuted row
+  // order or per-system values use the direct loop.  All threads of the block take part in the
+  // copies and barriers; lanes with k0 >= K only help copying.
+  template<int G_, int S_, int B_, int CAP_, typename Rows_, typename DT_, typename VT_, typename F>
+  __device__ __forceinline__ void for_block_rows_staged(const Rows_& A, u64 per, const DT_* __restrict__ x, VT_* lds, F&& f)
+  {
+    constexpr u32 BB = u32(B_ * B_);
+    constexpr unsigned R = 256 / G_;
+    const u64 b = dev::xcd_remap(blockIdx.x, gridDim.x);
+    const u64 beg = b * per;
+    const u64 end = beg + per < A.rows ? beg + per : A.rows;
+    const int k0 = int(threadIdx.x % G_) * S_;
+    const unsigned g = threadIdx.x / G_;
+    for(u64 q0 = beg; q0 < end; q0 += R)
+    {
+      const u64 q1 = q0 + R < end ? q0 + R : end;
+      const u32 e0 = A.rp[q0], e1 = A.rp[q1];
+      const bool staged = (e1 - e0) <= u32(CAP_);
+      __syncthreads();   // the previous round's readers are done with lds
+      if(staged)
+        for(u32 i = threadIdx.x; i < (e1 - e0) * BB; i += blockDim.x) lds[i] = A.val[u64(e0) * BB + i];
+      __syncthreads();
+      const u64 p = q0 + g;
+      if(p < q1 && k0 < A.K)
+      {
+        DT_ s[B_][S_];
+        A.row_sum(p, k0, x, s, staged ? lds : nullptr, e0);
+        f(p, k0, s);
+      }
+    }
+  }
+
   __device__ __forceinline__ bool masked(const u32* mask, u64 node) { return dev::masked(mask, node); }
   __device__ __forceinline__ bool idle(const int* any) { return any != nullptr && *any == 0; }
 
@@ -257,6 +325,36 @@ namespace batch
     }
   }
 
+  // entries of shared block values staged per round (LDS bytes: CAP * B * B * sizeof(value)).
+  // Measured on the 32-RHS 3x3 sweep (3D Q1 L6): staging FP64 values 725 -> 507 us (outer FP64
+  // SpMV 643 -> 490 us); staging FP32 values 260 -> 307 us (the 36 KB of LDS per block cost more
+  // occupancy than the broadcast loads they replace), so only 8-byte values are staged.
+  template<int B_, typename VT_>
+  struct Stage
+  {
+    static constexpr bool enabled = sizeof(VT_) == 8;
+    static constexpr int kBytes = 36 * 1024;
+    static constexpr int CAP = kBytes / int(B_ * B_ * sizeof(VT_));
+  };
+
+  // runs the row loop of a kernel, staged when the values are shared and the order ascending
+  template<int G_, int S_, int B_, typename Rows_, typename DT_, typename F>
+  __device__ __forceinline__ void row_loop(const Rows_& A, u64 per, const DT_* __restrict__ x, F&& f)
+  {
+    typedef typename std::remove_const<typename std::remove_pointer<decltype(A.val)>::type>::type VT;
+    if constexpr(Rows_::kShared && Stage<B_, VT>::enabled)
+    {
+      constexpr int CAP = Stage<B_, VT>::CAP;
+      __shared__ VT lds[CAP * B_ * B_];
+      if(A.ord == nullptr)
+      {
+        for_block_rows_staged<G_, S_, B_, CAP>(A, per, x, lds, f);
+        return;
+      }
+    }
+    for_block_rows<G_, S_, B_>(A, per, x, f);
+  }
+
   // x_out = x_in + dinv (rhs - A x_in)
   template<int G_, int S_, int B_, bool SHARED_, typename Rows_, typename DT_>
   F3_ROW_KERNEL void k_sweep(Rows_ A, u64 per, const DT_* __restrict__ dinv, const DT_* __restrict__ rhs,
@@ -264,7 +362,7 @@ namespace batch
   {
     if(idle(any)) return;
     const int K = A.K;
-    for_block_rows<G_, S_, B_>(A, per, xin, [&](u64 p, int k0, const DT_ (&s)[B_][S_]) {
+    row_loop<G_, S_, B_>(A, per, xin, [&](u64 p, int k0, const DT_ (&s)[B_][S_]) {
 #pragma unroll
       for(int h = 0; h < B_; ++h)
       {
@@ -287,7 +385,7 @@ namespace batch
   {
     if(idle(any)) return;
     const int K = A.K;
-    for_block_rows<G_, S_, B_>(A, per, x, [&](u64 p, int k0, const DT_ (&s)[B_][S_]) {
+    row_loop<G_, S_, B_>(A, per, x, [&](u64 p, int k0, const DT_ (&s)[B_][S_]) {
       const bool m = masked(mask, p);
 #pragma unroll
       for(int h = 0; h < B_; ++h)
@@ -418,7 +516,7 @@ namespace batch
     double v[S_];
 #pragma unroll
     for(int i = 0; i < S_; ++i) v[i] = 0.0;
-    for_block_rows<G_, S_, B_>(A, per, x, [&](u64 p, int k0, const double (&s)[B_][S_]) {
+    row_loop<G_, S_, B_>(A, per, x, [&](u64 p, int k0, const double (&s)[B_][S_]) {
       const bool m = masked(mask, p);
 #pragma unroll
       for(int h = 0; h < B_; ++h)
@@ -448,7 +546,7 @@ namespace batch
     double v[S_];
 #pragma unroll
     for(int i = 0; i < S_; ++i) v[i] = 0.0;
-    for_block_rows<G_, S_, B_>(A, per, p, [&](u64 row, int k0, const double (&s)[B_][S_]) {
+    row_loop<G_, S_, B_>(A, per, p, [&](u64 row, int k0, const double (&s)[B_][S_]) {
       const bool m = masked(mask, row);
 #pragma unroll
       for(int h = 0; h < B_; ++h)
