@@ -363,6 +363,17 @@ def main():
                "time_to_tol_ms": dt * 1e3}
         if fp64:
             cpu["rel_diff_gpu_fp64_vs_cpu"] = float(np.linalg.norm(fp64[1] - xo) / np.linalg.norm(xo))
+        cpu["rel_diff_headline_vs_cpu_fp64"] = float(np.linalg.norm(xm - xo) / np.linalg.norm(xo))
+        cpu["iterations_cpu"] = int(it)
+        # the same restatement with its row and BLAS-1 loops parallelised (OpenMP, all host threads
+        # the job may use); reported beside the single-core reference analogue
+        So = po.System.from_hierarchy(H, omp=True)
+        xp = np.zeros(n)
+        _, itp, _ = So.solve(xp, b, po.params())
+        dtp = po.lib(True).orc_last_solve_seconds()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu["all_cores"] = {"value": n * itp / dtp, "unit": "DOF*iter/s", "cores": threads,
+                            "time_to_tol_ms": dtp * 1e3, "iterations": int(itp)}
         log(f"cpu baseline: {cpu}")
 
     achieved = head["sweep_GBs"]
