@@ -150,14 +150,15 @@ def bench_vector32(args):
 
 def bench_coeff16(args):
     """16 coefficient fields k = exp(g), g Gaussian random fields (sigma 1, squared-exponential,
-    length 0.1, seeds 0..15), on the 2D Q1 level-10 mesh; u_obs from the field of seed 1000.  One
-    step = forward batched solve + adjoint batched solve + dL/dk kernel (the solver built from k
-    beforehand; host assembly of the 16 hierarchies reported separately)."""
+    length 0.1, seeds 0..15), on the 2D Q1 level-10 mesh; u_obs from the field of seed 1000; loss
+    1/2 sum_j |u_j - u_obs|^2.  One step = one differentiable solve: forward (device assembly of all
+    levels from k, batched solve) and backward (the same assembly again, adjoint batched solve,
+    dL/dk kernel), through torch.autograd.  The persistent solver object is built once beforehand
+    (reported separately, not timed)."""
     import numpy as np
     import torch
     from feat3_amd import problems as pb
-    from feat3_amd.autograd import DiffusionProblem, cell_gradient
-    from feat3_amd.batch import BatchMGPCG  # noqa: F401
+    from feat3_amd.autograd import DiffusionProblem, cell_gradient, diffusion_solve
 
     K, lvl = 16, args.level if args.level != 7 else 10
     prob = DiffusionProblem(dim=2, level=lvl, inner=args.inner)
@@ -165,41 +166,42 @@ def bench_coeff16(args):
     kf = np.exp(np.stack([pb.gaussian_random_field(2, lvl, seed=s) for s in range(K)]))
     kobs = np.exp(pb.gaussian_random_field(2, lvl, seed=1000))
     t_grf = time.time() - t
+    kt = torch.from_numpy(kf).cuda()
     t = time.time()
-    s = prob.make_solver(kf)
+    with torch.no_grad():
+        uobs = diffusion_solve(torch.from_numpy(kobs[None, :].copy()).cuda(), prob)[:, 0].clone()
+        prob._solver = None          # the 16-field solver below
+        diffusion_solve(kt, prob)    # builds the persistent 16-field solver
+    torch.cuda.synchronize()
     t_setup = time.time() - t
-    log(f"setup coeff16: GRF {t_grf:.1f} s, 16 hierarchies + solver {t_setup:.1f} s, n={prob.nodes}")
-    so = prob.make_solver(kobs[None, :])
-    b1 = torch.from_numpy(prob.b[:, None].copy()).cuda()
-    uobs = torch.zeros_like(b1)
-    so.correct(uobs, b1)
-    so.done()
-    rhs = torch.from_numpy(np.repeat(prob.b[:, None], K, axis=1)).cuda()
-    U = torch.zeros_like(rhs)
-    Lam = torch.zeros_like(rhs)
-    bnd = torch.from_numpy(prob.bnd.astype(np.int64)).cuda()
+    log(f"setup coeff16: GRF {t_grf:.1f} s, solver objects {t_setup:.1f} s, n={prob.nodes}")
 
     def step():
-        U.zero_()
-        st1, it1 = s.correct(U, rhs)
-        g_u = U - uobs
-        g_u[bnd] = 0.0
-        Lam.zero_()
-        st2, it2 = s.correct(Lam, g_u)
-        g = cell_gradient(prob, U, Lam)
-        assert all(int(v) == 2 for v in st1) and all(int(v) == 2 for v in st2)
-        return it1, it2, g
+        k = kt.clone().requires_grad_(True)
+        u = diffusion_solve(k, prob)
+        it1 = prob.last_iters.copy()
+        loss = 0.5 * ((u - uobs[:, None]) ** 2).sum()
+        loss.backward()
+        it2 = prob.last_iters.copy()
+        return it1, it2, k.grad, u.detach()
     times, outs = _timed(step, args.steps, args.warmup)
-    it1, it2, g = outs[-1]
+    it1, it2, g, U = outs[-1]
     iters = float(np.sum(it1) + np.sum(it2))
     ms = 1e3 * float(np.mean(times))
+    s = prob._solver
     sweep_ms, sweep_b = s.time_kernel(0, 20)
     Ug, gg = U.cpu().numpy(), g.cpu().numpy()
-    uo = uobs.cpu().numpy()[:, 0]
-    ident = max(abs(kf[j] @ gg[:, j] + np.dot(Ug[:, j] - uo, Ug[:, j])) / abs(np.dot(Ug[:, j] - uo, Ug[:, j]))
+    uo = uobs.cpu().numpy()
+    ident = max(abs(kf[j] @ gg[j] + np.dot(Ug[:, j] - uo, Ug[:, j])) / abs(np.dot(Ug[:, j] - uo, Ug[:, j]))
                 for j in range(K))
-    # the gradient kernel alone
+    # the parts: device assembly of all levels + value refresh, and the gradient kernel alone
     torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(5):
+        prob.solver_for(kt)
+    torch.cuda.synchronize()
+    asm_ms = 1e3 * (time.perf_counter() - t) / 5
+    Lam = U.clone()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(20):
@@ -211,11 +213,11 @@ def bench_coeff16(args):
     head = dict(ms_per_step=ms, forward_iterations=[int(v) for v in it1], adjoint_iterations=[int(v) for v in it2],
                 dof_iter_per_s=prob.nodes * iters / (ms * 1e-3), sweep_us=sweep_ms * 1e3, sweep_bytes=sweep_b,
                 sweep_GBs=sweep_b / (sweep_ms * 1e-3) / 1e9, grad_kernel_us=grad_ms * 1e3,
-                grad_kernel_GBs=grad_bytes / (grad_ms * 1e-3) / 1e9, max_rel_identity_error=ident,
-                host_setup_s=t_setup, grf_s=t_grf)
+                grad_kernel_GBs=grad_bytes / (grad_ms * 1e-3) / 1e9, assembly_and_refresh_ms=asm_ms,
+                max_rel_identity_error=ident, solver_setup_s=t_setup, grf_s=t_grf)
     log(f"coeff16: {ms:.2f} ms/step, fwd it {min(it1)}..{max(it1)}, adj it {min(it2)}..{max(it2)}, "
-        f"sweep {head['sweep_us']:.1f} us {head['sweep_GBs']:.0f} GB/s, dL/dk {grad_ms * 1e3:.1f} us, identity {ident:.1e}")
-    s.done()
+        f"sweep {head['sweep_us']:.1f} us {head['sweep_GBs']:.0f} GB/s, assembly+refresh {asm_ms:.2f} ms, "
+        f"dL/dk {grad_ms * 1e3:.1f} us, identity {ident:.1e}")
     cpu = None
     if not args.no_cpu:
         from oracle import pyoracle as po
@@ -232,7 +234,8 @@ def bench_coeff16(args):
             "kernel": "batch k_sweep<B=1, per-system values>, finest level (16 fields)",
             "bytes_per_launch": head["sweep_bytes"], "avg_launch_us": head["sweep_us"]}
     cfg = {"workload": f"2D variable-coefficient diffusion Q1 level {lvl} ({prob.nodes} DOF), 16 coefficient fields: "
-                       f"forward + adjoint batched PCG(FP64) + MG V(4,4), {args.inner} V-cycle, rel. tol 1e-8, dL/dk kernel",
+                       f"differentiable solve (device assembly, forward + adjoint batched PCG(FP64) + MG V(4,4), "
+                       f"{args.inner} V-cycle, rel. tol 1e-8, dL/dk kernel)",
            "dofs": prob.nodes, "fields": K}
     return _common(args, head["dof_iter_per_s"], ms, args.inner,
                    "synthetic (seeded Gaussian random fields, f = 1)", cfg, roof, {"headline": head}, cpu)

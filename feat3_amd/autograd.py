@@ -3,7 +3,10 @@ u = 0 on the boundary, one solution per coefficient field, as a torch.autograd.F
 
 forward:  k (K, cells) FP64 cuda tensor of per-cell coefficients (cells x fastest) ->
           u (nodes, K): K systems A(k_j) u_j = b solved together by the batched device MG-PCG
-          (one multigrid hierarchy per field, coarse coefficients = mean of the children).
+          (one multigrid hierarchy per field, coarse coefficients = mean of the children).  The
+          level matrices are assembled on the GPU from k (feat3_assemble_laplace_fields) and
+          loaded into one persistent solver object (feat3_batch_set_values); the host assembly
+          path stays available (device_assembly=False).
 backward: dL/du (nodes, K) -> the adjoint solves A(k_j) lambda_j = filter(dL/du_j) on the same
           solver object (A is symmetric), then the HIP kernel dL/dk_e = -lambda_e^T E u_e
           (E = h^(d-2) K_ref, the element stiffness of one cell).
@@ -35,23 +38,88 @@ class DiffusionProblem:
     tol_rel: float = 1e-8
     max_iter: int = 1000
     load: int = 1             # 1: f = 1, 0: sine bubble
+    device_assembly: bool = True   # matrices of k assembled on the GPU (False: on the host)
 
     def __post_init__(self):
         H = pb.hierarchy(self.dim, self.degree, self.level, self.level_coarse)
+        self.H = H
         self.nodes = H.n
         self.bnd = H.levels[0].bnd
         self.b = pb.filtered_rhs(H, self.load)
         self.cells = (1 << self.level) ** self.dim
         h = 1.0 / (1 << self.level)
-        self.elem = np.ascontiguousarray(pb.ref_stiffness(self.dim, self.degree) * h ** (self.dim - 2))
+        self.ref = np.ascontiguousarray(pb.ref_stiffness(self.dim, self.degree))
+        self.elem = np.ascontiguousarray(self.ref * h ** (self.dim - 2))
         self.last_status = None
         self.last_iters = None
+        self._solver = None
+        self._graphs = None
 
     def make_solver(self, kfields: np.ndarray) -> BatchMGPCG:
+        """A solver for the fields assembled on the host (the reference's assembly step)."""
         Hs = pb.field_hierarchies(self.dim, self.level, kfields, self.level_coarse, self.degree)
         vals = [pb.stacked_values([H.levels[i].A for H in Hs]) for i in range(len(Hs[0].levels))]
         return BatchMGPCG(Hs[0], len(Hs), values=vals, shared=False, inner=self.inner,
                           tol_rel=self.tol_rel, max_iter=self.max_iter).init()
+
+    def assemble(self, k):
+        """Level values [entry][field] of A(k) on the device for every level (finest first):
+        the fields are coarsened level by level (mean of the children) and each level assembled by
+        feat3_assemble_laplace_fields, bit-identical to the host assembly."""
+        import torch
+        K = k.shape[0]
+        if self._graphs is None:
+            self._graphs = [(torch.from_numpy(L.A.row_ptr.view(np.int32)).cuda(),
+                             torch.from_numpy(L.A.col_ind.view(np.int32)).cuda()) for L in self.H.levels]
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        kl = k.detach().contiguous()
+        out = []
+        for i, L in enumerate(self.H.levels):
+            lvl = self.level - i
+            if i > 0:
+                kc = torch.empty((K, (1 << lvl) ** self.dim), dtype=torch.float64, device=k.device)
+                capi.call("feat3_coarsen_cellfields", stream, self.dim, lvl + 1, K, ctypes.c_void_p(kl.data_ptr()),
+                          ctypes.c_void_p(kc.data_ptr()))
+                kl = kc
+            val = torch.empty(L.A.nnz * K, dtype=torch.float64, device=k.device)
+            rp, ci = self._graphs[i]
+            capi.call("feat3_assemble_laplace_fields", stream, self.dim, self.degree, lvl, K,
+                      ctypes.c_void_p(self.ref.ctypes.data), ctypes.c_void_p(kl.data_ptr()),
+                      ctypes.c_void_p(rp.data_ptr()), ctypes.c_void_p(ci.data_ptr()), L.A.rows,
+                      ctypes.c_void_p(val.data_ptr()))
+            out.append(val)
+        return out
+
+    def rhs_for(self, K: int, device):
+        """The filtered load for K systems, [node][system], kept on the device."""
+        import torch
+        key = (K, str(device))
+        if getattr(self, "_rhs_key", None) != key:
+            self._rhs = torch.from_numpy(np.ascontiguousarray(np.repeat(self.b[:, None], K, axis=1))).to(device)
+            self._rhs_key = key
+        return self._rhs
+
+    def bnd_index(self, device):
+        import torch
+        if getattr(self, "_bnd_dev", None) is None or self._bnd_dev.device != torch.device(device):
+            self._bnd_dev = torch.from_numpy(self.bnd.astype(np.int64)).to(device)
+        return self._bnd_dev
+
+    def solver_for(self, k) -> BatchMGPCG:
+        """The batched solver holding A(k) for the (K, cells) cuda tensor k."""
+        if not self.device_assembly:
+            return self.make_solver(k.detach().cpu().numpy())
+        import torch
+        K = k.shape[0]
+        if self._solver is None or self._solver.nsys != K:
+            vals = [np.ascontiguousarray(np.repeat(L.A.val[:, None], K, axis=1)) for L in self.H.levels]
+            self._solver = BatchMGPCG(self.H, K, values=vals, shared=False, inner=self.inner,
+                                      tol_rel=self.tol_rel, max_iter=self.max_iter).init()
+        vals = self.assemble(k)
+        torch.cuda.current_stream().synchronize()
+        for i, v in enumerate(vals):
+            capi.call("feat3_batch_set_values", self._solver._h, i, ctypes.c_void_p(v.data_ptr()))
+        return self._solver
 
 
 def _solve(torch, prob: DiffusionProblem, solver: BatchMGPCG, rhs):
@@ -71,20 +139,21 @@ def _make_function():
         def forward(ctx, k, prob: DiffusionProblem):
             if k.dim() != 2 or k.shape[1] != prob.cells or k.dtype != torch.float64 or not k.is_cuda:
                 raise ValueError("k must be a (K, cells) float64 cuda tensor")
-            solver = prob.make_solver(k.detach().cpu().numpy())
-            K = k.shape[0]
-            rhs = torch.from_numpy(np.repeat(prob.b[:, None], K, axis=1)).to(k.device)
+            solver = prob.solver_for(k)
+            rhs = prob.rhs_for(k.shape[0], k.device)
             u = _solve(torch, prob, solver, rhs)
             ctx.prob, ctx.solver = prob, solver
-            ctx.save_for_backward(u)
+            ctx.save_for_backward(u, k.detach())
             return u
 
         @staticmethod
         def backward(ctx, gu):
-            (u,) = ctx.saved_tensors
+            u, k = ctx.saved_tensors
             prob, solver = ctx.prob, ctx.solver
+            if prob.device_assembly:
+                solver = prob.solver_for(k)   # the shared solver may hold another k by now
             rhs = gu.detach().to(torch.float64).contiguous().clone()
-            rhs[torch.from_numpy(prob.bnd.astype(np.int64)).to(rhs.device)] = 0.0
+            rhs[prob.bnd_index(rhs.device)] = 0.0
             lam = _solve(torch, prob, solver, rhs)
             return cell_gradient(prob, u, lam).t().contiguous(), None
 

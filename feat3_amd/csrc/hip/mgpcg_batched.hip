@@ -134,7 +134,49 @@ namespace
     virtual int history(int k, double* out, u64 cap, u64* len) = 0;
     virtual int time_kernel(int which, int reps, double* ms, u64* bytes) = 0;
     virtual bool set_tolerances(const Config& c) = 0;
+    virtual int set_values(int level, const double* dvals) = 0;
   };
+
+  // device-side refresh of a level's values (feat3_batch_set_values): storage conversion and
+  // the Jacobi diagonal with the host setup's arithmetic (diagonal rounded to the storage type,
+  // dinv = omega_R * (omega_J / diag) in the inner type, 0 on Dirichlet nodes)
+  template<typename VT>
+  __global__ void k_to_storage(u64 n, const double* __restrict__ src, VT* __restrict__ dst)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      dst[i] = VT(float(src[i]));
+  }
+  template<>
+  __global__ void k_to_storage<double>(u64 n, const double* __restrict__ src, double* __restrict__ dst)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      dst[i] = src[i];
+  }
+
+  template<typename DT, typename VT, int B>
+  __global__ void k_batch_dinv(u64 nodes, u64 kv, const u32* __restrict__ rp, const u32* __restrict__ col,
+                               const double* __restrict__ val, const u32* __restrict__ mask, DT omega_j, DT omega_r,
+                               DT* __restrict__ dinv)
+  {
+    const u64 total = nodes * u64(B) * kv;
+    for(u64 t = u64(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += u64(gridDim.x) * blockDim.x)
+    {
+      const u64 k = t % kv;
+      const u64 nh = t / kv;
+      const u64 node = nh / u64(B);
+      const u64 hh = nh % u64(B);
+      DT d = DT(0);
+      for(u32 e = rp[node]; e < rp[node + 1]; ++e)
+        if(u64(col[e]) == node)
+        {
+          const double v = val[(u64(e) * u64(B * B) + hh * u64(B) + hh) * kv + k];
+          if constexpr(sizeof(VT) == 8) d = DT(v);
+          else d = DT(VT(float(v)));
+          break;
+        }
+      dinv[t] = dev::masked(mask, node) ? DT(0) : omega_r * (omega_j / d);
+    }
+  }
 
   template<int N>
   using ic = std::integral_constant<int, N>;
@@ -203,6 +245,7 @@ namespace
     double* x_cur = nullptr;
     double* x_graph = nullptr;
     std::vector<u64> last_iters;
+    HostLevel coarse_host;   // graph and Dirichlet list of the coarsest level (coarse inverses)
 
     explicit Engine(int k) : K(k)
     {
@@ -303,7 +346,18 @@ namespace
         L.push_back(std::move(l));
       }
       // FP64 inverses of the unit-filtered coarse matrices
-      const HostLevel& hc = lv.back();
+      coarse_host = lv.back();
+      F3_RC(coarse_inverses(coarse_host.A.val));
+      for(std::size_t i = 0; i + 1 < L.size(); ++i)
+        L[i]->xfinal = ((cfg.pre - 1 + cfg.post) % 2 == 0) ? L[i]->x.template as<DT>() : L[i]->t.template as<DT>();
+      L.back()->xfinal = L.back()->x.template as<DT>();
+      return init_outer(lv[0]);
+    }
+
+    // FP64 inverses of the unit-filtered coarse matrices from the coarsest level's values
+    int coarse_inverses(const std::vector<double>& cval)
+    {
+      const HostLevel& hc = coarse_host;
       ncoarse_rows = hc.nodes * B;
       const u64 mc = ncoarse_rows;
       if(mc > 4096) return rt::kErrArg;
@@ -317,7 +371,7 @@ namespace
             for(int hh = 0; hh < B; ++hh)
               for(int ww = 0; ww < B; ++ww)
                 a[(node * B + hh) * mc + u64(hc.A.col[e]) * B + ww] +=
-                  hc.A.val[(u64(e) * vpe() + u64(hh) * B + u64(ww)) * ninv + k];
+                  cval[(u64(e) * vpe() + u64(hh) * B + u64(ww)) * ninv + k];
         for(u32 f : hc.fidx)
           for(int hh = 0; hh < B; ++hh)
           {
@@ -329,12 +383,15 @@ namespace
         std::copy(a.begin(), a.end(), inv_all.begin() + k * mc * mc);
       }
       F3_RC(ainv.put(inv_all, s));
-      for(std::size_t i = 0; i + 1 < L.size(); ++i)
-        L[i]->xfinal = ((cfg.pre - 1 + cfg.post) % 2 == 0) ? L[i]->x.template as<DT>() : L[i]->t.template as<DT>();
-      L.back()->xfinal = L.back()->x.template as<DT>();
+      F3_CHECK(hipStreamSynchronize(s));   // inv_all goes out of scope
+      return rt::kOk;
+    }
+
+    int init_outer(const HostLevel& l0)
+    {
       // outer FP64
-      m0 = lv[0].nodes * B;
-      F3_RC(a64.put(lv[0].A.val, s));
+      m0 = l0.nodes * B;
+      F3_RC(a64.put(l0.A.val, s));
       const u64 nv = m0 * u64(K);
       F3_RC(r.alloc(sizeof(double) * nv)); F3_RC(p.alloc(sizeof(double) * nv)); F3_RC(q.alloc(sizeof(double) * nv));
       F3_RC(st.alloc(sizeof(batch::SysState) * K));
@@ -572,6 +629,32 @@ namespace
       return rt::kOk;
     }
 
+    int set_values(int level, const double* dv) override
+    {
+      if(level < 0 || level >= int(L.size()) || !dv) return rt::kErrArg;
+      Level& l = *L[level];
+      const u64 nval = l.nnz * vals_per_entry();
+      const unsigned nb = unsigned(std::max<u64>(1, std::min<u64>((nval + 255) / 256, 65536)));
+      k_to_storage<VT><<<nb, 256, 0, s>>>(nval, dv, l.aval.template as<VT>());
+      const u64 kv = SHARED ? 1 : u64(K);
+      const u64 nd = l.nodes * B * kv;
+      const unsigned nbd = unsigned(std::max<u64>(1, std::min<u64>((nd + 255) / 256, 65536)));
+      k_batch_dinv<DT, VT, B><<<nbd, 256, 0, s>>>(l.nodes, kv, l.arp.template as<u32>(), l.acol.template as<u32>(), dv,
+                                                   l.mask.template as<u32>(), DT(cfg.omega_j), DT(cfg.omega_r),
+                                                   l.dinv.template as<DT>());
+      F3_RC(rt::check_launch());
+      if(level == 0) F3_CHECK(hipMemcpyAsync(a64.p, dv, sizeof(double) * nval, hipMemcpyDeviceToDevice, s));
+      if(level + 1 == int(L.size()))
+      {
+        std::vector<double> cval(nval);
+        F3_CHECK(hipMemcpyAsync(cval.data(), dv, sizeof(double) * nval, hipMemcpyDeviceToHost, s));
+        F3_CHECK(hipStreamSynchronize(s));
+        F3_RC(coarse_inverses(cval));
+      }
+      F3_CHECK(hipStreamSynchronize(s));
+      return rt::kOk;
+    }
+
     // which: 0 finest smoothing sweep (all systems), 2 outer FP64 q = A p with dot, 3 one V-cycle
     int time_kernel(int which, int reps, double* ms, u64* bytes) override
     {
@@ -755,6 +838,12 @@ int feat3_batch_correct(feat3_batch* h, void* caller_stream, double* x, const do
   if(!h || !h->eng) return rt::kErrState;
   if(caller_stream && hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime;
   return h->eng->solve(x, b, status, iters);
+}
+
+int feat3_batch_set_values(feat3_batch* h, int level, const double* values)
+{
+  if(!h || !h->eng) return rt::kErrState;
+  return h->eng->set_values(level, values);
 }
 
 int feat3_batch_history(feat3_batch* h, int system, double* out, u64 cap, u64* len)

@@ -360,6 +360,11 @@ int feat3_batch_init(feat3_batch* h);
 /* x, b: FP64 device arrays [node][component][system], 16-byte aligned; status, iters: host
  * arrays of nsys */
 int feat3_batch_correct(feat3_batch* h, void* caller_stream, double* x, const double* b, int* status, uint64_t* iters);
+/* replaces the values of one level (0 = finest) from a device array in the layout of push_level
+ * (FP64; [entry][h][w] or [entry][system]): converts to the storage precision, recomputes the
+ * Jacobi diagonal, the FP64 outer copy (level 0) and the coarse inverses (coarsest level; a
+ * small device-to-host copy); graph, transfers and Dirichlet nodes are kept.  Blocks until done. */
+int feat3_batch_set_values(feat3_batch* h, int level, const double* values);
 /* defect norms of system `system` in the last solve */
 int feat3_batch_history(feat3_batch* h, int system, double* out_host, uint64_t cap, uint64_t* len);
 /* which: 0 finest smoothing sweep (all systems), 2 outer FP64 q = A p with q.p partials,
@@ -377,6 +382,20 @@ int feat3_batch_time_kernel(feat3_batch* h, int which, int reps, double* avg_ms,
 /* [node][system]; grad: device FP64 [cell][system], cells x fastest; enqueued on stream   */
 int feat3_adjoint_cell_gradient(void* stream, int dim, int degree, int level, int nsys, const double* elem_matrix,
                                 const double* u, const double* lam, double* grad);
+
+/* ===================================================================================== */
+/* Device-side assembly (the step before the hot path, for many coefficient fields): the   */
+/* coefficient-weighted Laplace values of feat3_setup_laplace, bit-identical, and the      */
+/* coarsening of cell fields of feat3_setup_coarsen_cellfield                              */
+/* ===================================================================================== */
+/* dim 2 (degree 1, 2) or 3 (degree 1); ref_stiffness: host array (feat3_setup_ref_stiffness);
+ * kfields: device [field][cell] (cells x fastest); row_ptr/col_ind: device graph of
+ * feat3_setup_laplace on this level (n rows); val: device [entry][field] */
+int feat3_assemble_laplace_fields(void* stream, int dim, int degree, int level, int nfields, const double* ref_stiffness,
+                                  const double* kfields, const uint32_t* row_ptr, const uint32_t* col_ind, uint64_t n,
+                                  double* val);
+/* kc[field][coarse cell] = mean of the 2^dim children on level level_fine (device arrays) */
+int feat3_coarsen_cellfields(void* stream, int dim, int level_fine, int nfields, const double* kf, double* kc);
 
 /* ===================================================================================== */
 /* Host-side problem setup (the step before the hot path: assembly, geometry)         */
