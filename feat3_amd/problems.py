@@ -290,3 +290,28 @@ def to_banded(A: Csr, max_diag: int = 64):
     val = np.zeros(offs.size * rows)
     val[d * rows + r] = A.val
     return (offs + rows - 1).astype(np.uint32), val
+
+
+# -------------------------------------------------------------------------------------------
+# the reference's on-disk matrix formats (Matrix Market, binary fm_csr)
+# -------------------------------------------------------------------------------------------
+def read_matrix(path: str) -> Csr:
+    """A CSR matrix from a Matrix Market file (.mtx) or the reference's binary fm_csr file."""
+    mtx = path.endswith(".mtx")
+    rows, cols, nnz = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    capi.call("feat3_io_mtx_sizes" if mtx else "feat3_io_csr_binary_sizes", path.encode(),
+              ctypes.byref(rows), ctypes.byref(cols), ctypes.byref(nnz))
+    rp = np.empty(rows.value + 1, U32)
+    col = np.empty(nnz.value, U32)
+    val = np.empty(nnz.value, F64)
+    capi.call("feat3_io_read_mtx" if mtx else "feat3_io_read_csr_binary", path.encode(), _p(rp), _p(col), _p(val))
+    return Csr(rows.value, cols.value, rp, col, val)
+
+
+def write_matrix(path: str, A: Csr, symmetric: bool = False) -> None:
+    """Writes A as Matrix Market (.mtx; symmetric: lower triangle only) or binary fm_csr."""
+    if path.endswith(".mtx"):
+        capi.call("feat3_io_write_mtx", path.encode(), A.rows, A.cols, _p(A.row_ptr), _p(A.col_ind), _p(A.val),
+                  int(symmetric))
+    else:
+        capi.call("feat3_io_write_csr_binary", path.encode(), A.rows, A.cols, _p(A.row_ptr), _p(A.col_ind), _p(A.val))

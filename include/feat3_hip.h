@@ -410,6 +410,19 @@ int feat3_setup_transpose(uint64_t rows, uint64_t cols, const uint32_t* row_ptr,
                           const double* val, uint32_t* t_row_ptr, uint32_t* t_col_ind, double* t_val);
 int feat3_setup_coarsen_cellfield(int dim, int level_fine, const double* kf, double* kc);
 
+/* The reference's on-disk formats of SparseMatrixCSR (host, u32 indices): Matrix Market
+ * coordinate real general/symmetric (sparse_matrix_csr.hpp:1125-1240, 1296-1340) and the binary
+ * fm_csr container (Container::_serialise<double, uint64_t>, container.hpp:320-447).  *_sizes
+ * reads the dimensions, read fills caller arrays (row_ptr rows + 1, col_ind and val nnz). */
+int feat3_io_mtx_sizes(const char* path, uint64_t* rows, uint64_t* cols, uint64_t* nnz);
+int feat3_io_read_mtx(const char* path, uint32_t* row_ptr, uint32_t* col_ind, double* val);
+int feat3_io_write_mtx(const char* path, uint64_t rows, uint64_t cols, const uint32_t* row_ptr, const uint32_t* col_ind,
+                       const double* val, int symmetric);
+int feat3_io_csr_binary_sizes(const char* path, uint64_t* rows, uint64_t* cols, uint64_t* nnz);
+int feat3_io_read_csr_binary(const char* path, uint32_t* row_ptr, uint32_t* col_ind, double* val);
+int feat3_io_write_csr_binary(const char* path, uint64_t rows, uint64_t cols, const uint32_t* row_ptr,
+                              const uint32_t* col_ind, const double* val);
+
 /* CSR -> SELL-C-sigma on the host (generalises SparseMatrixELL::convert,
  * lafem/sparse_matrix_ell.hpp:865-960): rows sorted by length inside windows of sigma rows,
  * perm[p] = row at position p, padding slots repeat a column of their row with value 0, and
