@@ -109,6 +109,10 @@ namespace feat3_amd
 
   namespace LAFEM
   {
+    // the file modes of lafem/base.hpp:31-46 (same values); this backend reads and writes fm_mtx
+    // and fm_csr
+    enum class FileMode { fm_exp = 0, fm_dv, fm_mtx, fm_ell, fm_csr, fm_coo, fm_bm, fm_dm, fm_sv, fm_dvb, fm_bcsr, fm_cscr, fm_binary };
+
     // ---- typed dispatch onto the C ABI ------------------------------------------------------
     namespace Arch
     {
@@ -294,6 +298,36 @@ namespace feat3_amd
         _rp.upload(rp.data());
         _col.upload(ci.data());
         _val.upload(v.data());
+      }
+      // from a file in the reference's formats (SparseMatrixCSR(FileMode, String),
+      // sparse_matrix_csr.hpp:1125-1240): fm_mtx (Matrix Market) or fm_csr (binary container)
+      SparseMatrixCSR(FileMode mode, const std::string& filename)
+      {
+        if(mode != FileMode::fm_mtx && mode != FileMode::fm_csr)
+          throw InternalError("SparseMatrixCSR(file): only fm_mtx and fm_csr are supported");
+        std::uint64_t r = 0, c = 0, z = 0;
+        const bool mtx = mode == FileMode::fm_mtx;
+        hip_check(mtx ? feat3_io_mtx_sizes(filename.c_str(), &r, &c, &z)
+                      : feat3_io_csr_binary_sizes(filename.c_str(), &r, &c, &z), "SparseMatrixCSR(file)");
+        std::vector<std::uint32_t> rp(r + 1), ci(z);
+        std::vector<double> v(z);
+        hip_check(mtx ? feat3_io_read_mtx(filename.c_str(), rp.data(), ci.data(), v.data())
+                      : feat3_io_read_csr_binary(filename.c_str(), rp.data(), ci.data(), v.data()), "SparseMatrixCSR(file)");
+        *this = SparseMatrixCSR(Index(r), Index(c), rp, ci, v);
+      }
+      // write_out(FileMode, String) (sparse_matrix_csr.hpp:1266-1340)
+      void write_out(FileMode mode, const std::string& filename, bool symmetric = false) const
+      {
+        if(mode != FileMode::fm_mtx && mode != FileMode::fm_csr)
+          throw InternalError("SparseMatrixCSR::write_out: only fm_mtx and fm_csr are supported");
+        const std::vector<IT_> rp = _rp.download(), ci = _col.download();
+        const std::vector<DT_> v = _val.download();
+        std::vector<std::uint32_t> rp32(rp.begin(), rp.end()), ci32(ci.begin(), ci.end());
+        std::vector<double> v64(v.begin(), v.end());
+        hip_check(mode == FileMode::fm_mtx
+                    ? feat3_io_write_mtx(filename.c_str(), _rows, _cols, rp32.data(), ci32.data(), v64.data(), symmetric ? 1 : 0)
+                    : feat3_io_write_csr_binary(filename.c_str(), _rows, _cols, rp32.data(), ci32.data(), v64.data()),
+                  "SparseMatrixCSR::write_out");
       }
       Index rows() const { return _rows; }
       Index columns() const { return _cols; }

@@ -199,6 +199,25 @@ int main(int argc, char** argv)
       // reported in units of 1e-16 of the largest entry, as an integer
       std::cout << "banded_vs_csr " << (long long)(std::ceil(dmax / cmax * 1e16)) << " " << B.num_of_offsets() << std::endl;
     }
+    {
+      // the reference's file formats: write the finest matrix as fm_csr and fm_mtx (symmetric),
+      // read both back into device matrices and compare their products with the original's
+      const DeviceLevel<double>::Matrix A(h0.n, h0.n, h0.rp, h0.col, h0.val);
+      A.write_out(LAFEM::FileMode::fm_csr, outdir + "/A.csr");
+      A.write_out(LAFEM::FileMode::fm_mtx, outdir + "/A.mtx", true);
+      const DeviceLevel<double>::Matrix Ab(LAFEM::FileMode::fm_csr, outdir + "/A.csr");
+      const DeviceLevel<double>::Matrix Am(LAFEM::FileMode::fm_mtx, outdir + "/A.mtx");
+      std::vector<double> xs(h0.n);
+      for(Index i = 0; i < h0.n; ++i) xs[i] = std::cos(0.11 * double(i));
+      LAFEM::DenseVector<Mem::HIP, double, u32> xv(xs), r0(h0.n, 0.0), rb(h0.n, 0.0), rm(h0.n, 0.0);
+      A.apply(r0, xv);
+      Ab.apply(rb, xv);
+      Am.apply(rm, xv);
+      const std::vector<double> h0v = r0.to_host(), hbv = rb.to_host(), hmv = rm.to_host();
+      const bool same = Ab.used_elements() == A.used_elements() && Am.used_elements() == A.used_elements() &&
+                        h0v == hbv && h0v == hmv;
+      std::cout << "file_roundtrip " << int(same) << " " << Ab.used_elements() << std::endl;
+    }
     std::cout << "leaked " << MemoryPool<Mem::HIP>::finalise() << std::endl;
   }
   catch(const std::exception& e)

@@ -51,4 +51,5 @@ def test_cpp_application_level_solvers(tmp_path, torch_cuda):
     assert res["banded_vs_csr"][0] <= 1000 and res["banded_vs_csr"][1] == 9   # within 1e-13 of max |entry|
     assert res["mixed32"][0] == 2 and abs(res["mixed32"][1] - io) <= 1
     assert rel(load("mixed32")) <= 1e-8
+    assert res["file_roundtrip"][0] == 1   # fm_csr and symmetric fm_mtx round trips, bitwise products
     assert res["leaked"] == [0]
