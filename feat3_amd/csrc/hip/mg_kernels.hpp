@@ -273,94 +273,129 @@ namespace mg
 
   // ---- coarse tail: all small levels in one workgroup ----------------------------------------
   //
-  // The levels below a size threshold cost one launch each per step (~4-5 us) while holding a
-  // few thousand rows.  k_coarse_tail runs the same V-cycle steps for levels tail[0] (the
-  // finest small level, whose first smoothing step was written by the restriction above it)
-  // down to the coarsest, with block barriers between the steps, in one 1024-thread
-  // workgroup.  Matrices are read in CSR (4 lanes per row).
+  // Below a few thousand rows a level step is a launch of ~4 us that moves a few KB.  k_tail
+  // runs every V-cycle step of the levels tail[0] (the finest small level) down to the coarsest
+  // and back in one 1024-thread workgroup with barriers between the steps.  All vectors of these
+  // levels live in LDS (x, its ping-pong partner, rhs, defect, dinv: 5 n values per level);
+  // the matrices are ELL with 16-bit columns (entry k of row r at k * n + r, padded with column
+  // 0 and value 0), so a step issues one round of independent global loads (L2-resident
+  // after the first V-cycle) and gathers from LDS.  The arithmetic of every step is that of
+  // k_sweep / k_defect / k_restrict / k_prolong / k_coarse; only the order of the terms of a
+  // row sum may differ from the level format's.
   constexpr int kTailMax = 12;
   constexpr int kTailThreads = 1024;
-  constexpr int kTailG = 4;
+
+  template<typename VT_>
+  struct TailEll
+  {
+    const std::uint16_t* col;
+    const VT_* val;
+    u32 n, w;   // rows, entries per row (padded)
+  };
 
   template<typename DT_, typename VT_>
   struct TailLevel
   {
-    u64 n;
-    const u32* arp; const u32* acol; const VT_* aval;
-    const u32* prp; const u32* pcol; const VT_* pval;   // to the next coarser tail level
-    const u32* rrp; const u32* rcol; const VT_* rval;
+    u32 n, off;             // rows; offset of the level's vectors in the LDS arena (DT_ elements)
+    TailEll<VT_> A, P, R;   // P: this level <- next coarser; R: next coarser <- this level
     const u32* mask;
     const DT_* dinv;
-    DT_* rhs; DT_* x; DT_* t; DT_* def;
   };
 
   template<typename DT_, typename VT_>
   struct TailDesc
   {
     int nlev, pre, post;
-    u64 ncoarse;
+    u32 ncoarse;
     const double* ainv;
+    const DT_* rhs_in;   // right-hand side of level tail[0], written by the restriction above
+    DT_* x_out;          // its solution after post-smoothing, read by the prolongation above
     TailLevel<DT_, VT_> lv[kTailMax];
   };
 
-  // calls f(row, row_sum) for every row of a CSR matrix, by the whole block
-  template<typename DT_, typename VT_, typename F>
-  __device__ __forceinline__ void tail_rows(u64 rows, const u32* rp, const u32* col, const VT_* val,
-                                            const DT_* __restrict__ x, F&& f)
+  template<typename DT_, typename VT_>
+  __device__ __forceinline__ DT_ tail_row(const TailEll<VT_>& M, u32 r, const DT_* xs)
   {
-    const CsrRows<DT_, VT_, kTailG> A{rp, col, val, rows};
-    const int lane = int(threadIdx.x % kTailG);
-    for(u64 p = threadIdx.x / kTailG; p < rows; p += kTailThreads / kTailG)
+    DT_ s = DT_(0);
+    const u64 n = M.n;
+#pragma unroll 9
+    for(u32 k = 0; k < M.w; ++k)
     {
-      const DT_ s = A.row_sum(p, lane, x);
-      if(lane == 0) f(p, s);
+      const u64 at = u64(k) * n + r;
+      s += dev::ld<DT_>(M.val, at) * xs[M.col[at]];
     }
+    return s;
   }
 
   template<typename DT_, typename VT_>
-  __global__ __launch_bounds__(1024) void k_coarse_tail(TailDesc<DT_, VT_> d, const int* status)
+  __global__ __launch_bounds__(kTailThreads) void k_tail(TailDesc<DT_, VT_> d, const int* status)
   {
     if(stopped(status)) return;
-    DT_* cur[kTailMax];
-    DT_* oth[kTailMax];
+    extern __shared__ double tail_arena_[];
+    DT_* arena = reinterpret_cast<DT_*>(tail_arena_);
+    const u32 tid = threadIdx.x;
+    // vectors of level l at arena + off: [0,n) x0, [n,2n) x1, [2n,3n) rhs, [3n,4n) def, [4n,5n) dinv
+    for(int i = 0; i < d.nlev; ++i)
+    {
+      const TailLevel<DT_, VT_>& l = d.lv[i];
+      DT_* v = arena + l.off;
+      for(u32 r = tid; r < l.n; r += kTailThreads)
+      {
+        const DT_ di = l.dinv[r];
+        v[4 * l.n + r] = di;
+        if(i == 0)
+        {
+          // the first smoothing step from x = 0 (as the restriction above writes it)
+          const DT_ b = d.rhs_in[r];
+          v[2 * l.n + r] = b;
+          v[r] = di * b;
+        }
+      }
+    }
+    __syncthreads();
+    // buffer (0: x0, 1: x1) holding a level's iterate after pre-smoothing and after post-smoothing
+    const u32 after_pre = u32(d.pre - 1) & 1u, after_post = u32(d.pre - 1 + d.post) & 1u;
     for(int i = 0; i + 1 < d.nlev; ++i)
     {
       const TailLevel<DT_, VT_>& f = d.lv[i];
       const TailLevel<DT_, VT_>& c = d.lv[i + 1];
-      cur[i] = f.x;
-      oth[i] = f.t;
+      DT_* fv = arena + f.off;
+      DT_* cv = arena + c.off;
+      const u32 n = f.n;
       for(int k = 1; k < d.pre; ++k)
       {
-        DT_* xi = cur[i];
-        DT_* xo = oth[i];
-        tail_rows<DT_, VT_>(f.n, f.arp, f.acol, f.aval, xi, [&](u64 r, DT_ s) { xo[r] = xi[r] + f.dinv[r] * (f.rhs[r] - s); });
+        const DT_* xi = fv + (u32(k - 1) & 1u) * n;
+        DT_* xo = fv + (u32(k) & 1u) * n;
+        for(u32 r = tid; r < n; r += kTailThreads)
+          xo[r] = xi[r] + fv[4 * n + r] * (fv[2 * n + r] - tail_row<DT_, VT_>(f.A, r, xi));
         __syncthreads();
-        cur[i] = xo;
-        oth[i] = xi;
       }
-      tail_rows<DT_, VT_>(f.n, f.arp, f.acol, f.aval, cur[i], [&](u64 r, DT_ s) {
-        f.def[r] = dev::masked(f.mask, r) ? DT_(0) : f.rhs[r] - s;
-      });
+      const DT_* xcur = fv + after_pre * n;
+      for(u32 r = tid; r < n; r += kTailThreads)
+      {
+        const DT_ s = tail_row<DT_, VT_>(f.A, r, xcur);
+        fv[3 * n + r] = dev::masked(f.mask, r) ? DT_(0) : fv[2 * n + r] - s;
+      }
       __syncthreads();
       const bool first_step = i + 2 < d.nlev;
-      tail_rows<DT_, VT_>(c.n, f.rrp, f.rcol, f.rval, f.def, [&](u64 r, DT_ s) {
+      for(u32 r = tid; r < c.n; r += kTailThreads)
+      {
+        const DT_ s = tail_row<DT_, VT_>(f.R, r, fv + 3 * n);
         const DT_ v = dev::masked(c.mask, r) ? DT_(0) : s;
-        c.rhs[r] = v;
-        if(first_step) c.x[r] = c.dinv[r] * v;
-      });
+        cv[2 * c.n + r] = v;
+        if(first_step) cv[r] = cv[4 * c.n + r] * v;
+      }
       __syncthreads();
     }
     {
-      // exact coarse solve with the FP64 inverse
+      // exact coarse solve with the FP64 inverse of the filtered coarsest matrix
       const TailLevel<DT_, VT_>& c = d.lv[d.nlev - 1];
-      __shared__ double sr[1024];
-      for(u64 j = threadIdx.x; j < d.ncoarse; j += blockDim.x) sr[j] = double(c.rhs[j]);
-      __syncthreads();
-      for(u64 i = threadIdx.x; i < d.ncoarse; i += blockDim.x)
+      DT_* cv = arena + c.off;
+      for(u32 i = tid; i < d.ncoarse; i += kTailThreads)
       {
         double s = 0.0;
-        for(u64 j = 0; j < d.ncoarse; ++j) s += d.ainv[i * d.ncoarse + j] * sr[j];
-        c.x[i] = dev::masked(c.mask, i) ? DT_(0) : DT_(s);
+        for(u32 j = 0; j < d.ncoarse; ++j) s += d.ainv[u64(i) * d.ncoarse + j] * double(cv[2 * c.n + j]);
+        cv[i] = dev::masked(c.mask, i) ? DT_(0) : DT_(s);
       }
       __syncthreads();
     }
@@ -368,22 +403,28 @@ namespace mg
     {
       const TailLevel<DT_, VT_>& f = d.lv[i];
       const TailLevel<DT_, VT_>& c = d.lv[i + 1];
-      const DT_* xc = (i + 1 == d.nlev - 1) ? c.x : cur[i + 1];
-      DT_* xf = cur[i];
-      tail_rows<DT_, VT_>(f.n, f.prp, f.pcol, f.pval, xc, [&](u64 r, DT_ s) {
-        if(!dev::masked(f.mask, r)) xf[r] = xf[r] + s;
-      });
+      DT_* fv = arena + f.off;
+      const DT_* xc = arena + c.off + (i + 2 == d.nlev ? 0u : after_post * c.n);
+      const u32 n = f.n;
+      DT_* xcur = fv + after_pre * n;
+      for(u32 r = tid; r < n; r += kTailThreads)
+      {
+        const DT_ s = tail_row<DT_, VT_>(f.P, r, xc);
+        if(!dev::masked(f.mask, r)) xcur[r] = xcur[r] + s;
+      }
       __syncthreads();
       for(int k = 0; k < d.post; ++k)
       {
-        DT_* xi = cur[i];
-        DT_* xo = oth[i];
-        tail_rows<DT_, VT_>(f.n, f.arp, f.acol, f.aval, xi, [&](u64 r, DT_ s) { xo[r] = xi[r] + f.dinv[r] * (f.rhs[r] - s); });
+        const DT_* xi = fv + ((after_pre + u32(k)) & 1u) * n;
+        DT_* xo = fv + ((after_pre + u32(k) + 1u) & 1u) * n;
+        for(u32 r = tid; r < n; r += kTailThreads)
+          xo[r] = xi[r] + fv[4 * n + r] * (fv[2 * n + r] - tail_row<DT_, VT_>(f.A, r, xi));
         __syncthreads();
-        cur[i] = xo;
-        oth[i] = xi;
       }
     }
+    const TailLevel<DT_, VT_>& t = d.lv[0];
+    const DT_* xf = arena + t.off + after_post * t.n;
+    for(u32 r = tid; r < t.n; r += kTailThreads) d.x_out[r] = xf[r];
   }
 
   // ---- setup kernels -----------------------------------------------------------------------

@@ -353,10 +353,9 @@ namespace
     int lanes = 0;            // forced lanes per CSR row (0 = by mean row length)
     bool index16 = true;      // SELL columns as 16-bit row offsets where they fit
     bool compact_transfer = true; // device P/R without the stored structural zeros
-    // levels with at most this many rows run in the single-workgroup coarse-tail kernel; off
-    // by default: on MI355X one CU stays latency-bound even at 27..729 rows and is slower
-    // than one full-chip launch per step
-    u64 tail_max_n = 0;
+    // levels with at most this many rows (and whose vectors fit in one workgroup's LDS) run in
+    // the single-workgroup coarse-tail kernel k_tail; 0 = every level keeps its own launches
+    u64 tail_max_n = 1100;
     u64 rows_per_block = 0;   // row positions per block of the row kernels (0: by row_grid)
     // SELL sort window of the transfer matrices: prolongation rows have 1/2/4/8 entries in an
     // alternating pattern, so sorting by length inside windows removes most of the padding
@@ -497,7 +496,7 @@ namespace
         L[i]->xfinal = (flips % 2 == 0) ? L[i]->x.template as<DT>() : L[i]->t.template as<DT>();
       }
       L.back()->xfinal = L.back()->x.template as<DT>();
-      setup_tail();
+      F3_RC(setup_tail(lv));
 
       // outer PCG
       const HostLevel& h0 = lv.front();
@@ -540,38 +539,99 @@ namespace
     const int* status_ptr() const { return reinterpret_cast<const int*>(&st()->status); }
 
     // ---- V-cycle -------------------------------------------------------------------------
-    // first level of the coarse tail (levels t0..nl-1 run in one workgroup); nl - 1 when
-    // only the coarse solve is left to a single kernel
+    // coarse tail: levels tail0..nl-1 run in one workgroup (k_tail) when tail0 < nl - 1;
+    // tail_lds bytes of LDS hold their vectors
     std::size_t tail0 = 0;
     mg::TailDesc<DT, VT> tail{};
+    u64 tail_lds = 0;
+    std::vector<std::unique_ptr<DevMem>> tail_mem;
 
-    void setup_tail()
+    // ELL copy for the tail kernel: 16-bit columns, exact zeros dropped, entry k of row r at
+    // k * rows + r, padded with column 0 and value 0
+    int tail_ell(const HostCsr& h, mg::TailEll<VT>& e)
+    {
+      u64 w = 0;
+      for(u64 r = 0; r < h.rows; ++r)
+      {
+        u64 c = 0;
+        for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k) c += h.val[k] != 0.0 ? 1 : 0;
+        w = std::max(w, c);
+      }
+      std::vector<std::uint16_t> col(w * h.rows, 0);
+      std::vector<VT> val(w * h.rows, to_storage<VT>(0.0));
+      for(u64 r = 0; r < h.rows; ++r)
+      {
+        u64 j = 0;
+        for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
+          if(h.val[k] != 0.0)
+          {
+            col[j * h.rows + r] = std::uint16_t(h.col[k]);
+            val[j * h.rows + r] = to_storage<VT>(h.val[k]);
+            ++j;
+          }
+      }
+      std::unique_ptr<DevMem> mc(new DevMem()), mv(new DevMem());
+      F3_RC(mc->alloc(sizeof(std::uint16_t) * col.size()));
+      F3_RC(mv->alloc(sizeof(VT) * val.size()));
+      if(!col.empty())
+      {
+        F3_CHECK(hipMemcpyAsync(mc->p, col.data(), sizeof(std::uint16_t) * col.size(), hipMemcpyHostToDevice, s));
+        F3_CHECK(hipMemcpyAsync(mv->p, val.data(), sizeof(VT) * val.size(), hipMemcpyHostToDevice, s));
+      }
+      F3_CHECK(hipStreamSynchronize(s));   // host staging vectors go out of scope
+      e = mg::TailEll<VT>{mc->template as<std::uint16_t>(), mv->template as<VT>(), u32(h.rows), u32(w)};
+      tail_mem.push_back(std::move(mc));
+      tail_mem.push_back(std::move(mv));
+      return rt::kOk;
+    }
+
+    int setup_tail(const std::vector<HostLevel>& lv)
     {
       const std::size_t nl = L.size();
+      tail_mem.clear();
+      int dev = 0, cap = 65536;
+      if(hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+      auto arena = [&](std::size_t t0) {
+        u64 b = 0;
+        for(std::size_t i = t0; i < nl; ++i) b += 5 * L[i]->n * sizeof(DT);
+        return b;
+      };
       tail0 = nl - 1;
-      while(tail0 > 0 && L[tail0 - 1]->n <= cfg.tail_max_n && nl - (tail0 - 1) <= std::size_t(mg::kTailMax))
+      while(tail0 > 0 && L[tail0 - 1]->n <= cfg.tail_max_n && L[tail0 - 1]->n < 65536 &&
+            nl - (tail0 - 1) <= std::size_t(mg::kTailMax) && arena(tail0 - 1) <= u64(cap))
         --tail0;
       tail = mg::TailDesc<DT, VT>{};
+      tail_lds = 0;
+      if(tail0 + 1 >= nl) return rt::kOk;
       tail.nlev = int(nl - tail0);
       tail.pre = cfg.pre;
       tail.post = cfg.post;
-      tail.ncoarse = ncoarse;
+      tail.ncoarse = u32(ncoarse);
       tail.ainv = ainv.template as<double>();
+      tail.rhs_in = L[tail0]->rhs.template as<DT>();
+      tail.x_out = L[tail0]->xfinal;
+      u64 off = 0;
       for(std::size_t i = tail0; i < nl; ++i)
       {
         Level<DT, VT>& l = *L[i];
         mg::TailLevel<DT, VT>& t = tail.lv[i - tail0];
-        t.n = l.n;
-        t.arp = l.A.rp.template as<u32>(); t.acol = l.A.col.template as<u32>(); t.aval = l.A.val.template as<VT>();
+        t.n = u32(l.n);
+        t.off = u32(off);
+        off += 5 * l.n;
+        F3_RC(tail_ell(lv[i].A, t.A));
         if(i + 1 < nl)
         {
-          t.prp = l.P.rp.template as<u32>(); t.pcol = l.P.col.template as<u32>(); t.pval = l.P.val.template as<VT>();
-          t.rrp = l.R.rp.template as<u32>(); t.rcol = l.R.col.template as<u32>(); t.rval = l.R.val.template as<VT>();
+          F3_RC(tail_ell(lv[i].P, t.P));
+          F3_RC(tail_ell(lv[i].R, t.R));
         }
         t.mask = l.mask.template as<u32>();
         t.dinv = l.dinv.template as<DT>();
-        t.rhs = l.rhs.template as<DT>(); t.x = l.x.template as<DT>(); t.t = l.t.template as<DT>(); t.def = l.def.template as<DT>();
       }
+      tail_lds = off * sizeof(DT);
+      if(tail_lds > 65536)
+        F3_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&mg::k_tail<DT, VT>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(tail_lds)));
+      return rt::kOk;
     }
 
     int enqueue_vcycle(const double* def, const int* status)
@@ -596,14 +656,16 @@ namespace
           mg::k_defect<<<g.blocks, kBlock, 0, s>>>(A, g.per, f.mask.template as<u32>(), f.rhs.template as<DT>(),
                                                    (const DT*)cur[i], f.def.template as<DT>(), status);
         }));
-        DT* xc = (i + 2 < nl) ? c.x.template as<DT>() : nullptr;
+        // the restriction also takes the next level's first smoothing step, unless that level
+        // is the coarsest or the first of the tail (the tail kernel takes it itself)
+        DT* xc = (i + 1 < tail0) ? c.x.template as<DT>() : nullptr;
         F3_RC(with_rows<DT, VT>(f.R, [&](auto R, mg::RowGrid g) {
           mg::k_restrict<<<g.blocks, kBlock, 0, s>>>(R, g.per, c.mask.template as<u32>(), f.def.template as<DT>(),
                                                      c.rhs.template as<DT>(), xc, c.dinv.template as<DT>(), status);
         }));
       }
       if(tail0 + 1 < nl)
-        mg::k_coarse_tail<DT, VT><<<1, mg::kTailThreads, 0, s>>>(tail, status);
+        mg::k_tail<DT, VT><<<1, mg::kTailThreads, tail_lds, s>>>(tail, status);
       else
       {
         Level<DT, VT>& cl = *L.back();
@@ -836,7 +898,7 @@ namespace
     // iteration excluding the V-cycle, [8] sizeof inner DT, [9] sizeof VT
     int info(u64* out, int cap) override
     {
-      std::vector<u64> v(14, 0);
+      std::vector<u64> v(16, 0);
       v[0] = L.size();
       v[1] = L.front()->n;
       v[2] = L.front()->A.nnz;
@@ -866,7 +928,9 @@ namespace
       v[11] = L.front()->A.sell ? (L.front()->n + L.front()->A.C - 1) / L.front()->A.C : 0;
       v[12] = L.front()->A.banded ? 1 : 0;
       v[13] = L.front()->A.ndiag;
-      for(int i = 0; i < cap && i < 14; ++i) out[i] = v[i];
+      v[14] = tail0 + 1 < L.size() ? L.size() - tail0 : 0;
+      v[15] = tail_lds;
+      for(int i = 0; i < cap && i < 16; ++i) out[i] = v[i];
       return rt::kOk;
     }
   };

@@ -121,8 +121,9 @@ class MGPCG:
         return ms.value, nb.value
 
     def info(self) -> dict:
-        out = np.zeros(14, np.uint64)
-        capi.call("feat3_mgpcg_info", self._h, _ptr(out), 14)
+        out = np.zeros(16, np.uint64)
+        capi.call("feat3_mgpcg_info", self._h, _ptr(out), 16)
         keys = ["levels", "n", "nnz", "lanes", "sell_size", "kernels_per_vcycle", "vcycle_bytes",
-                "outer_bytes", "sizeof_inner", "sizeof_value", "index16", "sell_chunks", "banded", "ndiag"]
+                "outer_bytes", "sizeof_inner", "sizeof_value", "index16", "sell_chunks", "banded", "ndiag",
+                "tail_levels", "tail_lds"]
         return {k: int(v) for k, v in zip(keys, out)}

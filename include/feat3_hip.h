@@ -300,7 +300,8 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
 /* named launch/format options (replaces the globals of lafem/tuning.hpp:72-163 that the
  * reference's block-size tuner sets): "index16" 0/1 (SELL columns as 16-bit row offsets where
  * they fit), "compact_transfer" 0/1 (device P/R without stored structural zeros), "sell_C",
- * "sell_sigma", "lanes", "tail_max_n", "rows_per_block" (row positions per workgroup of the
+ * "sell_sigma", "lanes", "tail_max_n" (levels of at most this many rows run in one LDS-resident
+ * workgroup; 0 = none; default 1100), "rows_per_block" (row positions per workgroup of the
  * level-matrix kernels, a multiple of 64; 0 = derived from the row count), "transfer_sigma" (SELL
  * sort window of the transfer matrices, default 1); takes effect at the next init */
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value);
@@ -320,7 +321,8 @@ int feat3_mgpcg_history(feat3_mgpcg* h, double* out_host, uint64_t cap, uint64_t
 int feat3_mgpcg_time_kernel(feat3_mgpcg* h, int which, int reps, double* avg_ms, uint64_t* bytes);
 /* [levels, n, nnz, lanes, sell size, kernels per V-cycle, V-cycle bytes, outer bytes,
  *  sizeof inner, sizeof stored value, 16-bit SELL columns on the finest level, SELL chunks
- *  of the finest level, finest level banded (0/1), its number of diagonals] */
+ *  of the finest level, finest level banded (0/1), its number of diagonals, levels run by the
+ *  single-workgroup coarse tail (0: none), LDS bytes of the tail] */
 int feat3_mgpcg_info(feat3_mgpcg* h, uint64_t* out, int cap);
 
 /* ===================================================================================== */

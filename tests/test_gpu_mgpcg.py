@@ -194,3 +194,30 @@ def test_banded_format_matches_oracle(torch, dim, deg, lvl, inner):
     else:
         _, im, _, _ = solve_orc(H, b, inner={"f32": 1, "f16": 2}[inner])
         assert abs(ig - im) <= 1 and rel(xg, xo) <= 1e-8
+
+
+@pytest.mark.parametrize("dim,deg,lvl,tail_levels", [(3, 1, 5, 3), (2, 2, 5, 4), (2, 1, 6, 5), (2, 1, 4, 4)])
+@pytest.mark.parametrize("inner", ["f64", "f32", "f16"])
+def test_coarse_tail_matches_per_level_launches(torch, dim, deg, lvl, tail_levels, inner):
+    """The single-workgroup coarse tail (levels of at most 1100 rows, vectors in LDS, ELL
+    matrices) takes the same V-cycle steps as the per-level launches: FP64 identical iteration
+    count and solutions within 1e-12 (only the order of a row's terms differs); FP32 / FP16-stored
+    V-cycles identical iteration count and within 1e-8 of the FP64 oracle solution.  On 2D Q1 L4
+    every level fits, so the tail holds the whole hierarchy."""
+    print(f"case tail {dim}D Q{deg} L{lvl} {inner}", flush=True)
+    from feat3_amd.solver import MGPCG
+    H = pb.hierarchy(dim, deg, lvl, 1)
+    b = pb.filtered_rhs(H, 0)
+    s = MGPCG(H, inner=inner).init()
+    info = s.info()
+    s.done()
+    assert info["tail_levels"] == tail_levels, info
+    assert info["tail_lds"] == 5 * {"f64": 8, "f32": 4, "f16": 4}[inner] * sum(L.A.rows for L in H.levels[-tail_levels:])
+    st0, it0, x0, _ = solve_gpu(torch, H, b, inner=inner, options=dict(tail_max_n=0))
+    st1, it1, x1, _ = solve_gpu(torch, H, b, inner=inner)
+    assert st0 == 2 and st1 == 2 and it1 == it0
+    if inner == "f64":
+        assert rel(x1, x0) <= 1e-12
+    else:
+        _, _, xo, _ = solve_orc(H, b)
+        assert rel(x1, xo) <= 1e-8
