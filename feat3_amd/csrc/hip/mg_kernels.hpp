@@ -453,7 +453,7 @@ namespace mg
     double tol_rel, tol_abs, tol_abs_low, div_rel, div_abs, stag_rate;
     u64 num_iter, min_iter, max_iter, min_stag_iter, num_stag_iter, hist_cap;
     int status;
-    int pad;
+    int first;   // set with the initial defect: the next beta finalisation starts the recurrence
   };
 
   // _set_initial_defect (iterative.hpp:744-766)
@@ -462,6 +462,7 @@ namespace mg
     st->def_init = st->def_cur = st->def_prev = d;
     st->num_iter = 0;
     st->num_stag_iter = 0;
+    st->first = 1;
     if(st->hist_cap > 0) hist[0] = d;
     const double eps = 2.220446049250313e-16;
     if(!isfinite(d)) st->status = kAborted;
@@ -615,6 +616,13 @@ namespace mg
     else if(OP == kFinAlpha) st->alpha = st->gamma / t;                     // pcg.hpp:240
     else if(OP == kFinNewDefect) set_new_defect(st, hist, sqrt(t));         // iterative.hpp:848-879
     else if(OP == kFinGamma) st->gamma = t;                                 // pcg.hpp:229
+    else if(st->first)
+    {
+      // the initial preconditioner application: gamma = r.z, p = z (pcg.hpp:225-231)
+      st->gamma = t;
+      st->beta = 0.0;
+      st->first = 0;
+    }
     else { const double g2 = st->gamma; st->gamma = t; st->beta = t / g2; } // pcg.hpp:273-279
   }
 

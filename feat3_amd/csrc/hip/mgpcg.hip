@@ -400,16 +400,18 @@ namespace
     DevMem r, p, q, zcg, state, hist, partials;
     const DevCsr<double>* A64 = nullptr;
     u64 n0 = 0, hist_cap = 0;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
-    int* pinned = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    // an iteration is captured as two graphs: [0] outer half (q = A p, alpha, x and r update,
+    // stopping criterion), [1] preconditioner half (z = M r, beta, p update)
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    static constexpr int kRing = 4;            // host copies of the solver state in flight
+    mg::PcgState* pinned = nullptr;
+    hipEvent_t ev[kRing] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t t0 = nullptr, t1 = nullptr;
 
     ~Engine() override
     {
-      if(gexec) (void)hipGraphExecDestroy(gexec);
-      if(graph) (void)hipGraphDestroy(graph);
+      drop_graphs();
       if(pinned) (void)hipHostFree(pinned);
       for(auto& e : ev) if(e) (void)hipEventDestroy(e);
       if(t0) (void)hipEventDestroy(t0);
@@ -525,14 +527,22 @@ namespace
       // 8192 blocks (mg::row_grid), the streaming ones at most 4096
       const u64 max_blocks = 8192;
       F3_RC(partials.alloc(sizeof(double) * max_blocks));
-      if(!pinned) F3_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), 64, hipHostMallocDefault));
+      if(!pinned) F3_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), sizeof(mg::PcgState) * kRing, hipHostMallocDefault));
       for(auto& e : ev) if(!e) F3_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       if(!t0) F3_CHECK(hipEventCreate(&t0));
       if(!t1) F3_CHECK(hipEventCreate(&t1));
       F3_CHECK(hipStreamSynchronize(s));
-      if(gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
-      if(graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+      drop_graphs();
       return rt::kOk;
+    }
+
+    void drop_graphs()
+    {
+      for(int i = 0; i < 2; ++i)
+      {
+        if(gexec[i]) { (void)hipGraphExecDestroy(gexec[i]); gexec[i] = nullptr; }
+        if(graph[i]) { (void)hipGraphDestroy(graph[i]); graph[i] = nullptr; }
+      }
     }
 
     mg::PcgState* st() const { return state.template as<mg::PcgState>(); }
@@ -736,7 +746,9 @@ namespace
       mg::k_pcg_fin<OP><<<1, kBlock, 0, s>>>(partials.template as<double>(), nb, st(), hist.template as<double>());
     }
 
-    int enqueue_iteration()
+    // outer half of a PCG iteration (pcg.hpp:236-262): q = A p, alpha, x += alpha p,
+    // r -= alpha q, the new defect and the stopping criterion
+    int enqueue_outer_half()
     {
       mg::PcgState* S = st();
       double* part = partials.template as<double>();
@@ -750,6 +762,14 @@ namespace
       mg::k_pcg_update_norm<<<stream_blocks(n0), kBlock, 0, s>>>(n0, p.template as<double>(), q.template as<double>(),
                                                                 x_cur, r.template as<double>(), S, part);
       fin<mg::kFinNewDefect>(stream_blocks(n0));
+      return rt::check_launch();
+    }
+
+    // preconditioner half (pcg.hpp:264-285): z = M r, gamma, beta, p = z + beta p
+    int enqueue_precond_half()
+    {
+      mg::PcgState* S = st();
+      double* part = partials.template as<double>();
       F3_RC(enqueue_precond(status_ptr()));
       F3_RC(with_z([&](auto z) {
         mg::k_pcg_gamma_beta<<<stream_blocks(n0), kBlock, 0, s>>>(n0, z, r.template as<double>(), S, part);
@@ -774,6 +794,27 @@ namespace
       mg::PcgState* S = st();
       double* part = partials.template as<double>();
       x_cur = x;
+      if(cfg.use_graph && (gexec[0] == nullptr || x_graph != x))
+      {
+        drop_graphs();
+        for(int half = 0; half < 2; ++half)
+        {
+          F3_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          const int rc = half == 0 ? enqueue_outer_half() : enqueue_precond_half();
+          hipGraph_t g = nullptr;
+          const hipError_t e = hipStreamEndCapture(s, &g);
+          if(rc) { if(g) (void)hipGraphDestroy(g); return rc; }
+          if(e != hipSuccess) return rt::kErrRuntime;
+          graph[half] = g;
+          F3_CHECK(hipGraphInstantiate(&gexec[half], graph[half], nullptr, nullptr, 0));
+        }
+        x_graph = x;
+      }
+      auto launch_half = [&](int half) -> int {
+        if(cfg.use_graph) F3_CHECK(hipGraphLaunch(gexec[half], s));
+        else F3_RC(half == 0 ? enqueue_outer_half() : enqueue_precond_half());
+        return rt::kOk;
+      };
       if(correct)
       {
         unsigned nb = 0;
@@ -791,44 +832,62 @@ namespace
         fin<mg::kFinInitDefect>(stream_blocks(n0));
       }
       F3_RC(rt::check_launch());
-      F3_RC(enqueue_precond(status_ptr()));
-      F3_RC(with_z([&](auto z) {
-        mg::k_pcg_p_gamma<<<stream_blocks(n0), kBlock, 0, s>>>(n0, z, r.template as<double>(), p.template as<double>(), S, part);
-        fin<mg::kFinGamma>(stream_blocks(n0));
-      }));
-      F3_RC(rt::check_launch());
+      // z = M r, gamma = r.z, p = z: the preconditioner half with beta = 0 (state flag "first")
+      F3_RC(launch_half(1));
 
-      if(cfg.use_graph && (gexec == nullptr || x_graph != x))
-      {
-        if(gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
-        if(graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
-        F3_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_iteration();
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(s, &g);
-        if(rc) { if(g) (void)hipGraphDestroy(g); return rc; }
-        if(e != hipSuccess) return rt::kErrRuntime;
-        graph = g;
-        F3_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-        x_graph = x;
-      }
-      // host loop: launch iteration k+1 before looking at the status of iteration k
-      const u64 max_launch = cfg.max_iter + 1;
-      u64 launched = 0;
+      // host copy of the solver state after iteration k (0: after the initial defect)
+      auto post_state = [&](u64 k) -> int {
+        F3_CHECK(hipMemcpyAsync(&pinned[k % kRing], S, sizeof(mg::PcgState), hipMemcpyDeviceToHost, s));
+        F3_CHECK(hipEventRecord(ev[k % kRing], s));
+        return rt::kOk;
+      };
+      std::vector<double> seen_def;   // defects of iterations 0..known, as read by the host
       int seen = mg::kProgress;
-      while(launched < max_launch)
+      // reads the state of the next iteration not read yet (states are read in order)
+      auto read_state = [&]() -> int {
+        const u64 k = seen_def.size();
+        F3_CHECK(hipEventSynchronize(ev[k % kRing]));
+        const mg::PcgState& h = pinned[k % kRing];
+        seen = h.status;
+        seen_def.push_back(h.def_cur);
+        return rt::kOk;
+      };
+      // Host loop.  Far from convergence an iteration is launched whole while the previous
+      // one runs, and the state of the one before is read (one iteration in flight ahead).
+      // When the defect history read so far predicts that iteration k may meet the criterion,
+      // only its outer half is launched and its state read before the preconditioner half, so
+      // that no iteration of skipped kernels (each kernel checks the status and returns) is
+      // queued behind the last one.
+      const u64 max_launch = cfg.max_iter + 1;
+      F3_RC(post_state(0));
+      u64 k = 0;   // iterations launched
+      while(k < max_launch && seen == mg::kProgress)
       {
-        if(cfg.use_graph) F3_CHECK(hipGraphLaunch(gexec, s));
-        else F3_RC(enqueue_iteration());
-        F3_CHECK(hipMemcpyAsync(&pinned[launched % 2], &S->status, sizeof(int), hipMemcpyDeviceToHost, s));
-        F3_CHECK(hipEventRecord(ev[launched % 2], s));
-        ++launched;
-        if(launched >= 2)
+        const u64 next = k + 1;
+        bool near = next >= cfg.max_iter;
+        if(seen_def.size() >= 2)
         {
-          F3_CHECK(hipEventSynchronize(ev[(launched - 2) % 2]));
-          seen = pinned[(launched - 2) % 2];
-          if(seen != mg::kProgress) break;
+          const std::size_t j = seen_def.size() - 1;   // latest iteration read
+          const double d0 = seen_def[0], dj = seen_def[j], dp = seen_def[j - 1];
+          const double rate = dp > 0.0 ? dj / dp : 1.0;
+          const double goal = std::min(cfg.tol_abs, std::max(cfg.tol_rel * d0, cfg.tol_abs_low));
+          const double pred = dj * std::pow(std::min(rate, 1.0), double(next - j));
+          near = near || pred <= 10.0 * goal;
         }
+        F3_RC(launch_half(0));
+        F3_RC(post_state(next));
+        if(near)
+        {
+          // read every state up to this iteration's, then decide on its second half
+          while(seen_def.size() <= next && seen == mg::kProgress) F3_RC(read_state());
+          if(seen != mg::kProgress) { k = next; break; }
+          F3_RC(launch_half(1));
+          k = next;
+          continue;
+        }
+        F3_RC(launch_half(1));
+        k = next;
+        while(seen_def.size() < k && seen == mg::kProgress) F3_RC(read_state());
       }
       F3_CHECK(hipStreamSynchronize(s));
       mg::PcgState out{};
