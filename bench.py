@@ -241,6 +241,116 @@ def bench_coeff16(args):
                    "synthetic (seeded Gaussian random fields, f = 1)", cfg, roof, {"headline": head}, cpu)
 
 
+def bench_spmv(args):
+    """The drop-in Arch::Apply kernels (lafem_hip_apply_*: r = A x, beta = 0) and BLAS-1 at full
+    size: the scalar formats on the 3D Q1 level-7 Laplacian (2,146,689 rows, 57,066,625 nnz), the
+    blocked ones on the 3D Q1 level-6 vector Laplacian (274,625 block rows of 3x3 blocks).  Each
+    kernel: 20 launches between two HIP events on the launching stream after 3 warm-ups; bytes are
+    the algorithmic ones (unpadded nnz; x and r once), so padding and re-reads show as lost
+    efficiency.  One step = one pass over the whole table."""
+    import ctypes
+    import numpy as np
+    import torch
+    from feat3_amd import capi
+    from feat3_amd import problems as pb
+
+    lvl = args.level
+    t = time.time()
+    A, _ = pb.laplace(3, 1, lvl)
+    Av, _ = pb.laplace(3, 1, max(lvl - 1, 1))
+    log(f"setup spmv: n={A.rows}, nnz={A.nnz}; vector Laplacian {Av.rows} block rows ({time.time() - t:.1f} s)")
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    keep = []
+
+    def dev(a):
+        d = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+        keep.append(d)
+        return ctypes.c_void_p(d.data_ptr())
+
+    def timed(fn, reps=20):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3   # seconds per launch
+
+    n, nnz = A.rows, A.nnz
+    rng = np.random.default_rng(0)
+    rows = []
+
+    def add(name, fmt, dtype, bytes_, fn):
+        sec = timed(fn)
+        gbs = bytes_ / sec / 1e9
+        rows.append(dict(kernel=name, format=fmt, dtype=dtype, us=sec * 1e6, bytes=int(bytes_), GBs=gbs,
+                         frac_of_peak=gbs / HBM_PEAK_GBS))
+        log(f"{name:38s} {sec * 1e6:9.1f} us {bytes_ / 1e6:9.1f} MB {gbs:7.0f} GB/s  {gbs / HBM_PEAK_GBS:.3f} of peak")
+
+    for dt, npdt, s_v, tag in ((np.float64, "f64", 8, "f64"), (np.float32, "f32", 4, "f32")):
+        x = dev(rng.uniform(-1, 1, A.cols).astype(dt))
+        r = dev(np.zeros(n, dt))
+        val = dev(A.val.astype(dt))
+        col, rp = dev(A.col_ind), dev(A.row_ptr)
+        add(f"lafem_hip_apply_csr_{tag}_u32", "csr", tag, nnz * (s_v + 4) + (n + 1) * 4 + 2 * n * s_v,
+            lambda: capi.call(f"lafem_hip_apply_csr_{tag}_u32", sp, r, 1.0, x, 0.0, r, val, col, rp, n, A.cols, nnz, 0))
+        if tag == "f32":
+            hv = dev(A.val.astype(np.float16))
+            add("lafem_hip_apply_csr_h16f32_u32", "csr", "f16 values / f32", nnz * (2 + 4) + (n + 1) * 4 + 2 * n * 4,
+                lambda: capi.call("lafem_hip_apply_csr_h16f32_u32", sp, r, 1.0, x, 0.0, r, hv, col, rp, n, A.cols, nnz))
+        for C, sig in ((32, 1), (64, 64)):
+            cs, cl, perm, ecol, evals, rl = pb.to_sell(A, C, sig)
+            dcs, dcl, dcol, dval = dev(cs), dev(cl), dev(ecol), dev(evals.astype(dt))
+            nch = cl.size
+            if sig == 1:
+                add(f"lafem_hip_apply_ell_{tag}_u32 (C={C})", "ell", tag, nnz * (s_v + 4) + 8 * nch + 2 * n * s_v,
+                    lambda: capi.call(f"lafem_hip_apply_ell_{tag}_u32", sp, r, 1.0, x, 0.0, r, dval, dcol, dcs, dcl, C, n))
+            else:
+                drl, dperm = dev(rl), dev(perm[:n])
+                add(f"lafem_hip_apply_sell_{tag}_u32 (C={C}, s={sig})", "sell", tag,
+                    nnz * (s_v + 4) + 8 * nch + 8 * n + 2 * n * s_v,
+                    lambda: capi.call(f"lafem_hip_apply_sell_{tag}_u32", sp, r, 1.0, x, 0.0, r, dval, dcol, dcs, dcl, drl,
+                                      dperm, C, n))
+        offs, bval = pb.to_banded(A)
+        doff, dbv = dev(offs), dev(bval.astype(dt))
+        add(f"lafem_hip_apply_banded_{tag}_u32 ({offs.size} diag.)", "banded", tag, nnz * s_v + 4 * offs.size + 2 * n * s_v,
+            lambda: capi.call(f"lafem_hip_apply_banded_{tag}_u32", sp, r, 1.0, x, 0.0, r, dbv, doff, offs.size, n, A.cols))
+        # blocked: the vector Laplacian (3x3 blocks a_ij I) and the scalar matrix times 3-vectors
+        nb, nnzb = Av.rows, Av.nnz
+        xb = dev(rng.uniform(-1, 1, 3 * nb).astype(dt))
+        rb = dev(np.zeros(3 * nb, dt))
+        bv = dev(pb.block_identity_values(Av, 3).reshape(-1).astype(dt))
+        bcol, brp = dev(Av.col_ind), dev(Av.row_ptr)
+        add(f"lafem_hip_apply_csrb_{tag}_u32 (3x3)", "bcsr", tag, nnzb * (9 * s_v + 4) + (nb + 1) * 4 + 2 * 3 * nb * s_v,
+            lambda: capi.call(f"lafem_hip_apply_csrb_{tag}_u32", sp, rb, 1.0, xb, 0.0, rb, bv, bcol, brp, nb, nb, nnzb, 3, 3))
+        sv = dev(Av.val.astype(dt))
+        add(f"lafem_hip_apply_csrsb_{tag}_u32 (3)", "csrsb", tag, nnzb * (s_v + 4) + (nb + 1) * 4 + 2 * 3 * nb * s_v,
+            lambda: capi.call(f"lafem_hip_apply_csrsb_{tag}_u32", sp, rb, 1.0, xb, 0.0, rb, sv, bcol, brp, nb, nb, nnzb, 3))
+        y = dev(rng.uniform(-1, 1, n).astype(dt))
+        add(f"lafem_hip_axpy_{tag}", "blas1", tag, 3 * n * s_v,
+            lambda: capi.call(f"lafem_hip_axpy_{tag}", sp, r, 2.0, x, y, n))
+        res = dev(np.zeros(1, dt))
+        add(f"lafem_hip_dot_{tag} (device scalar)", "blas1", tag, 2 * n * s_v,
+            lambda: capi.call(f"lafem_hip_dot_{tag}", sp, x, y, n, res))
+        keep.clear()
+        torch.cuda.synchronize()
+    spmv = [r for r in rows if r["format"] not in ("blas1",)]
+    best = max(spmv, key=lambda r: r["GBs"])
+    head = next(r for r in rows if r["kernel"] == "lafem_hip_apply_csr_f64_u32")
+    roof = {"bound": "hbm", "achieved": head["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": head["frac_of_peak"],
+            "traffic": None, "kernel": head["kernel"] + " (3D Q1 level 7, r = A x)", "bytes_per_launch": head["bytes"],
+            "avg_launch_us": head["us"]}
+    cfg = {"workload": f"drop-in Arch::Apply SpMV and BLAS-1 kernels, 3D Q1 level {lvl} Laplacian ({n} rows, {nnz} nnz) "
+                       f"and the level-{max(lvl - 1, 1)} 3x3 vector Laplacian ({Av.rows} block rows)", "rows": n, "nnz": nnz}
+    total_ms = sum(r["us"] for r in rows) * 1e-3
+    return _common(args, head["GBs"], total_ms, "f64", "synthetic (assembled Laplacians, uniform[-1,1] vectors)", cfg, roof,
+                   {"unit_note": "value = GB/s of the FP64 CSR apply; kernels[] lists every entry point",
+                    "kernels": rows, "best_spmv": best["kernel"]}, None) | {"unit": "GB/s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,13 +363,14 @@ def main():
                     help="level-matrix format of the V-cycle (banded falls back to SELL-C-sigma where a matrix is not banded)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP64 / FP16 comparison solves")
-    ap.add_argument("--workload", default="poisson3d", choices=["poisson3d", "poisson2d", "vector32", "coeff16"],
+    ap.add_argument("--workload", default="poisson3d", choices=["poisson3d", "poisson2d", "vector32", "coeff16", "spmv"],
                     help="poisson3d: the headline 3D Q1 L7 solve; poisson2d: 2D Q2 L9 (1,050,625 DOF); vector32: 32 "
                          "right-hand sides of the 3x3-block vector Laplacian (3D Q1 L6) in one batched solve; coeff16: "
-                         "forward + adjoint solves and dL/dk of 16 coefficient fields (2D Q1 L10)")
+                         "forward + adjoint solves and dL/dk of 16 coefficient fields (2D Q1 L10); spmv: every drop-in "
+                         "Arch::Apply SpMV and BLAS-1 kernel at full size (GB/s against the HBM peak)")
     args = ap.parse_args()
     if args.level is None:
-        args.level = {"poisson3d": 7, "poisson2d": 9, "vector32": 6, "coeff16": 10}[args.workload]
+        args.level = {"poisson3d": 7, "poisson2d": 9, "vector32": 6, "coeff16": 10, "spmv": 7}[args.workload]
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -274,8 +385,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
-    if args.workload in ("vector32", "coeff16"):
-        out = (bench_vector32 if args.workload == "vector32" else bench_coeff16)(args)
+    if args.workload in ("vector32", "coeff16", "spmv"):
+        out = {"vector32": bench_vector32, "coeff16": bench_coeff16, "spmv": bench_spmv}[args.workload](args)
         print(json.dumps(out), flush=True)
         return
     dim, deg = (3, 1) if args.workload == "poisson3d" else (2, 2)

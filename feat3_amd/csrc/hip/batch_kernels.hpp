@@ -664,9 +664,18 @@ namespace batch
     __shared__ double red[256];
     const int k = int(threadIdx.x % 64);
     const unsigned slice = threadIdx.x / 64;
-    double s = 0.0;
+    // eight accumulators per thread: eight loads in flight (the partials come from memory)
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if(k < K)
-      for(unsigned b = slice; b < nb; b += blockDim.x / 64) s += partials[u64(b) * u64(K) + u64(k)];
+    {
+      const unsigned step = blockDim.x / 64;
+      unsigned b = slice;
+      for(; b + 7 * step < nb; b += 8 * step)
+#pragma unroll
+        for(unsigned j = 0; j < 8; ++j) acc[j] += partials[u64(b + j * step) * u64(K) + u64(k)];
+      for(; b < nb; b += step) acc[0] += partials[u64(b) * u64(K) + u64(k)];
+    }
+    const double s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     red[threadIdx.x] = s;
     __syncthreads();
     if(int(threadIdx.x) < K)

@@ -86,13 +86,23 @@ namespace dev
     if(threadIdx.x == 0) partials[blockIdx.x] = bs;
   }
 
-  // sum of partials[0, n) in a fixed order; valid in thread 0 (one block)
+  // sum of partials[0, n) in a fixed order; valid in thread 0 (one block).  Each thread keeps
+  // eight accumulators so that eight of its loads are in flight at once (the partials were
+  // written by the previous kernel and come from memory, not from this CU's cache).
   template<typename T>
   __device__ __forceinline__ T sum_partials(const T* partials, unsigned int n)
   {
-    T s = T(0);
-    for(unsigned int i = threadIdx.x; i < n; i += blockDim.x)
-      s += partials[i];
+    constexpr unsigned int U = 8;
+    T acc[U];
+#pragma unroll
+    for(unsigned int j = 0; j < U; ++j) acc[j] = T(0);
+    const unsigned int step = blockDim.x;
+    unsigned int i = threadIdx.x;
+    for(; i + (U - 1) * step < n; i += U * step)
+#pragma unroll
+      for(unsigned int j = 0; j < U; ++j) acc[j] += partials[i + j * step];
+    for(; i < n; i += step) acc[0] += partials[i];
+    const T s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     return block_sum(s);
   }
 

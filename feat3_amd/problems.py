@@ -292,6 +292,24 @@ def to_banded(A: Csr, max_diag: int = 64):
     return (offs + rows - 1).astype(np.uint32), val
 
 
+def to_sell(A: Csr, C: int = 32, sigma: int = 1):
+    """SELL-C-sigma arrays of a CSR matrix (feat3_sell_build; sigma = 1 is the reference's
+    SparseMatrixELL layout, sparse_matrix_ell.hpp:865-960): chunk starts cs, chunk widths cl,
+    row at each position perm, true row length at each position rl, columns, values."""
+    nch, size = ctypes.c_uint64(), ctypes.c_uint64()
+    capi.call("feat3_sell_sizes", A.rows, _p(A.row_ptr), C, sigma, ctypes.byref(nch), ctypes.byref(size))
+    cs = np.empty(nch.value + 1, U32)
+    cl = np.empty(max(nch.value, 1), U32)
+    perm = np.empty(max(A.rows, 1), U32)
+    col = np.empty(max(size.value, 1), U32)
+    val = np.empty(max(size.value, 1), F64)
+    used = ctypes.c_int()
+    capi.call("feat3_sell_build", A.rows, A.cols, _p(A.row_ptr), _p(A.col_ind), _p(A.val), C, sigma, _p(cs), _p(cl),
+              _p(perm), _p(col), _p(val), None, ctypes.byref(used))
+    rl = np.diff(A.row_ptr.astype(np.int64))[perm[: A.rows]].astype(U32)
+    return cs, cl, perm, col, val, rl
+
+
 # -------------------------------------------------------------------------------------------
 # the reference's on-disk matrix formats (Matrix Market, binary fm_csr)
 # -------------------------------------------------------------------------------------------

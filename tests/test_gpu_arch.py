@@ -90,6 +90,57 @@ def test_csr_apply_f64(T, itype, ab, shape):
     assert err <= 1e-13, err
 
 
+@pytest.mark.parametrize("itype", [np.uint32, np.uint64])
+def test_csr_apply_long_and_empty_rows(T, itype):
+    """Rows of very different lengths (empty, short, and a few with thousands of entries): the
+    streaming CSR kernel stages a block's products in LDS when its entries fit and sums rows
+    with 8 lanes each when they do not; both paths (and blocks mixing them) match the oracle."""
+    torch, capi = T
+    rng = np.random.default_rng(11)
+    rows, cols = 3000, 9000
+    lens = rng.integers(0, 4, rows)
+    lens[[5, 700, 701, 2999]] = [6000, 9000, 4500, 7000]
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(itype)
+    col = np.concatenate([np.sort(rng.choice(cols, size=int(k), replace=False)) for k in lens]).astype(itype)
+    val = rng.uniform(-1, 1, int(rp[-1]))
+    x = rng.uniform(-1, 1, cols)
+    y = rng.uniform(-1, 1, rows)
+    it = np.int64 if itype == np.uint64 else np.int32
+    suf = "u64" if itype == np.uint64 else "u32"
+    for a, b in [(1.0, 0.0), (-0.5, 2.0)]:
+        ref = y.copy()
+        po.csr_apply(ref, a, x, b, ref, val, col, rp, rows, cols)
+        dr = dev(torch, y.copy())
+        capi.call(f"lafem_hip_apply_csr_f64_{suf}", None, vp(dr), a, vp(dev(torch, x)), b, vp(dr), vp(dev(torch, val)),
+                  vp(dev(torch, col.astype(it))), vp(dev(torch, rp.astype(it))), rows, cols, int(rp[-1]), 0)
+        torch.cuda.synchronize()
+        assert rel(host(dr), ref) <= 1e-13
+    if suf != "u32":
+        return
+    col32, rp32 = col.astype(np.uint32), rp.astype(np.uint32)
+    # FP32 values and vectors
+    x32, v32 = x.astype(np.float32), val.astype(np.float32)
+    ref32 = y.astype(np.float32)
+    po.lib().orc_csr_apply_f32_u32(po.P(ref32), 1.0, po.P(x32), 0.0, po.P(ref32), po.P(v32), po.P(col32), po.P(rp32),
+                                   rows, cols, 0)
+    dr = dev(torch, y.astype(np.float32))
+    capi.call("lafem_hip_apply_csr_f32_u32", None, vp(dr), 1.0, vp(dev(torch, x32)), 0.0, vp(dr), vp(dev(torch, v32)),
+              vp(dev(torch, col.astype(np.int32))), vp(dev(torch, rp.astype(np.int32))), rows, cols, int(rp[-1]), 0)
+    torch.cuda.synchronize()
+    assert rel(host(dr), ref32) <= 1e-5
+    # csrsb with 3-vectors
+    x3 = rng.uniform(-1, 1, 3 * cols)
+    y3 = rng.uniform(-1, 1, 3 * rows)
+    ref = y3.copy()
+    po.lib().orc_csrsb_apply_f64_u32(po.P(ref), 1.5, po.P(x3), -1.0, po.P(ref), po.P(val), po.P(col32),
+                                     po.P(rp32), rows, 3)
+    dr = dev(torch, y3.copy())
+    capi.call("lafem_hip_apply_csrsb_f64_u32", None, vp(dr), 1.5, vp(dev(torch, x3)), -1.0, vp(dr), vp(dev(torch, val)),
+              vp(dev(torch, col.astype(np.int32))), vp(dev(torch, rp.astype(np.int32))), rows, cols, int(rp[-1]), 3)
+    torch.cuda.synchronize()
+    assert rel(host(dr), ref) <= 1e-13
+
+
 def test_csr_apply_transposed_and_f32(T):
     torch, capi = T
     rng = np.random.default_rng(7)
