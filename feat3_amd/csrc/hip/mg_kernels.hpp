@@ -46,6 +46,30 @@ namespace mg
     }
   };
 
+  // CSR evaluated by streaming: the rows of a block are processed in rounds of R consecutive
+  // rows whose entries [rp[r0], rp[r1]) are contiguous; the 256 threads read values and columns
+  // with unit stride (8 independent loads per thread in flight) and stage the products
+  // val * x[col] in LDS, then thread t adds up row r0 + t in entry order.  A round whose
+  // entries exceed the LDS capacity is summed with 8 lanes per row.  R is chosen on the host
+  // so that a round's entries fill about 85% of the capacity.  for_rows has the block-level
+  // loop; row_sum is not used.
+  template<typename DT_, typename VT_>
+  struct CsrStreamRows
+  {
+    static constexpr int G = 1;
+    static constexpr int kCapBytes = 16 * 1024;
+    static constexpr int CAP = kCapBytes / int(sizeof(DT_));
+    const u32* rp;
+    const u32* col;
+    const VT_* val;
+    u64 rows;
+    u64 R;
+    __device__ __forceinline__ u64 row(u64 p) const { return p; }
+  };
+
+  template<typename T> struct IsStream { static constexpr bool value = false; };
+  template<typename DT_, typename VT_> struct IsStream<CsrStreamRows<DT_, VT_>> { static constexpr bool value = true; };
+
   // column storage of the derived SELL format: absolute u32 indices, or 16-bit signed offsets
   // from the row index (usable when every |col - row| < 2^15, e.g. lexicographic 3D grids up
   // to 129^3 nodes); a padding slot holds offset 0 / value 0
@@ -163,6 +187,68 @@ namespace mg
     return g;
   }
 
+  // the block-level loop of CsrStreamRows (all threads of the block take part)
+  template<typename DT_, typename VT_, typename F>
+  __device__ __forceinline__ void for_rows_stream(const CsrStreamRows<DT_, VT_>& A, u64 beg, u64 end,
+                                                  const DT_* __restrict__ x, F&& f)
+  {
+    constexpr int CAP = CsrStreamRows<DT_, VT_>::CAP;
+    constexpr int U = 8;
+    constexpr unsigned NT = 256;
+    __shared__ DT_ prod[CAP];
+    for(u64 r0 = beg; r0 < end; r0 += A.R)
+    {
+      const u64 r1 = r0 + A.R < end ? r0 + A.R : end;
+      const u64 e0 = u64(A.rp[r0]), ne = u64(A.rp[r1]) - e0;
+      __syncthreads();   // the previous round's readers are done with prod
+      if(ne <= u64(CAP))
+      {
+        const u64 my = r0 + threadIdx.x;
+        const u64 mk0 = my < r1 ? u64(A.rp[my]) : 0, mk1 = my < r1 ? u64(A.rp[my + 1]) : 0;
+        for(u64 k0 = threadIdx.x; k0 < ne; k0 += u64(NT) * U)
+        {
+          DT_ v[U];
+          u64 c[U];
+#pragma unroll
+          for(int j = 0; j < U; ++j)
+          {
+            const u64 k = k0 + u64(j) * NT;
+            v[j] = k < ne ? DT_(dev::ld_stream(A.val + e0 + k)) : DT_(0);
+            c[j] = k < ne ? u64(dev::ld_stream(A.col + e0 + k)) : 0;
+          }
+          DT_ xv[U];
+#pragma unroll
+          for(int j = 0; j < U; ++j) xv[j] = k0 + u64(j) * NT < ne ? x[c[j]] : DT_(0);
+#pragma unroll
+          for(int j = 0; j < U; ++j)
+            if(k0 + u64(j) * NT < ne) prod[k0 + u64(j) * NT] = v[j] * xv[j];
+        }
+        __syncthreads();
+        for(u64 row = my; row < r1; row += NT)
+        {
+          const bool first = row == my;
+          const u64 k0 = (first ? mk0 : u64(A.rp[row])) - e0, k1 = (first ? mk1 : u64(A.rp[row + 1])) - e0;
+          DT_ s = DT_(0);
+          for(u64 k = k0; k < k1; ++k) s += prod[k];
+          f(row, s, true);
+        }
+      }
+      else
+      {
+        constexpr int G = 8;
+        const int lane = int(threadIdx.x % G);
+        for(u64 row = r0 + threadIdx.x / G; row < r1; row += NT / G)
+        {
+          DT_ s = DT_(0);
+          for(u64 i = u64(A.rp[row]) + u64(lane); i < u64(A.rp[row + 1]); i += G)
+            s += dev::ld<DT_>(A.val, i) * x[A.col[i]];
+          s = dev::group_sum<G>(s);
+          f(row, s, lane == 0);
+        }
+      }
+    }
+  }
+
   // calls f(position, row_sum, is_group_leader) for every row position of this block
   template<typename Rows_, typename DT_, typename F>
   __device__ __forceinline__ void for_rows(const Rows_& A, u64 per, const DT_* __restrict__ x, F&& f)
@@ -171,12 +257,17 @@ namespace mg
     const u64 b = dev::xcd_remap(blockIdx.x, gridDim.x);
     const u64 beg = b * per;
     const u64 end = beg + per < A.rows ? beg + per : A.rows;
-    const int lane = int(threadIdx.x % G);
-    const u64 ngrp = blockDim.x / G;
-    for(u64 p = beg + threadIdx.x / G; p < end; p += ngrp)
+    if constexpr(IsStream<Rows_>::value)
+      for_rows_stream(A, beg, end, x, f);
+    else
     {
-      const DT_ s = A.row_sum(p, lane, x);
-      f(p, s, lane == 0);
+      const int lane = int(threadIdx.x % G);
+      const u64 ngrp = blockDim.x / G;
+      for(u64 p = beg + threadIdx.x / G; p < end; p += ngrp)
+      {
+        const DT_ s = A.row_sum(p, lane, x);
+        f(p, s, lane == 0);
+      }
     }
   }
 

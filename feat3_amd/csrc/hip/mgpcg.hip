@@ -122,12 +122,9 @@ namespace
         F3_CHECK(hipMemcpyAsync(val.p, v.data(), sizeof(VT) * nnz, hipMemcpyHostToDevice, s));
       }
       F3_CHECK(hipStreamSynchronize(s)); // host staging vectors go out of scope
-      const double mean = rows ? double(nnz) / double(rows) : 1.0;
-      // about four entries per lane: enough independent loads in flight per lane while the
-      // group reduction stays short
-      lanes = 2;
-      while(lanes < 64 && double(lanes) * 4.0 < mean) lanes *= 2;
-      if(force_lanes > 0) lanes = force_lanes;
+      // lanes = 0: the streaming evaluator (CsrStreamRows); a forced lane count (tuner) selects
+      // the G-lanes-per-row evaluator instead
+      lanes = force_lanes > 0 ? force_lanes : 0;
       if(want_banded) F3_RC(build_banded(h, s));
       sell = want_sell && !banded;
       if(sell) F3_RC(build_sell(h, s, sell_C, sell_sigma, allow_d16));
@@ -277,6 +274,15 @@ namespace
     const VT* val = m.val.template as<VT>();
     switch(m.lanes)
     {
+      case 0:
+      {
+        // streaming evaluator: rounds of R rows filling about 85% of the LDS products
+        const double mean = m.rows ? double(m.nnz) / double(m.rows) : 1.0;
+        const double fit = 0.85 * double(mg::CsrStreamRows<DT, VT>::CAP) / (mean > 1.0 ? mean : 1.0);
+        const u64 R = fit < 1.0 ? 1 : (fit > 1024.0 ? 1024 : u64(fit));
+        f(mg::CsrStreamRows<DT, VT>{rp, col, val, m.rows, R}, grid(1));
+        break;
+      }
       case 2: f(mg::CsrRows<DT, VT, 2>{rp, col, val, m.rows}, grid(2)); break;
       case 4: f(mg::CsrRows<DT, VT, 4>{rp, col, val, m.rows}, grid(4)); break;
       case 8: f(mg::CsrRows<DT, VT, 8>{rp, col, val, m.rows}, grid(8)); break;

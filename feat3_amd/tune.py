@@ -22,7 +22,7 @@ from typing import Dict, List, Optional
 from .problems import Hierarchy
 
 FORMATS: List[dict] = (
-    [dict(fmt="banded")]
+    [dict(fmt="banded"), dict(fmt="csr")]
     + [dict(fmt="csr", lanes=L) for L in (4, 8, 16)]
     + [dict(fmt="sell", sell_C=C, sell_sigma=s) for C in (32, 64, 128, 256) for s in (1, 64)]
 )
