@@ -337,7 +337,18 @@ def bench_spmv(args):
             lambda: capi.call(f"lafem_hip_dot_{tag}", sp, x, y, n, res))
         keep.clear()
         torch.cuda.synchronize()
-    spmv = [r for r in rows if r["format"] not in ("blas1",)]
+    # attainable HBM rates on this GPU: a pure read (the norm of a 2 GiB vector) and a device copy
+    big = torch.empty(1 << 28, dtype=torch.float64, device="cuda").uniform_()
+    big2 = torch.empty_like(big)
+    res = torch.zeros(1, dtype=torch.float64, device="cuda")
+    pb_, pb2, pr = ctypes.c_void_p(big.data_ptr()), ctypes.c_void_p(big2.data_ptr()), ctypes.c_void_p(res.data_ptr())
+    nbig = big.numel()
+    add("calibration: read (lafem_hip_norm2_f64, 2 GiB)", "calibration", "f64", 8 * nbig,
+        lambda: capi.call("lafem_hip_norm2_f64", sp, pb_, nbig, pr))
+    add("calibration: copy (feat3_hip_copy, 2 GiB)", "calibration", "f64", 16 * nbig,
+        lambda: capi.call("feat3_hip_copy", sp, pb2, pb_, 8 * nbig))
+    del big, big2
+    spmv = [r for r in rows if r["format"] not in ("blas1", "calibration")]
     best = max(spmv, key=lambda r: r["GBs"])
     head = next(r for r in rows if r["kernel"] == "lafem_hip_apply_csr_f64_u32")
     roof = {"bound": "hbm", "achieved": head["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": head["frac_of_peak"],

@@ -126,7 +126,11 @@ namespace
       // the bounds of this thread's first row, loaded with the entries
       const u64 my = r0 + threadIdx.x;
       const u64 mk0 = my < r1 ? u64(rp[my]) : 0, mk1 = my < r1 ? u64(rp[my + 1]) : 0;
-      constexpr int U = BS == 1 ? F3_STREAM_U : (F3_STREAM_U + 1) / 2;
+      if constexpr(BS == 1)
+        dev::stage_products<F3_STREAM_U, kBlock>(val, col, e0, ne, x, prod);
+      else
+      {
+      constexpr int U = (F3_STREAM_U + 1) / 2;
       for(u64 k0 = threadIdx.x; k0 < ne; k0 += u64(kBlock) * U)
       {
         DT v[U];
@@ -151,6 +155,7 @@ namespace
 #pragma unroll
             for(int h = 0; h < BS; ++h) prod[k * BS + h] = v[j] * xv[j][h];
         }
+      }
       }
       __syncthreads();
       for(u64 row = r0 + threadIdx.x; row < r1; row += kBlock)
@@ -431,6 +436,27 @@ namespace
     for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
       r[i] = (a * x[i]) + y[i];
   }
+  // 16-byte accesses (V = 16 / sizeof(DT) elements) when r, x, y are aligned for them; the
+  // n % V trailing elements by the first threads of the grid
+  template<typename DT, int V>
+  __global__ __launch_bounds__(kBlock) void k_axpy_vec(DT* r, DT a, const DT* x, const DT* y, u64 n)
+  {
+    typedef DT vt __attribute__((ext_vector_type(V)));
+    const u64 nv = n / V;
+    const u64 step = u64(gridDim.x) * blockDim.x;
+    const u64 gid = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    for(u64 i = gid; i < nv; i += step)
+    {
+      const vt xv = reinterpret_cast<const vt*>(x)[i], yv = reinterpret_cast<const vt*>(y)[i];
+      vt o;
+#pragma unroll
+      for(int q = 0; q < V; ++q) o[q] = (a * xv[q]) + yv[q];
+      reinterpret_cast<vt*>(r)[i] = o;
+    }
+    const u64 t = nv * V + gid;
+    if(t < n) r[t] = (a * x[t]) + y[t];
+  }
+
   template<typename DT>
   __global__ void k_cprod(DT* r, const DT* x, const DT* y, u64 n)
   {
@@ -487,6 +513,38 @@ namespace
     for(; i < n; i += step) s0 += x[i] * y[i];
     dev::block_partial((s0 + s1) + (s2 + s3), partials);
   }
+  // the same with 16-byte loads of x and y (aligned for V = 16 / sizeof(DT) elements)
+  template<typename DT, int V>
+  __global__ __launch_bounds__(kBlock) void k_dot_vec(const DT* x, const DT* y, u64 n, DT* partials)
+  {
+    typedef DT vt __attribute__((ext_vector_type(V)));
+    const u64 nv = n / V;
+    const u64 step = u64(gridDim.x) * blockDim.x;
+    const u64 gid = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    DT s[2][V];
+#pragma unroll
+    for(int q = 0; q < V; ++q) { s[0][q] = DT(0); s[1][q] = DT(0); }
+    u64 i = gid;
+    for(; i + step < nv; i += 2 * step)
+    {
+      const vt x0 = reinterpret_cast<const vt*>(x)[i], y0 = reinterpret_cast<const vt*>(y)[i];
+      const vt x1 = reinterpret_cast<const vt*>(x)[i + step], y1 = reinterpret_cast<const vt*>(y)[i + step];
+#pragma unroll
+      for(int q = 0; q < V; ++q) { s[0][q] += x0[q] * y0[q]; s[1][q] += x1[q] * y1[q]; }
+    }
+    if(i < nv)
+    {
+      const vt x0 = reinterpret_cast<const vt*>(x)[i], y0 = reinterpret_cast<const vt*>(y)[i];
+#pragma unroll
+      for(int q = 0; q < V; ++q) s[0][q] += x0[q] * y0[q];
+    }
+    const u64 t = nv * V + gid;
+    if(t < n) s[1][0] += x[t] * y[t];
+    DT tot = DT(0);
+#pragma unroll
+    for(int q = 0; q < V; ++q) tot += s[0][q] + s[1][q];
+    dev::block_partial(tot, partials);
+  }
   template<typename DT, bool SQRT>
   __global__ __launch_bounds__(kBlock) void k_dot_fin(const DT* partials, unsigned nb, DT* out)
   {
@@ -503,7 +561,11 @@ namespace
     // finalisation sums few partials
     const unsigned nb = unsigned(std::min<u64>(std::min<u64>(stream_blocks(n), 2048), rt::Workspace::kMaxPartials));
     DT* part = reinterpret_cast<DT*>(ws->partials);
-    k_dot<DT><<<nb, kBlock, 0, s>>>(x, y, n, part);
+    constexpr int V = 16 / int(sizeof(DT));
+    if(reinterpret_cast<std::uintptr_t>(x) % 16 == 0 && reinterpret_cast<std::uintptr_t>(y) % 16 == 0)
+      k_dot_vec<DT, V><<<nb, kBlock, 0, s>>>(x, y, n, part);
+    else
+      k_dot<DT><<<nb, kBlock, 0, s>>>(x, y, n, part);
     k_dot_fin<DT, SQRT><<<1, kBlock, 0, s>>>(part, nb, out);
     return rt::check_launch();
   }
@@ -724,7 +786,11 @@ F3_DEF_CSRSB(f32_u64, float, u64)
   {                                                                                                           \
     if(n == 0) return 0;                                                                                      \
     if(!r || !x || !y) return rt::kErrArg;                                                                    \
-    k_axpy<DT><<<stream_blocks(n), kBlock, 0, S_>>>(r, a, x, y, n);                                           \
+    if(reinterpret_cast<std::uintptr_t>(r) % 16 == 0 && reinterpret_cast<std::uintptr_t>(x) % 16 == 0 &&         \
+       reinterpret_cast<std::uintptr_t>(y) % 16 == 0)                                                         \
+      k_axpy_vec<DT, 16 / int(sizeof(DT))><<<stream_blocks((n + 1) / 2), kBlock, 0, S_>>>(r, a, x, y, n);     \
+    else                                                                                                      \
+      k_axpy<DT><<<stream_blocks(n), kBlock, 0, S_>>>(r, a, x, y, n);                                         \
     return rt::check_launch();                                                                                \
   }                                                                                                           \
   int lafem_hip_dot_##SUF(void* stream, const DT* x, const DT* y, u64 n, DT* result_dev)                      \

@@ -193,7 +193,6 @@ namespace mg
                                                   const DT_* __restrict__ x, F&& f)
   {
     constexpr int CAP = CsrStreamRows<DT_, VT_>::CAP;
-    constexpr int U = 8;
     constexpr unsigned NT = 256;
     __shared__ DT_ prod[CAP];
     for(u64 r0 = beg; r0 < end; r0 += A.R)
@@ -205,24 +204,7 @@ namespace mg
       {
         const u64 my = r0 + threadIdx.x;
         const u64 mk0 = my < r1 ? u64(A.rp[my]) : 0, mk1 = my < r1 ? u64(A.rp[my + 1]) : 0;
-        for(u64 k0 = threadIdx.x; k0 < ne; k0 += u64(NT) * U)
-        {
-          DT_ v[U];
-          u64 c[U];
-#pragma unroll
-          for(int j = 0; j < U; ++j)
-          {
-            const u64 k = k0 + u64(j) * NT;
-            v[j] = k < ne ? DT_(dev::ld_stream(A.val + e0 + k)) : DT_(0);
-            c[j] = k < ne ? u64(dev::ld_stream(A.col + e0 + k)) : 0;
-          }
-          DT_ xv[U];
-#pragma unroll
-          for(int j = 0; j < U; ++j) xv[j] = k0 + u64(j) * NT < ne ? x[c[j]] : DT_(0);
-#pragma unroll
-          for(int j = 0; j < U; ++j)
-            if(k0 + u64(j) * NT < ne) prod[k0 + u64(j) * NT] = v[j] * xv[j];
-        }
+        dev::stage_products<8, NT>(A.val, A.col, e0, ne, x, prod);
         __syncthreads();
         for(u64 row = my; row < r1; row += NT)
         {

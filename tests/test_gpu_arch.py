@@ -366,6 +366,38 @@ def test_blas1_f64(T, n):
         assert np.array_equal(host(dsc), sc)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_blas1_unaligned_pointers(T, dtype):
+    """axpy and dot use 16-byte accesses when every pointer is 16-byte aligned and element
+    accesses otherwise: views starting one element into a buffer take the second path, with
+    the same results as the aligned call on the same data."""
+    torch, capi = T
+    rng = np.random.default_rng(3)
+    n = 100_003
+    suf = "f64" if dtype == np.float64 else "f32"
+    tol = 1e-15 if dtype == np.float64 else 1e-6
+    x = rng.uniform(-1, 1, n + 1).astype(dtype)
+    y = rng.uniform(-1, 1, n + 1).astype(dtype)
+    dx, dy = dev(torch, x), dev(torch, y)
+    es = dx.element_size()
+    px, py = ctypes.c_void_p(dx.data_ptr() + es), ctypes.c_void_p(dy.data_ptr() + es)
+    out = torch.zeros(n + 1, dtype=dx.dtype, device="cuda")
+    capi.call(f"lafem_hip_axpy_{suf}", None, ctypes.c_void_p(out.data_ptr() + es), 0.25, px, py, n)
+    res = torch.zeros(2, dtype=dx.dtype, device="cuda")
+    capi.call(f"lafem_hip_dot_{suf}", None, px, py, n, ctypes.c_void_p(res.data_ptr()))
+    xa, ya = dev(torch, x[1:].copy()), dev(torch, y[1:].copy())
+    outa = torch.zeros(n, dtype=dx.dtype, device="cuda")
+    capi.call(f"lafem_hip_axpy_{suf}", None, vp(outa), 0.25, vp(xa), vp(ya), n)
+    capi.call(f"lafem_hip_dot_{suf}", None, vp(xa), vp(ya), n, ctypes.c_void_p(res.data_ptr() + es))
+    torch.cuda.synchronize()
+    ref = (0.25 * x[1:].astype(np.float64)) + y[1:]
+    assert rel(host(out)[1:], ref) <= tol and rel(host(outa), ref) <= tol
+    d = float(np.dot(x[1:].astype(np.float64), y[1:].astype(np.float64)))
+    r = host(res).astype(np.float64)
+    assert abs(r[0] - d) <= (1e-12 if dtype == np.float64 else 1e-3) * np.abs(x * y).sum()
+    assert abs(r[1] - d) <= (1e-12 if dtype == np.float64 else 1e-3) * np.abs(x * y).sum()
+
+
 def test_blas1_f32_and_convert(T):
     torch, capi = T
     rng = np.random.default_rng(5)
