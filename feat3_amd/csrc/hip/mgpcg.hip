@@ -366,6 +366,10 @@ namespace
     // SELL sort window of the transfer matrices: prolongation rows have 1/2/4/8 entries in an
     // alternating pattern, so sorting by length inside windows removes most of the padding
     u64 transfer_sigma = 1;
+    // transfer matrices (P, R) in CSR, evaluated by streaming, instead of the level format:
+    // prolongation rows hold 1..8 entries (Q1), which SELL pads to the chunk's widest row
+    // (measured 3D Q1 L7 FP32 V-cycle 0.603 -> 0.583 ms, 2D Q2 L9 0.643 -> 0.610 ms)
+    bool transfer_csr = true;
     double tol_rel = 1e-8, tol_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16), tol_abs_low = 0.0;
     double div_rel = 1.0 / 2.220446049250313e-16, div_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16);
     double stag_rate = 0.95;
@@ -455,8 +459,9 @@ namespace
         {
           if(h.P.rows != l->n || h.R.cols != l->n || h.P.cols != lv[i + 1].A.rows || h.R.rows != lv[i + 1].A.rows)
             return rt::kErrArg;
-          F3_RC(l->P.upload(h.P, s, want_sell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16));
-          F3_RC(l->R.upload(h.R, s, want_sell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16));
+          const bool tsell = want_sell && !cfg.transfer_csr;
+          F3_RC(l->P.upload(h.P, s, tsell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16));
+          F3_RC(l->R.upload(h.R, s, tsell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16));
         }
         const std::vector<u32> m = make_mask(l->n, h.fidx);
         F3_RC(l->mask.alloc(sizeof(u32) * m.size()));
@@ -1123,7 +1128,7 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, u64 sell_C, u64
 
 // named options: "index16" (0/1), "compact_transfer" (0/1), "sell_C", "sell_sigma", "lanes",
 // "tail_max_n", "rows_per_block" (level matrices; 0 = derived from the row count),
-// "transfer_sigma" (SELL sort window of P and R)
+// "transfer_sigma" (SELL sort window of P and R), "transfer_csr" (P and R in CSR, default 1)
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
 {
   if(!h || !name) return rt::kErrArg;
@@ -1132,6 +1137,7 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
   else if(k == "compact_transfer") h->cfg.compact_transfer = value != 0.0;
   else if(k == "sell_C" && value >= 1 && value <= 1024) h->cfg.sell_C = u64(value);
   else if(k == "tail_max_n" && value >= 0) h->cfg.tail_max_n = u64(value);
+  else if(k == "transfer_csr") h->cfg.transfer_csr = value != 0.0;
   else if(k == "rows_per_block" && value >= 0) h->cfg.rows_per_block = u64(value);
   else if(k == "transfer_sigma" && value >= 1) h->cfg.transfer_sigma = u64(value);
   else if(k == "sell_sigma" && value >= 1) h->cfg.sell_sigma = u64(value);

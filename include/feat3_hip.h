@@ -302,8 +302,10 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
  * they fit), "compact_transfer" 0/1 (device P/R without stored structural zeros), "sell_C",
  * "sell_sigma", "lanes", "tail_max_n" (levels of at most this many rows run in one LDS-resident
  * workgroup; 0 = none; default 1100), "rows_per_block" (row positions per workgroup of the
- * level-matrix kernels, a multiple of 64; 0 = derived from the row count), "transfer_sigma" (SELL
- * sort window of the transfer matrices, default 1); takes effect at the next init */
+ * level-matrix kernels, a multiple of 64; 0 = derived from the row count), "transfer_csr" (0/1:
+ * transfer matrices in CSR evaluated by streaming, default 1, or in the level format),
+ * "transfer_sigma" (SELL sort window of the transfer matrices when they are SELL, default 1);
+ * takes effect at the next init */
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value);
 /* init_symbolic + init_numeric (solver/base.hpp:227-260) */
 int feat3_mgpcg_init(feat3_mgpcg* h);

@@ -103,7 +103,8 @@ def test_sell_graph_and_lanes_variants_agree(torch):
     for kw in [dict(fmt="sell"), dict(graph=False), dict(lanes=16), dict(fmt="sell", sell_C=32, sell_sigma=1),
                dict(fmt="sell", sell_sigma=64), dict(options=dict(tail_max_n=0)), dict(options=dict(tail_max_n=10 ** 6)),
                dict(fmt="sell", options=dict(index16=0, compact_transfer=0)),
-               dict(fmt="sell", options=dict(transfer_sigma=64)), dict(fmt="banded", options=dict(transfer_sigma=256)),
+               dict(fmt="sell", options=dict(transfer_csr=0, transfer_sigma=64)),
+               dict(fmt="banded", options=dict(transfer_csr=0, transfer_sigma=256)), dict(fmt="sell", options=dict(transfer_csr=0)),
                dict(fmt="banded", options=dict(rows_per_block=128))]:
         st, it, x, _ = solve_gpu(torch, H, b, **kw)
         assert st == 2 and it == ref[1], kw
