@@ -72,6 +72,7 @@ namespace feat3_amd
     public:
       void set_tol_rel(DataType t) { _tol_rel = t; }
       void set_tol_abs(DataType t) { _tol_abs = t; }
+      void set_tol_abs_low(DataType t) { _tol_abs_low = t; }
       void set_min_iter(Index i) { _min_iter = i; }
       void set_max_iter(Index i) { _max_iter = i; }
       Index get_num_iter() const { return _num_iter; }
@@ -414,6 +415,15 @@ namespace feat3_amd
       void set_smoother(double omega_jacobi, double omega_richardson, int pre, int post)
       {
         hip_check(feat3_mgpcg_set_smoother(_h, omega_jacobi, omega_richardson, pre, post), "FusedMGPCG::set_smoother");
+      }
+      // plain CG (no preconditioner) instead of the V-cycle
+      void set_precond_none() { hip_check(feat3_mgpcg_set_precond(_h, 1), "FusedMGPCG::set_precond_none"); }
+      // graph replay of the iteration (default on)
+      void set_graph(bool on) { hip_check(feat3_mgpcg_set_launch(_h, on ? 1 : 0, 0, 64, 1), "FusedMGPCG::set_graph"); }
+      // a named launch/format option of feat3_mgpcg_set_option
+      void set_option(const std::string& name, double value)
+      {
+        hip_check(feat3_mgpcg_set_option(_h, name.c_str(), value), "FusedMGPCG::set_option");
       }
       void init_numeric() override
       {

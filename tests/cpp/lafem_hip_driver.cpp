@@ -9,6 +9,7 @@
 // <outdir>/<name>.bin for the test to compare against the oracle.
 #include "feat3_amd/lafem_hip.hpp"
 #include "feat3_amd/solver_hip.hpp"
+#include "feat3_amd/solver_factory_hip.hpp"
 
 #include <cmath>
 #include <cstdio>
@@ -164,6 +165,35 @@ int main(int argc, char** argv)
       const Solver::Status st = fused.correct(x, b);
       std::cout << "fused64 " << int(st) << " " << fused.get_num_iter() << std::endl;
       write(outdir + "/fused64.bin", x.to_host());
+    }
+    {
+      // the same fused solver built by the SolverFactory from an INI configuration
+      PropertyMap pm;
+      pm.parse("[linsolver]\ntype = pcg\ntol_rel = 1e-8\nmax_iter = 1000\nprecon = mgv\n"
+               "[mgv]\ntype = mg\ncycle = v\nhierarchy = h1\nmemorytype = hip\ndatatype = double\nformat = sell\n"
+               "[h1]\ntype = hierarchy\nsmoother = rich\ncoarse = coarse\n"
+               "[rich]\ntype = richardson\nmax_iter = 4\nmin_iter = 4\nprecon = jac  # damped Jacobi\n"
+               "[jac]\ntype = jacobi\nomega = " + std::to_string(omega) + "\n[coarse]\ntype = umfpack\n");
+      auto fs = Solver::SolverFactory::create_scalar_solver(pm, "linsolver");
+      for(const HostLevel& h : host)
+        fs->push_level(h.n, h.rp, h.col, h.val, h.bnd, h.nc, h.prp, h.pcol, h.pval, h.rrp, h.rcol, h.rval);
+      fs->init();
+      LAFEM::DenseVector<Mem::HIP, double, u32> x(h0.n, 0.0), b(rhs);
+      const Solver::Status st = fs->correct(x, b);
+      std::cout << "factory64 " << int(st) << " " << fs->get_num_iter() << std::endl;
+      write(outdir + "/factory64.bin", x.to_host());
+      bool rejected = false;
+      try
+      {
+        PropertyMap bad;
+        bad.parse("[s]\ntype = pcg\nprecon = m\n[m]\ntype = mg\ncycle = w\n");
+        Solver::SolverFactory::create_scalar_solver(bad, "s");
+      }
+      catch(const InternalError&)
+      {
+        rejected = true;
+      }
+      std::cout << "factory_rejects_w_cycle " << int(rejected) << std::endl;
     }
     {
       // the same solver over the banded level format (SparseMatrixBanded layout)
