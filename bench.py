@@ -405,10 +405,25 @@ def main():
 
     t = time.time()
     H = pb.hierarchy(dim, deg, args.level, 1)
+    host_setup_s = time.time() - t
     b = pb.filtered_rhs(H, 0)
     n, nnz = H.n, H.levels[0].A.nnz
-    log(f"setup: {len(H.levels)} levels, n={n}, nnz={nnz} ({time.time() - t:.1f} s)")
+    log(f"setup: {len(H.levels)} levels, n={n}, nnz={nnz} ({host_setup_s:.1f} s)")
+    # the same hierarchy assembled on the device (bit-identical; the solver's format conversion
+    # still runs on the host, so the arrays are copied back)
+    pb.hierarchy_device(dim, deg, 2, 1)   # warm-up (module load)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    levels_dev = pb.hierarchy_device(dim, deg, args.level, 1)
+    torch.cuda.synchronize()
+    dev_setup_ms = 1e3 * (time.perf_counter() - t)
+    t = time.perf_counter()
+    pb.device_levels_to_host(dim, deg, args.level, 1, levels_dev)
+    d2h_ms = 1e3 * (time.perf_counter() - t)
+    del levels_dev
+    log(f"setup on the device: {dev_setup_ms:.1f} ms (+ {d2h_ms:.0f} ms copy to the host) vs {host_setup_s:.2f} s on the host")
 
+    setup = {"host_hierarchy_s": host_setup_s, "device_hierarchy_ms": dev_setup_ms, "device_to_host_copy_ms": d2h_ms}
     db = torch.from_numpy(b).to(dev)
     x = torch.zeros(n, dtype=torch.float64, device=dev)
 
@@ -529,6 +544,7 @@ def main():
         "headline": head,
         "variants": {k: v[0] for k, v in variants.items()},
         "speedup_vs_fp64": (fp64[0]["ms_per_solve"] / ms_per_step) if fp64 else None,
+        "setup": setup,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

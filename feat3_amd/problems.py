@@ -136,6 +136,95 @@ def hierarchy(dim: int, degree: int, level_fine: int, level_coarse: int = 1,
     return H
 
 
+# -------------------------------------------------------------------------------------------
+# the same hierarchy assembled on the device (bit-identical arrays)
+# -------------------------------------------------------------------------------------------
+@dataclass
+class DeviceCsr:
+    """CSR arrays in device memory (torch tensors; the u32 index arrays held as int32)."""
+    rows: int
+    cols: int
+    row_ptr: object
+    col_ind: object
+    val: object
+
+    def to_host(self) -> Csr:
+        return Csr(self.rows, self.cols, self.row_ptr.cpu().numpy().view(U32), self.col_ind.cpu().numpy().view(U32),
+                   self.val.cpu().numpy())
+
+
+def _dptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def laplace_device(dim: int, degree: int, level: int, kcell=None, stream=None):
+    """(DeviceCsr A, Dirichlet list) assembled on the device: feat3_device_laplace, the same
+    arrays as laplace(); kcell: device coefficient per cell or None for k = 1."""
+    import torch
+    n, nnz, nb = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    capi.call("feat3_device_laplace_sizes", dim, degree, level, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(nb))
+    dev = torch.device("cuda")
+    rp = torch.empty(n.value + 1, dtype=torch.int32, device=dev)
+    col = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=dev)
+    val = torch.empty(max(nnz.value, 1), dtype=torch.float64, device=dev)
+    bnd = torch.empty(max(nb.value, 1), dtype=torch.int32, device=dev)
+    K = ref_stiffness(dim, degree)
+    capi.call("feat3_device_laplace", _dptr(stream) if stream is not None else None, dim, degree, level, _p(K), _dptr(kcell),
+              _dptr(rp), _dptr(col), _dptr(val), _dptr(bnd))
+    return DeviceCsr(n.value, n.value, rp, col[: nnz.value], val[: nnz.value]), bnd[: nb.value]
+
+
+def transfer_device(dim: int, degree: int, level_fine: int, transpose: bool = False) -> DeviceCsr:
+    """P (fine x coarse) or R = P^T assembled on the device (feat3_device_transfer), the same
+    arrays as prolongation() / transpose(prolongation())."""
+    import torch
+    nf, nc, nnz = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    capi.call("feat3_device_prol_sizes", dim, degree, level_fine, ctypes.byref(nf), ctypes.byref(nc), ctypes.byref(nnz))
+    rows, cols = (nc.value, nf.value) if transpose else (nf.value, nc.value)
+    dev = torch.device("cuda")
+    rp = torch.empty(rows + 1, dtype=torch.int32, device=dev)
+    col = torch.empty(nnz.value, dtype=torch.int32, device=dev)
+    val = torch.empty(nnz.value, dtype=torch.float64, device=dev)
+    capi.call("feat3_device_transfer", None, dim, degree, level_fine, int(transpose), _dptr(rp), _dptr(col), _dptr(val))
+    return DeviceCsr(rows, cols, rp, col, val)
+
+
+@dataclass
+class DeviceLevel:
+    A: DeviceCsr
+    bnd: object
+    P: Optional[DeviceCsr] = None
+    R: Optional[DeviceCsr] = None
+
+
+def hierarchy_device(dim: int, degree: int, level_fine: int, level_coarse: int = 1, kcell=None):
+    """The levels of hierarchy() assembled on the device (finest first); kcell: device
+    coefficient per finest cell or None.  to_host(levels) gives the Hierarchy the solver takes."""
+    import torch
+    levels = []
+    k = kcell
+    for lvl in range(level_fine, level_coarse - 1, -1):
+        A, bnd = laplace_device(dim, degree, lvl, k)
+        L = DeviceLevel(A, bnd)
+        if lvl > level_coarse:
+            L.P = transfer_device(dim, degree, lvl, False)
+            L.R = transfer_device(dim, degree, lvl, True)
+            if k is not None:
+                kc = torch.empty(k.numel() >> dim, dtype=torch.float64, device=k.device)
+                capi.call("feat3_coarsen_cellfields", None, dim, lvl, 1, _dptr(k), _dptr(kc))
+                k = kc
+        levels.append(L)
+    return levels
+
+
+def device_levels_to_host(dim: int, degree: int, level_fine: int, level_coarse: int, levels) -> Hierarchy:
+    H = Hierarchy(dim, degree, [], level_fine, level_coarse)
+    for L in levels:
+        H.levels.append(Level(L.A.to_host(), L.bnd.cpu().numpy().view(U32),
+                              None if L.P is None else L.P.to_host(), None if L.R is None else L.R.to_host()))
+    return H
+
+
 def filtered_rhs(H: Hierarchy, func: int = 0) -> np.ndarray:
     """Load vector on the finest level with homogeneous Dirichlet values (filter_rhs)."""
     b = load_vector(H.dim, H.degree, H.level_fine, func)

@@ -401,6 +401,24 @@ int feat3_assemble_laplace_fields(void* stream, int dim, int degree, int level, 
 /* kc[field][coarse cell] = mean of the 2^dim children on level level_fine (device arrays) */
 int feat3_coarsen_cellfields(void* stream, int dim, int level_fine, int nfields, const double* kf, double* kc);
 
+/* Symbolic and numeric assembly of a level on the device (the host setup's feat3_setup_laplace,
+ * feat3_setup_prol and feat3_setup_transpose, bit for bit: the reference's
+ * SymbolicAssembler::assemble_matrix_std1 / assemble_matrix_2lvl, symbolic_assembler.hpp:59-110,
+ * 319-343, BilinearOperatorAssembler with the Laplace operator, UnitFilterAssembler,
+ * unit_filter_assembler.hpp:101-130, and the transposed prolongation,
+ * sparse_matrix_csr.hpp:1781-1844).  Sizes are closed forms computed on the host. */
+int feat3_device_laplace_sizes(int dim, int degree, int level, uint64_t* n, uint64_t* nnz, uint64_t* nbnd);
+/* row_ptr[n+1], col_ind[nnz], val[nnz], bnd[nbnd]: device; ref_stiffness: host
+ * (feat3_setup_ref_stiffness); kcell: device coefficient per cell (x fastest) or NULL for k = 1 */
+int feat3_device_laplace(void* stream, int dim, int degree, int level, const double* ref_stiffness, const double* kcell,
+                         uint32_t* row_ptr, uint32_t* col_ind, double* val, uint32_t* bnd);
+/* prolongation level_fine-1 -> level_fine: fine rows nf, coarse columns nc, entries nnz (R = P^T
+ * has nc rows and the same entries) */
+int feat3_device_prol_sizes(int dim, int degree, int level_fine, uint64_t* nf, uint64_t* nc, uint64_t* nnz);
+/* transpose = 0: P (row_ptr[nf+1]); transpose = 1: R = P^T (row_ptr[nc+1]); device arrays */
+int feat3_device_transfer(void* stream, int dim, int degree, int level_fine, int transpose, uint32_t* row_ptr,
+                          uint32_t* col_ind, double* val);
+
 /* ===================================================================================== */
 /* Host-side problem setup (the step before the hot path: assembly, geometry)         */
 /* ===================================================================================== */

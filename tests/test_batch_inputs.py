@@ -95,3 +95,21 @@ def test_tile_order_is_compact_permutation(dim, lvl, tile):
     ext = coords.max(axis=0) - coords.min(axis=0) + 1
     assert np.prod(ext) <= 2 * tile
     assert np.all(ext <= int(np.ceil(tile ** (1.0 / dim))) + 1)
+
+
+@pytest.mark.parametrize("dim,deg", [(2, 1), (2, 2), (3, 1)])
+def test_device_assembly_sizes_match_the_host_setup(dim, deg):
+    """The closed-form sizes of the device assembly (host-side, no GPU) equal the host setup's
+    counted sizes for every level."""
+    import ctypes
+    from feat3_amd import capi
+    for lvl in range(0, 7 if dim == 2 else 5):
+        a = [ctypes.c_uint64() for _ in range(3)]
+        b = [ctypes.c_uint64() for _ in range(3)]
+        capi.call("feat3_setup_laplace_sizes", dim, deg, lvl, *[ctypes.byref(v) for v in a])
+        capi.call("feat3_device_laplace_sizes", dim, deg, lvl, *[ctypes.byref(v) for v in b])
+        assert [v.value for v in a] == [v.value for v in b], (dim, deg, lvl)
+        if lvl >= 1:
+            capi.call("feat3_setup_prol_sizes", dim, deg, lvl, *[ctypes.byref(v) for v in a])
+            capi.call("feat3_device_prol_sizes", dim, deg, lvl, *[ctypes.byref(v) for v in b])
+            assert [v.value for v in a] == [v.value for v in b], (dim, deg, lvl)
