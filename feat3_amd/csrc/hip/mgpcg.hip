@@ -64,8 +64,102 @@ namespace
     u64 rows = 0, cols = 0;
     std::vector<u32> rp, col;
     std::vector<double> val;
-    u64 nnz() const { return rp.empty() ? 0 : rp[rows]; }
+    // device-resident source (feat3_mgpcg_push_level_device): the arrays above stay empty
+    const u32* drp = nullptr;
+    const u32* dcol = nullptr;
+    const double* dval = nullptr;
+    u64 dnnz = 0;
+    bool on_device() const { return drp != nullptr; }
+    u64 nnz() const { return on_device() ? dnnz : (rp.empty() ? 0 : rp[rows]); }
   };
+
+  // ---- setup kernels for device-resident level matrices -----------------------------------
+  __global__ void k_validate_csr(u64 rows, u64 cols, u64 nnz, const u32* __restrict__ rp, const u32* __restrict__ col,
+                                 int* bad)
+  {
+    for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
+    {
+      const u64 a = rp[r], b = rp[r + 1];
+      bool ok = a <= b && b <= nnz && (r > 0 || a == 0) && (r + 1 < rows || b == nnz);
+      // strictly ascending columns inside the row, all in range
+      for(u64 k = a; ok && k < b; ++k) ok = u64(col[k]) < cols && (k == a || col[k] > col[k - 1]);
+      if(!ok) atomicOr(bad, 1);
+    }
+  }
+  __global__ void k_count_nonzero(u64 rows, const u32* __restrict__ rp, const double* __restrict__ val, u32* __restrict__ cnt)
+  {
+    for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
+    {
+      u32 c = 0;
+      for(u64 k = rp[r]; k < rp[r + 1]; ++k) c += val[k] != 0.0 ? 1u : 0u;
+      cnt[r] = c;
+    }
+  }
+  __global__ void k_compact(u64 rows, const u32* __restrict__ rp, const u32* __restrict__ col, const double* __restrict__ val,
+                            const u32* __restrict__ nrp, u32* __restrict__ ncol, double* __restrict__ nval)
+  {
+    for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
+    {
+      u64 j = nrp[r];
+      for(u64 k = rp[r]; k < rp[r + 1]; ++k)
+        if(val[k] != 0.0) { ncol[j] = col[k]; nval[j] = val[k]; ++j; }
+    }
+  }
+  template<typename VT>
+  __global__ void k_to_storage(u64 n, const double* __restrict__ in, VT* __restrict__ out)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      out[i] = VT(float(in[i]));
+  }
+  template<>
+  __global__ void k_to_storage<double>(u64 n, const double* __restrict__ in, double* __restrict__ out)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      out[i] = in[i];
+  }
+  // bit (col - row + rows - 1) for every entry: the set of diagonals
+  __global__ void k_diag_bitmap(u64 rows, const u32* __restrict__ rp, const u32* __restrict__ col, u32* bits)
+  {
+    for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
+      for(u64 k = rp[r]; k < rp[r + 1]; ++k)
+      {
+        const u64 o = u64(col[k]) + rows - 1 - r;
+        atomicOr(&bits[o >> 5], 1u << (o & 31u));
+      }
+  }
+  // chunk-major banded values from CSR (offsets stored as off + rows - 1, ascending)
+  template<typename VT>
+  __global__ void k_banded_scatter(u64 rows, const u32* __restrict__ rp, const u32* __restrict__ col,
+                                   const double* __restrict__ val, const u32* __restrict__ boff, u32 nd, VT* __restrict__ bval)
+  {
+    for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
+      for(u64 k = rp[r]; k < rp[r + 1]; ++k)
+      {
+        const u32 key = u32(u64(col[k]) + rows - 1 - r);
+        u32 lo = 0, hi = nd;
+        while(lo < hi) { const u32 mid = (lo + hi) / 2; if(boff[mid] < key) lo = mid + 1; else hi = mid; }
+        bval[(r >> 6) * u64(nd) * 64 + u64(lo) * 64 + (r & 63)] = VT(float(val[k]));
+      }
+  }
+  template<>
+  __global__ void k_banded_scatter<double>(u64 rows, const u32* __restrict__ rp, const u32* __restrict__ col,
+                                           const double* __restrict__ val, const u32* __restrict__ boff, u32 nd,
+                                           double* __restrict__ bval)
+  {
+    for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
+      for(u64 k = rp[r]; k < rp[r + 1]; ++k)
+      {
+        const u32 key = u32(u64(col[k]) + rows - 1 - r);
+        u32 lo = 0, hi = nd;
+        while(lo < hi) { const u32 mid = (lo + hi) / 2; if(boff[mid] < key) lo = mid + 1; else hi = mid; }
+        bval[(r >> 6) * u64(nd) * 64 + u64(lo) * 64 + (r & 63)] = val[k];
+      }
+  }
+  inline unsigned setup_blocks(u64 n)
+  {
+    const u64 b = (n + 255) / 256;
+    return unsigned(b < 1 ? 1 : (b > 16384 ? 16384 : b));
+  }
 
   struct HostLevel
   {
@@ -128,6 +222,119 @@ namespace
       if(want_banded) F3_RC(build_banded(h, s));
       sell = want_sell && !banded;
       if(sell) F3_RC(build_sell(h, s, sell_C, sell_sigma, allow_d16));
+      return rt::kOk;
+    }
+
+    // The same from a device-resident source (validated on the device: rows ascending, columns
+    // strictly ascending and in range).  The banded copy and the compacted transfers are built
+    // on the device; only the diagonal bitmap (2 rows bits) and the per-row entry counts of a
+    // compaction come back to the host.  A matrix that needs the SELL-C-sigma copy is brought
+    // to the host and takes upload().
+    int upload_dev(const HostCsr& h0, hipStream_t s, bool want_sell, u64 sell_C, u64 sell_sigma, int force_lanes,
+                   bool drop_zeros, bool allow_d16, bool want_banded)
+    {
+      rows = h0.rows; cols = h0.cols;
+      u64 n_in = h0.dnnz;
+      DevMem bad;
+      F3_RC(bad.alloc(sizeof(int)));
+      F3_CHECK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+      if(rows) k_validate_csr<<<setup_blocks(rows), 256, 0, s>>>(rows, cols, n_in, h0.drp, h0.dcol, bad.template as<int>());
+      int hbad = 0;
+      F3_CHECK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      F3_CHECK(hipStreamSynchronize(s));
+      if(hbad) return rt::kErrArg;
+      // compaction (exact zeros dropped) into FP64 device arrays
+      DevMem crp, ccol, cval;
+      const u32* srp = h0.drp;
+      const u32* scol_ = h0.dcol;
+      const double* sval_ = h0.dval;
+      if(drop_zeros && rows)
+      {
+        DevMem cnt;
+        F3_RC(cnt.alloc(sizeof(u32) * rows));
+        k_count_nonzero<<<setup_blocks(rows), 256, 0, s>>>(rows, h0.drp, h0.dval, cnt.template as<u32>());
+        std::vector<u32> hc(rows), hrp(rows + 1, 0);
+        F3_CHECK(hipMemcpyAsync(hc.data(), cnt.p, sizeof(u32) * rows, hipMemcpyDeviceToHost, s));
+        F3_CHECK(hipStreamSynchronize(s));
+        for(u64 r = 0; r < rows; ++r) hrp[r + 1] = hrp[r] + hc[r];
+        n_in = hrp[rows];
+        F3_RC(crp.alloc(sizeof(u32) * (rows + 1)));
+        F3_RC(ccol.alloc(sizeof(u32) * std::max<u64>(n_in, 1)));
+        F3_RC(cval.alloc(sizeof(double) * std::max<u64>(n_in, 1)));
+        F3_CHECK(hipMemcpyAsync(crp.p, hrp.data(), sizeof(u32) * (rows + 1), hipMemcpyHostToDevice, s));
+        k_compact<<<setup_blocks(rows), 256, 0, s>>>(rows, h0.drp, h0.dcol, h0.dval, crp.template as<u32>(),
+                                                     ccol.template as<u32>(), cval.template as<double>());
+        F3_CHECK(hipStreamSynchronize(s));   // hrp leaves scope
+        srp = crp.template as<u32>();
+        scol_ = ccol.template as<u32>();
+        sval_ = cval.template as<double>();
+      }
+      nnz = n_in;
+      F3_RC(rp.alloc(sizeof(u32) * (rows + 1)));
+      F3_RC(col.alloc(sizeof(u32) * std::max<u64>(nnz, 1)));
+      F3_RC(val.alloc(sizeof(VT) * std::max<u64>(nnz, 1)));
+      F3_CHECK(hipMemcpyAsync(rp.p, srp, sizeof(u32) * (rows + 1), hipMemcpyDeviceToDevice, s));
+      if(nnz)
+      {
+        F3_CHECK(hipMemcpyAsync(col.p, scol_, sizeof(u32) * nnz, hipMemcpyDeviceToDevice, s));
+        k_to_storage<VT><<<setup_blocks(nnz), 256, 0, s>>>(nnz, sval_, val.template as<VT>());
+        F3_RC(rt::check_launch());
+      }
+      lanes = force_lanes > 0 ? force_lanes : 0;
+      banded = false;
+      if(want_banded) F3_RC(build_banded_dev(srp, scol_, sval_, s));
+      sell = want_sell && !banded;
+      if(sell)
+      {
+        HostCsr h;
+        h.rows = rows; h.cols = cols;
+        h.rp.resize(rows + 1); h.col.resize(nnz); h.val.resize(nnz);
+        F3_CHECK(hipMemcpyAsync(h.rp.data(), srp, sizeof(u32) * (rows + 1), hipMemcpyDeviceToHost, s));
+        if(nnz)
+        {
+          F3_CHECK(hipMemcpyAsync(h.col.data(), scol_, sizeof(u32) * nnz, hipMemcpyDeviceToHost, s));
+          F3_CHECK(hipMemcpyAsync(h.val.data(), sval_, sizeof(double) * nnz, hipMemcpyDeviceToHost, s));
+        }
+        F3_CHECK(hipStreamSynchronize(s));
+        F3_RC(build_sell(h, s, sell_C, sell_sigma, allow_d16));
+      }
+      F3_CHECK(hipStreamSynchronize(s));   // the compaction buffers leave scope
+      return rt::kOk;
+    }
+
+    // banded copy from device CSR arrays: the diagonal set from a device bitmap, the same
+    // acceptance rule as build_banded, values scattered on the device
+    int build_banded_dev(const u32* drp, const u32* dcol, const double* dval, hipStream_t s)
+    {
+      banded = false;
+      if(rows != cols || rows == 0 || rows >= (u64(1) << 31)) return rt::kOk;
+      const u64 nbits = 2 * rows - 1, nw = (nbits + 31) / 32;
+      DevMem bits;
+      F3_RC(bits.alloc(sizeof(u32) * nw));
+      F3_CHECK(hipMemsetAsync(bits.p, 0, sizeof(u32) * nw, s));
+      k_diag_bitmap<<<setup_blocks(rows), 256, 0, s>>>(rows, drp, dcol, bits.template as<u32>());
+      std::vector<u32> hb(nw);
+      F3_CHECK(hipMemcpyAsync(hb.data(), bits.p, sizeof(u32) * nw, hipMemcpyDeviceToHost, s));
+      F3_CHECK(hipStreamSynchronize(s));
+      std::vector<u32> ho;
+      for(u64 w = 0; w < nw; ++w)
+        for(u32 b = hb[w]; b; b &= b - 1)
+        {
+          if(ho.size() >= kMaxDiag) return rt::kOk;
+          ho.push_back(u32(w * 32 + u64(__builtin_ctz(b))));
+        }
+      if(double(ho.size()) * double(rows) > 1.25 * double(std::max<u64>(nnz, 1))) return rt::kOk;
+      const u64 nd = ho.size(), padded = (rows + 63) / 64 * 64;
+      F3_RC(boff.alloc(sizeof(u32) * nd));
+      F3_RC(bval.alloc(sizeof(VT) * nd * padded));
+      F3_CHECK(hipMemcpyAsync(boff.p, ho.data(), sizeof(u32) * nd, hipMemcpyHostToDevice, s));
+      F3_CHECK(hipMemsetAsync(bval.p, 0, sizeof(VT) * nd * padded, s));
+      k_banded_scatter<VT><<<setup_blocks(rows), 256, 0, s>>>(rows, drp, dcol, dval, boff.template as<u32>(), u32(nd),
+                                                             bval.template as<VT>());
+      F3_RC(rt::check_launch());
+      F3_CHECK(hipStreamSynchronize(s));   // ho leaves scope
+      ndiag = u32(nd);
+      banded = true;
       return rt::kOk;
     }
 
@@ -453,15 +660,26 @@ namespace
         std::unique_ptr<Level<DT, VT>> l(new Level<DT, VT>());
         const HostLevel& h = lv[i];
         l->n = h.A.rows;
-        F3_RC(l->A.upload(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
+        if(h.A.on_device())
+          F3_RC(l->A.upload_dev(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
+        else
+          F3_RC(l->A.upload(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
         l->A.rows_per_block = cfg.rows_per_block;
         if(i + 1 < lv.size())
         {
           if(h.P.rows != l->n || h.R.cols != l->n || h.P.cols != lv[i + 1].A.rows || h.R.rows != lv[i + 1].A.rows)
             return rt::kErrArg;
           const bool tsell = want_sell && !cfg.transfer_csr;
-          F3_RC(l->P.upload(h.P, s, tsell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16));
-          F3_RC(l->R.upload(h.R, s, tsell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16));
+          if(h.P.on_device())
+          {
+            F3_RC(l->P.upload_dev(h.P, s, tsell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16, false));
+            F3_RC(l->R.upload_dev(h.R, s, tsell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16, false));
+          }
+          else
+          {
+            F3_RC(l->P.upload(h.P, s, tsell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16));
+            F3_RC(l->R.upload(h.R, s, tsell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16));
+          }
         }
         const std::vector<u32> m = make_mask(l->n, h.fidx);
         F3_RC(l->mask.alloc(sizeof(u32) * m.size()));
@@ -524,7 +742,10 @@ namespace
       }
       if(!shared)
       {
-        F3_RC(A64own.upload(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
+        if(h0.A.on_device())
+          F3_RC(A64own.upload_dev(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
+        else
+          F3_RC(A64own.upload(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
         A64 = &A64own;
       }
       F3_RC(r.alloc(sizeof(double) * n0));
@@ -1074,6 +1295,82 @@ int feat3_mgpcg_push_level(feat3_mgpcg* h, u64 n, const u32* row_ptr, const u32*
     L.R.val.assign(r_val, r_val + r_row_ptr[nc]);
     for(u32 c : L.P.col) if(c >= nc) return rt::kErrArg;
     for(u32 c : L.R.col) if(c >= n) return rt::kErrArg;
+  }
+  h->levels.push_back(std::move(L));
+  h->eng.reset();
+  return rt::kOk;
+}
+
+// the same level from device arrays (kept by the caller until feat3_mgpcg_init returns); levels
+// of at most 65536 rows are copied to the host here (the coarse solve and the single-workgroup
+// tail are set up there), larger ones are converted on the device by init
+int feat3_mgpcg_push_level_device(feat3_mgpcg* h, u64 n, const u32* row_ptr, const u32* col_ind, const double* val,
+                                  u64 nfilt, const u32* filter_idx, u64 nc, const u32* p_row_ptr, const u32* p_col_ind,
+                                  const double* p_val, const u32* r_row_ptr, const u32* r_col_ind, const double* r_val)
+{
+  if(!h || !row_ptr || n == 0 || !col_ind || !val) return rt::kErrArg;
+  if(nfilt && !filter_idx) return rt::kErrArg;
+  if(nc > 0 && (!p_row_ptr || !p_col_ind || !p_val || !r_row_ptr || !r_col_ind || !r_val)) return rt::kErrArg;
+  auto last = [&](const u32* rp, u64 rows, u64& out) -> int {
+    u32 v = 0;
+    F3_CHECK(hipMemcpy(&v, rp + rows, sizeof(u32), hipMemcpyDeviceToHost));
+    out = v;
+    return rt::kOk;
+  };
+  u64 nnz = 0, pn = 0, rn = 0;
+  F3_RC(last(row_ptr, n, nnz));
+  if(nc > 0) { F3_RC(last(p_row_ptr, n, pn)); F3_RC(last(r_row_ptr, nc, rn)); }
+  std::vector<u32> fidx(nfilt);
+  if(nfilt) F3_CHECK(hipMemcpy(fidx.data(), filter_idx, sizeof(u32) * nfilt, hipMemcpyDeviceToHost));
+  if(n <= 65536)
+  {
+    // small: through the host path
+    std::vector<u32> rp(n + 1), ci(nnz), prp, pci, rrp, rci;
+    std::vector<double> v(nnz), pv, rv;
+    F3_CHECK(hipMemcpy(rp.data(), row_ptr, sizeof(u32) * (n + 1), hipMemcpyDeviceToHost));
+    if(nnz)
+    {
+      F3_CHECK(hipMemcpy(ci.data(), col_ind, sizeof(u32) * nnz, hipMemcpyDeviceToHost));
+      F3_CHECK(hipMemcpy(v.data(), val, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    }
+    if(nc > 0)
+    {
+      prp.resize(n + 1); pci.resize(pn); pv.resize(pn); rrp.resize(nc + 1); rci.resize(rn); rv.resize(rn);
+      F3_CHECK(hipMemcpy(prp.data(), p_row_ptr, sizeof(u32) * (n + 1), hipMemcpyDeviceToHost));
+      F3_CHECK(hipMemcpy(rrp.data(), r_row_ptr, sizeof(u32) * (nc + 1), hipMemcpyDeviceToHost));
+      if(pn)
+      {
+        F3_CHECK(hipMemcpy(pci.data(), p_col_ind, sizeof(u32) * pn, hipMemcpyDeviceToHost));
+        F3_CHECK(hipMemcpy(pv.data(), p_val, sizeof(double) * pn, hipMemcpyDeviceToHost));
+      }
+      if(rn)
+      {
+        F3_CHECK(hipMemcpy(rci.data(), r_col_ind, sizeof(u32) * rn, hipMemcpyDeviceToHost));
+        F3_CHECK(hipMemcpy(rv.data(), r_val, sizeof(double) * rn, hipMemcpyDeviceToHost));
+      }
+    }
+    if(rp[0] != 0 || rp[n] != nnz) return rt::kErrArg;
+    for(u64 r = 0; r < n; ++r) if(rp[r] > rp[r + 1]) return rt::kErrArg;
+    if(nc > 0)
+    {
+      if(prp[0] != 0 || rrp[0] != 0) return rt::kErrArg;
+      for(u64 r = 0; r < n; ++r) if(prp[r] > prp[r + 1]) return rt::kErrArg;
+      for(u64 r = 0; r < nc; ++r) if(rrp[r] > rrp[r + 1]) return rt::kErrArg;
+    }
+    return feat3_mgpcg_push_level(h, n, rp.data(), ci.data(), v.data(), nfilt, fidx.data(), nc, prp.data(), pci.data(),
+                                  pv.data(), rrp.data(), rci.data(), rv.data());
+  }
+  for(u32 i : fidx) if(i >= n) return rt::kErrArg;
+  HostLevel L;
+  L.A.rows = n; L.A.cols = n;
+  L.A.drp = row_ptr; L.A.dcol = col_ind; L.A.dval = val; L.A.dnnz = nnz;
+  L.fidx = std::move(fidx);
+  if(nc > 0)
+  {
+    L.P.rows = n; L.P.cols = nc;
+    L.P.drp = p_row_ptr; L.P.dcol = p_col_ind; L.P.dval = p_val; L.P.dnnz = pn;
+    L.R.rows = nc; L.R.cols = n;
+    L.R.drp = r_row_ptr; L.R.dcol = r_col_ind; L.R.dval = r_val; L.R.dnnz = rn;
   }
   h->levels.push_back(std::move(L));
   h->eng.reset();

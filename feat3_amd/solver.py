@@ -51,8 +51,11 @@ class MGPCG:
         self._h = h
         self.inner = inner
         self.max_iter = max_iter
-        for L in hierarchy.levels:
+        # a Hierarchy (host arrays) or the device-assembled levels of problems.hierarchy_device
+        levels = hierarchy.levels if hasattr(hierarchy, "levels") else hierarchy
+        for L in levels:
             A = L.A
+            on_device = not isinstance(A.row_ptr, np.ndarray)
             if L.P is not None:
                 P, R = L.P, L.R
                 nc = P.cols
@@ -60,8 +63,9 @@ class MGPCG:
             else:
                 nc = 0
                 pa = (ctypes.c_void_p(0),) * 6
-            capi.call("feat3_mgpcg_push_level", h, A.rows, _ptr(A.row_ptr), _ptr(A.col_ind), _ptr(A.val),
-                      L.bnd.size, _ptr(L.bnd), nc, *pa)
+            nb = L.bnd.numel() if on_device else L.bnd.size
+            capi.call("feat3_mgpcg_push_level_device" if on_device else "feat3_mgpcg_push_level", h, A.rows,
+                      _ptr(A.row_ptr), _ptr(A.col_ind), _ptr(A.val), nb, _ptr(L.bnd), nc, *pa)
         capi.call("feat3_mgpcg_set_smoother", h, omega_jacobi, omega_richardson, pre, post)
         capi.call("feat3_mgpcg_set_tolerances", h, tol_rel, tol_abs, tol_abs_low, min_iter, max_iter)
         capi.call("feat3_mgpcg_set_precond", h, {"mg": 0, "none": 1}[precond])
@@ -73,6 +77,8 @@ class MGPCG:
 
     # ---- lifecycle ------------------------------------------------------------------------
     def init(self) -> "MGPCG":
+        """init_symbolic + init_numeric; device-assembled levels are converted on the device here
+        (they must stay alive until init returns)."""
         capi.call("feat3_mgpcg_init", self._h)
         return self
 

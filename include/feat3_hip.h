@@ -287,6 +287,14 @@ int feat3_mgpcg_push_level(feat3_mgpcg* h, uint64_t n, const uint32_t* row_ptr, 
                            const double* val, uint64_t nfilt, const uint32_t* filter_idx, uint64_t nc,
                            const uint32_t* p_row_ptr, const uint32_t* p_col_ind, const double* p_val,
                            const uint32_t* r_row_ptr, const uint32_t* r_col_ind, const double* r_val);
+/* the same with device arrays (e.g. from feat3_device_laplace / feat3_device_transfer), valid until
+ * feat3_mgpcg_init returns: init validates them on the device (rows ascending, columns strictly
+ * ascending and in range) and builds the level formats there; levels of at most 65536 rows are
+ * copied to the host at this call.  No reference counterpart (device-resident setup). */
+int feat3_mgpcg_push_level_device(feat3_mgpcg* h, uint64_t n, const uint32_t* row_ptr, const uint32_t* col_ind,
+                                  const double* val, uint64_t nfilt, const uint32_t* filter_idx, uint64_t nc,
+                                  const uint32_t* p_row_ptr, const uint32_t* p_col_ind, const double* p_val,
+                                  const uint32_t* r_row_ptr, const uint32_t* r_col_ind, const double* r_val);
 /* Jacobi damping, Richardson damping, pre/post smoothing steps (>= 1) */
 int feat3_mgpcg_set_smoother(feat3_mgpcg* h, double omega_jacobi, double omega_richardson, int pre, int post);
 /* IterativeSolver::set_tol_rel/set_tol_abs/set_min_iter/set_max_iter (iterative.hpp);

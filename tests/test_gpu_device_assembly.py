@@ -51,3 +51,40 @@ def test_hierarchy_bitwise(torch_cuda, dim, deg, lvl):
         if b.P is not None:
             same_csr(a.P, b.P, f"P level {i}")
             same_csr(a.R, b.R, f"R level {i}")
+
+
+@pytest.mark.parametrize("dim,deg,lvl,fmt,inner", [(3, 1, 6, "banded", "f32"), (3, 1, 6, "csr", "f64"),
+                                                   (3, 1, 6, "sell", "f16"), (2, 2, 7, "banded", "f32"),
+                                                   (2, 1, 9, "banded", "f64")])
+def test_solver_from_device_levels(torch_cuda, dim, deg, lvl, fmt, inner):
+    """A solver initialised from the device-assembled levels (formats built on the device for
+    levels above 65536 rows) solves bitwise like the one initialised from the host hierarchy."""
+    torch = torch_cuda
+    print(f"case device init {dim}D Q{deg} L{lvl} {fmt} {inner}", flush=True)
+    from feat3_amd.solver import MGPCG
+    H = pb.hierarchy(dim, deg, lvl, 1)
+    levels = pb.hierarchy_device(dim, deg, lvl, 1)
+    db = torch.from_numpy(pb.filtered_rhs(H, 0)).cuda()
+    out = []
+    for src in (H, levels):
+        s = MGPCG(src, inner=inner, fmt=fmt).init()
+        info = s.info()
+        x = torch.zeros(H.n, dtype=torch.float64, device="cuda")
+        st, it = s.correct(x, db)
+        out.append((st, it, x.cpu().numpy(), info))
+        s.done()
+    assert out[0][0] == out[1][0] == 2 and out[0][1] == out[1][1]
+    assert out[0][3] == out[1][3]
+    assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_device_levels_are_validated(torch_cuda):
+    """A device level whose columns leave the matrix is rejected by init (validated on the
+    device before any kernel uses it)."""
+    torch = torch_cuda
+    from feat3_amd import capi
+    from feat3_amd.solver import MGPCG
+    levels = pb.hierarchy_device(3, 1, 6, 1)
+    levels[0].A.col_ind[7] = levels[0].A.rows + 5
+    with pytest.raises(capi.Feat3HipError):
+        MGPCG(levels, inner="f32", fmt="banded").init()
