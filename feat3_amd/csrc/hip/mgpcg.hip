@@ -117,14 +117,18 @@ namespace
     for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
       out[i] = in[i];
   }
-  // bit (col - row + rows - 1) for every entry: the set of diagonals
+  // bit (col - row + rows - 1) for every entry: the set of diagonals.  Banded matrices set the
+  // same few bits from every row, so a bit is tested with a plain load first and the atomic
+  // issued only while it reads clear (a repeated OR is harmless; unconditional atomics on ~27
+  // words serialised 57 M entries: 202 ms for 3D Q1 L7)
   __global__ void k_diag_bitmap(u64 rows, const u32* __restrict__ rp, const u32* __restrict__ col, u32* bits)
   {
     for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
       for(u64 k = rp[r]; k < rp[r + 1]; ++k)
       {
         const u64 o = u64(col[k]) + rows - 1 - r;
-        atomicOr(&bits[o >> 5], 1u << (o & 31u));
+        const u32 m = 1u << (o & 31u);
+        if((__atomic_load_n(&bits[o >> 5], __ATOMIC_RELAXED) & m) == 0u) atomicOr(&bits[o >> 5], m);
       }
   }
   // chunk-major banded values from CSR (offsets stored as off + rows - 1, ascending)
