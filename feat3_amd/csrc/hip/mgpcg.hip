@@ -490,8 +490,17 @@ namespace
         // streaming evaluator: rounds of R rows filling about 85% of the LDS products
         const double mean = m.rows ? double(m.nnz) / double(m.rows) : 1.0;
         const double fit = 0.85 * double(mg::CsrStreamRows<DT, VT>::CAP) / (mean > 1.0 ? mean : 1.0);
-        const u64 R = fit < 1.0 ? 1 : (fit > 1024.0 ? 1024 : u64(fit));
-        f(mg::CsrStreamRows<DT, VT>{rp, col, val, m.rows, R}, grid(1));
+        u64 R = fit < 1.0 ? 1 : (fit > 1024.0 ? 1024 : u64(fit));
+        if(R >= 64) R = R / 64 * 64;
+        // one round of R rows per block (a full LDS of products each) where rounds are short
+        // (R <= 256 rows: one row per thread in the summing phase) and that still gives at least
+        // 2048 blocks (8 per CU); otherwise the default grid.  Measured on the restriction of
+        // 2D Q2 L9 (V-cycle 0.61 -> 0.59 ms); full 1024-row rounds of the 3D prolongation and
+        // too few blocks on mid-size levels were slower.
+        const bool full_rounds = R >= 64 && R <= 256 && m.rows >= R * 2048;
+        const mg::RowGrid g = m.rows_per_block ? mg::row_grid_fixed(m.rows, m.rows_per_block)
+                                               : (full_rounds ? mg::row_grid_fixed(m.rows, R) : grid(1));
+        f(mg::CsrStreamRows<DT, VT>{rp, col, val, m.rows, R}, g);
         break;
       }
       case 2: f(mg::CsrRows<DT, VT, 2>{rp, col, val, m.rows}, grid(2)); break;
