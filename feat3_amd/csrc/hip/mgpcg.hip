@@ -1477,7 +1477,9 @@ int feat3_mgpcg_init(feat3_mgpcg* h)
 int feat3_mgpcg_correct(feat3_mgpcg* h, void* caller_stream, double* x, const double* b, int* status, u64* iters)
 {
   if(!h || !h->eng) return rt::kErrState;
-  if(caller_stream) { if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime; }
+  // the caller's stream (NULL: the null stream) has produced the inputs: wait for it (the
+  // solver's own stream does not synchronise with it implicitly)
+  if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime;
   return h->eng->solve(x, b, true, status, iters);
 }
 
@@ -1485,7 +1487,9 @@ int feat3_mgpcg_correct(feat3_mgpcg* h, void* caller_stream, double* x, const do
 int feat3_mgpcg_apply(feat3_mgpcg* h, void* caller_stream, double* cor, const double* def, int* status, u64* iters)
 {
   if(!h || !h->eng) return rt::kErrState;
-  if(caller_stream) { if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime; }
+  // the caller's stream (NULL: the null stream) has produced the inputs: wait for it (the
+  // solver's own stream does not synchronise with it implicitly)
+  if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime;
   return h->eng->solve(cor, def, false, status, iters);
 }
 
@@ -1493,7 +1497,9 @@ int feat3_mgpcg_apply(feat3_mgpcg* h, void* caller_stream, double* cor, const do
 int feat3_mgpcg_vcycle(feat3_mgpcg* h, void* caller_stream, double* cor, const double* def)
 {
   if(!h || !h->eng) return rt::kErrState;
-  if(caller_stream) { if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime; }
+  // the caller's stream (NULL: the null stream) has produced the inputs: wait for it (the
+  // solver's own stream does not synchronise with it implicitly)
+  if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime;
   return h->eng->vcycle(cor, def);
 }
 

@@ -238,10 +238,22 @@ namespace
     std::vector<std::unique_ptr<Level>> L;
     DevMem ainv, a64, r, p, q, st, cr, hist, partials, any;
     u64 m0 = 0, hist_cap = 0, ncoarse_rows = 0;
-    int* pinned = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr}, t0 = nullptr, t1 = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    // host copies of the system states (K SysState each) of the last kRing iterations
+    static constexpr int kRing = 4;
+    batch::SysState* pinned = nullptr;
+    hipEvent_t ev[kRing] = {nullptr, nullptr, nullptr, nullptr}, t0 = nullptr, t1 = nullptr;
+    // an iteration as two graphs: [0] outer half (q = A p, alpha, x and r, criteria),
+    // [1] preconditioner half (V-cycle, beta, p)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    void drop_graphs()
+    {
+      for(int i = 0; i < 2; ++i)
+      {
+        if(gexec[i]) { (void)hipGraphExecDestroy(gexec[i]); gexec[i] = nullptr; }
+        if(graph[i]) { (void)hipGraphDestroy(graph[i]); graph[i] = nullptr; }
+      }
+    }
     double* x_cur = nullptr;
     double* x_graph = nullptr;
     std::vector<u64> last_iters;
@@ -255,8 +267,7 @@ namespace
     }
     ~Engine() override
     {
-      if(gexec) (void)hipGraphExecDestroy(gexec);
-      if(graph) (void)hipGraphDestroy(graph);
+      drop_graphs();
       if(pinned) (void)hipHostFree(pinned);
       for(auto& e : ev) if(e) (void)hipEventDestroy(e);
       if(t0) (void)hipEventDestroy(t0);
@@ -400,13 +411,13 @@ namespace
       F3_RC(hist.alloc(sizeof(double) * hist_cap * K));
       F3_RC(partials.alloc(sizeof(double) * 8192 * u64(K)));
       F3_RC(any.alloc(64));
-      if(!pinned) F3_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), 64, hipHostMallocDefault));
+      if(pinned) { (void)hipHostFree(pinned); pinned = nullptr; }
+      F3_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), sizeof(batch::SysState) * u64(K) * kRing, hipHostMallocDefault));
       for(auto& e : ev) if(!e) F3_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       if(!t0) F3_CHECK(hipEventCreate(&t0));
       if(!t1) F3_CHECK(hipEventCreate(&t1));
       F3_CHECK(hipStreamSynchronize(s));
-      if(gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
-      if(graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+      drop_graphs();
       return rt::kOk;
     }
 
@@ -525,7 +536,8 @@ namespace
       });
     }
 
-    int enqueue_iteration()
+    // outer half of a PCG iteration: q = A p, alpha, x and r updates, the criteria
+    int enqueue_outer_half()
     {
       double* part = partials.template as<double>();
       batch::SysState* Sst = st.template as<batch::SysState>();
@@ -533,13 +545,22 @@ namespace
       F3_RC(outer_spmv_dot(nb));
       fin<mg::kFinAlpha>(nb);
       const unsigned sb = stream_blocks(m0, G);
-      const DT* z = L[0]->xfinal;
       F3_RC(with_stream(G, [&](auto gg) {
         constexpr int Gc = decltype(gg)::value;
         batch::k_pcg_update<Gc, B><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, p.template as<double>(), q.template as<double>(), x_cur,
                                                          r.template as<double>(), Sst, anyp(), part);
       }));
       fin<mg::kFinNewDefect>(sb);
+      return rt::check_launch();
+    }
+
+    // preconditioner half: z = MG(r), beta, p = z + beta p
+    int enqueue_precond_half()
+    {
+      double* part = partials.template as<double>();
+      batch::SysState* Sst = st.template as<batch::SysState>();
+      const unsigned sb = stream_blocks(m0, G);
+      const DT* z = L[0]->xfinal;
       F3_RC(enqueue_vcycle(r.template as<double>(), anyp()));
       F3_RC(with_stream(G, [&](auto gg) {
         constexpr int Gc = decltype(gg)::value;
@@ -578,33 +599,85 @@ namespace
       }));
       fin<mg::kFinGamma>(sb);
       F3_RC(rt::check_launch());
-      if(cfg.use_graph && (gexec == nullptr || x_graph != x))
+      if(cfg.use_graph && (gexec[0] == nullptr || x_graph != x))
       {
-        if(gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
-        if(graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
-        F3_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_iteration();
-        hipGraph_t gr = nullptr;
-        const hipError_t e = hipStreamEndCapture(s, &gr);
-        if(rc) { if(gr) (void)hipGraphDestroy(gr); return rc; }
-        if(e != hipSuccess) return rt::kErrRuntime;
-        graph = gr;
-        F3_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        drop_graphs();
+        for(int half = 0; half < 2; ++half)
+        {
+          F3_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          const int rc = half == 0 ? enqueue_outer_half() : enqueue_precond_half();
+          hipGraph_t gr = nullptr;
+          const hipError_t e = hipStreamEndCapture(s, &gr);
+          if(rc) { if(gr) (void)hipGraphDestroy(gr); return rc; }
+          if(e != hipSuccess) return rt::kErrRuntime;
+          graph[half] = gr;
+          F3_CHECK(hipGraphInstantiate(&gexec[half], graph[half], nullptr, nullptr, 0));
+        }
         x_graph = x;
       }
-      u64 launched = 0;
-      while(launched < cfg.max_iter + 1)
-      {
-        if(cfg.use_graph) F3_CHECK(hipGraphLaunch(gexec, s));
-        else F3_RC(enqueue_iteration());
-        F3_CHECK(hipMemcpyAsync(&pinned[launched % 2], any.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        F3_CHECK(hipEventRecord(ev[launched % 2], s));
-        ++launched;
-        if(launched >= 2)
+      auto launch_half = [&](int half) -> int {
+        if(cfg.use_graph) F3_CHECK(hipGraphLaunch(gexec[half], s));
+        else F3_RC(half == 0 ? enqueue_outer_half() : enqueue_precond_half());
+        return rt::kOk;
+      };
+      auto post_state = [&](u64 k) -> int {
+        F3_CHECK(hipMemcpyAsync(pinned + (k % kRing) * u64(K), st.p, sizeof(batch::SysState) * u64(K), hipMemcpyDeviceToHost, s));
+        F3_CHECK(hipEventRecord(ev[k % kRing], s));
+        return rt::kOk;
+      };
+      // defects of every system after iterations 0..known, as read by the host
+      std::vector<std::vector<double>> seen;
+      bool running = true;
+      auto read_state = [&]() -> int {
+        const u64 k = seen.size();
+        F3_CHECK(hipEventSynchronize(ev[k % kRing]));
+        const batch::SysState* h = pinned + (k % kRing) * u64(K);
+        std::vector<double> d(K);
+        running = false;
+        for(int j = 0; j < K; ++j)
         {
-          F3_CHECK(hipEventSynchronize(ev[(launched - 2) % 2]));
-          if(pinned[(launched - 2) % 2] == 0) break;
+          d[j] = h[j].status == mg::kProgress ? h[j].def_cur : -1.0;
+          running = running || h[j].status == mg::kProgress;
         }
+        seen.push_back(std::move(d));
+        return rt::kOk;
+      };
+      // the same host loop as the single-system solver: one iteration in flight ahead; when every
+      // running system's defect history predicts that the next iteration may meet its criterion,
+      // that iteration's outer half is read before its preconditioner half is launched
+      const u64 max_launch = cfg.max_iter + 1;
+      F3_RC(post_state(0));
+      u64 k = 0;
+      while(k < max_launch && running)
+      {
+        const u64 next = k + 1;
+        bool near = next >= cfg.max_iter;
+        if(!near && seen.size() >= 2)
+        {
+          const std::size_t jj = seen.size() - 1;
+          near = true;
+          for(int j = 0; j < K && near; ++j)
+          {
+            const double dj = seen[jj][j], dp = seen[jj - 1][j], d0 = seen[0][j];
+            if(dj < 0.0) continue;   // this system has stopped
+            const double rate = dp > 0.0 ? dj / dp : 1.0;
+            const double goal = std::min(cfg.tol_abs, std::max(cfg.tol_rel * d0, cfg.tol_abs_low));
+            near = dj * std::pow(std::min(rate, 1.0), double(next - jj)) <= 10.0 * goal;
+          }
+        }
+        F3_RC(launch_half(0));
+        F3_RC(post_state(next));
+        if(near)
+        {
+          while(seen.size() <= next && running) F3_RC(read_state());
+          if(!running) { k = next; break; }
+          F3_RC(launch_half(1));
+          k = next;
+          continue;
+        }
+        F3_RC(launch_half(1));
+        k = next;
+        while(seen.size() < k && running) F3_RC(read_state());
       }
       F3_CHECK(hipStreamSynchronize(s));
       std::vector<batch::SysState> out(K);
@@ -836,13 +909,16 @@ int feat3_batch_init(feat3_batch* h)
 int feat3_batch_correct(feat3_batch* h, void* caller_stream, double* x, const double* b, int* status, u64* iters)
 {
   if(!h || !h->eng) return rt::kErrState;
-  if(caller_stream && hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime;
+  // the caller's stream (NULL: the null stream) has produced x and b: wait for it
+  if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime;
   return h->eng->solve(x, b, status, iters);
 }
 
 int feat3_batch_set_values(feat3_batch* h, int level, const double* values)
 {
   if(!h || !h->eng) return rt::kErrState;
+  // values produced on the null stream are complete before the solver's stream reads them
+  if(hipStreamSynchronize(nullptr) != hipSuccess) return rt::kErrRuntime;
   return h->eng->set_values(level, values);
 }
 

@@ -318,7 +318,8 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value);
 /* init_symbolic + init_numeric (solver/base.hpp:227-260) */
 int feat3_mgpcg_init(feat3_mgpcg* h);
 /* Solver::solve / IterativeSolver::correct (iterative.hpp:600,926-935): x, b FP64 device
- * vectors; caller_stream is synchronised first; blocks until the solve has ended */
+ * vectors; caller_stream (NULL: the null stream) is synchronised first, since the solver runs
+ * on a stream of its own; blocks until the solve has ended */
 int feat3_mgpcg_correct(feat3_mgpcg* h, void* caller_stream, double* x, const double* b, int* status, uint64_t* iters);
 /* SolverBase::apply(cor, def) (pcg.hpp:141-156) */
 int feat3_mgpcg_apply(feat3_mgpcg* h, void* caller_stream, double* cor, const double* def, int* status, uint64_t* iters);
@@ -375,7 +376,8 @@ int feat3_batch_correct(feat3_batch* h, void* caller_stream, double* x, const do
 /* replaces the values of one level (0 = finest) from a device array in the layout of push_level
  * (FP64; [entry][h][w] or [entry][system]): converts to the storage precision, recomputes the
  * Jacobi diagonal, the FP64 outer copy (level 0) and the coarse inverses (coarsest level; a
- * small device-to-host copy); graph, transfers and Dirichlet nodes are kept.  Blocks until done. */
+ * small device-to-host copy); graph, transfers and Dirichlet nodes are kept.  The null stream is
+ * synchronised first (values produced on another stream must be complete); blocks until done. */
 int feat3_batch_set_values(feat3_batch* h, int level, const double* values);
 /* defect norms of system `system` in the last solve */
 int feat3_batch_history(feat3_batch* h, int system, double* out_host, uint64_t cap, uint64_t* len);

@@ -158,3 +158,29 @@ def test_tile_order_does_not_change_results(torch):
     b = gpu_batch(torch, H, B, 8, values=vals, block=3, inner="f64", options={"tile_rows": 64})
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     assert rel(b[2], a[2]) <= 1e-13
+
+
+def test_inputs_from_the_default_stream_are_awaited(torch):
+    """Regression: the solver runs on a stream of its own and must wait for the caller's stream
+    also when that is the default (null) stream, whose handle is 0.  x is refilled with zeros
+    behind a long kernel on the default stream; the solve has to start from zero, i.e. repeat
+    the reference iterations and solution exactly."""
+    from feat3_amd.batch import BatchMGPCG
+    H = pb.hierarchy(2, 1, 7, 1)
+    nsys = 4
+    b = pb.filtered_rhs(H, 0)
+    B = torch.from_numpy(np.ascontiguousarray(np.repeat(b[:, None, None], nsys, axis=2))).cuda()
+    s = BatchMGPCG(H, nsys, inner="f64").init()
+    x = torch.zeros(s.shape, dtype=torch.float64, device="cuda")
+    st0, it0 = s.correct(x, B)
+    ref = x.clone()
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(3):
+        # x holds the converged solution; keep the default stream busy, then refill x with zeros
+        big = torch.randn(4096, 4096, device="cuda")
+        for _ in range(4):
+            big = big @ big * 1e-3
+        x.zero_()
+        st, it = s.correct(x, B, stream=stream)
+        assert list(it) == list(it0) and np.array_equal(x.cpu().numpy(), ref.cpu().numpy())
+    s.done()
