@@ -90,21 +90,8 @@ namespace
   // 85% of the capacity (16 KB of products: up to 8 blocks per CU).  Each thread issues the
   // loads of U entries (values and columns, then the x gathers) before using any of them.
   // -------------------------------------------------------------------------------------
-#ifndef F3_STREAM_KB
-#define F3_STREAM_KB 16
-#endif
-#ifndef F3_STREAM_U
-#define F3_STREAM_U 8
-#endif
-#ifndef F3_STREAM_NT
-#define F3_STREAM_NT 1
-#endif
-#if F3_STREAM_NT
-#define F3_STREAM_LD(p) dev::ld_stream(p)
-#else
-#define F3_STREAM_LD(p) (*(p))
-#endif
-  constexpr int kStreamBytes = F3_STREAM_KB * 1024;
+  constexpr int kStreamBytes = 16 * 1024;   // products per block in LDS (8 blocks per CU)
+  constexpr int kStreamLoads = 8;           // entries per thread whose loads are in flight together
 
   template<int BS, typename DT>
   constexpr int stream_cap() { return kStreamBytes / (BS * int(sizeof(DT))); }
@@ -127,10 +114,10 @@ namespace
       const u64 my = r0 + threadIdx.x;
       const u64 mk0 = my < r1 ? u64(rp[my]) : 0, mk1 = my < r1 ? u64(rp[my + 1]) : 0;
       if constexpr(BS == 1)
-        dev::stage_products<F3_STREAM_U, kBlock>(val, col, e0, ne, x, prod);
+        dev::stage_products<kStreamLoads, kBlock>(val, col, e0, ne, x, prod);
       else
       {
-      constexpr int U = (F3_STREAM_U + 1) / 2;
+      constexpr int U = kStreamLoads / 2;
       for(u64 k0 = threadIdx.x; k0 < ne; k0 += u64(kBlock) * U)
       {
         DT v[U];
@@ -139,8 +126,8 @@ namespace
         for(int j = 0; j < U; ++j)
         {
           const u64 k = k0 + u64(j) * kBlock;
-          v[j] = k < ne ? DT(F3_STREAM_LD(val + e0 + k)) : DT(0);
-          c[j] = k < ne ? u64(F3_STREAM_LD(col + e0 + k)) : 0;
+          v[j] = k < ne ? DT(dev::ld_stream(val + e0 + k)) : DT(0);
+          c[j] = k < ne ? u64(dev::ld_stream(col + e0 + k)) : 0;
         }
         DT xv[U][BS];
 #pragma unroll
