@@ -446,14 +446,14 @@ def main():
             assert st == 2, f"solve did not converge (status {st})"
         return times, iters
 
-    def variant(inner, steps, warmup, fmt=None):
+    def variant(inner, steps, warmup, fmt=None, cycle="v"):
         fmt = fmt or args.fmt
         t0 = time.time()
-        s = MGPCG(H, inner=inner, fmt=fmt).init()
+        s = MGPCG(H, inner=inner, fmt=fmt, cycle=cycle).init()
         init_s = time.time() - t0
-        if inner == args.inner and fmt == args.fmt:
+        if inner == args.inner and fmt == args.fmt and cycle == "v":
             setup["host_init_s"] = init_s
-        log(f"init {inner}/{fmt}: {init_s:.1f} s")
+        log(f"init {inner}/{fmt}/{cycle}: {init_s:.1f} s")
         times, iters = run(s, steps, warmup)
         hist = s.history()
         sol = x.cpu().numpy().copy()
@@ -467,8 +467,8 @@ def main():
                  dof_iter_per_s=n * sum(iters) / sum(times), final_rel_residual=float(hist[-1] / hist[0]),
                  sweep_us=sweep_ms * 1e3, sweep_bytes=sb, sweep_GBs=sb / (sweep_ms * 1e-3) / 1e9,
                  vcycle_ms=vc_ms, kernels_per_vcycle=info["kernels_per_vcycle"])
-        log(f"{inner}/{fmt}: {ms:.3f} ms/solve, it {iters}, rel.res {r['final_rel_residual']:.2e}, "
-            f"sweep {r['sweep_us']:.1f} us {r['sweep_GBs']:.0f} GB/s, V-cycle {vc_ms:.3f} ms")
+        log(f"{inner}/{fmt}/{cycle}: {ms:.3f} ms/solve, it {iters}, rel.res {r['final_rel_residual']:.2e}, "
+            f"sweep {r['sweep_us']:.1f} us {r['sweep_GBs']:.0f} GB/s, {cycle.upper()}-cycle {vc_ms:.3f} ms")
         return r, sol, times, iters
 
     head, xm, times, iters = variant(args.inner, args.steps, args.warmup)
@@ -489,6 +489,10 @@ def main():
                 continue
             r, _, _, _ = variant(args.inner, max(2, args.steps // 2), 1, fmt=fmt)
             variants[f"{args.inner}/{fmt}"] = (r, None)
+        # the reference's F- and W-cycles as the preconditioner (multigrid.hpp:1366-1519)
+        for cycle in ("f", "w"):
+            r, _, _, _ = variant(args.inner, max(2, args.steps // 2), 1, cycle=cycle)
+            variants[f"{args.inner}/{args.fmt}/{cycle}-cycle"] = (r, None)
     fp64 = variants.get("f64")
     if fp64:
         x64 = fp64[1]

@@ -7,7 +7,7 @@
 // and configures the fused solver for the shape it implements:
 //
 //   root:      type = pcg                      tol_rel, tol_abs, tol_abs_low, max_iter, min_iter
-//   precon:    type = mg, cycle = v            hierarchy, lvl_min = -1, lvl_max = 0,
+//   precon:    type = mg, cycle = v | f | w    hierarchy, lvl_min = -1, lvl_max = 0,
 //                                              memorytype = hip, datatype = float | double
 //                                              (+ valuetype = half: binary16 matrix values)
 //   hierarchy: type = hierarchy                smoother, coarse
@@ -133,17 +133,22 @@ namespace feat3_amd
         const std::string pre = get(root, "precon", "none");
         double oj = 0.5, orr = 1.0;
         int steps = 4;
+        MultiGridCycle cycle = MultiGridCycle::V;
         if(pre != "none")
         {
           const Section& mg = section(base, pre, name + ".precon");
           if(get(mg, "type", "") != "mg") throw InternalError(pre + ".type: the preconditioner must be a multigrid (type = mg)");
+          if(!has(mg, "cycle")) throw InternalError(pre + ": mg section without cycle key is not allowed");
           const std::string cyc = get(mg, "cycle", "");
-          if(cyc != "v" && cyc != "V") throw InternalError(pre + ".cycle = " + cyc + ": the device multigrid runs V-cycles");
+          if(cyc == "v" || cyc == "V") cycle = MultiGridCycle::V;
+          else if(cyc == "f" || cyc == "F") cycle = MultiGridCycle::F;
+          else if(cyc == "w" || cyc == "W") cycle = MultiGridCycle::W;
+          else throw InternalError(pre + ".cycle = " + cyc + ": mg cycle unknown (v, f or w)");
           if(num(mg, pre, "lvl_max", 0.0) != 0.0 || num(mg, pre, "lvl_min", -1.0) != -1.0)
-            throw InternalError(pre + ": the V-cycle spans the whole hierarchy (lvl_max = 0, lvl_min = -1)");
+            throw InternalError(pre + ": the cycle spans the whole hierarchy (lvl_max = 0, lvl_min = -1)");
           if(get(mg, "adapt_cgc", "fixed") != "fixed") throw InternalError(pre + ".adapt_cgc: the correction is fixed");
           const std::string mem = get(mg, "memorytype", "hip");
-          if(mem != "hip" && mem != "main") throw InternalError(pre + ".memorytype = " + mem + ": the V-cycle runs on the device");
+          if(mem != "hip" && mem != "main") throw InternalError(pre + ".memorytype = " + mem + ": the cycle runs on the device");
           const std::string dt = get(mg, "datatype", "double"), vt = get(mg, "valuetype", "");
           if(vt == "half") inner = FusedMGPCG::fp16;
           else if(!vt.empty()) throw InternalError(pre + ".valuetype = " + vt + ": half or absent");
@@ -179,7 +184,7 @@ namespace feat3_amd
             steps = int(mx);
           }
           else
-            throw InternalError(sn + ".type = " + st + ": the device V-cycle smooths with richardson(jacobi) or jacobi");
+            throw InternalError(sn + ".type = " + st + ": the device multigrid smooths with richardson(jacobi) or jacobi");
           const std::string cn = get(h, "coarse", "");
           const std::string ct = get(section(base, cn, hn + ".coarse"), "type", "");
           if(ct != "umfpack" && ct != "cusolver" && ct != "direct")
@@ -192,6 +197,7 @@ namespace feat3_amd
         auto solver = std::make_shared<FusedMGPCG>(inner, f);
         solver->set_smoother(oj, orr, steps, steps);
         if(pre == "none") solver->set_precond_none();
+        else if(cycle != MultiGridCycle::V) solver->set_cycle(cycle);
         if(has(root, "tol_rel")) solver->set_tol_rel(num(root, name, "tol_rel", 0.0));
         if(has(root, "tol_abs")) solver->set_tol_abs(num(root, name, "tol_abs", 0.0));
         if(has(root, "tol_abs_low")) solver->set_tol_abs_low(num(root, name, "tol_abs_low", 0.0));

@@ -386,8 +386,11 @@ namespace feat3_amd
     // ---- the fused device-resident solver ---------------------------------------------------
     //
     // PCG( MultiGrid-V( Richardson(omega_R, Jacobi(omega_J)) x pre/post ), coarse = exact FP64 ),
-    // FP64 outer iteration, V-cycle in FP64 / FP32 / FP16-stored values.  The level data is
+    // FP64 outer iteration, multigrid cycle (V, F or W) in FP64 / FP32 / FP16-stored values.  The level data is
     // given as host CSR arrays (finest first), exactly what the hierarchy above holds.
+    // multigrid cycle kinds (multigrid.hpp:26-34)
+    enum class MultiGridCycle { V, F, W };
+
     class FusedMGPCG : public IterativeSolver<LAFEM::DenseVector<Mem::HIP, double, std::uint32_t>>
     {
       feat3_mgpcg* _h = nullptr;
@@ -416,8 +419,10 @@ namespace feat3_amd
       {
         hip_check(feat3_mgpcg_set_smoother(_h, omega_jacobi, omega_richardson, pre, post), "FusedMGPCG::set_smoother");
       }
-      // plain CG (no preconditioner) instead of the V-cycle
+      // plain CG (no preconditioner) instead of the multigrid cycle
       void set_precond_none() { hip_check(feat3_mgpcg_set_precond(_h, 1), "FusedMGPCG::set_precond_none"); }
+      // the multigrid cycle (default V); takes effect at the next init
+      void set_cycle(MultiGridCycle c) { set_option("cycle", c == MultiGridCycle::V ? 0.0 : c == MultiGridCycle::F ? 1.0 : 2.0); }
       // graph replay of the iteration (default on)
       void set_graph(bool on) { hip_check(feat3_mgpcg_set_launch(_h, on ? 1 : 0, 0, 64, 1), "FusedMGPCG::set_graph"); }
       // a named launch/format option of feat3_mgpcg_set_option

@@ -326,6 +326,15 @@ namespace mg
     });
   }
 
+  // dst = src (a level solution that ended in the other buffer of its pair)
+  template<typename DT_>
+  __global__ void k_copy_vec(u64 n, const DT_* __restrict__ src, DT_* __restrict__ dst, const int* status)
+  {
+    if(stopped(status)) return;
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      dst[i] = src[i];
+  }
+
   // exact coarse solve with the FP64 inverse of the unit-filtered coarse matrix, one
   // workgroup: x = filter(Ainv * rhs)
   template<typename DT_>

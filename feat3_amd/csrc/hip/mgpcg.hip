@@ -572,7 +572,8 @@ namespace
   {
     double omega_j = 0.5, omega_r = 1.0;
     int pre = 4, post = 4;
-    int precond = 0;          // 0 MG V-cycle, 1 none (CG)
+    int precond = 0;          // 0 multigrid cycle, 1 none (CG)
+    int cycle = 0;            // 0 V, 1 F, 2 W (multigrid.hpp:1333-1519)
     bool use_graph = true;
     int format = 0;           // 0 CSR, 1 SELL-C-sigma, 2 banded where it fits (else SELL)
     u64 sell_C = 64, sell_sigma = 64;
@@ -741,6 +742,7 @@ namespace
       }
       L.back()->xfinal = L.back()->x.template as<DT>();
       F3_RC(setup_tail(lv));
+      build_cycle();
 
       // outer PCG
       const HostLevel& h0 = lv.front();
@@ -852,7 +854,8 @@ namespace
         return b;
       };
       tail0 = nl - 1;
-      while(tail0 > 0 && L[tail0 - 1]->n <= cfg.tail_max_n && L[tail0 - 1]->n < 65536 &&
+      // the tail kernel runs a V-cycle; F- and W-cycles launch every level operation
+      while(cfg.cycle == 0 && tail0 > 0 && L[tail0 - 1]->n <= cfg.tail_max_n && L[tail0 - 1]->n < 65536 &&
             nl - (tail0 - 1) <= std::size_t(mg::kTailMax) && arena(tail0 - 1) <= u64(cap))
         --tail0;
       tail = mg::TailDesc<DT, VT>{};
@@ -889,59 +892,163 @@ namespace
       return rt::kOk;
     }
 
+    // ---- multigrid cycle ------------------------------------------------------------------
+    // A cycle is a fixed sequence of level operations, built once at init from the cycle
+    // kind (multigrid.hpp:1333-1519, _apply_cycle_v/_f/_w over _apply_rest, _apply_coarse,
+    // _apply_prol and _apply_smooth_peak).  The bottom operation solves level tail0: the coarse
+    // level (k_coarse) or, for the V-cycle, the first level of the coarse tail (k_tail).
+    enum class Op : int { entry, sweep, defect, rest, rest_step, bottom, prol, copy_top };
+    struct CycleOp { Op op; std::size_t lvl; };
+    std::vector<CycleOp> cycle_ops;
+
+    void build_cycle()
+    {
+      std::vector<CycleOp>& o = cycle_ops;
+      o.clear();
+      const std::size_t b = tail0;
+      // per level: the solution sits in t (1) or x (0); the first smoothing step writes x
+      std::vector<int> in_t(L.size(), 0);
+      auto sweeps = [&](std::size_t i, int k) {
+        for(int j = 0; j < k; ++j) o.push_back({Op::sweep, i});
+        in_t[i] ^= k & 1;
+      };
+      // _apply_rest(cur, smooth), multigrid.hpp:1711-1832: pre-smoothing from zero and defect
+      // on every level below cur, and on cur itself when smooth (else cur holds the defect of
+      // its peak smoothing); the restriction takes the next level's first smoothing step
+      auto rest = [&](std::size_t cur, bool smooth) {
+        for(std::size_t i = cur; i < b; ++i)
+        {
+          if(smooth || i > cur)
+          {
+            sweeps(i, cfg.pre - 1);
+            o.push_back({Op::defect, i});
+          }
+          if(i + 1 < b) { o.push_back({Op::rest_step, i}); in_t[i + 1] = 0; }
+          else o.push_back({Op::rest, i});
+        }
+      };
+      // _apply_prol(cur, smooth), multigrid.hpp:1843-2000 (fixed coarse-grid correction)
+      auto prol = [&](std::size_t cur, bool smooth) {
+        for(std::size_t i = b; i-- > cur;)
+        {
+          o.push_back({Op::prol, i});
+          if(smooth || i > cur) sweeps(i, cfg.post);
+        }
+      };
+      // _apply_smooth_peak(lvl), multigrid.hpp:1630-1700: pre- then post-smoother (no peak
+      // smoother is configured), then the defect
+      auto peak = [&](std::size_t l) {
+        sweeps(l, cfg.pre + cfg.post);
+        o.push_back({Op::defect, l});
+      };
+      auto coarse = [&]() { o.push_back({Op::bottom, b}); };
+
+      o.push_back({Op::entry, 0});
+      rest(0, true);
+      if(cfg.cycle == 1)
+      {
+        // F: one peak level after the other, from the coarsest but one up to the level below
+        // the top (multigrid.hpp:1366-1406)
+        for(std::size_t pk = b > 0 ? b - 1 : 0; pk > 0; --pk)
+        {
+          coarse();
+          prol(pk, false);
+          peak(pk);
+          rest(pk, false);
+        }
+        coarse();
+      }
+      else if(cfg.cycle == 2)
+      {
+        // W: 2^(levels below the top) coarse solves; the next peak level is the lowest level
+        // whose counter is 0, the counters below it are reset (multigrid.hpp:1409-1519)
+        coarse();
+        std::vector<int> cnt(b + 1, 0);
+        const std::size_t num = std::size_t(1) << b;
+        for(std::size_t c = 1; c < num; ++c)
+        {
+          std::size_t pk = b;
+          while(pk > 0)
+            if(cnt[--pk] == 0) break;
+          for(std::size_t i = b - 1; i > pk; --i) cnt[i] = 0;
+          ++cnt[pk];
+          prol(pk, false);
+          peak(pk);
+          rest(pk, false);
+          coarse();
+        }
+      }
+      else
+        coarse();
+      prol(0, true);
+      // the top level's solution must end in its xfinal buffer, which the PCG reads
+      if(b > 0 && (in_t[0] != 0) != (L[0]->xfinal == L[0]->t.template as<DT>()))
+        o.push_back({Op::copy_top, 0});
+    }
+
     int enqueue_vcycle(const double* def, const int* status)
     {
       const std::size_t nl = L.size();
-      Level<DT, VT>& top = *L[0];
-      mg::k_mg_entry<DT><<<stream_blocks(top.n), kBlock, 0, s>>>(top.n, def, top.rhs.template as<DT>(),
-                                                                top.x.template as<DT>(), top.dinv.template as<DT>(), status);
       std::vector<DT*> cur(nl, nullptr), oth(nl, nullptr);
-      for(std::size_t i = 0; i < tail0; ++i)
+      for(const CycleOp& c : cycle_ops)
       {
+        const std::size_t i = c.lvl;
         Level<DT, VT>& f = *L[i];
-        Level<DT, VT>& c = *L[i + 1];
-        cur[i] = f.x.template as<DT>();
-        oth[i] = f.t.template as<DT>();
-        for(int k = 1; k < cfg.pre; ++k)
+        switch(c.op)
         {
-          F3_RC(sweep(f, cur[i], oth[i], status));
-          std::swap(cur[i], oth[i]);
+          case Op::entry:
+            mg::k_mg_entry<DT><<<stream_blocks(f.n), kBlock, 0, s>>>(f.n, def, f.rhs.template as<DT>(), f.x.template as<DT>(),
+                                                                    f.dinv.template as<DT>(), status);
+            cur[0] = f.x.template as<DT>();
+            oth[0] = f.t.template as<DT>();
+            break;
+          case Op::sweep:
+            F3_RC(sweep(f, cur[i], oth[i], status));
+            std::swap(cur[i], oth[i]);
+            break;
+          case Op::defect:
+            F3_RC(with_rows<DT, VT>(f.A, [&](auto A, mg::RowGrid g) {
+              mg::k_defect<<<g.blocks, kBlock, 0, s>>>(A, g.per, f.mask.template as<u32>(), f.rhs.template as<DT>(),
+                                                       (const DT*)cur[i], f.def.template as<DT>(), status);
+            }));
+            break;
+          case Op::rest:
+          case Op::rest_step:
+          {
+            Level<DT, VT>& cl = *L[i + 1];
+            DT* xc = c.op == Op::rest_step ? cl.x.template as<DT>() : nullptr;
+            F3_RC(with_rows<DT, VT>(f.R, [&](auto R, mg::RowGrid g) {
+              mg::k_restrict<<<g.blocks, kBlock, 0, s>>>(R, g.per, cl.mask.template as<u32>(), f.def.template as<DT>(),
+                                                         cl.rhs.template as<DT>(), xc, cl.dinv.template as<DT>(), status);
+            }));
+            if(xc)
+            {
+              cur[i + 1] = xc;
+              oth[i + 1] = cl.t.template as<DT>();
+            }
+            break;
+          }
+          case Op::bottom:
+            if(tail0 + 1 < nl)
+              mg::k_tail<DT, VT><<<1, mg::kTailThreads, tail_lds, s>>>(tail, status);
+            else
+              mg::k_coarse<DT><<<1, kBlock, 0, s>>>(f.n, ainv.template as<double>(), f.mask.template as<u32>(),
+                                                    f.rhs.template as<DT>(), f.x.template as<DT>(), status);
+            cur[i] = f.xfinal;
+            break;
+          case Op::prol:
+            F3_RC(with_rows<DT, VT>(f.P, [&](auto P, mg::RowGrid g) {
+              mg::k_prolong<<<g.blocks, kBlock, 0, s>>>(P, g.per, f.mask.template as<u32>(), (const DT*)cur[i + 1], cur[i],
+                                                        status);
+            }));
+            break;
+          case Op::copy_top:
+            mg::k_copy_vec<DT><<<stream_blocks(f.n), kBlock, 0, s>>>(f.n, cur[0], f.xfinal, status);
+            std::swap(cur[0], oth[0]);
+            break;
         }
-        F3_RC(with_rows<DT, VT>(f.A, [&](auto A, mg::RowGrid g) {
-          mg::k_defect<<<g.blocks, kBlock, 0, s>>>(A, g.per, f.mask.template as<u32>(), f.rhs.template as<DT>(),
-                                                   (const DT*)cur[i], f.def.template as<DT>(), status);
-        }));
-        // the restriction also takes the next level's first smoothing step, unless that level
-        // is the coarsest or the first of the tail (the tail kernel takes it itself)
-        DT* xc = (i + 1 < tail0) ? c.x.template as<DT>() : nullptr;
-        F3_RC(with_rows<DT, VT>(f.R, [&](auto R, mg::RowGrid g) {
-          mg::k_restrict<<<g.blocks, kBlock, 0, s>>>(R, g.per, c.mask.template as<u32>(), f.def.template as<DT>(),
-                                                     c.rhs.template as<DT>(), xc, c.dinv.template as<DT>(), status);
-        }));
       }
-      if(tail0 + 1 < nl)
-        mg::k_tail<DT, VT><<<1, mg::kTailThreads, tail_lds, s>>>(tail, status);
-      else
-      {
-        Level<DT, VT>& cl = *L.back();
-        mg::k_coarse<DT><<<1, kBlock, 0, s>>>(cl.n, ainv.template as<double>(), cl.mask.template as<u32>(),
-                                              cl.rhs.template as<DT>(), cl.x.template as<DT>(), status);
-      }
-      for(std::size_t ii = tail0; ii > 0; --ii)
-      {
-        const std::size_t i = ii - 1;
-        Level<DT, VT>& f = *L[i];
-        Level<DT, VT>& c = *L[i + 1];
-        F3_RC(with_rows<DT, VT>(f.P, [&](auto P, mg::RowGrid g) {
-          mg::k_prolong<<<g.blocks, kBlock, 0, s>>>(P, g.per, f.mask.template as<u32>(), (const DT*)c.xfinal, cur[i], status);
-        }));
-        for(int k = 0; k < cfg.post; ++k)
-        {
-          F3_RC(sweep(f, cur[i], oth[i], status));
-          std::swap(cur[i], oth[i]);
-        }
-        if(cur[i] != f.xfinal) return rt::kErrState;
-      }
+      if(cur[0] != L[0]->xfinal) return rt::kErrState;
       return rt::check_launch();
     }
 
@@ -1208,21 +1315,25 @@ namespace
       v[2] = L.front()->A.nnz;
       v[3] = u64(L.front()->A.lanes);
       v[4] = L.front()->A.sell_size;
-      u64 kern = 1, bytes = 0;
+      u64 kern = 0, bytes = 0;
       const u64 vb = sizeof(DT);
-      bytes += L.front()->n * (8 + vb * 3); // entry
-      for(std::size_t i = 0; i < tail0; ++i)
+      for(const CycleOp& c : cycle_ops)
       {
-        const Level<DT, VT>& f = *L[i];
-        const u64 sweeps = u64(cfg.pre - 1 + cfg.post);
-        kern += sweeps + 3;
-        bytes += sweeps * (f.A.matrix_bytes() + f.n * vb * 4);
-        bytes += f.A.matrix_bytes() + f.n * vb * 3;                       // defect
-        bytes += f.R.matrix_bytes() + f.n * vb + L[i + 1]->n * vb * 4;     // restrict (+first step)
-        bytes += f.P.matrix_bytes() + L[i + 1]->n * vb + f.n * vb * 2;     // prolong
+        const Level<DT, VT>& f = *L[c.lvl];
+        const u64 nc = c.lvl + 1 < L.size() ? L[c.lvl + 1]->n : 0;
+        ++kern;
+        switch(c.op)
+        {
+          case Op::entry: bytes += f.n * (8 + vb * 3); break;
+          case Op::sweep: bytes += f.A.matrix_bytes() + f.n * vb * 4; break;
+          case Op::defect: bytes += f.A.matrix_bytes() + f.n * vb * 3; break;
+          case Op::rest: bytes += f.R.matrix_bytes() + f.n * vb + nc * vb * 2; break;
+          case Op::rest_step: bytes += f.R.matrix_bytes() + f.n * vb + nc * vb * 4; break;
+          case Op::bottom: bytes += ncoarse * ncoarse * 8 + ncoarse * vb * 2; break;
+          case Op::prol: bytes += f.P.matrix_bytes() + nc * vb + f.n * vb * 2; break;
+          case Op::copy_top: bytes += f.n * vb * 2; break;
+        }
       }
-      kern += 1; // coarse solve or coarse tail
-      bytes += ncoarse * ncoarse * 8 + ncoarse * vb * 2;
       v[5] = kern;
       v[6] = bytes;
       v[7] = outer_matrix().matrix_bytes() + n0 * 8 * 2 + n0 * 8 * 5 + n0 * (8 + vb) * 2 + n0 * (8 + vb * 2);
@@ -1448,6 +1559,7 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
   else if(k == "sell_C" && value >= 1 && value <= 1024) h->cfg.sell_C = u64(value);
   else if(k == "tail_max_n" && value >= 0) h->cfg.tail_max_n = u64(value);
   else if(k == "transfer_csr") h->cfg.transfer_csr = value != 0.0;
+  else if(k == "cycle" && (value == 0 || value == 1 || value == 2)) h->cfg.cycle = int(value);
   else if(k == "rows_per_block" && value >= 0) h->cfg.rows_per_block = u64(value);
   else if(k == "transfer_sigma" && value >= 1) h->cfg.transfer_sigma = u64(value);
   else if(k == "sell_sigma" && value >= 1) h->cfg.sell_sigma = u64(value);

@@ -8,9 +8,9 @@ comment), as read by the reference's PropertyMap.  ``create_scalar_solver(hierar
 accepts the shape this path implements::
 
     [linsolver]            type = pcg        (tolerances and iteration limits of the outer PCG)
-      precon = mgv  ->     type = mg         cycle = v, hierarchy = h1, lvl_min = -1, lvl_max = 0,
+      precon = mgv  ->     type = mg         cycle = v | f | w, hierarchy = h1, lvl_min = -1, lvl_max = 0,
                                              memorytype = hip, datatype = float | double
-                                             (the V-cycle's precision; the extension key
+                                             (the cycle's precision; the extension key
                                              valuetype = half stores its matrices in binary16)
         hierarchy = h1 ->  type = hierarchy  smoother = rich, coarse = coarse
           smoother ->      type = richardson max_iter = min_iter = steps (pre = post), omega,
@@ -22,8 +22,8 @@ accepts the shape this path implements::
 (IterativeSolver: tol_rel, tol_abs, tol_abs_low, div_rel, div_abs, stag_rate, max_iter,
 min_iter, min_stag_iter, iterative.hpp:252-301; Richardson/Jacobi: omega, richardson.hpp:113,
 jacobi_precond.hpp:106) keep their meaning.  A tree outside this shape (another Krylov solver, an
-F or W cycle, an iterative coarse solver, a smoother with a stopping tolerance) raises
-``FactoryError`` naming the section: the fused path does not approximate it.  The extension keys
+iterative coarse solver, a smoother with a stopping tolerance) raises ``FactoryError`` naming the
+section: the fused path does not approximate it.  The extension keys
 ``format = csr | sell | banded`` (root or mg section) and ``graph = 0 | 1`` select the level-matrix
 format and graph replay.
 """
@@ -32,7 +32,7 @@ from __future__ import annotations
 from typing import Dict, Optional, Tuple
 
 from .problems import Hierarchy
-from .solver import MGPCG
+from .solver import CYCLE, MGPCG
 
 
 class FactoryError(ValueError):
@@ -90,7 +90,7 @@ def _smoother(base: PropertyMap, name: str) -> Tuple[float, float, int]:
     if t == "jacobi":
         return _num(sec, name, "omega", float, 1.0), 1.0, 1
     if t != "richardson":
-        raise FactoryError(f"{name}.type = {t}: the device V-cycle smooths with richardson(jacobi) or jacobi")
+        raise FactoryError(f"{name}.type = {t}: the device multigrid smooths with richardson(jacobi) or jacobi")
     mx, mn = _num(sec, name, "max_iter", int, 100), _num(sec, name, "min_iter", int, 0)
     if mx != mn:
         raise FactoryError(f"{name}: a smoother runs a fixed number of steps here (min_iter = max_iter), "
@@ -134,15 +134,20 @@ def solver_kwargs(base: PropertyMap, section: str = "linsolver") -> dict:
         mg = _section(base, pre, f"{section}.precon")
         if mg.get("type") != "mg":
             raise FactoryError(f"{pre}.type = {mg.get('type')}: the preconditioner must be a multigrid (type = mg)")
-        if mg.get("cycle", "").lower() != "v":
-            raise FactoryError(f"{pre}.cycle = {mg.get('cycle')}: the device multigrid runs V-cycles")
+        if "cycle" not in mg:
+            raise FactoryError(f"{pre}: mg section without cycle key is not allowed")
+        cyc = mg["cycle"].lower()
+        if cyc not in CYCLE:
+            raise FactoryError(f"{pre}.cycle = {mg['cycle']}: mg cycle unknown (v, f or w)")
+        if cyc != "v":
+            kw["cycle"] = cyc
         if int(_num(mg, pre, "lvl_max", float, 0)) != 0 or int(_num(mg, pre, "lvl_min", float, -1)) != -1:
-            raise FactoryError(f"{pre}: the V-cycle spans the whole hierarchy (lvl_max = 0, lvl_min = -1)")
+            raise FactoryError(f"{pre}: the cycle spans the whole hierarchy (lvl_max = 0, lvl_min = -1)")
         if mg.get("adapt_cgc", "fixed") != "fixed":
             raise FactoryError(f"{pre}.adapt_cgc: the coarse-grid correction is fixed (omega = 1)")
         mem = mg.get("memorytype", "hip")
         if mem not in ("hip", "main"):
-            raise FactoryError(f"{pre}.memorytype = {mem}: the V-cycle runs on the device (memorytype = hip)")
+            raise FactoryError(f"{pre}.memorytype = {mem}: the cycle runs on the device (memorytype = hip)")
         key = (mg.get("datatype", "double"), mg.get("valuetype"))
         if key not in _INNER:
             raise FactoryError(f"{pre}: datatype = {key[0]}, valuetype = {key[1]} is not one of double, float, float+half")

@@ -19,6 +19,7 @@ from .problems import Hierarchy
 STATUS = {0: "undefined", 1: "progress", 2: "success", 3: "aborted", 4: "diverged", 5: "max-iter", 6: "stagnated"}
 INNER = {"f64": 0, "f32": 1, "f16": 2}
 FORMAT = {"csr": 0, "sell": 1, "banded": 2}
+CYCLE = {"v": 0, "f": 1, "w": 2}          # MultiGridCycle, kernel/solver/multigrid.hpp:26-34
 
 
 def status_success(status: int) -> bool:
@@ -37,14 +38,15 @@ def _ptr(t) -> ctypes.c_void_p:
 
 
 class MGPCG:
-    """PCG preconditioned by a device multigrid V-cycle (or plain CG with precond='none')."""
+    """PCG preconditioned by a device multigrid cycle (V by default, cycle='f' or 'w' for the
+    reference's F- and W-cycles; plain CG with precond='none')."""
 
     def __init__(self, hierarchy: Hierarchy, inner: str = "f64", fmt: str = "csr",
                  omega_jacobi: float = 0.5, omega_richardson: float = 1.0, pre: int = 4, post: int = 4,
                  tol_rel: float = 1e-8, tol_abs: float = 0.0, tol_abs_low: float = 0.0,
                  min_iter: int = 0, max_iter: int = 1000, precond: str = "mg",
                  graph: bool = True, lanes: int = 0, sell_C: int = 64, sell_sigma: int = 1,
-                 options: Optional[dict] = None):
+                 options: Optional[dict] = None, cycle: str = "v"):
         self.hierarchy = hierarchy
         h = ctypes.c_void_p()
         capi.call("feat3_mgpcg_create", ctypes.byref(h), INNER[inner], FORMAT[fmt])
@@ -70,6 +72,8 @@ class MGPCG:
         capi.call("feat3_mgpcg_set_tolerances", h, tol_rel, tol_abs, tol_abs_low, min_iter, max_iter)
         capi.call("feat3_mgpcg_set_precond", h, {"mg": 0, "none": 1}[precond])
         capi.call("feat3_mgpcg_set_launch", h, int(graph), lanes, sell_C, sell_sigma)
+        if cycle != "v":
+            capi.call("feat3_mgpcg_set_option", h, b"cycle", float(CYCLE[cycle]))
         for k, v in (options or {}).items():
             capi.call("feat3_mgpcg_set_option", h, k.encode(), float(v))
         self.status = 0

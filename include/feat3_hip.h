@@ -301,7 +301,7 @@ int feat3_mgpcg_set_smoother(feat3_mgpcg* h, double omega_jacobi, double omega_r
  * tol_abs <= 0 keeps the default 1/eps^2 */
 int feat3_mgpcg_set_tolerances(feat3_mgpcg* h, double tol_rel, double tol_abs, double tol_abs_low, uint64_t min_iter,
                                uint64_t max_iter);
-/* 0: MG V-cycle preconditioner, 1: none (plain CG) */
+/* 0: multigrid preconditioner (V-cycle unless the option "cycle" says otherwise), 1: none (plain CG) */
 int feat3_mgpcg_set_precond(feat3_mgpcg* h, int kind);
 /* graph replay on/off, lanes per CSR row (0 auto), SELL chunk height and sort window */
 int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C, uint64_t sell_sigma);
@@ -312,7 +312,9 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
  * workgroup; 0 = none; default 1100), "rows_per_block" (row positions per workgroup of the
  * level-matrix kernels, a multiple of 64; 0 = derived from the row count), "transfer_csr" (0/1:
  * transfer matrices in CSR evaluated by streaming, default 1, or in the level format),
- * "transfer_sigma" (SELL sort window of the transfer matrices when they are SELL, default 1);
+ * "transfer_sigma" (SELL sort window of the transfer matrices when they are SELL, default 1),
+ * "cycle" (the multigrid cycle, MultiGridCycle of multigrid.hpp:26-34: 0 V, default, 1 F,
+ * 2 W; F and W launch every level operation, the coarse tail serves the V-cycle only);
  * takes effect at the next init */
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value);
 /* init_symbolic + init_numeric (solver/base.hpp:227-260) */
@@ -323,14 +325,15 @@ int feat3_mgpcg_init(feat3_mgpcg* h);
 int feat3_mgpcg_correct(feat3_mgpcg* h, void* caller_stream, double* x, const double* b, int* status, uint64_t* iters);
 /* SolverBase::apply(cor, def) (pcg.hpp:141-156) */
 int feat3_mgpcg_apply(feat3_mgpcg* h, void* caller_stream, double* cor, const double* def, int* status, uint64_t* iters);
-/* the MultiGrid preconditioner alone: cor = one V-cycle applied to def (multigrid.hpp:1268-1329) */
+/* the MultiGrid preconditioner alone: cor = one cycle (V, F or W, option "cycle") applied to
+ * def (multigrid.hpp:1268-1519) */
 int feat3_mgpcg_vcycle(feat3_mgpcg* h, void* caller_stream, double* cor, const double* def);
 /* defect norms of the last solve: [0] initial, [k] after iteration k */
 int feat3_mgpcg_history(feat3_mgpcg* h, double* out_host, uint64_t cap, uint64_t* len);
 /* average duration of `reps` launches of one hot-path kernel on the solver's stream
- * (which: 0 finest smoothing sweep, 1 finest defect, 3 full V-cycle) and its algorithmic bytes */
+ * (which: 0 finest smoothing sweep, 1 finest defect, 3 full cycle) and its algorithmic bytes */
 int feat3_mgpcg_time_kernel(feat3_mgpcg* h, int which, int reps, double* avg_ms, uint64_t* bytes);
-/* [levels, n, nnz, lanes, sell size, kernels per V-cycle, V-cycle bytes, outer bytes,
+/* [levels, n, nnz, lanes, sell size, kernels per cycle, cycle bytes, outer bytes,
  *  sizeof inner, sizeof stored value, 16-bit SELL columns on the finest level, SELL chunks
  *  of the finest level, finest level banded (0/1), its number of diagonals, levels run by the
  *  single-workgroup coarse tail (0: none), LDS bytes of the tail] */

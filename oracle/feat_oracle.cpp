@@ -21,6 +21,9 @@
 //                                (CG 28, PCG-JAC 28, Richardson-JAC 1175, each +-2)
 //   MultiGrid V-cycle + PCG   -- applications/poisson_dirichlet.cpp:266-276 (4 +- 1 iterations
 //                                at level 5) and exact two-grid/coarse-solve identities
+//   MultiGrid F-/W-cycle      -- recursive definitions evaluated independently in numpy
+//                                (F = V on two levels, W on two levels = two V-cycles in
+//                                defect correction, recursion on 3-5 levels)
 //   transpose                 -- R = P^T identity on random patterns
 //   half conversion           -- exhaustive round trip of all 65536 binary16 patterns
 //   cell_gradient (adjoint)   -- no reference counterpart; pinned by central finite differences
@@ -968,21 +971,27 @@ namespace orc
         if(l->coarse && l->coarse != l->smoother_pre) l->coarse->init();
       }
     }
-    // multigrid.hpp:1711-1832
-    void rest()
+    int cycle = 0;  // 0 V, 1 F, 2 W (MultiGridCycle, multigrid.hpp:26-34)
+
+    // multigrid.hpp:1711-1832: restriction from level cur down to the coarse level; the
+    // pre-smoother runs on every level below cur, and on cur itself only when smooth
+    void rest(std::size_t cur, bool smooth)
     {
-      for(std::size_t i(0); i + 1 < lv.size(); ++i)
+      for(std::size_t i(cur); i + 1 < lv.size(); ++i)
       {
         MGLevel<DT_>& f = *lv[i];
-        if(f.smoother_pre)
+        if(smooth || i > cur)
         {
-          f.smoother_pre->apply(f.sol.p(), f.rhs.p());
-          f.A->apply(f.def.p(), f.sol.p(), f.rhs.p(), DT_(-1));
-        }
-        else
-        {
-          std::fill(f.sol.v.begin(), f.sol.v.end(), DT_(0));
-          f.def.v = f.rhs.v;
+          if(f.smoother_pre)
+          {
+            f.smoother_pre->apply(f.sol.p(), f.rhs.p());
+            f.A->apply(f.def.p(), f.sol.p(), f.rhs.p(), DT_(-1));
+          }
+          else
+          {
+            std::fill(f.sol.v.begin(), f.sol.v.end(), DT_(0));
+            f.def.v = f.rhs.v;
+          }
         }
         f.F.filter_def(f.def.p());
         MGLevel<DT_>& c = *lv[i + 1];
@@ -1006,10 +1015,12 @@ namespace orc
       }
       return Status::success;
     }
-    // multigrid.hpp:1843-2000 (fixed coarse-grid correction, omega_cgc = 1)
-    void prol()
+    // multigrid.hpp:1843-2000 (fixed coarse-grid correction, omega_cgc = 1): prolongation from
+    // the coarse level up to level cur; the post-smoother runs on every level below cur, and
+    // on cur itself only when smooth
+    void prol(std::size_t cur, bool smooth)
     {
-      for(std::size_t i(lv.size() - 1); i > 0;)
+      for(std::size_t i(lv.size() - 1); i > cur;)
       {
         --i;
         MGLevel<DT_>& f = *lv[i];
@@ -1018,7 +1029,7 @@ namespace orc
         f.P->apply(f.cor.p(), c.sol.p());
         f.F.filter_cor(f.cor.p());
         axpy(f.sol.p(), DT_(1), f.cor.p(), f.sol.p(), n);
-        if(f.smoother_post)
+        if(f.smoother_post && (smooth || i > cur))
         {
           f.A->apply(f.def.p(), f.sol.p(), f.rhs.p(), DT_(-1));
           f.F.filter_def(f.def.p());
@@ -1027,15 +1038,73 @@ namespace orc
         }
       }
     }
+    // multigrid.hpp:1589-1621: one smoother on the defect, correction added, defect updated
+    void smooth_def(MGLevel<DT_>& l, SolverBase<DT_>& sm)
+    {
+      const Index n = l.A->size_l();
+      sm.apply(l.cor.p(), l.def.p());
+      l.F.filter_cor(l.cor.p());
+      axpy(l.sol.p(), DT_(1), l.cor.p(), l.sol.p(), n);
+      l.A->apply(l.def.p(), l.sol.p(), l.rhs.p(), DT_(-1));
+      l.F.filter_def(l.def.p());
+    }
+    // multigrid.hpp:1630-1700 with no peak smoother: the pre- and then the post-smoother
+    void smooth_peak(std::size_t i)
+    {
+      MGLevel<DT_>& l = *lv[i];
+      l.A->apply(l.def.p(), l.sol.p(), l.rhs.p(), DT_(-1));
+      l.F.filter_def(l.def.p());
+      if(l.smoother_pre) smooth_def(l, *l.smoother_pre);
+      if(l.smoother_post) smooth_def(l, *l.smoother_post);
+    }
     Status apply(DT_* cor, const DT_* def) override
     {
       MGLevel<DT_>& top = *lv.front();
       std::copy(def, def + top.A->size_l(), top.rhs.p());
-      rest();
-      Status st = coarse();
-      prol();
+      const std::size_t last = lv.size() - 1;
+      Status st = Status::success;
+      rest(0, true);
+      if(cycle == 1)
+      {
+        // multigrid.hpp:1366-1406: intermediate peak levels from the coarsest but one upwards
+        for(std::size_t pk = last > 0 ? last - 1 : 0; pk > 0; --pk)
+        {
+          coarse();
+          prol(pk, false);
+          smooth_peak(pk);
+          rest(pk, false);
+        }
+        st = coarse();
+      }
+      else if(cycle == 2)
+      {
+        // multigrid.hpp:1409-1519: 2^last coarse solves; each round's peak level is the lowest
+        // level above the coarse one whose counter is 0, the counters below it are reset
+        st = coarse();
+        std::vector<int> counter(last + 1, 0);
+        const std::size_t num_cgs = std::size_t(1) << last;
+        for(std::size_t cgs(1); cgs < num_cgs; ++cgs)
+        {
+          std::size_t pk = last;
+          while(pk > 0)
+          {
+            if(counter[--pk] == 0)
+              break;
+          }
+          for(std::size_t i = last - 1; i > pk; --i)
+            counter[i] = 0;
+          ++counter[pk];
+          prol(pk, false);
+          smooth_peak(pk);
+          rest(pk, false);
+          st = coarse();
+        }
+      }
+      else
+        st = coarse();
+      prol(0, true);
       std::copy(top.sol.v.begin(), top.sol.v.end(), cor);
-      return st == Status::success ? Status::success : st;
+      return st;
     }
   };
 
@@ -1205,6 +1274,7 @@ namespace
     double tol_abs;
     u64 min_iter;
     int use_def;       // 1: solve via correct(x, b) (Solver::solve); 0: apply(cor, def)
+    int cycle;         // multigrid cycle: 0 V, 1 F, 2 W
   };
 
   Params parse(const double* p)
@@ -1213,7 +1283,7 @@ namespace
     q.kind = int(p[0]); q.tol_rel = p[1]; q.max_iter = u64(p[2]);
     q.omega_jac = p[3]; q.omega_rich = p[4]; q.pre = u64(p[5]); q.post = u64(p[6]);
     q.coarse = int(p[7]); q.inner = int(p[8]); q.tol_abs = p[9]; q.min_iter = u64(p[10]);
-    q.use_def = int(p[11]);
+    q.use_def = int(p[11]); q.cycle = int(p[12]);
     return q;
   }
 
@@ -1233,6 +1303,7 @@ namespace
   std::shared_ptr<orc::MultiGrid<DT_>> build_mg(const OrcSystem& s, const Params& q)
   {
     auto mg = std::make_shared<orc::MultiGrid<DT_>>();
+    mg->cycle = q.cycle;
     for(std::size_t i(0); i < s.levels.size(); ++i)
     {
       const OrcLevelData& L = s.levels[i];
@@ -1390,8 +1461,8 @@ u64 orc_system_solve(void* h, const double* praw, double* x, const double* b, in
   return iters;
 }
 
-// One application of the MG preconditioner (a single V-cycle) in the requested precision:
-// cor = MG(def).  Used to check the device V-cycle in isolation.
+// One application of the MG preconditioner (one cycle of the kind the parameters name) in the
+// requested precision: cor = MG(def).  Used to check the device cycle in isolation.
 int orc_system_vcycle(void* h, const double* praw, double* cor, const double* def)
 {
   const OrcSystem& s = *static_cast<OrcSystem*>(h);

@@ -125,10 +125,13 @@ PCG_MG, PCG_JACOBI, CG, RICHARDSON_JACOBI = 0, 1, 2, 3
 COARSE_EXACT, COARSE_SMOOTHER, COARSE_NONE = 0, 1, 2
 
 
+CYCLE_V, CYCLE_F, CYCLE_W = 0, 1, 2
+
+
 def params(kind=PCG_MG, tol_rel=1e-8, max_iter=1000, omega_jac=0.5, omega_rich=1.0, pre=4, post=4,
-           coarse=COARSE_EXACT, inner=0, tol_abs=0.0, min_iter=0, use_def=1):
+           coarse=COARSE_EXACT, inner=0, tol_abs=0.0, min_iter=0, use_def=1, cycle=CYCLE_V):
     return np.array([kind, tol_rel, max_iter, omega_jac, omega_rich, pre, post, coarse, inner, tol_abs,
-                     min_iter, use_def], np.float64)
+                     min_iter, use_def, cycle], np.float64)
 
 
 class System:

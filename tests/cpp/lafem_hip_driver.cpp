@@ -168,12 +168,15 @@ int main(int argc, char** argv)
     }
     {
       // the same fused solver built by the SolverFactory from an INI configuration
+      const std::string ini = "[linsolver]\ntype = pcg\ntol_rel = 1e-8\nmax_iter = 1000\nprecon = mgv\n"
+                              "[mgv]\ntype = mg\ncycle = v\nhierarchy = h1\nmemorytype = hip\ndatatype = double\nformat = sell\n"
+                              "[h1]\ntype = hierarchy\nsmoother = rich\ncoarse = coarse\n"
+                              "[rich]\ntype = richardson\nmax_iter = 4\nmin_iter = 4\nprecon = jac  # damped Jacobi\n"
+                              "[jac]\ntype = jacobi\nomega = " + std::to_string(omega) + "\n[coarse]\ntype = umfpack\n";
+      std::string ini_w = ini;
+      ini_w.replace(ini_w.find("cycle = v"), 9, "cycle = w");
       PropertyMap pm;
-      pm.parse("[linsolver]\ntype = pcg\ntol_rel = 1e-8\nmax_iter = 1000\nprecon = mgv\n"
-               "[mgv]\ntype = mg\ncycle = v\nhierarchy = h1\nmemorytype = hip\ndatatype = double\nformat = sell\n"
-               "[h1]\ntype = hierarchy\nsmoother = rich\ncoarse = coarse\n"
-               "[rich]\ntype = richardson\nmax_iter = 4\nmin_iter = 4\nprecon = jac  # damped Jacobi\n"
-               "[jac]\ntype = jacobi\nomega = " + std::to_string(omega) + "\n[coarse]\ntype = umfpack\n");
+      pm.parse(ini);
       auto fs = Solver::SolverFactory::create_scalar_solver(pm, "linsolver");
       for(const HostLevel& h : host)
         fs->push_level(h.n, h.rp, h.col, h.val, h.bnd, h.nc, h.prp, h.pcol, h.pval, h.rrp, h.rcol, h.rval);
@@ -182,18 +185,31 @@ int main(int argc, char** argv)
       const Solver::Status st = fs->correct(x, b);
       std::cout << "factory64 " << int(st) << " " << fs->get_num_iter() << std::endl;
       write(outdir + "/factory64.bin", x.to_host());
+      // the same tree with a W-cycle (multigrid.hpp:1409-1519)
+      PropertyMap pw;
+      pw.parse(ini_w);
+      auto fw = Solver::SolverFactory::create_scalar_solver(pw, "linsolver");
+      for(const HostLevel& h : host)
+        fw->push_level(h.n, h.rp, h.col, h.val, h.bnd, h.nc, h.prp, h.pcol, h.pval, h.rrp, h.rcol, h.rval);
+      fw->init();
+      LAFEM::DenseVector<Mem::HIP, double, u32> xw(h0.n, 0.0);
+      const Solver::Status stw = fw->correct(xw, b);
+      std::cout << "factoryw64 " << int(stw) << " " << fw->get_num_iter() << std::endl;
+      write(outdir + "/factoryw64.bin", xw.to_host());
       bool rejected = false;
       try
       {
         PropertyMap bad;
-        bad.parse("[s]\ntype = pcg\nprecon = m\n[m]\ntype = mg\ncycle = w\n");
-        Solver::SolverFactory::create_scalar_solver(bad, "s");
+        std::string t = ini_w;
+        t.replace(t.find("cycle = w"), 9, "cycle = x");
+        bad.parse(t);
+        Solver::SolverFactory::create_scalar_solver(bad, "linsolver");
       }
       catch(const InternalError&)
       {
         rejected = true;
       }
-      std::cout << "factory_rejects_w_cycle " << int(rejected) << std::endl;
+      std::cout << "factory_rejects_unknown_cycle " << int(rejected) << std::endl;
     }
     {
       // the same solver over the banded level format (SparseMatrixBanded layout)

@@ -65,6 +65,8 @@ def test_poisson_tree_maps_onto_the_fused_solver():
      dict(pre=2, post=2, omega_richardson=0.9)),
     (("[linsolver]", "precon = mgv", "precon = none"), dict(precond="none")),
     (("[mgv]", "cycle = v", "cycle = V\nformat = banded"), dict(fmt="banded")),
+    (("[mgv]", "cycle = v", "cycle = F"), dict(cycle="f")),
+    (("[mgv]", "cycle = v", "cycle = w"), dict(cycle="w")),
     (("[h1]", "smoother = rich", "smoother = jacobi"), dict(pre=1, post=1, omega_jacobi=0.5, omega_richardson=1.0)),
     (("[coarse]", "type = umfpack", "type = cusolver"), dict()),
 ])
@@ -77,7 +79,8 @@ def test_variants(edit, expect):
 
 @pytest.mark.parametrize("old,new,msg", [
     ("type = pcg", "type = gropppcg", "linsolver.type"),
-    ("cycle = v", "cycle = f", "mgv.cycle"),
+    ("cycle = v", "cycle = x", "mgv.cycle"),
+    ("cycle = v", "", "mgv: mg section without cycle key"),
     ("type = umfpack", "type = pcg\nprecon = jacobi", "coarse.type"),
     ("max_iter = 4\nmin_iter = 4", "max_iter = 4", "rich"),
     ("datatype = float", "datatype = float\nmemorytype = cuda", "mgv.memorytype"),

@@ -55,5 +55,11 @@ def test_cpp_application_level_solvers(tmp_path, torch_cuda):
     # the SolverFactory-built solver is the directly built one (same INI parameters)
     assert res["factory64"] == res["fused64"]
     assert np.array_equal(load("factory64"), load("fused64"))
-    assert res["factory_rejects_w_cycle"] == [1]
+    assert res["factory_rejects_unknown_cycle"] == [1]
+    # cycle = w: the oracle's W-cycle PCG
+    xw = np.zeros(H.n)
+    sw, iw, _ = S.solve(xw, b, po.params(omega_jac=0.7, coarse=po.COARSE_SMOOTHER, cycle=po.CYCLE_W))
+    assert sw == 2 and iw <= io
+    assert res["factoryw64"] == [2, iw]
+    assert np.linalg.norm(load("factoryw64") - xw) <= 1e-10 * np.linalg.norm(xw)
     assert res["leaked"] == [0]

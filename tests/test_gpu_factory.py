@@ -34,14 +34,15 @@ def test_factory_solver_equals_direct_construction(torch_cuda):
     assert np.array_equal(xf, xd)
 
 
-def test_factory_fp64_meets_the_oracle(torch_cuda):
+@pytest.mark.parametrize("cycle,code", [("v", po.CYCLE_V), ("f", po.CYCLE_F), ("w", po.CYCLE_W)])
+def test_factory_fp64_meets_the_oracle(torch_cuda, cycle, code):
     torch = torch_cuda
     H = pb.hierarchy(2, 2, 5, 1)
     b = pb.filtered_rhs(H, 0)
-    cfg = POISSON.replace("datatype = float", "datatype = double")
+    cfg = POISSON.replace("datatype = float", "datatype = double").replace("cycle = v", f"cycle = {cycle}")
     st, it, x = _solve(torch, fa.create_scalar_solver(H, cfg), H, b)
     S = po.System.from_hierarchy(H)
     xo = np.zeros(H.n)
-    so, io, _ = S.solve(xo, b, po.params())
+    so, io, _ = S.solve(xo, b, po.params(cycle=code))
     assert st == so == 2 and it == io
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-10
