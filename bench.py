@@ -67,7 +67,23 @@ def _timed(fn, steps, warmup):
     return times, outs
 
 
+def _host_cpu():
+    """CPU model and the threads visible to this process (for the cpu_baseline record)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}; {len(os.sched_getaffinity(0))} threads visible, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}"
+
+
 def _common(args, metric_value, ms, dtype, data, config, roof, extra, cpu):
+    if cpu is not None:
+        cpu.setdefault("host", _host_cpu())
     return {
         "metric": "time to rel. residual 1e-8 & DOF·iter/s, FP32-inner vs FP64; SpMV GB/s",
         "value": metric_value, "unit": "DOF*iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -283,11 +299,16 @@ def bench_spmv(args):
     rng = np.random.default_rng(0)
     rows = []
 
-    def add(name, fmt, dtype, bytes_, fn):
+    def add(name, fmt, dtype, bytes_, fn, ref_bytes=None):
         sec = timed(fn)
         gbs = bytes_ / sec / 1e9
-        rows.append(dict(kernel=name, format=fmt, dtype=dtype, us=sec * 1e6, bytes=int(bytes_), GBs=gbs,
-                         frac_of_peak=gbs / HBM_PEAK_GBS))
+        row = dict(kernel=name, format=fmt, dtype=dtype, us=sec * 1e6, bytes=int(bytes_), GBs=gbs,
+                   frac_of_peak=gbs / HBM_PEAK_GBS)
+        if ref_bytes is not None:
+            # the reference's own SpMV benchmark count, used_elements * (s_v + s_i) + n * s_v
+            # (benchmarks/product_matvec-bench.cpp:130-133), for comparison with its published rates
+            row.update(ref_formula_bytes=int(ref_bytes), ref_formula_GBs=ref_bytes / sec / 1e9)
+        rows.append(row)
         log(f"{name:38s} {sec * 1e6:9.1f} us {bytes_ / 1e6:9.1f} MB {gbs:7.0f} GB/s  {gbs / HBM_PEAK_GBS:.3f} of peak")
 
     for dt, npdt, s_v, tag in ((np.float64, "f64", 8, "f64"), (np.float32, "f32", 4, "f32")):
@@ -296,7 +317,8 @@ def bench_spmv(args):
         val = dev(A.val.astype(dt))
         col, rp = dev(A.col_ind), dev(A.row_ptr)
         add(f"lafem_hip_apply_csr_{tag}_u32", "csr", tag, nnz * (s_v + 4) + (n + 1) * 4 + 2 * n * s_v,
-            lambda: capi.call(f"lafem_hip_apply_csr_{tag}_u32", sp, r, 1.0, x, 0.0, r, val, col, rp, n, A.cols, nnz, 0))
+            lambda: capi.call(f"lafem_hip_apply_csr_{tag}_u32", sp, r, 1.0, x, 0.0, r, val, col, rp, n, A.cols, nnz, 0),
+            ref_bytes=nnz * (s_v + 4) + n * s_v)
         if tag == "f32":
             hv = dev(A.val.astype(np.float16))
             add("lafem_hip_apply_csr_h16f32_u32", "csr", "f16 values / f32", nnz * (2 + 4) + (n + 1) * 4 + 2 * n * 4,
@@ -307,7 +329,8 @@ def bench_spmv(args):
             nch = cl.size
             if sig == 1:
                 add(f"lafem_hip_apply_ell_{tag}_u32 (C={C})", "ell", tag, nnz * (s_v + 4) + 8 * nch + 2 * n * s_v,
-                    lambda: capi.call(f"lafem_hip_apply_ell_{tag}_u32", sp, r, 1.0, x, 0.0, r, dval, dcol, dcs, dcl, C, n))
+                    lambda: capi.call(f"lafem_hip_apply_ell_{tag}_u32", sp, r, 1.0, x, 0.0, r, dval, dcol, dcs, dcl, C, n),
+                    ref_bytes=nnz * (s_v + 4) + n * s_v)
             else:
                 drl, dperm = dev(rl), dev(perm[:n])
                 add(f"lafem_hip_apply_sell_{tag}_u32 (C={C}, s={sig})", "sell", tag,
@@ -524,6 +547,7 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cpu["all_cores"] = {"value": n * itp / dtp, "unit": "DOF*iter/s", "cores": threads,
                             "time_to_tol_ms": dtp * 1e3, "iterations": int(itp)}
+        cpu["host"] = _host_cpu()
         log(f"cpu baseline: {cpu}")
 
     achieved = head["sweep_GBs"]
