@@ -1,4 +1,4 @@
-// mg_kernels.hpp -- fused kernels of the device V-cycle and the device-resident PCG.
+// mg_kernels.hpp -- fused kernels of the device multigrid cycles and the device-resident PCG.
 //
 // The V-cycle follows MultiGrid::_apply_cycle_v (solver/multigrid.hpp:1333-1351,1711-2000)
 // with Richardson(omega_R, Jacobi(omega_J)) smoothers (richardson.hpp:187-228,
@@ -356,16 +356,29 @@ namespace mg
   // ---- coarse tail: all small levels in one workgroup ----------------------------------------
   //
   // Below a few thousand rows a level step is a launch of ~4 us that moves a few KB.  k_tail
-  // runs every V-cycle step of the levels tail[0] (the finest small level) down to the coarsest
-  // and back in one 1024-thread workgroup with barriers between the steps.  All vectors of these
-  // levels live in LDS (x, its ping-pong partner, rhs, defect, dinv: 5 n values per level);
-  // the matrices are ELL with 16-bit columns (entry k of row r at k * n + r, padded with column
-  // 0 and value 0), so a step issues one round of independent global loads (L2-resident
-  // after the first V-cycle) and gathers from LDS.  The arithmetic of every step is that of
-  // k_sweep / k_defect / k_restrict / k_prolong / k_coarse; only the order of the terms of a
-  // row sum may differ from the level format's.
+  // runs a segment of a multigrid cycle -- every step on the levels tail[0] (the finest small
+  // level) down to the coarsest and back, and for F- and W-cycles the peak smoothing and repeated
+  // coarse solves between them -- in one 1024-thread workgroup with barriers between the steps.
+  // A segment starts when the restriction above has written tail[0]'s right-hand side, takes
+  // tail[0]'s first smoothing step from x = 0, runs its list of steps and ends by writing
+  // tail[0]'s iterate for the prolongation above.  All vectors of these levels live in LDS (x,
+  // its ping-pong partner, rhs, defect, dinv: 5 n values per level); the matrices are ELL with
+  // 16-bit columns (entry k of row r at k * n + r, padded with column 0 and value 0), so a step
+  // issues one round of independent global loads (L2-resident after the first cycle) and gathers
+  // from LDS.  The arithmetic of every step is that of k_sweep / k_defect / k_restrict /
+  // k_prolong / k_coarse; only the order of the terms of a row sum may differ from the level
+  // format's.
   constexpr int kTailMax = 12;
   constexpr int kTailThreads = 1024;
+
+  // one step of a tail segment: kind | level << 4 | a << 8 | b << 9 | step << 10, where a is the
+  // buffer (0: x0, 1: x1) of the level's iterate that the step reads, b that of the next
+  // coarser level (prolongation), step the fused first smoothing step of a restriction
+  enum TailKind : u32 { kTailSweep = 0, kTailDefect = 1, kTailRest = 2, kTailCoarse = 3, kTailProl = 4 };
+  __host__ __device__ constexpr u32 tail_op(u32 kind, u32 lvl, u32 a, u32 b, u32 step)
+  {
+    return kind | (lvl << 4) | (a << 8) | (b << 9) | (step << 10);
+  }
 
   template<typename VT_>
   struct TailEll
@@ -387,11 +400,11 @@ namespace mg
   template<typename DT_, typename VT_>
   struct TailDesc
   {
-    int nlev, pre, post;
+    int nlev;
     u32 ncoarse;
     const double* ainv;
     const DT_* rhs_in;   // right-hand side of level tail[0], written by the restriction above
-    DT_* x_out;          // its solution after post-smoothing, read by the prolongation above
+    DT_* x_out;          // its iterate at the end of a segment, read by the prolongation above
     TailLevel<DT_, VT_> lv[kTailMax];
   };
 
@@ -409,8 +422,10 @@ namespace mg
     return s;
   }
 
+  // ops[0, nops): the segment's steps; out_buf: the buffer of tail[0]'s iterate at the end
   template<typename DT_, typename VT_>
-  __global__ __launch_bounds__(kTailThreads) void k_tail(TailDesc<DT_, VT_> d, const int* status)
+  __global__ __launch_bounds__(kTailThreads) void k_tail(TailDesc<DT_, VT_> d, const u32* __restrict__ ops, u32 nops,
+                                                         u32 out_buf, const int* status)
   {
     if(stopped(status)) return;
     extern __shared__ double tail_arena_[];
@@ -435,77 +450,67 @@ namespace mg
       }
     }
     __syncthreads();
-    // buffer (0: x0, 1: x1) holding a level's iterate after pre-smoothing and after post-smoothing
-    const u32 after_pre = u32(d.pre - 1) & 1u, after_post = u32(d.pre - 1 + d.post) & 1u;
-    for(int i = 0; i + 1 < d.nlev; ++i)
+    for(u32 o = 0; o < nops; ++o)
     {
-      const TailLevel<DT_, VT_>& f = d.lv[i];
-      const TailLevel<DT_, VT_>& c = d.lv[i + 1];
+      const u32 op = ops[o];
+      const u32 kind = op & 15u, li = (op >> 4) & 15u, a = (op >> 8) & 1u, b = (op >> 9) & 1u;
+      const TailLevel<DT_, VT_>& f = d.lv[li];
       DT_* fv = arena + f.off;
-      DT_* cv = arena + c.off;
       const u32 n = f.n;
-      for(int k = 1; k < d.pre; ++k)
+      if(kind == kTailSweep)
       {
-        const DT_* xi = fv + (u32(k - 1) & 1u) * n;
-        DT_* xo = fv + (u32(k) & 1u) * n;
+        const DT_* xi = fv + a * n;
+        DT_* xo = fv + (a ^ 1u) * n;
         for(u32 r = tid; r < n; r += kTailThreads)
           xo[r] = xi[r] + fv[4 * n + r] * (fv[2 * n + r] - tail_row<DT_, VT_>(f.A, r, xi));
-        __syncthreads();
       }
-      const DT_* xcur = fv + after_pre * n;
-      for(u32 r = tid; r < n; r += kTailThreads)
+      else if(kind == kTailDefect)
       {
-        const DT_ s = tail_row<DT_, VT_>(f.A, r, xcur);
-        fv[3 * n + r] = dev::masked(f.mask, r) ? DT_(0) : fv[2 * n + r] - s;
-      }
-      __syncthreads();
-      const bool first_step = i + 2 < d.nlev;
-      for(u32 r = tid; r < c.n; r += kTailThreads)
-      {
-        const DT_ s = tail_row<DT_, VT_>(f.R, r, fv + 3 * n);
-        const DT_ v = dev::masked(c.mask, r) ? DT_(0) : s;
-        cv[2 * c.n + r] = v;
-        if(first_step) cv[r] = cv[4 * c.n + r] * v;
-      }
-      __syncthreads();
-    }
-    {
-      // exact coarse solve with the FP64 inverse of the filtered coarsest matrix
-      const TailLevel<DT_, VT_>& c = d.lv[d.nlev - 1];
-      DT_* cv = arena + c.off;
-      for(u32 i = tid; i < d.ncoarse; i += kTailThreads)
-      {
-        double s = 0.0;
-        for(u32 j = 0; j < d.ncoarse; ++j) s += d.ainv[u64(i) * d.ncoarse + j] * double(cv[2 * c.n + j]);
-        cv[i] = dev::masked(c.mask, i) ? DT_(0) : DT_(s);
-      }
-      __syncthreads();
-    }
-    for(int i = d.nlev - 2; i >= 0; --i)
-    {
-      const TailLevel<DT_, VT_>& f = d.lv[i];
-      const TailLevel<DT_, VT_>& c = d.lv[i + 1];
-      DT_* fv = arena + f.off;
-      const DT_* xc = arena + c.off + (i + 2 == d.nlev ? 0u : after_post * c.n);
-      const u32 n = f.n;
-      DT_* xcur = fv + after_pre * n;
-      for(u32 r = tid; r < n; r += kTailThreads)
-      {
-        const DT_ s = tail_row<DT_, VT_>(f.P, r, xc);
-        if(!dev::masked(f.mask, r)) xcur[r] = xcur[r] + s;
-      }
-      __syncthreads();
-      for(int k = 0; k < d.post; ++k)
-      {
-        const DT_* xi = fv + ((after_pre + u32(k)) & 1u) * n;
-        DT_* xo = fv + ((after_pre + u32(k) + 1u) & 1u) * n;
+        const DT_* xc = fv + a * n;
         for(u32 r = tid; r < n; r += kTailThreads)
-          xo[r] = xi[r] + fv[4 * n + r] * (fv[2 * n + r] - tail_row<DT_, VT_>(f.A, r, xi));
-        __syncthreads();
+        {
+          const DT_ s = tail_row<DT_, VT_>(f.A, r, xc);
+          fv[3 * n + r] = dev::masked(f.mask, r) ? DT_(0) : fv[2 * n + r] - s;
+        }
       }
+      else if(kind == kTailRest)
+      {
+        const TailLevel<DT_, VT_>& c = d.lv[li + 1];
+        DT_* cv = arena + c.off;
+        const bool first_step = ((op >> 10) & 1u) != 0u;
+        for(u32 r = tid; r < c.n; r += kTailThreads)
+        {
+          const DT_ s = tail_row<DT_, VT_>(f.R, r, fv + 3 * n);
+          const DT_ v = dev::masked(c.mask, r) ? DT_(0) : s;
+          cv[2 * c.n + r] = v;
+          if(first_step) cv[r] = cv[4 * c.n + r] * v;
+        }
+      }
+      else if(kind == kTailCoarse)
+      {
+        // exact coarse solve with the FP64 inverse of the filtered coarsest matrix
+        for(u32 i = tid; i < d.ncoarse; i += kTailThreads)
+        {
+          double s = 0.0;
+          for(u32 j = 0; j < d.ncoarse; ++j) s += d.ainv[u64(i) * d.ncoarse + j] * double(fv[2 * n + j]);
+          fv[i] = dev::masked(f.mask, i) ? DT_(0) : DT_(s);
+        }
+      }
+      else
+      {
+        const TailLevel<DT_, VT_>& c = d.lv[li + 1];
+        const DT_* xc = arena + c.off + b * c.n;
+        DT_* xcur = fv + a * n;
+        for(u32 r = tid; r < n; r += kTailThreads)
+        {
+          const DT_ s = tail_row<DT_, VT_>(f.P, r, xc);
+          if(!dev::masked(f.mask, r)) xcur[r] = xcur[r] + s;
+        }
+      }
+      __syncthreads();
     }
     const TailLevel<DT_, VT_>& t = d.lv[0];
-    const DT_* xf = arena + t.off + after_post * t.n;
+    const DT_* xf = arena + t.off + out_buf * t.n;
     for(u32 r = tid; r < t.n; r += kTailThreads) d.x_out[r] = xf[r];
   }
 

@@ -742,7 +742,7 @@ namespace
       }
       L.back()->xfinal = L.back()->x.template as<DT>();
       F3_RC(setup_tail(lv));
-      build_cycle();
+      F3_RC(build_cycle());
 
       // outer PCG
       const HostLevel& h0 = lv.front();
@@ -854,16 +854,13 @@ namespace
         return b;
       };
       tail0 = nl - 1;
-      // the tail kernel runs a V-cycle; F- and W-cycles launch every level operation
-      while(cfg.cycle == 0 && tail0 > 0 && L[tail0 - 1]->n <= cfg.tail_max_n && L[tail0 - 1]->n < 65536 &&
+      while(tail0 > 0 && L[tail0 - 1]->n <= cfg.tail_max_n && L[tail0 - 1]->n < 65536 &&
             nl - (tail0 - 1) <= std::size_t(mg::kTailMax) && arena(tail0 - 1) <= u64(cap))
         --tail0;
       tail = mg::TailDesc<DT, VT>{};
       tail_lds = 0;
       if(tail0 + 1 >= nl) return rt::kOk;
       tail.nlev = int(nl - tail0);
-      tail.pre = cfg.pre;
-      tail.post = cfg.post;
       tail.ncoarse = u32(ncoarse);
       tail.ainv = ainv.template as<double>();
       tail.rhs_in = L[tail0]->rhs.template as<DT>();
@@ -895,22 +892,24 @@ namespace
     // ---- multigrid cycle ------------------------------------------------------------------
     // A cycle is a fixed sequence of level operations, built once at init from the cycle
     // kind (multigrid.hpp:1333-1519, _apply_cycle_v/_f/_w over _apply_rest, _apply_coarse,
-    // _apply_prol and _apply_smooth_peak).  The bottom operation solves level tail0: the coarse
-    // level (k_coarse) or, for the V-cycle, the first level of the coarse tail (k_tail).
+    // _apply_prol and _apply_smooth_peak).  The operations on the levels of the coarse tail
+    // (tail0 and below) are grouped into segments that one k_tail launch each runs; a bottom
+    // operation stands for the coarse solve (k_coarse) or for such a segment.
     enum class Op : int { entry, sweep, defect, rest, rest_step, bottom, prol, copy_top };
-    struct CycleOp { Op op; std::size_t lvl; };
+    struct CycleOp { Op op; std::size_t lvl; u32 seg; };
+    struct TailSeg { u32 first, count, out_buf; };
     std::vector<CycleOp> cycle_ops;
+    std::vector<TailSeg> tail_segs;
+    DevMem tail_ops;
 
-    void build_cycle()
+    // the whole cycle over every level, the coarse solve at the coarsest
+    std::vector<CycleOp> cycle_schedule() const
     {
-      std::vector<CycleOp>& o = cycle_ops;
-      o.clear();
-      const std::size_t b = tail0;
-      // per level: the solution sits in t (1) or x (0); the first smoothing step writes x
-      std::vector<int> in_t(L.size(), 0);
+      std::vector<CycleOp> o;
+      const std::size_t b = L.size() - 1;
+      auto push = [&](Op op, std::size_t l) { o.push_back(CycleOp{op, l, 0}); };
       auto sweeps = [&](std::size_t i, int k) {
-        for(int j = 0; j < k; ++j) o.push_back({Op::sweep, i});
-        in_t[i] ^= k & 1;
+        for(int j = 0; j < k; ++j) push(Op::sweep, i);
       };
       // _apply_rest(cur, smooth), multigrid.hpp:1711-1832: pre-smoothing from zero and defect
       // on every level below cur, and on cur itself when smooth (else cur holds the defect of
@@ -921,17 +920,16 @@ namespace
           if(smooth || i > cur)
           {
             sweeps(i, cfg.pre - 1);
-            o.push_back({Op::defect, i});
+            push(Op::defect, i);
           }
-          if(i + 1 < b) { o.push_back({Op::rest_step, i}); in_t[i + 1] = 0; }
-          else o.push_back({Op::rest, i});
+          push(i + 1 < b ? Op::rest_step : Op::rest, i);
         }
       };
       // _apply_prol(cur, smooth), multigrid.hpp:1843-2000 (fixed coarse-grid correction)
       auto prol = [&](std::size_t cur, bool smooth) {
         for(std::size_t i = b; i-- > cur;)
         {
-          o.push_back({Op::prol, i});
+          push(Op::prol, i);
           if(smooth || i > cur) sweeps(i, cfg.post);
         }
       };
@@ -939,11 +937,11 @@ namespace
       // smoother is configured), then the defect
       auto peak = [&](std::size_t l) {
         sweeps(l, cfg.pre + cfg.post);
-        o.push_back({Op::defect, l});
+        push(Op::defect, l);
       };
-      auto coarse = [&]() { o.push_back({Op::bottom, b}); };
+      auto coarse = [&]() { push(Op::bottom, b); };
 
-      o.push_back({Op::entry, 0});
+      push(Op::entry, 0);
       rest(0, true);
       if(cfg.cycle == 1)
       {
@@ -981,9 +979,82 @@ namespace
       else
         coarse();
       prol(0, true);
-      // the top level's solution must end in its xfinal buffer, which the PCG reads
-      if(b > 0 && (in_t[0] != 0) != (L[0]->xfinal == L[0]->t.template as<DT>()))
-        o.push_back({Op::copy_top, 0});
+      return o;
+    }
+
+    int build_cycle()
+    {
+      const std::vector<CycleOp> full = cycle_schedule();
+      const std::size_t nl = L.size();
+      cycle_ops.clear();
+      tail_segs.clear();
+      if(tail0 + 1 >= nl)
+        cycle_ops = full;
+      else
+      {
+        // levels tail0.. run inside k_tail: a segment starts after the restriction into tail0
+        // (the tail takes tail0's first smoothing step itself) and ends before the prolongation
+        // out of it; every step records which buffer of its level's pair it reads
+        std::vector<u32> ops;
+        std::vector<u32> tin(nl, 0);
+        bool open = false;
+        auto close = [&]() {
+          TailSeg& g = tail_segs.back();
+          g.count = u32(ops.size()) - g.first;
+          g.out_buf = tin[tail0];
+          open = false;
+        };
+        for(const CycleOp& c : full)
+        {
+          const bool in_tail = c.op != Op::entry && c.lvl >= tail0;
+          if(!in_tail)
+          {
+            if(open) close();
+            CycleOp u = c;
+            if(u.op == Op::rest_step && u.lvl + 1 == tail0) u.op = Op::rest;
+            cycle_ops.push_back(u);
+            continue;
+          }
+          if(!open)
+          {
+            cycle_ops.push_back(CycleOp{Op::bottom, tail0, u32(tail_segs.size())});
+            tail_segs.push_back(TailSeg{u32(ops.size()), 0, 0});
+            tin[tail0] = 0;
+            open = true;
+          }
+          const u32 l = u32(c.lvl - tail0);
+          switch(c.op)
+          {
+            case Op::sweep: ops.push_back(mg::tail_op(mg::kTailSweep, l, tin[c.lvl], 0, 0)); tin[c.lvl] ^= 1u; break;
+            case Op::defect: ops.push_back(mg::tail_op(mg::kTailDefect, l, tin[c.lvl], 0, 0)); break;
+            case Op::rest: ops.push_back(mg::tail_op(mg::kTailRest, l, 0, 0, 0)); break;
+            case Op::rest_step: ops.push_back(mg::tail_op(mg::kTailRest, l, 0, 0, 1)); tin[c.lvl + 1] = 0; break;
+            case Op::bottom: ops.push_back(mg::tail_op(mg::kTailCoarse, l, 0, 0, 0)); tin[c.lvl] = 0; break;
+            case Op::prol: ops.push_back(mg::tail_op(mg::kTailProl, l, tin[c.lvl], tin[c.lvl + 1], 0)); break;
+            default: return rt::kErrState;
+          }
+        }
+        if(open) close();
+        F3_RC(tail_ops.alloc(sizeof(u32) * std::max<std::size_t>(ops.size(), 1)));
+        if(!ops.empty())
+        {
+          F3_CHECK(hipMemcpyAsync(tail_ops.p, ops.data(), sizeof(u32) * ops.size(), hipMemcpyHostToDevice, s));
+          F3_CHECK(hipStreamSynchronize(s));
+        }
+      }
+      // the top level's iterate must end in its xfinal buffer, which the PCG reads: follow it
+      // through the upper operations (a bottom operation on level 0 writes xfinal itself)
+      int where = 0;   // 0: x, 1: t, 2: xfinal
+      for(const CycleOp& c : cycle_ops)
+      {
+        if(c.lvl != 0) continue;
+        if(c.op == Op::entry) where = 0;
+        else if(c.op == Op::sweep && where != 2) where ^= 1;
+        else if(c.op == Op::bottom) where = 2;
+      }
+      if(where != 2 && (where == 1) != (L[0]->xfinal == L[0]->t.template as<DT>()))
+        cycle_ops.push_back(CycleOp{Op::copy_top, 0, 0});
+      return rt::kOk;
     }
 
     int enqueue_vcycle(const double* def, const int* status)
@@ -1030,7 +1101,11 @@ namespace
           }
           case Op::bottom:
             if(tail0 + 1 < nl)
-              mg::k_tail<DT, VT><<<1, mg::kTailThreads, tail_lds, s>>>(tail, status);
+            {
+              const TailSeg& g = tail_segs[c.seg];
+              mg::k_tail<DT, VT><<<1, mg::kTailThreads, tail_lds, s>>>(tail, tail_ops.template as<u32>() + g.first, g.count,
+                                                                      g.out_buf, status);
+            }
             else
               mg::k_coarse<DT><<<1, kBlock, 0, s>>>(f.n, ainv.template as<double>(), f.mask.template as<u32>(),
                                                     f.rhs.template as<DT>(), f.x.template as<DT>(), status);

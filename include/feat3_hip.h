@@ -314,7 +314,7 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
  * transfer matrices in CSR evaluated by streaming, default 1, or in the level format),
  * "transfer_sigma" (SELL sort window of the transfer matrices when they are SELL, default 1),
  * "cycle" (the multigrid cycle, MultiGridCycle of multigrid.hpp:26-34: 0 V, default, 1 F,
- * 2 W; F and W launch every level operation, the coarse tail serves the V-cycle only);
+ * 2 W; the coarse-tail kernel runs the cycle's steps on the small levels for each of them);
  * takes effect at the next init */
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value);
 /* init_symbolic + init_numeric (solver/base.hpp:227-260) */
