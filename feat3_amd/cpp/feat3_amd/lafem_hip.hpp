@@ -200,6 +200,14 @@ namespace feat3_amd
     class DenseVector<Mem::HIP, DT_, IT_>
     {
       DeviceArray<DT_> _a;
+      // the FileMode value the library's vector files take ("Filemode not supported!" else)
+      static int file_mode(FileMode mode)
+      {
+        if(mode == FileMode::fm_exp) return 0;
+        if(mode == FileMode::fm_dv || mode == FileMode::fm_binary) return 1;
+        if(mode == FileMode::fm_mtx) return 2;
+        throw InternalError("DenseVector: Filemode not supported!");
+      }
     public:
       typedef Mem::HIP MemType;
       typedef DT_ DataType;
@@ -210,10 +218,32 @@ namespace feat3_amd
       DenseVector(Index size, DT_ value) : _a(size) { format(value); }
       explicit DenseVector(const std::vector<DT_>& host) : _a(host.size()) { _a.upload(host.data()); }
 
+      // from a file in the reference's formats (DenseVector(FileMode, String), read_from,
+      // dense_vector.hpp:587-718): fm_exp, fm_dv (or fm_binary), fm_mtx
+      DenseVector(FileMode mode, const std::string& filename) { read_from(mode, filename); }
+
       Index size() const { return _a.size(); }
       DT_* elements() { return _a.data(); }
       const DT_* elements() const { return _a.data(); }
       std::vector<DT_> to_host() const { return _a.download(); }
+
+      void read_from(FileMode mode, const std::string& filename)
+      {
+        const int m = file_mode(mode);
+        std::uint64_t n = 0;
+        hip_check(feat3_io_vector_size(filename.c_str(), m, &n), "DenseVector::read_from");
+        std::vector<double> v(n);
+        hip_check(feat3_io_read_vector(filename.c_str(), m, v.data(), n), "DenseVector::read_from");
+        *this = DenseVector(std::vector<DT_>(v.begin(), v.end()));
+      }
+      // write_out(FileMode, String) (dense_vector.hpp:721-780); text modes keep 6 significant digits
+      void write_out(FileMode mode, const std::string& filename) const
+      {
+        const int m = file_mode(mode);
+        const std::vector<DT_> h = to_host();
+        const std::vector<double> v(h.begin(), h.end());
+        hip_check(feat3_io_write_vector(filename.c_str(), m, v.size(), v.data()), "DenseVector::write_out");
+      }
 
       DenseVector clone(CloneMode mode = CloneMode::Deep) const
       {

@@ -1,5 +1,6 @@
-// io_formats.cpp -- the reference's on-disk formats of SparseMatrixCSR, so that matrices written by
-// a FEAT build (its solvers' inputs, matrix_info, fixtures) feed the HIP path and back:
+// io_formats.cpp -- the reference's on-disk formats of SparseMatrixCSR and DenseVector, so that
+// matrices and vectors written by a FEAT build (its solvers' inputs, matrix_info, fixtures) feed
+// the HIP path and back:
 //
 //  * binary fm_csr (sparse_matrix_csr.hpp:1266-1296 -> Container::_serialise<double, uint64_t>,
 //    container.hpp:320-447): 8-byte words [total bytes, FileMode fm_csr = 4, feature hash of the
@@ -11,6 +12,19 @@
 //    1296-1340 writer): 1-based "row col value" lines after the "rows cols nnz" line, '%' comments;
 //    a symmetric file lists the lower triangle and is expanded.  Entries are sorted by row and
 //    column; of repeated (row, col) pairs the first is kept (the reference inserts into a map).
+//
+// and of DenseVector (dense_vector.hpp:587-780), mode = the FileMode value:
+//
+//  * fm_exp (0): one value per line in C++'s std::scientific (6 digits); the reader skips every
+//    line holding a '#', reads the first number of every other line (an empty line counts as 0)
+//    and, as the reference's getline loop does, drops a last line that has no newline;
+//  * fm_dv (1): Container::_serialise<double, uint64_t>(fm_dv): 8-byte words [total bytes, 1,
+//    feature hash of the value type, feature hash of the index type, #elements = 1, #indices = 0,
+//    #elements_size = 1, #indices_size = 0, #scalar_index = 1, #scalar_dt = 0], elements_size {n},
+//    scalar_index {n}, the n values (FP64) and 16 bytes of padding;
+//  * fm_mtx (2): "%%MatrixMarket matrix array real general", '%' comment lines, "n 1", then one
+//    value per line (std::scientific); a file whose value count differs from n is rejected (the
+//    reference would read past its buffer).
 //
 // Indices are u32 in this library: files with more than 2^32 - 1 rows, columns or entries are
 // rejected.
@@ -132,9 +146,134 @@ namespace
     std::memcpy(&w, &d, sizeof(w));
     return w;
   }
+  constexpr int kFmExp = 0, kFmDv = 1, kFmMtx = 2;   // FileMode values (lafem/base.hpp:31-46)
+
+  // dense_vector.hpp:669-705 (fm_exp) and 609-668 (fm_mtx, the values after the size line)
+  bool read_vector_text(const char* path, int mode, std::vector<double>& v)
+  {
+    std::ifstream f(path);
+    if(!f.is_open()) return false;
+    v.clear();
+    std::string line;
+    u64 rows = 0;
+    if(mode == kFmMtx)
+    {
+      if(!std::getline(f, line) || line.find("%%MatrixMarket matrix array real general") == std::string::npos)
+        return false;
+      bool have = false;
+      while(std::getline(f, line))
+      {
+        if(f.eof()) return false;   // the reference: "Input-file is empty"
+        const std::size_t b = line.find_first_not_of(' ');
+        if(b != std::string::npos && line[b] == '%') continue;
+        std::istringstream is(line);
+        u64 cols = 0;
+        if(!(is >> rows >> cols) || cols != 1) return false;
+        have = true;
+        break;
+      }
+      if(!have) return false;
+    }
+    while(true)
+    {
+      std::getline(f, line);
+      if(mode == kFmExp && line.find('#') != std::string::npos)
+      {
+        if(f.eof()) break;
+        continue;
+      }
+      if(f.eof() || f.fail()) break;
+      const std::size_t b = line.find_first_not_of(' ');
+      const std::string t = b == std::string::npos ? std::string() : line.substr(b);
+      v.push_back(std::atof(t.c_str()));
+    }
+    return mode != kFmMtx || v.size() == rows;
+  }
+
+  bool read_vector_dv(const char* path, std::vector<double>& v)
+  {
+    std::ifstream f(path, std::ios::binary);
+    if(!f.is_open()) return false;
+    u64 total = 0;
+    f.read(reinterpret_cast<char*>(&total), sizeof(u64));
+    if(!f.good() || total < 14 * 8 || total % 8 != 0) return false;
+    std::vector<u64> w(total / 8, 0);
+    f.seekg(0);
+    f.read(reinterpret_cast<char*>(w.data()), std::streamsize(total));
+    if(!f.good()) return false;
+    if(w[1] != u64(kFmDv) || w[4] != 1 || w[5] != 0 || w[6] != 1 || w[7] != 0) return false;
+    if((w[2] >> 33 & 1) == 0 || (w[2] & 0xFFFFFFFFull) == 0) return false;   // floating value type
+    if((w[3] >> 32 & 1) == 0) return false;                                   // integral index type
+    const u64 n = w[10];
+    const u64 at = feat_offset(10 + w[6] + w[7] + w[8]) + w[9];
+    if(at + n > w.size()) return false;
+    v.resize(n);
+    for(u64 i = 0; i < n; ++i) v[i] = as_double(w[at + i]);
+    return true;
+  }
+
+  bool read_vector(const char* path, int mode, std::vector<double>& v)
+  {
+    if(!path) return false;
+    if(mode == kFmDv) return read_vector_dv(path, v);
+    if(mode == kFmExp || mode == kFmMtx) return read_vector_text(path, mode, v);
+    return false;
+  }
 } // namespace
 
 extern "C" {
+
+// DenseVector files: number of values, then the values (host memory, n = the size returned)
+int feat3_io_vector_size(const char* path, int mode, uint64_t* n)
+{
+  std::vector<double> v;
+  if(!n || !read_vector(path, mode, v)) return 1;
+  *n = v.size();
+  return 0;
+}
+
+int feat3_io_read_vector(const char* path, int mode, double* out, uint64_t n)
+{
+  std::vector<double> v;
+  if(!read_vector(path, mode, v) || v.size() != n || (n > 0 && !out)) return 1;
+  std::copy(v.begin(), v.end(), out);
+  return 0;
+}
+
+int feat3_io_write_vector(const char* path, int mode, uint64_t n, const double* v)
+{
+  if(!path || (n > 0 && !v)) return 1;
+  if(mode == kFmDv)
+  {
+    // _serialised_size: 10 header words, elements_size and scalar_index, the values, 16 bytes
+    const u64 at = feat_offset(12);
+    const u64 words = std::max<u64>(12 + n + 2, at + n);
+    std::vector<u64> w(words, 0);
+    w[0] = words * 8;
+    w[1] = u64(kFmDv);
+    w[2] = kHashDouble;
+    w[3] = kHashIndex;
+    w[4] = 1; w[5] = 0; w[6] = 1; w[7] = 0; w[8] = 1; w[9] = 0;
+    w[10] = n;
+    w[11] = n;
+    for(u64 i = 0; i < n; ++i) w[at + i] = as_word(v[i]);
+    std::ofstream f(path, std::ios::binary);
+    if(!f.is_open()) return 1;
+    f.write(reinterpret_cast<const char*>(w.data()), std::streamsize(words * 8));
+    return f.good() ? 0 : 1;
+  }
+  if(mode != kFmExp && mode != kFmMtx) return 1;
+  FILE* f = std::fopen(path, "w");
+  if(!f) return 1;
+  if(mode == kFmMtx)
+  {
+    std::fprintf(f, "%%%%MatrixMarket matrix array real general\n");
+    std::fprintf(f, "%llu 1\n", (unsigned long long)n);
+  }
+  for(u64 i = 0; i < n; ++i) std::fprintf(f, "%e\n", v[i]);   // std::scientific, precision 6
+  return std::fclose(f) == 0 ? 0 : 1;
+}
+
 
 // Matrix Market: sizes (nnz after symmetric expansion and removal of repeated entries)
 int feat3_io_mtx_sizes(const char* path, uint64_t* rows, uint64_t* cols, uint64_t* nnz)

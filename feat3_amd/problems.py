@@ -415,6 +415,24 @@ def read_matrix(path: str) -> Csr:
     return Csr(rows.value, cols.value, rp, col, val)
 
 
+VECTOR_MODES = {"exp": 0, "dv": 1, "mtx": 2}   # FileMode fm_exp, fm_dv, fm_mtx
+
+
+def read_vector(path: str, mode: str) -> np.ndarray:
+    """A DenseVector file of the reference (mode "exp", "dv" or "mtx") as FP64 values."""
+    n = ctypes.c_uint64()
+    capi.call("feat3_io_vector_size", path.encode(), VECTOR_MODES[mode], ctypes.byref(n))
+    out = np.empty(n.value, F64)
+    capi.call("feat3_io_read_vector", path.encode(), VECTOR_MODES[mode], _p(out), n.value)
+    return out
+
+
+def write_vector(path: str, v, mode: str) -> None:
+    """Writes v as the reference's DenseVector file (text modes keep 6 significant digits)."""
+    v = np.ascontiguousarray(v, F64)
+    capi.call("feat3_io_write_vector", path.encode(), VECTOR_MODES[mode], v.size, _p(v))
+
+
 def write_matrix(path: str, A: Csr, symmetric: bool = False) -> None:
     """Writes A as Matrix Market (.mtx; symmetric: lower triangle only) or binary fm_csr."""
     if path.endswith(".mtx"):

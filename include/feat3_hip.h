@@ -457,6 +457,14 @@ int feat3_io_csr_binary_sizes(const char* path, uint64_t* rows, uint64_t* cols, 
 int feat3_io_read_csr_binary(const char* path, uint32_t* row_ptr, uint32_t* col_ind, double* val);
 int feat3_io_write_csr_binary(const char* path, uint64_t rows, uint64_t cols, const uint32_t* row_ptr,
                               const uint32_t* col_ind, const double* val);
+/* DenseVector files (DenseVector::read_from / write_out, dense_vector.hpp:587-780), host memory;
+ * mode = the FileMode value: 0 fm_exp (one std::scientific value per line, '#' lines skipped),
+ * 1 fm_dv (binary Container::_serialise<double, uint64_t>, container.hpp:320-447), 2 fm_mtx
+ * (Matrix Market "array real general", n x 1).  Text files are written with 6 significant
+ * digits, as the reference writes them.  Return 0, or 1 for an unreadable or malformed file. */
+int feat3_io_vector_size(const char* path, int mode, uint64_t* n);
+int feat3_io_read_vector(const char* path, int mode, double* out, uint64_t n);
+int feat3_io_write_vector(const char* path, int mode, uint64_t n, const double* v);
 
 /* CSR -> SELL-C-sigma on the host (generalises SparseMatrixELL::convert,
  * lafem/sparse_matrix_ell.hpp:865-960): rows sorted by length inside windows of sigma rows,

@@ -263,6 +263,19 @@ int main(int argc, char** argv)
       const bool same = Ab.used_elements() == A.used_elements() && Am.used_elements() == A.used_elements() &&
                         h0v == hbv && h0v == hmv;
       std::cout << "file_roundtrip " << int(same) << " " << Ab.used_elements() << std::endl;
+      // vectors: fm_dv bitwise, fm_exp / fm_mtx to the 6 digits the reference writes
+      r0.write_out(LAFEM::FileMode::fm_dv, outdir + "/r.dv");
+      r0.write_out(LAFEM::FileMode::fm_exp, outdir + "/r.exp");
+      r0.write_out(LAFEM::FileMode::fm_mtx, outdir + "/r.mtx");
+      const LAFEM::DenseVector<Mem::HIP, double, u32> vd(LAFEM::FileMode::fm_dv, outdir + "/r.dv");
+      const LAFEM::DenseVector<Mem::HIP, double, u32> ve(LAFEM::FileMode::fm_exp, outdir + "/r.exp");
+      LAFEM::DenseVector<Mem::HIP, double, u32> vm;
+      vm.read_from(LAFEM::FileMode::fm_mtx, outdir + "/r.mtx");
+      const std::vector<double> hd = vd.to_host(), he = ve.to_host(), hm = vm.to_host();
+      bool vsame = hd == h0v && he.size() == h0v.size() && hm.size() == h0v.size();
+      for(std::size_t i = 0; vsame && i < h0v.size(); ++i)
+        vsame = std::abs(he[i] - h0v[i]) <= 5e-7 * std::abs(h0v[i]) && he[i] == hm[i];
+      std::cout << "vector_files " << int(vsame) << " " << vd.size() << std::endl;
     }
     std::cout << "leaked " << MemoryPool<Mem::HIP>::finalise() << std::endl;
   }

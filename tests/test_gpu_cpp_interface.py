@@ -52,6 +52,7 @@ def test_cpp_application_level_solvers(tmp_path, torch_cuda):
     assert res["mixed32"][0] == 2 and abs(res["mixed32"][1] - io) <= 1
     assert rel(load("mixed32")) <= 1e-8
     assert res["file_roundtrip"][0] == 1   # fm_csr and symmetric fm_mtx round trips, bitwise products
+    assert res["vector_files"] == [1, H.n]  # DenseVector fm_dv bitwise, fm_exp / fm_mtx to 6 digits
     # the SolverFactory-built solver is the directly built one (same INI parameters)
     assert res["factory64"] == res["fused64"]
     assert np.array_equal(load("factory64"), load("fused64"))

@@ -82,3 +82,74 @@ def test_mtx_parsing_rules(tmp_path):
         pb.read_matrix(bad)
     with pytest.raises(capi.Feat3HipError):
         pb.read_matrix(str(tmp_path / "missing.csr"))
+
+
+# ---- DenseVector files (dense_vector.hpp:587-780) ---------------------------------------------
+def reference_dv_bytes(v):
+    """fm_dv bytes of a DenseVector<double> written by the reference, restated from
+    container.hpp:320-447: 10 header words, elements_size {n}, scalar_index {n}, the values,
+    16 bytes of padding."""
+    n = v.size
+    gsize = (4 + 6 + 1 + 1) * 8 + n * 8 + 16
+    hdr = [gsize, 1, 8 | 1 << 33 | 1 << 34, 8 | 1 << 32, 1, 0, 1, 0, 1, 0, n, n]
+    out = bytearray(gsize)
+    struct.pack_into(f"<{len(hdr)}Q", out, 0, *hdr)
+    struct.pack_into(f"<{n}d", out, feat_offset(12) * 8, *v)
+    return bytes(out)
+
+
+def test_vector_dv_layout_bytes_and_round_trip(tmp_path):
+    v = np.random.default_rng(3).uniform(-1, 1, 37)
+    v[5] = 0.0
+    v[6] = -1e-300
+    f = str(tmp_path / "v.dv")
+    pb.write_vector(f, v, "dv")
+    with open(f, "rb") as fh:
+        assert fh.read() == reference_dv_bytes(v)
+    assert np.array_equal(pb.read_vector(f, "dv"), v)
+    g = str(tmp_path / "w.dv")
+    w = np.arange(5, dtype=np.float64) - 2.5
+    with open(g, "wb") as fh:
+        fh.write(reference_dv_bytes(w))
+    assert np.array_equal(pb.read_vector(g, "dv"), w)
+    # empty vector
+    pb.write_vector(g, np.zeros(0), "dv")
+    assert pb.read_vector(g, "dv").size == 0
+
+
+def test_vector_exp_and_mtx_text(tmp_path):
+    v = np.array([1.0, -2.5e-7, 3.14159265358979, 0.0, 6.02214076e23])
+    f = str(tmp_path / "v.exp")
+    pb.write_vector(f, v, "exp")
+    # std::scientific with the default precision 6, one value per line
+    assert open(f).read() == "".join(f"{x:e}\n" for x in v)
+    assert np.allclose(pb.read_vector(f, "exp"), v, rtol=5e-7, atol=0)
+    g = str(tmp_path / "v.mtx")
+    pb.write_vector(g, v, "mtx")
+    assert open(g).read() == "%%MatrixMarket matrix array real general\n5 1\n" + "".join(f"{x:e}\n" for x in v)
+    assert np.allclose(pb.read_vector(g, "mtx"), v, rtol=5e-7, atol=0)
+
+
+def test_vector_exp_reader_rules(tmp_path):
+    """The reference's fm_exp loop: a line holding '#' is skipped, leading blanks are ignored,
+    an empty line reads as 0, and a last line without a newline is dropped."""
+    f = str(tmp_path / "r.exp")
+    with open(f, "w") as fh:
+        fh.write("# header\n  1.5\n\n-2e3   # trailing comment skips the line\n4.25\n7.0")
+    assert np.array_equal(pb.read_vector(f, "exp"), np.array([1.5, 0.0, 4.25]))
+
+
+def test_vector_mtx_reader_rules(tmp_path):
+    f = str(tmp_path / "r.mtx")
+    with open(f, "w") as fh:
+        fh.write("%%MatrixMarket matrix array real general\n% a comment\n3 1\n1.0\n  2.0 extra\n3.0\n")
+    assert np.array_equal(pb.read_vector(f, "mtx"), np.array([1.0, 2.0, 3.0]))
+    for bad in ("%%MatrixMarket matrix array real general\n3 2\n1\n2\n3\n",           # not n x 1
+                "%%MatrixMarket matrix array real general\n3 1\n1\n2\n",              # too few values
+                "%%MatrixMarket matrix coordinate real general\n3 1 1\n1 1 1.0\n"):   # not an array file
+        with open(f, "w") as fh:
+            fh.write(bad)
+        with pytest.raises(Exception):
+            pb.read_vector(f, "mtx")
+    with pytest.raises(Exception):
+        pb.read_vector(str(tmp_path / "missing.dv"), "dv")
