@@ -84,3 +84,24 @@ def test_pcg_with_cycle_matches_oracle(torch, cycle, name, dim, deg, lvl, tail):
     assert np.allclose(hg[: io + 1], ho[: io + 1], rtol=1e-9, atol=1e-14 * ho[0])
     assert (info["tail_levels"] > 0) == tail
 
+
+
+@pytest.mark.parametrize("tail", [True, False])
+@pytest.mark.parametrize("cycle", ["f", "w"])
+@pytest.mark.parametrize("fine,coarse", [(2, 2), (3, 2), (4, 2)])
+def test_short_hierarchies(torch, cycle, fine, coarse, tail):
+    """One level (the cycle is the exact coarse solve), two levels (F = V; the W-cycle's one peak
+    is the top level itself) and three levels, FP64, against the oracle."""
+    from feat3_amd.solver import MGPCG
+    print(f"cycle {cycle} levels {fine}..{coarse} tail {tail}", flush=True)
+    H = pb.hierarchy(2, 1, fine, coarse)
+    d = _defect(H, fine)
+    S = po.System.from_hierarchy(H)
+    cor = np.zeros(H.n)
+    S.vcycle(cor, d, po.params(pre=2, post=3, cycle=CYC[cycle]))
+    s = MGPCG(H, pre=2, post=3, cycle=cycle, options=None if tail else dict(tail_max_n=0)).init()
+    dd = torch.from_numpy(d).cuda()
+    dc = torch.zeros_like(dd)
+    s.vcycle(dc, dd)
+    s.done()
+    assert rel(dc.cpu().numpy(), cor) <= 1e-12
