@@ -34,6 +34,14 @@ namespace dev
   template<>
   __device__ __forceinline__ float ldc<float, half>(cptr<half> p, std::uint64_t i) { return float(p[i]); }
 
+  // nontemporal load with conversion into the arithmetic type (binary16: an ordinary load)
+  template<typename DT_, typename VT_>
+  __device__ __forceinline__ DT_ ld_nt(const VT_* p, std::uint64_t i) { return DT_(__builtin_nontemporal_load(p + i)); }
+  template<>
+  __device__ __forceinline__ float ld_nt<float, half>(const half* p, std::uint64_t i) { return float(p[i]); }
+  template<>
+  __device__ __forceinline__ double ld_nt<double, half>(const half* p, std::uint64_t i) { return double(float(p[i])); }
+
   // nontemporal streaming loads for matrix arrays that are read exactly once per sweep
   template<typename T>
   __device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
