@@ -577,7 +577,7 @@ namespace
     bool use_graph = true;
     int format = 0;           // 0 CSR, 1 SELL-C-sigma, 2 banded where it fits (else SELL)
     u64 sell_C = 64, sell_sigma = 64;
-    int lanes = 0;            // forced lanes per CSR row (0 = by mean row length)
+    int lanes = 0;            // forced lanes per CSR row (0 = the streaming evaluator)
     bool index16 = true;      // SELL columns as 16-bit row offsets where they fit
     bool compact_transfer = true; // device P/R without the stored structural zeros
     // levels with at most this many rows (and whose vectors fit in one workgroup's LDS) run in
@@ -591,6 +591,12 @@ namespace
     // prolongation rows hold 1..8 entries (Q1), which SELL pads to the chunk's widest row
     // (measured 3D Q1 L7 FP32 V-cycle 0.603 -> 0.583 ms, 2D Q2 L9 0.643 -> 0.610 ms)
     bool transfer_csr = true;
+    // launch shape of the transfer matrices: lanes per row (-1: as the level matrices) and row
+    // positions per block (0: derived from the row count).  128 rows per block (more, smaller
+    // blocks than the default grid on the mid-size levels) measured V-cycle 3D Q1 L7 584 -> 580 us,
+    // 2D Q2 L9 590 -> 576 us; 64 rows, or lifting the 8192-block grid limit, were slower.
+    int transfer_lanes = -1;
+    u64 transfer_rows_per_block = 128;
     double tol_rel = 1e-8, tol_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16), tol_abs_low = 0.0;
     double div_rel = 1.0 / 2.220446049250313e-16, div_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16);
     double stag_rate = 0.95;
@@ -684,16 +690,19 @@ namespace
           if(h.P.rows != l->n || h.R.cols != l->n || h.P.cols != lv[i + 1].A.rows || h.R.rows != lv[i + 1].A.rows)
             return rt::kErrArg;
           const bool tsell = want_sell && !cfg.transfer_csr;
+          const int tlanes = cfg.transfer_lanes >= 0 ? cfg.transfer_lanes : cfg.lanes;
           if(h.P.on_device())
           {
-            F3_RC(l->P.upload_dev(h.P, s, tsell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16, false));
-            F3_RC(l->R.upload_dev(h.R, s, tsell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16, false));
+            F3_RC(l->P.upload_dev(h.P, s, tsell, cfg.sell_C, cfg.transfer_sigma, tlanes, cfg.compact_transfer, cfg.index16, false));
+            F3_RC(l->R.upload_dev(h.R, s, tsell, cfg.sell_C, cfg.transfer_sigma, tlanes, cfg.compact_transfer, cfg.index16, false));
           }
           else
           {
-            F3_RC(l->P.upload(h.P, s, tsell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16));
-            F3_RC(l->R.upload(h.R, s, tsell, cfg.sell_C, cfg.transfer_sigma, cfg.lanes, cfg.compact_transfer, cfg.index16));
+            F3_RC(l->P.upload(h.P, s, tsell, cfg.sell_C, cfg.transfer_sigma, tlanes, cfg.compact_transfer, cfg.index16));
+            F3_RC(l->R.upload(h.R, s, tsell, cfg.sell_C, cfg.transfer_sigma, tlanes, cfg.compact_transfer, cfg.index16));
           }
+          l->P.rows_per_block = cfg.transfer_rows_per_block;
+          l->R.rows_per_block = cfg.transfer_rows_per_block;
         }
         const std::vector<u32> m = make_mask(l->n, h.fidx);
         F3_RC(l->mask.alloc(sizeof(u32) * m.size()));
@@ -1607,7 +1616,7 @@ int feat3_mgpcg_set_precond(feat3_mgpcg* h, int kind)
 }
 
 // graph: replay the captured iteration (1) or launch kernel by kernel (0);
-// lanes: lanes per CSR row for all levels (0 = by mean row length);
+// lanes: lanes per CSR row for all levels (0 = the streaming evaluator);
 // sell_C / sell_sigma: chunk height and sort window of the SELL format (format 1)
 int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, u64 sell_C, u64 sell_sigma)
 {
@@ -1634,6 +1643,10 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
   else if(k == "sell_C" && value >= 1 && value <= 1024) h->cfg.sell_C = u64(value);
   else if(k == "tail_max_n" && value >= 0) h->cfg.tail_max_n = u64(value);
   else if(k == "transfer_csr") h->cfg.transfer_csr = value != 0.0;
+  else if(k == "transfer_lanes" && (value == -1 || value == 0 || value == 2 || value == 4 || value == 8 || value == 16 ||
+                                    value == 32 || value == 64))
+    h->cfg.transfer_lanes = int(value);
+  else if(k == "transfer_rows_per_block" && value >= 0) h->cfg.transfer_rows_per_block = u64(value);
   else if(k == "cycle" && (value == 0 || value == 1 || value == 2)) h->cfg.cycle = int(value);
   else if(k == "rows_per_block" && value >= 0) h->cfg.rows_per_block = u64(value);
   else if(k == "transfer_sigma" && value >= 1) h->cfg.transfer_sigma = u64(value);

@@ -313,6 +313,9 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
  * level-matrix kernels, a multiple of 64; 0 = derived from the row count), "transfer_csr" (0/1:
  * transfer matrices in CSR evaluated by streaming, default 1, or in the level format),
  * "transfer_sigma" (SELL sort window of the transfer matrices when they are SELL, default 1),
+ * "transfer_lanes" (lanes per row of CSR transfer matrices: -1 as "lanes", default; 0 streaming),
+ * "transfer_rows_per_block" (row positions per workgroup of the transfer kernels, default 128;
+ * 0 = derived from the row count),
  * "cycle" (the multigrid cycle, MultiGridCycle of multigrid.hpp:26-34: 0 V, default, 1 F,
  * 2 W; the coarse-tail kernel runs the cycle's steps on the small levels for each of them);
  * takes effect at the next init */

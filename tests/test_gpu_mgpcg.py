@@ -105,7 +105,9 @@ def test_sell_graph_and_lanes_variants_agree(torch):
                dict(fmt="sell", options=dict(index16=0, compact_transfer=0)),
                dict(fmt="sell", options=dict(transfer_csr=0, transfer_sigma=64)),
                dict(fmt="banded", options=dict(transfer_csr=0, transfer_sigma=256)), dict(fmt="sell", options=dict(transfer_csr=0)),
-               dict(fmt="banded", options=dict(rows_per_block=128))]:
+               dict(fmt="banded", options=dict(rows_per_block=128)),
+               dict(options=dict(transfer_lanes=4, transfer_rows_per_block=0)),
+               dict(fmt="banded", options=dict(transfer_rows_per_block=512))]:
         st, it, x, _ = solve_gpu(torch, H, b, **kw)
         assert st == 2 and it == ref[1], kw
         assert rel(x, ref[2]) <= 1e-12, kw
