@@ -481,6 +481,7 @@ def main():
         hist = s.history()
         sol = x.cpu().numpy().copy()
         sweep_ms, _ = s.time_kernel(0, 50)
+        outer_ms, outer_bytes = s.time_kernel(2, 50)
         vc_ms, _ = s.time_kernel(3, 10)
         info = s.info()
         s.done()
@@ -489,7 +490,9 @@ def main():
         r = dict(ms_per_solve=ms, ms_min=1e3 * min(times), iterations=iters,
                  dof_iter_per_s=n * sum(iters) / sum(times), final_rel_residual=float(hist[-1] / hist[0]),
                  sweep_us=sweep_ms * 1e3, sweep_bytes=sb, sweep_GBs=sb / (sweep_ms * 1e-3) / 1e9,
-                 vcycle_ms=vc_ms, kernels_per_vcycle=info["kernels_per_vcycle"])
+                 vcycle_ms=vc_ms, kernels_per_vcycle=info["kernels_per_vcycle"],
+                 outer_product_us=outer_ms * 1e3, outer_product_bytes=outer_bytes,
+                 outer_product_GBs=outer_bytes / (outer_ms * 1e-3) / 1e9)
         log(f"{inner}/{fmt}/{cycle}: {ms:.3f} ms/solve, it {iters}, rel.res {r['final_rel_residual']:.2e}, "
             f"sweep {r['sweep_us']:.1f} us {r['sweep_GBs']:.0f} GB/s, {cycle.upper()}-cycle {vc_ms:.3f} ms")
         return r, sol, times, iters

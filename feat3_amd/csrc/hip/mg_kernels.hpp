@@ -127,9 +127,10 @@ namespace mg
   {
     static constexpr int G = 1;
     // FP64 values are read with nontemporal loads: at 3D Q1 level 7 (464 MB, larger than the
-    // 256 MB Infinity Cache) the FP64 sweep went 117 -> 108 us and the outer PCG product with it
-    // (headline solve 3.40 -> 3.32 ms).  FP32 values (232 MB) stay ordinary loads: consecutive
-    // sweeps reuse them from the Infinity Cache, which nontemporal loads give up (39.7 -> 45.5 us).
+    // 256 MB Infinity Cache) the FP64 sweep went 117 -> 107 us and the outer PCG product
+    // 112 -> 106 us (FP32-inner solve 3.31 -> 3.26 ms, same GPU, alternating builds).  FP32
+    // values (232 MB) stay ordinary loads: consecutive sweeps reuse them from the Infinity Cache,
+    // which nontemporal loads give up (39.7 -> 45.5 us).
     static constexpr bool kNonTemporal = sizeof(VT_) == 8;
     const u32* off;
     const VT_* val;

@@ -645,6 +645,7 @@ namespace
     mg::PcgState* pinned = nullptr;
     hipEvent_t ev[kRing] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t t0 = nullptr, t1 = nullptr;
+    DevMem timing_state;   // solver state "in progress" for timing the outer product alone
 
     ~Engine() override
     {
@@ -1367,6 +1368,24 @@ namespace
                                                              (const DT*)f.x.template as<DT>(), f.def.template as<DT>(), nullptr);
               }));
               break;
+            case 2:
+            {
+              // the outer FP64 product q = filter(A p) with its block partials of q.p, on a
+              // state of its own that says "in progress"
+              if(!timing_state.p)
+              {
+                mg::PcgState h{};
+                h.status = mg::kProgress;
+                F3_RC(timing_state.alloc(sizeof(mg::PcgState)));
+                F3_CHECK(hipMemcpyAsync(timing_state.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+              }
+              F3_RC(with_rows<double, double>(outer_matrix(), [&](auto A, mg::RowGrid g) {
+                mg::k_pcg_spmv_dot<<<g.blocks, kBlock, 0, s>>>(A, g.per, mask0_ptr(), p.template as<double>(),
+                                                               q.template as<double>(), timing_state.template as<mg::PcgState>(),
+                                                               partials.template as<double>());
+              }));
+              break;
+            }
             case 3: F3_RC(enqueue_vcycle(r.template as<double>(), nullptr)); break;
             default: return rt::kErrArg;
           }
@@ -1383,6 +1402,7 @@ namespace
         // sweep: matrix + x gathered once + rhs + dinv + x_out; defect: matrix + x + rhs + def (+mask)
         if(which == 0) *bytes = f.A.matrix_bytes() + f.n * vb * 4;
         else if(which == 1) *bytes = f.A.matrix_bytes() + f.n * vb * 3 + (f.n + 31) / 32 * 4;
+        else if(which == 2) *bytes = outer_matrix().matrix_bytes() + n0 * 8 * 2 + (n0 + 31) / 32 * 4;
         else *bytes = 0;
       }
       return rt::kOk;

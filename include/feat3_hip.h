@@ -334,7 +334,8 @@ int feat3_mgpcg_vcycle(feat3_mgpcg* h, void* caller_stream, double* cor, const d
 /* defect norms of the last solve: [0] initial, [k] after iteration k */
 int feat3_mgpcg_history(feat3_mgpcg* h, double* out_host, uint64_t cap, uint64_t* len);
 /* average duration of `reps` launches of one hot-path kernel on the solver's stream
- * (which: 0 finest smoothing sweep, 1 finest defect, 3 full cycle) and its algorithmic bytes */
+ * (which: 0 finest smoothing sweep, 1 finest defect, 2 outer FP64 product q = A p with its dot
+ * partials, 3 full cycle) and its algorithmic bytes */
 int feat3_mgpcg_time_kernel(feat3_mgpcg* h, int which, int reps, double* avg_ms, uint64_t* bytes);
 /* [levels, n, nnz, lanes, sell size, kernels per cycle, cycle bytes, outer bytes,
  *  sizeof inner, sizeof stored value, 16-bit SELL columns on the finest level, SELL chunks

@@ -224,3 +224,25 @@ def test_coarse_tail_matches_per_level_launches(torch, dim, deg, lvl, tail_level
     else:
         _, _, xo, _ = solve_orc(H, b)
         assert rel(x1, xo) <= 1e-8
+
+
+def test_time_kernel_entries(torch):
+    """feat3_mgpcg_time_kernel: every hot-path kernel it times runs and reports its algorithmic
+    bytes; timing leaves a following solve unchanged (bitwise)."""
+    from feat3_amd.solver import MGPCG
+    H = pb.hierarchy(3, 1, 4, 1)
+    b = pb.filtered_rhs(H, 0)
+    ref = solve_gpu(torch, H, b, inner="f32", fmt="banded")
+    s = MGPCG(H, inner="f32", fmt="banded").init()
+    n, nnz = H.n, H.levels[0].A.nnz
+    for which, min_bytes in ((0, nnz * 4 + 4 * n * 4), (1, nnz * 4 + 3 * n * 4), (2, nnz * 8 + 2 * n * 8), (3, 0)):
+        ms, nb = s.time_kernel(which, 3)
+        assert ms > 0.0, which
+        assert nb >= min_bytes, (which, nb)
+    x = torch.zeros(H.n, dtype=torch.float64, device="cuda")
+    st, it = s.correct(x, torch.from_numpy(b).cuda())
+    s.done()
+    assert st == 2 and it == ref[1]
+    assert np.array_equal(x.cpu().numpy(), ref[2])
+    with pytest.raises(Exception):
+        MGPCG(H).init().time_kernel(7, 1)
