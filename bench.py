@@ -370,6 +370,8 @@ def bench_spmv(args):
         lambda: capi.call("lafem_hip_norm2_f64", sp, pb_, nbig, pr))
     add("calibration: copy (feat3_hip_copy, 2 GiB)", "calibration", "f64", 16 * nbig,
         lambda: capi.call("feat3_hip_copy", sp, pb2, pb_, 8 * nbig))
+    add("calibration: read (feat3_hip_read_probe, nontemporal, 2 GiB)", "calibration", "f64", 8 * nbig,
+        lambda: capi.call("feat3_hip_read_probe", sp, pb_, 8 * nbig, pr))
     del big, big2
     spmv = [r for r in rows if r["format"] not in ("blas1", "calibration")]
     best = max(spmv, key=lambda r: r["GBs"])

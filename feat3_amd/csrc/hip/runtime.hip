@@ -76,6 +76,35 @@ namespace rt
     return kOk;
   }
 } // namespace rt
+
+namespace
+{
+  typedef double d2 __attribute__((ext_vector_type(2)));
+
+  // read-rate calibration: every 16-byte word of the buffer read once with nontemporal loads,
+  // four in flight per thread; the sum is stored only in a case the data never produces, which
+  // keeps the loads without a reduction
+  __global__ __launch_bounds__(256) void k_read_probe(const d2* __restrict__ a, std::uint64_t n2, double* out)
+  {
+    const std::uint64_t stride = std::uint64_t(gridDim.x) * 256;
+    std::uint64_t i = std::uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    double s = 0.0;
+    for(; i + 3 * stride < n2; i += 4 * stride)
+    {
+      d2 v[4];
+#pragma unroll
+      for(int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(a + i + u * stride);
+#pragma unroll
+      for(int u = 0; u < 4; ++u) s += v[u].x + v[u].y;
+    }
+    for(; i < n2; i += stride)
+    {
+      const d2 v = a[i];
+      s += v.x + v.y;
+    }
+    if(s == -1.0e308) *out = s;
+  }
+} // namespace
 } // namespace feat3_amd
 
 using namespace feat3_amd;
@@ -167,6 +196,16 @@ int feat3_hip_copy(void* stream, void* dst, const void* src, std::uint64_t bytes
   if(bytes == 0) return rt::kOk;
   return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)) == hipSuccess
            ? rt::kOk : rt::kErrRuntime;
+}
+// HBM read-rate calibration (no counterpart in the reference): reads bytes (a multiple of 16,
+// 16-byte aligned) once; out is a device double that the kernel does not write for finite data
+int feat3_hip_read_probe(void* stream, const void* p, std::uint64_t bytes, double* out)
+{
+  if(!p || !out || bytes % 16 != 0 || reinterpret_cast<std::uintptr_t>(p) % 16 != 0) return rt::kErrArg;
+  if(bytes == 0) return rt::kOk;
+  feat3_amd::k_read_probe<<<16384, 256, 0, static_cast<hipStream_t>(stream)>>>(static_cast<const feat3_amd::d2*>(p),
+                                                                               bytes / 16, out);
+  return hipGetLastError() == hipSuccess ? rt::kOk : rt::kErrRuntime;
 }
 int feat3_hip_synchronise(void* stream)
 {

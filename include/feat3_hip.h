@@ -258,6 +258,10 @@ int feat3_hip_upload(void* stream, void* dst, const void* src, uint64_t bytes);
 int feat3_hip_download(void* stream, void* dst, const void* src, uint64_t bytes);
 int feat3_hip_copy(void* stream, void* dst, const void* src, uint64_t bytes);
 int feat3_hip_synchronise(void* stream);
+/* HBM read-rate calibration, no counterpart in the reference (for measuring the other kernels
+ * against what the GPU sustains): reads `bytes` (a multiple of 16, 16-byte aligned) once with
+ * nontemporal 16-byte loads; `out` is a device double that finite data never causes to be written */
+int feat3_hip_read_probe(void* stream, const void* p, uint64_t bytes, double* out);
 /* lanes per row of the CSR/BCSR apply kernels (0 = by mean row length); replaces the block
  * size globals that lafem/tuning.hpp:72-163 sets */
 int feat3_hip_set_csr_lanes(int lanes);
