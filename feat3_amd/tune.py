@@ -7,11 +7,12 @@ the fastest:
     per row, or SELL-C-sigma with chunk height C in {32, 64, 128, 256} and sort window sigma in
     {1, 64};
   * rows per workgroup of the row kernels (0 = derived from the row count, or 128..2048);
+  * rows per workgroup of the transfer kernels (restriction, prolongation);
   * for the batched solver, the locality tile of the row order.
 The objective is the measured time of one full V-cycle (feat3_*_time_kernel(3)), which covers
 every level, the transfers and the coarse solve, i.e. what a PCG iteration spends in the
 preconditioner.  Stage 1 searches format and chunk shape at the default grid, stage 2 the rows
-per workgroup for the winner.  Results can be cached in a JSON file keyed by the problem shape.
+per workgroup for the winner, stage 3 the rows per workgroup of its transfer kernels.  Results can be cached in a JSON file keyed by the problem shape.
 """
 from __future__ import annotations
 
@@ -27,6 +28,8 @@ FORMATS: List[dict] = (
     + [dict(fmt="sell", sell_C=C, sell_sigma=s) for C in (32, 64, 128, 256) for s in (1, 64)]
 )
 ROWS_PER_BLOCK = (0, 128, 256, 512, 1024, 2048)
+TRANSFER_ROWS_PER_BLOCK = (0, 256, 512)   # the solver's default, 128, is the reference point
+_OPTION_KEYS = ("rows_per_block", "transfer_rows_per_block")
 TILE_ROWS = (0, 512, 4096)
 
 
@@ -51,12 +54,8 @@ def _store(cache: Optional[str], table: Dict[str, dict]) -> None:
 def time_vcycle(H: Hierarchy, inner: str, reps: int = 10, **cfg) -> float:
     """Average milliseconds of one V-cycle of a solver built with the given configuration."""
     from .solver import MGPCG
-    kw = dict(cfg)
-    opts = dict(kw.pop("options", {}) or {})
-    rpb = kw.pop("rows_per_block", 0)
-    if rpb:
-        opts["rows_per_block"] = rpb
-    s = MGPCG(H, inner=inner, options=opts, **kw).init()
+    kw = solver_kwargs({"config": cfg})
+    s = MGPCG(H, inner=inner, **kw).init()
     try:
         s.time_kernel(3, 2)
         ms, _ = s.time_kernel(3, reps)
@@ -66,7 +65,8 @@ def time_vcycle(H: Hierarchy, inner: str, reps: int = 10, **cfg) -> float:
 
 
 def tune_mgpcg(H: Hierarchy, inner: str = "f32", reps: int = 10, cache: Optional[str] = None,
-               formats: Optional[List[dict]] = None, rows_per_block=ROWS_PER_BLOCK, log=None) -> dict:
+               formats: Optional[List[dict]] = None, rows_per_block=ROWS_PER_BLOCK,
+               transfer_rows_per_block=TRANSFER_ROWS_PER_BLOCK, log=None) -> dict:
     """Returns {"config": best keyword arguments for solver.MGPCG, "ms": V-cycle time,
     "trials": [(config, ms), ...]}."""
     key = _key(H, inner)
@@ -92,6 +92,14 @@ def tune_mgpcg(H: Hierarchy, inner: str = "f32", reps: int = 10, cache: Optional
             log(f"tune {cfg}: {ms:.3f} ms")
         if ms < best_ms:
             best, best_ms = cfg, ms
+    for t in transfer_rows_per_block:
+        cfg = dict(best, transfer_rows_per_block=t)
+        ms = time_vcycle(H, inner, reps, **cfg)
+        trials.append((cfg, ms))
+        if log:
+            log(f"tune {cfg}: {ms:.3f} ms")
+        if ms < best_ms:
+            best, best_ms = cfg, ms
     out = {"config": best, "ms": best_ms, "trials": trials}
     table[key] = out
     _store(cache, table)
@@ -101,9 +109,14 @@ def tune_mgpcg(H: Hierarchy, inner: str = "f32", reps: int = 10, cache: Optional
 def solver_kwargs(result: dict) -> dict:
     """Turns a tuning result into keyword arguments of solver.MGPCG."""
     cfg = dict(result["config"])
-    rpb = cfg.pop("rows_per_block", 0)
-    if rpb:
-        cfg["options"] = dict(cfg.get("options", {}) or {}, rows_per_block=rpb)
+    opts = dict(cfg.pop("options", {}) or {})
+    for k in _OPTION_KEYS:
+        if k in cfg:
+            v = cfg.pop(k)
+            if v or k == "transfer_rows_per_block":   # 0 there selects the derived grid
+                opts[k] = v
+    if opts:
+        cfg["options"] = opts
     return cfg
 
 

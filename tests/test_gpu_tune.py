@@ -34,9 +34,9 @@ def test_tuner_picks_fastest_trial_and_tuned_solver_matches_oracle(torch, tmp_pa
     res = tune.tune_mgpcg(H, inner="f64", reps=3, cache=cache,
                           formats=[dict(fmt="csr", lanes=8), dict(fmt="sell", sell_C=64, sell_sigma=1),
                                    dict(fmt="sell", sell_C=128, sell_sigma=64)],
-                          rows_per_block=(0, 256, 1024))
+                          rows_per_block=(0, 256, 1024), transfer_rows_per_block=(0, 512))
     ms = [t[1] for t in res["trials"]]
-    assert len(ms) == 5 and all(m > 0 for m in ms)
+    assert len(ms) == 7 and all(m > 0 for m in ms)
     assert res["ms"] == min(ms)
     with open(cache) as f:
         assert len(json.load(f)) == 1
