@@ -117,47 +117,64 @@ namespace
     for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
       out[i] = in[i];
   }
-  // bit (col - row + rows - 1) for every entry: the set of diagonals.  Banded matrices set the
-  // same few bits from every row, so a bit is tested with a plain load first and the atomic
+  // Entries the banded copy of an inner-precision level matrix leaves out: |a_ij| <= eps2 |a_ii|
+  // with eps2 the square of the inner precision's epsilon (FP32: 1.4e-14; FP64 keeps every
+  // nonzero).  Such a coupling changes a row sum by less than eps^2 of its diagonal term, far
+  // below the rounding of the inner arithmetic itself (binary16 storage flushes such values to
+  // zero anyway).  The 3D Q1 Laplacian's 6 face couplings per row are roundoff residues of about
+  // 2e-17 of the diagonal, so its FP32 band has 21 diagonals instead of 27 (22% fewer bytes per
+  // sweep).  Host and device builders evaluate the same expression on the same FP64 values.
+  __host__ __device__ inline bool band_keeps(double v, double diag, double eps2)
+  {
+    return (v < 0.0 ? -v : v) > eps2 * (diag < 0.0 ? -diag : diag);
+  }
+  __device__ inline double row_diagonal(const u32* rp, const u32* col, const double* val, u64 r)
+  {
+    for(u64 k = rp[r]; k < rp[r + 1]; ++k)
+      if(u64(col[k]) == r) return val[k];
+    return 0.0;
+  }
+
+  // bit (col - row + rows - 1) for every kept entry: the set of diagonals.  Banded matrices set
+  // the same few bits from every row, so a bit is tested with a plain load first and the atomic
   // issued only while it reads clear (a repeated OR is harmless; unconditional atomics on ~27
   // words serialised 57 M entries: 202 ms for 3D Q1 L7)
-  __global__ void k_diag_bitmap(u64 rows, const u32* __restrict__ rp, const u32* __restrict__ col, u32* bits)
+  __global__ void k_diag_bitmap(u64 rows, const u32* __restrict__ rp, const u32* __restrict__ col,
+                                const double* __restrict__ val, double eps2, u32* bits)
   {
     for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
+    {
+      const double d = row_diagonal(rp, col, val, r);
       for(u64 k = rp[r]; k < rp[r + 1]; ++k)
       {
+        if(!band_keeps(val[k], d, eps2)) continue;
         const u64 o = u64(col[k]) + rows - 1 - r;
         const u32 m = 1u << (o & 31u);
         if((__atomic_load_n(&bits[o >> 5], __ATOMIC_RELAXED) & m) == 0u) atomicOr(&bits[o >> 5], m);
       }
+    }
   }
-  // chunk-major banded values from CSR (offsets stored as off + rows - 1, ascending)
+  // chunk-major banded values from CSR (offsets stored as off + rows - 1, ascending); the left
+  // out entries are skipped (their diagonal need not be stored; the band is zero-filled)
+  template<typename VT> __device__ inline VT band_value(double v) { return VT(float(v)); }
+  template<> __device__ inline double band_value<double>(double v) { return v; }
   template<typename VT>
   __global__ void k_banded_scatter(u64 rows, const u32* __restrict__ rp, const u32* __restrict__ col,
-                                   const double* __restrict__ val, const u32* __restrict__ boff, u32 nd, VT* __restrict__ bval)
+                                   const double* __restrict__ val, double eps2, const u32* __restrict__ boff, u32 nd,
+                                   VT* __restrict__ bval)
   {
     for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
+    {
+      const double d = row_diagonal(rp, col, val, r);
       for(u64 k = rp[r]; k < rp[r + 1]; ++k)
       {
+        if(!band_keeps(val[k], d, eps2)) continue;
         const u32 key = u32(u64(col[k]) + rows - 1 - r);
         u32 lo = 0, hi = nd;
         while(lo < hi) { const u32 mid = (lo + hi) / 2; if(boff[mid] < key) lo = mid + 1; else hi = mid; }
-        bval[(r >> 6) * u64(nd) * 64 + u64(lo) * 64 + (r & 63)] = VT(float(val[k]));
+        bval[(r >> 6) * u64(nd) * 64 + u64(lo) * 64 + (r & 63)] = band_value<VT>(val[k]);
       }
-  }
-  template<>
-  __global__ void k_banded_scatter<double>(u64 rows, const u32* __restrict__ rp, const u32* __restrict__ col,
-                                           const double* __restrict__ val, const u32* __restrict__ boff, u32 nd,
-                                           double* __restrict__ bval)
-  {
-    for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
-      for(u64 k = rp[r]; k < rp[r + 1]; ++k)
-      {
-        const u32 key = u32(u64(col[k]) + rows - 1 - r);
-        u32 lo = 0, hi = nd;
-        while(lo < hi) { const u32 mid = (lo + hi) / 2; if(boff[mid] < key) lo = mid + 1; else hi = mid; }
-        bval[(r >> 6) * u64(nd) * 64 + u64(lo) * 64 + (r & 63)] = val[k];
-      }
+    }
   }
   inline unsigned setup_blocks(u64 n)
   {
@@ -316,7 +333,7 @@ namespace
       DevMem bits;
       F3_RC(bits.alloc(sizeof(u32) * nw));
       F3_CHECK(hipMemsetAsync(bits.p, 0, sizeof(u32) * nw, s));
-      k_diag_bitmap<<<setup_blocks(rows), 256, 0, s>>>(rows, drp, dcol, bits.template as<u32>());
+      k_diag_bitmap<<<setup_blocks(rows), 256, 0, s>>>(rows, drp, dcol, dval, band_eps2(), bits.template as<u32>());
       std::vector<u32> hb(nw);
       F3_CHECK(hipMemcpyAsync(hb.data(), bits.p, sizeof(u32) * nw, hipMemcpyDeviceToHost, s));
       F3_CHECK(hipStreamSynchronize(s));
@@ -333,8 +350,8 @@ namespace
       F3_RC(bval.alloc(sizeof(VT) * nd * padded));
       F3_CHECK(hipMemcpyAsync(boff.p, ho.data(), sizeof(u32) * nd, hipMemcpyHostToDevice, s));
       F3_CHECK(hipMemsetAsync(bval.p, 0, sizeof(VT) * nd * padded, s));
-      k_banded_scatter<VT><<<setup_blocks(rows), 256, 0, s>>>(rows, drp, dcol, dval, boff.template as<u32>(), u32(nd),
-                                                             bval.template as<VT>());
+      k_banded_scatter<VT><<<setup_blocks(rows), 256, 0, s>>>(rows, drp, dcol, dval, band_eps2(), boff.template as<u32>(),
+                                                             u32(nd), bval.template as<VT>());
       F3_RC(rt::check_launch());
       F3_CHECK(hipStreamSynchronize(s));   // ho leaves scope
       ndiag = u32(nd);
@@ -346,15 +363,27 @@ namespace
     // (col - row) and the zero fill stays small: at most kMaxDiag diagonals and
     // ndiag * rows <= 1.25 * nnz.  Otherwise the matrix keeps its other format.
     static constexpr u32 kMaxDiag = 64;
+    // eps^2 of the inner arithmetic (binary16 storage computes in FP32)
+    static double band_eps2()
+    {
+      const double e = sizeof(VT) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
+      return e * e;
+    }
     int build_banded(const HostCsr& h, hipStream_t s)
     {
       banded = false;
       if(rows != cols || rows == 0 || rows >= (u64(1) << 31)) return rt::kOk;
+      const double eps2 = band_eps2();
+      std::vector<double> dg(rows, 0.0);
+      for(u64 r = 0; r < rows; ++r)
+        for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
+          if(u64(h.col[k]) == r) { dg[r] = h.val[k]; break; }
       std::vector<std::int64_t> offs;
       auto find = [&](std::int64_t o) { return std::lower_bound(offs.begin(), offs.end(), o); };
       for(u64 r = 0; r < rows; ++r)
         for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
         {
+          if(!band_keeps(h.val[k], dg[r], eps2)) continue;
           const std::int64_t o = std::int64_t(h.col[k]) - std::int64_t(r);
           auto it = find(o);
           if(it == offs.end() || *it != o)
@@ -373,6 +402,7 @@ namespace
       for(u64 r = 0; r < rows; ++r)
         for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
         {
+          if(!band_keeps(h.val[k], dg[r], eps2)) continue;
           const u64 d = u64(find(std::int64_t(h.col[k]) - std::int64_t(r)) - offs.begin());
           const u64 at = (r / 64) * nd * 64 + d * 64 + (r % 64);
           if(used[at]) return rt::kOk;   // duplicate entry: keep the general format
