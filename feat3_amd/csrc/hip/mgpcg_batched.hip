@@ -96,6 +96,37 @@ namespace
     return c;
   }
 
+  // The inner-precision graph of shared block values (FP32 / binary16 arithmetic) leaves out the
+  // entries whose largest value is at most eps_f32^2 of the largest diagonal value of their row's
+  // diagonal block: such a block changes a row sum by less than eps^2 of the diagonal term, far
+  // below the rounding of FP32 arithmetic (the face couplings of the 3D Q1 vector Laplacian are
+  // roundoff residues of about 6e-17 of the diagonal: 21 of 27 blocks per row stay).  The outer
+  // FP64 product keeps the full graph.
+  HostGraph thin_graph(const HostGraph& h, u64 bb, int B)
+  {
+    const double eps2 = 1.1920928955078125e-07 * 1.1920928955078125e-07;
+    HostGraph c;
+    c.rows = h.rows; c.cols = h.cols;
+    c.rp.assign(h.rows + 1, 0);
+    for(u64 r = 0; r < h.rows; ++r)
+    {
+      double d = 0.0;
+      for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
+        if(u64(h.col[k]) == r)
+          for(int q = 0; q < B; ++q) d = std::max(d, std::fabs(h.val[u64(k) * bb + u64(q) * u64(B) + u64(q)]));
+      for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
+      {
+        double m = 0.0;
+        for(u64 j = 0; j < bb; ++j) m = std::max(m, std::fabs(h.val[u64(k) * bb + j]));
+        if(u64(h.col[k]) != r && m <= eps2 * d) continue;
+        c.col.push_back(h.col[k]);
+        for(u64 j = 0; j < bb; ++j) c.val.push_back(h.val[u64(k) * bb + j]);
+      }
+      c.rp[r + 1] = u32(c.col.size());
+    }
+    return c;
+  }
+
   bool invert_dense(std::vector<double>& a, u64 n)
   {
     std::vector<double> inv(n * n, 0.0);
@@ -228,6 +259,7 @@ namespace
       u64 nodes = 0, nnz = 0;
       DevMem arp, acol, aval, prp, pcol, pval, rrp, rcol, rval, mask, dinv, rhs, x, t, def, ord;
       u64 nc = 0;
+      bool thinned = false;   // the graph holds fewer entries than the pushed one (thin_graph)
       DT* xfinal = nullptr;
       const u32* order() const { return ord.p ? ord.template as<u32>() : nullptr; }
     };
@@ -237,6 +269,7 @@ namespace
     int G = 16, GV = 16, S = 1, GO = 16, SO = 1;   // streaming, V-cycle rows, outer FP64 rows
     std::vector<std::unique_ptr<Level>> L;
     DevMem ainv, a64, r, p, q, st, cr, hist, partials, any;
+    DevMem orp, ocol;   // the outer product's graph where level 0's inner graph is thinned
     u64 m0 = 0, hist_cap = 0, ncoarse_rows = 0;
     // host copies of the system states (K SysState each) of the last kRing iterations
     static constexpr int kRing = 4;
@@ -299,7 +332,9 @@ namespace
     }
     template<int S_> batch::BlockRows<double, double, B, SHARED, S_> rows64() const
     {
-      return {L[0]->arp.template as<u32>(), L[0]->acol.template as<u32>(), a64.template as<double>(), L[0]->nodes, K,
+      const bool own = L[0]->thinned;
+      return {(own ? orp : L[0]->arp).template as<u32>(), (own ? ocol : L[0]->acol).template as<u32>(), a64.template as<double>(),
+              L[0]->nodes, K,
               L[0]->order()};
     }
 
@@ -314,10 +349,23 @@ namespace
         const HostLevel& h = lv[i];
         std::unique_ptr<Level> l(new Level());
         l->nodes = h.nodes;
-        l->nnz = h.A.rp[h.nodes];
-        F3_RC(l->arp.put(h.A.rp, s));
-        F3_RC(l->acol.put(h.A.col, s));
-        F3_RC(l->aval.put(conv<VT>(h.A.val), s));
+        if(SHARED && sizeof(VT) < 8)
+        {
+          const HostGraph T = thin_graph(h.A, vpe(), B);
+          l->thinned = T.rp[h.nodes] != h.A.rp[h.nodes];
+          l->nnz = T.rp[h.nodes];
+          F3_RC(l->arp.put(T.rp, s));
+          F3_RC(l->acol.put(T.col, s));
+          F3_RC(l->aval.put(conv<VT>(T.val), s));
+          F3_CHECK(hipStreamSynchronize(s));   // T leaves scope
+        }
+        else
+        {
+          l->nnz = h.A.rp[h.nodes];
+          F3_RC(l->arp.put(h.A.rp, s));
+          F3_RC(l->acol.put(h.A.col, s));
+          F3_RC(l->aval.put(conv<VT>(h.A.val), s));
+        }
         if(i + 1 < lv.size())
         {
           const HostGraph P = compact(h.P), R = compact(h.R);
@@ -403,6 +451,11 @@ namespace
       // outer FP64
       m0 = l0.nodes * B;
       F3_RC(a64.put(l0.A.val, s));
+      if(L[0]->thinned)
+      {
+        F3_RC(orp.put(l0.A.rp, s));
+        F3_RC(ocol.put(l0.A.col, s));
+      }
       const u64 nv = m0 * u64(K);
       F3_RC(r.alloc(sizeof(double) * nv)); F3_RC(p.alloc(sizeof(double) * nv)); F3_RC(q.alloc(sizeof(double) * nv));
       F3_RC(st.alloc(sizeof(batch::SysState) * K));
@@ -706,6 +759,8 @@ namespace
     {
       if(level < 0 || level >= int(L.size()) || !dv) return rt::kErrArg;
       Level& l = *L[level];
+      // a thinned graph was chosen for the values given at init; other values need a new solver
+      if(l.thinned) return rt::kErrState;
       const u64 nval = l.nnz * vals_per_entry();
       const unsigned nb = unsigned(std::max<u64>(1, std::min<u64>((nval + 255) / 256, 65536)));
       k_to_storage<VT><<<nb, 256, 0, s>>>(nval, dv, l.aval.template as<VT>());

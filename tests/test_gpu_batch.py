@@ -184,3 +184,24 @@ def test_inputs_from_the_default_stream_are_awaited(torch):
         st, it = s.correct(x, B, stream=stream)
         assert list(it) == list(it0) and np.array_equal(x.cpu().numpy(), ref.cpu().numpy())
     s.done()
+
+
+def test_thinned_graph_refuses_new_values(torch):
+    """With shared FP32 values the inner graph of the 3D vector Laplacian leaves out the face
+    blocks (roundoff residues below eps_f32^2 of the diagonal); the solve keeps the parity bar
+    (test_vector_laplacian_bcsr) and a later set_values on such a level is refused, since the
+    thinning was chosen for the values given at init."""
+    import ctypes
+    from feat3_amd import capi
+    from feat3_amd.batch import BatchMGPCG
+    H = pb.hierarchy(3, 1, 3, 1)
+    vals = [pb.block_identity_values(L.A, 3) for L in H.levels]
+    s = BatchMGPCG(H, 4, values=vals, block=3, inner="f32").init()
+    dv = torch.from_numpy(np.ascontiguousarray(vals[0], np.float64)).cuda()
+    with pytest.raises(Exception):
+        capi.call("feat3_batch_set_values", s._h, 0, ctypes.c_void_p(dv.data_ptr()))
+    s.done()
+    # FP64 keeps the full graph, so new values are taken
+    s = BatchMGPCG(H, 4, values=vals, block=3, inner="f64").init()
+    capi.call("feat3_batch_set_values", s._h, 0, ctypes.c_void_p(dv.data_ptr()))
+    s.done()
