@@ -388,7 +388,9 @@ int feat3_batch_correct(feat3_batch* h, void* caller_stream, double* x, const do
  * (FP64; [entry][h][w] or [entry][system]): converts to the storage precision, recomputes the
  * Jacobi diagonal, the FP64 outer copy (level 0) and the coarse inverses (coarsest level; a
  * small device-to-host copy); graph, transfers and Dirichlet nodes are kept.  The null stream is
- * synchronised first (values produced on another stream must be complete); blocks until done. */
+ * synchronised first (values produced on another stream must be complete); blocks until done.
+ * Refused (invalid object state) for a level whose FP32 inner graph was thinned at init (shared
+ * values: blocks below eps_f32^2 of the diagonal left out, chosen for the values given then). */
 int feat3_batch_set_values(feat3_batch* h, int level, const double* values);
 /* defect norms of system `system` in the last solve */
 int feat3_batch_history(feat3_batch* h, int system, double* out_host, uint64_t cap, uint64_t* len);
