@@ -37,24 +37,30 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def band_nnz(A, inner):
+def band_nnz(A, inner, upper=False):
     """Entries the solver's banded copy of A keeps in the given inner precision: those with
-    |a_ij| > eps^2 |a_ii| (eps of the inner arithmetic; binary16 storage computes in FP32)."""
+    |a_ij| > eps^2 |a_ii| (eps of the inner arithmetic; binary16 storage computes in FP32);
+    upper: only those with j >= i, which is what the symmetric band storage holds."""
     import numpy as np
     rows = np.repeat(np.arange(A.rows), np.diff(A.row_ptr.astype(np.int64)))
+    cols = A.col_ind.astype(np.int64)
     diag = np.zeros(A.rows)
-    on = A.col_ind.astype(np.int64) == rows
+    on = cols == rows
     diag[rows[on]] = A.val[on]
     eps = np.finfo(np.float64 if inner == "f64" else np.float32).eps
-    return int(np.count_nonzero(np.abs(A.val) > eps * eps * np.abs(diag[rows])))
+    keep = np.abs(A.val) > eps * eps * np.abs(diag[rows])
+    if upper:
+        keep &= cols >= rows
+    return int(np.count_nonzero(keep))
 
 
 def sweep_algorithmic_bytes(info, n, nnz, nchunks):
     """Fused damped-Jacobi sweep x_out = x + dinv (b - A x): the CSR defect formula over the
     unpadded nnz with the stored value and index sizes -- for SELL-C the chunk descriptors (cs, cl)
     replace row_ptr, the banded format stores no indices at all (its offsets, 4 bytes per
-    diagonal, replace them; nnz is then the count of the entries the band keeps, band_nnz) -- x
-    counted once, plus dinv, b and x_out."""
+    diagonal, replace them; nnz is then the count of the entries the band keeps, band_nnz, and for
+    the symmetric band storage those on and above the diagonal: the lower half is read from the
+    L2) -- x counted once, plus dinv, b and x_out."""
     s_v = info["sizeof_value"]
     s_x = info["sizeof_inner"]
     if info.get("banded"):
@@ -501,8 +507,8 @@ def main():
         info = s.info()
         s.done()
         ms = 1e3 * sum(times) / len(times)
-        sb = sweep_algorithmic_bytes(info, n, band_nnz(H.levels[0].A, inner) if info["banded"] else nnz,
-                                     info["sell_chunks"])
+        sb = sweep_algorithmic_bytes(info, n, band_nnz(H.levels[0].A, inner, upper=info["banded"] == 2)
+                                     if info["banded"] else nnz, info["sell_chunks"])
         r = dict(ms_per_solve=ms, ms_min=1e3 * min(times), iterations=iters,
                  dof_iter_per_s=n * sum(iters) / sum(times), final_rel_residual=float(hist[-1] / hist[0]),
                  sweep_us=sweep_ms * 1e3, sweep_bytes=sb, sweep_GBs=sb / (sweep_ms * 1e-3) / 1e9,

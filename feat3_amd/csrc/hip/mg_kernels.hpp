@@ -122,7 +122,13 @@ namespace mg
   // order would make every wave read ndiag streams one matrix-length apart.  One lane per row,
   // 64 consecutive x per diagonal, no column indices; the offsets are wave-uniform (scalar
   // loads).
-  template<typename DT_, typename VT_>
+  // SYM_ (FP64 bands): symmetric storage of a band that equals its mirror image bit for bit (a_pc = a_cp):
+  // only the center and the upper diagonals are stored (nds of them, chunk-major as above), and
+  // sx[d] is the stored diagonal that holds diagonal d.  A lower diagonal's value for row p is
+  // the upper one stored at row c (its column), the same number, so the row sum takes the same
+  // terms in the same order as the full band; the half of the values read a second time come
+  // from the L2 (row c lies at most one node plane before p).
+  template<typename DT_, typename VT_, bool SYM_ = false>
   struct BandedRows
   {
     static constexpr int G = 1;
@@ -131,17 +137,35 @@ namespace mg
     // 112 -> 106 us (FP32-inner solve 3.31 -> 3.26 ms, same GPU, alternating builds).  FP32
     // values (232 MB) stay ordinary loads: consecutive sweeps reuse them from the Infinity Cache,
     // which nontemporal loads give up (39.7 -> 45.5 us).
-    static constexpr bool kNonTemporal = sizeof(VT_) == 8;
+    static constexpr bool kNonTemporal = sizeof(VT_) == 8 && !SYM_;
     const u32* off;
     const VT_* val;
     u64 rows, cols;
     u32 ndiag;
+    const u32* sx = nullptr;   // SYM_: stored diagonal of each diagonal
+    u32 nds = 0;               // SYM_: stored diagonals
     __device__ __forceinline__ u64 row(u64 p) const { return p; }
     __device__ __forceinline__ DT_ row_sum(u64 p, int, const DT_* __restrict__ x) const
     {
       const dev::cptr<u32> co = dev::to_const(off);
-      const u64 base = (p >> 6) * u64(ndiag) * 64 + (p & 63);
       DT_ s = DT_(0);
+      if constexpr(SYM_)
+      {
+        const dev::cptr<u32> cs = dev::to_const(sx);
+#pragma unroll 9
+        for(u32 d = 0; d < ndiag; ++d)
+        {
+          const u64 c = p + u64(co[d]) + 1 - rows;
+          const bool in = c < cols;
+          const u64 vr = (in && c < p) ? c : p;   // lower diagonal: the value stored at row c
+          const u64 at = (vr >> 6) * u64(nds) * 64 + u64(cs[d]) * 64 + (vr & 63);
+          const DT_ a = kNonTemporal ? dev::ld_nt<DT_>(val, at) : dev::ld<DT_>(val, at);
+          const DT_ t = a * x[in ? c : p];
+          s += in ? t : DT_(0);
+        }
+        return s;
+      }
+      const u64 base = (p >> 6) * u64(ndiag) * 64 + (p & 63);
 #pragma unroll 9
       for(u32 d = 0; d < ndiag; ++d)
       {

@@ -176,6 +176,44 @@ namespace
       }
     }
   }
+  // bit patterns for the exact comparison of band values
+  template<typename VT> struct Bits;
+  template<> struct Bits<double> { typedef std::uint64_t T; };
+  template<> struct Bits<float> { typedef std::uint32_t T; };
+  template<> struct Bits<half> { typedef std::uint16_t T; };
+
+  // any lower-diagonal value of the chunk-major band that differs (bit for bit) from its mirror
+  // image, the upper value stored at row c = r + off (off < 0), raises *bad
+  template<typename VT>
+  __global__ void k_band_sym_check(u64 padded, u32 nd, const VT* __restrict__ bval, const std::int64_t* __restrict__ offs,
+                                   const u32* __restrict__ mirror, int* bad)
+  {
+    typedef typename Bits<VT>::T B;
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < padded * u64(nd); i += u64(gridDim.x) * blockDim.x)
+    {
+      const u64 r = i / nd;
+      const u32 d = u32(i % nd);
+      const std::int64_t o = offs[d];
+      if(o >= 0 || std::int64_t(r) + o < 0) continue;
+      const u64 c = u64(std::int64_t(r) + o);
+      const B a = reinterpret_cast<const B*>(bval)[(r >> 6) * nd * 64 + u64(d) * 64 + (r & 63)];
+      const B b = reinterpret_cast<const B*>(bval)[(c >> 6) * nd * 64 + u64(mirror[d]) * 64 + (c & 63)];
+      if(a != b) atomicOr(bad, 1);
+    }
+  }
+  // the center and upper diagonals (pos[k], k < ns) of the band into the symmetric storage
+  template<typename VT>
+  __global__ void k_band_sym_pack(u64 padded, u32 nd, const VT* __restrict__ bval, u32 ns, const u32* __restrict__ pos,
+                                  VT* __restrict__ sval)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < padded * u64(ns); i += u64(gridDim.x) * blockDim.x)
+    {
+      const u64 r = i / ns;
+      const u32 k = u32(i % ns);
+      sval[(r >> 6) * ns * 64 + u64(k) * 64 + (r & 63)] = bval[(r >> 6) * nd * 64 + u64(pos[k]) * 64 + (r & 63)];
+    }
+  }
+
   inline unsigned setup_blocks(u64 n)
   {
     const u64 b = (n + 255) / 256;
@@ -202,6 +240,11 @@ namespace
     bool banded = false;
     u32 ndiag = 0;
     DevMem boff, bval;
+    // symmetric band storage (mg::BandedRows<.., true>): nds stored diagonals, bsx[d] the stored
+    // diagonal holding diagonal d; bval then holds nds diagonals
+    bool bsym = false;
+    u32 nds = 0;
+    DevMem bsx;
 
     // drop_zeros: upload a compact copy without the stored exact zeros (used for the
     // transfer matrices, whose two-level pattern holds structural zeros; a zero entry adds
@@ -356,6 +399,52 @@ namespace
       F3_CHECK(hipStreamSynchronize(s));   // ho leaves scope
       ndiag = u32(nd);
       banded = true;
+      // symmetric storage when the offsets are symmetric and every value equals its mirror image
+      // (FP64 values, as build_banded)
+      bsym = false;
+      if(sizeof(VT) != 8) return rt::kOk;
+      std::vector<std::int64_t> offs(nd);
+      for(u64 d = 0; d < nd; ++d) offs[d] = std::int64_t(ho[d]) - std::int64_t(rows) + 1;
+      std::vector<u32> mirror(nd), pos, sx(nd);
+      for(u64 d = 0; d < nd; ++d)
+      {
+        auto it = std::lower_bound(offs.begin(), offs.end(), -offs[d]);
+        if(it == offs.end() || *it != -offs[d]) return rt::kOk;
+        mirror[d] = u32(it - offs.begin());
+        if(offs[d] >= 0) pos.push_back(u32(d));
+      }
+      for(u64 k = 0; k < pos.size(); ++k) sx[pos[k]] = u32(k);
+      for(u64 d = 0; d < nd; ++d)
+        if(offs[d] < 0) sx[d] = sx[mirror[d]];
+      DevMem doffs, dmirror, dbad;
+      F3_RC(doffs.alloc(sizeof(std::int64_t) * nd));
+      F3_RC(dmirror.alloc(sizeof(u32) * nd));
+      F3_RC(dbad.alloc(sizeof(int)));
+      F3_CHECK(hipMemcpyAsync(doffs.p, offs.data(), sizeof(std::int64_t) * nd, hipMemcpyHostToDevice, s));
+      F3_CHECK(hipMemcpyAsync(dmirror.p, mirror.data(), sizeof(u32) * nd, hipMemcpyHostToDevice, s));
+      F3_CHECK(hipMemsetAsync(dbad.p, 0, sizeof(int), s));
+      k_band_sym_check<VT><<<setup_blocks(padded * nd), 256, 0, s>>>(padded, u32(nd), bval.template as<VT>(),
+                                                                    doffs.template as<std::int64_t>(),
+                                                                    dmirror.template as<u32>(), dbad.template as<int>());
+      F3_RC(rt::check_launch());
+      int hbad = 0;
+      F3_CHECK(hipMemcpyAsync(&hbad, dbad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      F3_CHECK(hipStreamSynchronize(s));
+      if(hbad) return rt::kOk;
+      const u32 ns = u32(pos.size());
+      DevMem dpos, sv;
+      F3_RC(dpos.alloc(sizeof(u32) * ns));
+      F3_RC(sv.alloc(sizeof(VT) * u64(ns) * padded));
+      F3_RC(bsx.alloc(sizeof(u32) * nd));
+      F3_CHECK(hipMemcpyAsync(dpos.p, pos.data(), sizeof(u32) * ns, hipMemcpyHostToDevice, s));
+      F3_CHECK(hipMemcpyAsync(bsx.p, sx.data(), sizeof(u32) * nd, hipMemcpyHostToDevice, s));
+      k_band_sym_pack<VT><<<setup_blocks(padded * ns), 256, 0, s>>>(padded, u32(nd), bval.template as<VT>(), ns,
+                                                                   dpos.template as<u32>(), sv.template as<VT>());
+      F3_RC(rt::check_launch());
+      F3_CHECK(hipStreamSynchronize(s));   // host vectors leave scope
+      std::swap(bval.p, sv.p);             // sv frees the full band
+      nds = ns;
+      bsym = true;
       return rt::kOk;
     }
 
@@ -411,14 +500,68 @@ namespace
         }
       std::vector<u32> ho(nd);
       for(u64 d = 0; d < nd; ++d) ho[d] = u32(offs[d] + std::int64_t(rows) - 1);
+      // symmetric storage when the band equals its mirror image bit for bit, for FP64 values
+      // (3D Q1 L7: FP64 sweep 107 -> 85 us, outer PCG product likewise).  FP32 and binary16 bands
+      // stay full: their values are served partly from the Infinity Cache, and the mirrored
+      // reads cost about what the halved stream saves (FP32 sweep 32.5 -> 32.3 us; binary16
+      // 23.5 -> 32.0 us)
+      std::vector<u32> sx;
+      const std::vector<VT> sv = sizeof(VT) == 8 ? symmetric_band(hv, offs, padded, sx) : std::vector<VT>();
+      const bool sym = !sx.empty();
+      const std::vector<VT>& up = sym ? sv : hv;
       F3_RC(boff.alloc(sizeof(u32) * nd));
-      F3_RC(bval.alloc(sizeof(VT) * nd * padded));
+      F3_RC(bval.alloc(sizeof(VT) * up.size()));
       F3_CHECK(hipMemcpyAsync(boff.p, ho.data(), sizeof(u32) * nd, hipMemcpyHostToDevice, s));
-      F3_CHECK(hipMemcpyAsync(bval.p, hv.data(), sizeof(VT) * nd * padded, hipMemcpyHostToDevice, s));
+      F3_CHECK(hipMemcpyAsync(bval.p, up.data(), sizeof(VT) * up.size(), hipMemcpyHostToDevice, s));
+      if(sym)
+      {
+        F3_RC(bsx.alloc(sizeof(u32) * nd));
+        F3_CHECK(hipMemcpyAsync(bsx.p, sx.data(), sizeof(u32) * nd, hipMemcpyHostToDevice, s));
+      }
       F3_CHECK(hipStreamSynchronize(s));
       ndiag = u32(nd);
+      bsym = sym;
+      nds = sym ? u32(up.size() / padded) : 0;
       banded = true;
       return rt::kOk;
+    }
+
+    // the band's symmetric storage (center and upper diagonals, chunk-major) and sx, or an empty
+    // sx when the offsets are not symmetric or some value differs from its mirror image
+    // (compared bit for bit, so that the row sums are unchanged)
+    static std::vector<VT> symmetric_band(const std::vector<VT>& hv, const std::vector<std::int64_t>& offs, u64 padded,
+                                          std::vector<u32>& sx)
+    {
+      sx.clear();
+      const u64 nd = offs.size(), rows_p = padded;
+      std::vector<u64> mirror(nd), pos;
+      for(u64 d = 0; d < nd; ++d)
+      {
+        auto it = std::lower_bound(offs.begin(), offs.end(), -offs[d]);
+        if(it == offs.end() || *it != -offs[d]) return {};
+        mirror[d] = u64(it - offs.begin());
+        if(offs[d] >= 0) pos.push_back(d);
+      }
+      auto at = [&](u64 r, u64 d) { return (r / 64) * nd * 64 + d * 64 + (r % 64); };
+      const std::int64_t nrows = std::int64_t(rows_p);
+      for(u64 d = 0; d < nd; ++d)
+      {
+        if(offs[d] >= 0) continue;
+        for(std::int64_t r = -offs[d]; r < nrows; ++r)
+        {
+          const u64 c = u64(r + offs[d]);
+          if(std::memcmp(&hv[at(u64(r), d)], &hv[at(c, mirror[d])], sizeof(VT)) != 0) return {};
+        }
+      }
+      const u64 ns = pos.size();
+      std::vector<u64> sidx(nd);
+      for(u64 k = 0; k < ns; ++k) sidx[pos[k]] = k;
+      sx.resize(nd);
+      for(u64 d = 0; d < nd; ++d) sx[d] = u32(offs[d] >= 0 ? sidx[d] : sidx[mirror[d]]);
+      std::vector<VT> sv(ns * padded);
+      for(u64 r = 0; r < padded; ++r)
+        for(u64 k = 0; k < ns; ++k) sv[(r / 64) * ns * 64 + k * 64 + (r % 64)] = hv[at(r, pos[k])];
+      return sv;
     }
 
     // SELL-C-sigma from the host converter feat3_sell_build (no permutation is stored for
@@ -469,7 +612,7 @@ namespace
     // pointers / chunk descriptors, SELL padding included), excluding the vector traffic
     u64 matrix_bytes() const
     {
-      if(banded) return u64(ndiag) * rows * sizeof(VT) + u64(ndiag) * 4;
+      if(banded) return u64(bsym ? nds : ndiag) * rows * sizeof(VT) + u64(ndiag) * 4 * (bsym ? 2 : 1);
       if(sell)
       {
         const u64 nch = (rows + C - 1) / C;
@@ -486,7 +629,12 @@ namespace
     auto grid = [&](int G) { return m.rows_per_block ? mg::row_grid_fixed(m.rows, m.rows_per_block) : mg::row_grid(m.rows, G); };
     if(m.banded)
     {
-      f(mg::BandedRows<DT, VT>{m.boff.template as<u32>(), m.bval.template as<VT>(), m.rows, m.cols, m.ndiag}, grid(1));
+      if(m.bsym)
+        f(mg::BandedRows<DT, VT, true>{m.boff.template as<u32>(), m.bval.template as<VT>(), m.rows, m.cols, m.ndiag,
+                                       m.bsx.template as<u32>(), m.nds},
+          grid(1));
+      else
+        f(mg::BandedRows<DT, VT>{m.boff.template as<u32>(), m.bval.template as<VT>(), m.rows, m.cols, m.ndiag}, grid(1));
       return rt::kOk;
     }
     if(m.sell)
@@ -1475,7 +1623,7 @@ namespace
       v[9] = sizeof(VT);
       v[10] = L.front()->A.sell_d16 ? 1 : 0;
       v[11] = L.front()->A.sell ? (L.front()->n + L.front()->A.C - 1) / L.front()->A.C : 0;
-      v[12] = L.front()->A.banded ? 1 : 0;
+      v[12] = L.front()->A.banded ? (L.front()->A.bsym ? 2 : 1) : 0;
       v[13] = L.front()->A.ndiag;
       v[14] = tail0 + 1 < L.size() ? L.size() - tail0 : 0;
       v[15] = tail_lds;

@@ -343,7 +343,8 @@ int feat3_mgpcg_history(feat3_mgpcg* h, double* out_host, uint64_t cap, uint64_t
 int feat3_mgpcg_time_kernel(feat3_mgpcg* h, int which, int reps, double* avg_ms, uint64_t* bytes);
 /* [levels, n, nnz, lanes, sell size, kernels per cycle, cycle bytes, outer bytes,
  *  sizeof inner, sizeof stored value, 16-bit SELL columns on the finest level, SELL chunks
- *  of the finest level, finest level banded (0/1), its number of diagonals, levels run by the
+ *  of the finest level, finest level banded (0 no, 1 yes, 2 with symmetric storage: center and
+ *  upper diagonals only), its number of diagonals, levels run by the
  *  single-workgroup coarse tail (0: none), LDS bytes of the tail] */
 int feat3_mgpcg_info(feat3_mgpcg* h, uint64_t* out, int cap);
 
