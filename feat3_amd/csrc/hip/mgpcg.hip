@@ -245,6 +245,7 @@ namespace
     bool bsym = false;
     u32 nds = 0;
     DevMem bsx;
+    bool allow_sym = true;    // symmetric storage allowed (option "symmetric_band")
 
     // drop_zeros: upload a compact copy without the stored exact zeros (used for the
     // transfer matrices, whose two-level pattern holds structural zeros; a zero entry adds
@@ -402,7 +403,7 @@ namespace
       // symmetric storage when the offsets are symmetric and every value equals its mirror image
       // (FP64 values, as build_banded)
       bsym = false;
-      if(sizeof(VT) != 8) return rt::kOk;
+      if(sizeof(VT) != 8 || !allow_sym) return rt::kOk;
       std::vector<std::int64_t> offs(nd);
       for(u64 d = 0; d < nd; ++d) offs[d] = std::int64_t(ho[d]) - std::int64_t(rows) + 1;
       std::vector<u32> mirror(nd), pos, sx(nd);
@@ -506,7 +507,7 @@ namespace
       // reads cost about what the halved stream saves (FP32 sweep 32.5 -> 32.3 us; binary16
       // 23.5 -> 32.0 us)
       std::vector<u32> sx;
-      const std::vector<VT> sv = sizeof(VT) == 8 ? symmetric_band(hv, offs, padded, sx) : std::vector<VT>();
+      const std::vector<VT> sv = sizeof(VT) == 8 && allow_sym ? symmetric_band(hv, offs, padded, sx) : std::vector<VT>();
       const bool sym = !sx.empty();
       const std::vector<VT>& up = sym ? sv : hv;
       F3_RC(boff.alloc(sizeof(u32) * nd));
@@ -775,6 +776,9 @@ namespace
     // 2D Q2 L9 590 -> 576 us; 64 rows, or lifting the 8192-block grid limit, were slower.
     int transfer_lanes = -1;
     u64 transfer_rows_per_block = 128;
+    // FP64 bands that equal their mirror image bit for bit store only the center and upper
+    // diagonals (0: always the full band; the results are identical either way)
+    bool symmetric_band = true;
     double tol_rel = 1e-8, tol_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16), tol_abs_low = 0.0;
     double div_rel = 1.0 / 2.220446049250313e-16, div_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16);
     double stag_rate = 0.95;
@@ -859,6 +863,7 @@ namespace
         std::unique_ptr<Level<DT, VT>> l(new Level<DT, VT>());
         const HostLevel& h = lv[i];
         l->n = h.A.rows;
+        l->A.allow_sym = cfg.symmetric_band;
         if(h.A.on_device())
           F3_RC(l->A.upload_dev(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
         else
@@ -945,6 +950,7 @@ namespace
       }
       if(!shared)
       {
+        A64own.allow_sym = cfg.symmetric_band;
         if(h0.A.on_device())
           F3_RC(A64own.upload_dev(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
         else
@@ -1845,6 +1851,7 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
                                     value == 32 || value == 64))
     h->cfg.transfer_lanes = int(value);
   else if(k == "transfer_rows_per_block" && value >= 0) h->cfg.transfer_rows_per_block = u64(value);
+  else if(k == "symmetric_band") h->cfg.symmetric_band = value != 0.0;
   else if(k == "cycle" && (value == 0 || value == 1 || value == 2)) h->cfg.cycle = int(value);
   else if(k == "rows_per_block" && value >= 0) h->cfg.rows_per_block = u64(value);
   else if(k == "transfer_sigma" && value >= 1) h->cfg.transfer_sigma = u64(value);

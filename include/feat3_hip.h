@@ -319,7 +319,8 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
  * "transfer_sigma" (SELL sort window of the transfer matrices when they are SELL, default 1),
  * "transfer_lanes" (lanes per row of CSR transfer matrices: -1 as "lanes", default; 0 streaming),
  * "transfer_rows_per_block" (row positions per workgroup of the transfer kernels, default 128;
- * 0 = derived from the row count),
+ * 0 = derived from the row count), "symmetric_band" (1, default: an FP64 band equal to its mirror
+ * image bit for bit stores only its center and upper diagonals; 0: the full band; same results),
  * "cycle" (the multigrid cycle, MultiGridCycle of multigrid.hpp:26-34: 0 V, default, 1 F,
  * 2 W; the coarse-tail kernel runs the cycle's steps on the small levels for each of them);
  * takes effect at the next init */

@@ -260,3 +260,46 @@ def test_time_kernel_entries(torch):
     assert np.array_equal(x.cpu().numpy(), ref[2])
     with pytest.raises(Exception):
         MGPCG(H).init().time_kernel(7, 1)
+
+
+@pytest.mark.parametrize("dim,lvl", [(3, 4), (2, 6)])
+def test_symmetric_band_is_bitwise_the_full_band(torch, dim, lvl):
+    """The FP64 band of a matrix equal to its transpose bit for bit stores only the center and
+    upper diagonals; the row sums take the same values in the same order, so the FP64 solve is
+    bitwise that of the full band (option symmetric_band = 0), for the host- and the
+    device-built solver."""
+    print(f"case symmetric band {dim}D Q1 L{lvl}", flush=True)
+    from feat3_amd.solver import MGPCG
+    H = pb.hierarchy(dim, 1, lvl, 1)
+    b = pb.filtered_rhs(H, 0)
+    s = MGPCG(H, fmt="banded").init()
+    assert s.info()["banded"] == 2
+    s.done()
+    ref = solve_gpu(torch, H, b, fmt="banded", options=dict(symmetric_band=0))
+    sym = solve_gpu(torch, H, b, fmt="banded")
+    assert ref[1] == sym[1] and np.array_equal(ref[2], sym[2])
+    levels = pb.hierarchy_device(dim, 1, lvl, 1)
+    sd = MGPCG(levels, fmt="banded").init()
+    assert sd.info()["banded"] == 2
+    x = torch.zeros(H.n, dtype=torch.float64, device="cuda")
+    st, it = sd.correct(x, torch.from_numpy(b).cuda())
+    sd.done()
+    assert it == ref[1] and np.array_equal(x.cpu().numpy(), ref[2])
+
+
+def test_nonsymmetric_band_keeps_the_full_band(torch):
+    """A banded matrix that is not symmetric keeps the full band and the FP64 parity bars."""
+    print("case non-symmetric band 2D Q1 L5", flush=True)
+    from feat3_amd.solver import MGPCG
+    H = pb.hierarchy(2, 1, 5, 1)
+    A = H.levels[0].A
+    rows = np.repeat(np.arange(A.rows), np.diff(A.row_ptr.astype(np.int64)))
+    A.val = np.where(A.col_ind > rows, A.val * (1.0 + 1e-3), A.val)
+    b = pb.filtered_rhs(H, 0)
+    s = MGPCG(H, fmt="banded").init()
+    assert s.info()["banded"] == 1
+    s.done()
+    so, io, xo, _ = solve_orc(H, b)
+    sg, ig, xg, _ = solve_gpu(torch, H, b, fmt="banded")
+    assert so == sg == 2 and ig == io
+    assert rel(xg, xo) <= 1e-10
