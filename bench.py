@@ -38,8 +38,9 @@ def log(msg):
 
 
 def band_nnz(A, inner, upper=False):
-    """Entries the solver's banded copy of A keeps in the given inner precision: those with
-    |a_ij| > eps^2 |a_ii| (eps of the inner arithmetic; binary16 storage computes in FP32);
+    """Entries the solver's copy of a level matrix keeps in the given inner precision (every format):
+    those with |a_ij| > eps^2 |a_ii| (eps of the inner arithmetic; binary16 storage computes in
+    FP32; in FP64 that is every stored nonzero);
     upper: only those with j >= i, which is what the symmetric band storage holds."""
     import numpy as np
     rows = np.repeat(np.arange(A.rows), np.diff(A.row_ptr.astype(np.int64)))
@@ -56,11 +57,11 @@ def band_nnz(A, inner, upper=False):
 
 def sweep_algorithmic_bytes(info, n, nnz, nchunks):
     """Fused damped-Jacobi sweep x_out = x + dinv (b - A x): the CSR defect formula over the
-    unpadded nnz with the stored value and index sizes -- for SELL-C the chunk descriptors (cs, cl)
-    replace row_ptr, the banded format stores no indices at all (its offsets, 4 bytes per
-    diagonal, replace them; nnz is then the count of the entries the band keeps, band_nnz, and for
-    the symmetric band storage those on and above the diagonal: the lower half is read from the
-    L2) -- x counted once, plus dinv, b and x_out."""
+    unpadded nnz (the entries the level copy keeps, band_nnz) with the stored value and index
+    sizes -- for SELL-C the chunk descriptors (cs, cl) replace row_ptr, the banded format stores no
+    indices at all (its offsets, 4 bytes per diagonal, replace them; for the symmetric band storage
+    nnz counts the entries on and above the diagonal: the lower half is read from the L2) -- x
+    counted once, plus dinv, b and x_out."""
     s_v = info["sizeof_value"]
     s_x = info["sizeof_inner"]
     if info.get("banded"):
@@ -507,8 +508,8 @@ def main():
         info = s.info()
         s.done()
         ms = 1e3 * sum(times) / len(times)
-        sb = sweep_algorithmic_bytes(info, n, band_nnz(H.levels[0].A, inner, upper=info["banded"] == 2)
-                                     if info["banded"] else nnz, info["sell_chunks"])
+        sb = sweep_algorithmic_bytes(info, n, band_nnz(H.levels[0].A, inner, upper=info["banded"] == 2),
+                                     info["sell_chunks"])
         r = dict(ms_per_solve=ms, ms_min=1e3 * min(times), iterations=iters,
                  dof_iter_per_s=n * sum(iters) / sum(times), final_rel_residual=float(hist[-1] / hist[0]),
                  sweep_us=sweep_ms * 1e3, sweep_bytes=sb, sweep_GBs=sb / (sweep_ms * 1e-3) / 1e9,

@@ -86,37 +86,6 @@ namespace
       if(!ok) atomicOr(bad, 1);
     }
   }
-  __global__ void k_count_nonzero(u64 rows, const u32* __restrict__ rp, const double* __restrict__ val, u32* __restrict__ cnt)
-  {
-    for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
-    {
-      u32 c = 0;
-      for(u64 k = rp[r]; k < rp[r + 1]; ++k) c += val[k] != 0.0 ? 1u : 0u;
-      cnt[r] = c;
-    }
-  }
-  __global__ void k_compact(u64 rows, const u32* __restrict__ rp, const u32* __restrict__ col, const double* __restrict__ val,
-                            const u32* __restrict__ nrp, u32* __restrict__ ncol, double* __restrict__ nval)
-  {
-    for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
-    {
-      u64 j = nrp[r];
-      for(u64 k = rp[r]; k < rp[r + 1]; ++k)
-        if(val[k] != 0.0) { ncol[j] = col[k]; nval[j] = val[k]; ++j; }
-    }
-  }
-  template<typename VT>
-  __global__ void k_to_storage(u64 n, const double* __restrict__ in, VT* __restrict__ out)
-  {
-    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
-      out[i] = VT(float(in[i]));
-  }
-  template<>
-  __global__ void k_to_storage<double>(u64 n, const double* __restrict__ in, double* __restrict__ out)
-  {
-    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
-      out[i] = in[i];
-  }
   // Entries the banded copy of an inner-precision level matrix leaves out: |a_ij| <= eps2 |a_ii|
   // with eps2 the square of the inner precision's epsilon (FP32: 1.4e-14; FP64 keeps every
   // nonzero).  Such a coupling changes a row sum by less than eps^2 of its diagonal term, far
@@ -135,6 +104,43 @@ namespace
     return 0.0;
   }
 
+  // entries a thinned copy keeps: band_keeps(v, diagonal of the row, eps2) -- eps2 = 0 keeps the
+  // nonzeros (the transfers' compaction), eps2 = eps_inner^2 the couplings the inner arithmetic
+  // can resolve (inner-precision level matrices)
+  __global__ void k_count_kept(u64 rows, const u32* __restrict__ rp, const u32* __restrict__ col,
+                               const double* __restrict__ val, double eps2, u32* __restrict__ cnt)
+  {
+    for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
+    {
+      const double d = row_diagonal(rp, col, val, r);
+      u32 c = 0;
+      for(u64 k = rp[r]; k < rp[r + 1]; ++k) c += band_keeps(val[k], d, eps2) ? 1u : 0u;
+      cnt[r] = c;
+    }
+  }
+  __global__ void k_compact(u64 rows, const u32* __restrict__ rp, const u32* __restrict__ col, const double* __restrict__ val,
+                            double eps2, const u32* __restrict__ nrp, u32* __restrict__ ncol, double* __restrict__ nval)
+  {
+    for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += u64(gridDim.x) * blockDim.x)
+    {
+      const double d = row_diagonal(rp, col, val, r);
+      u64 j = nrp[r];
+      for(u64 k = rp[r]; k < rp[r + 1]; ++k)
+        if(band_keeps(val[k], d, eps2)) { ncol[j] = col[k]; nval[j] = val[k]; ++j; }
+    }
+  }
+  template<typename VT>
+  __global__ void k_to_storage(u64 n, const double* __restrict__ in, VT* __restrict__ out)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      out[i] = VT(float(in[i]));
+  }
+  template<>
+  __global__ void k_to_storage<double>(u64 n, const double* __restrict__ in, double* __restrict__ out)
+  {
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+      out[i] = in[i];
+  }
   // bit (col - row + rows - 1) for every kept entry: the set of diagonals.  Banded matrices set
   // the same few bits from every row, so a bit is tested with a plain load first and the atomic
   // issued only while it reads clear (a repeated OR is harmless; unconditional atomics on ~27
@@ -247,22 +253,27 @@ namespace
     DevMem bsx;
     bool allow_sym = true;    // symmetric storage allowed (option "symmetric_band")
 
-    // drop_zeros: upload a compact copy without the stored exact zeros (used for the
-    // transfer matrices, whose two-level pattern holds structural zeros; a zero entry adds
-    // nothing to a finite product, so the result is unchanged)
+    // thin >= 0: upload a compact copy with the entries band_keeps(v, row diagonal, thin) only:
+    // thin = 0 drops the stored exact zeros (the transfer matrices, whose two-level pattern holds
+    // structural zeros; a zero adds nothing to a finite product), thin = eps_inner^2 also the
+    // couplings below eps^2 of the diagonal that the inner arithmetic cannot resolve (inner-precision
+    // level matrices); thin < 0 keeps every entry
     int upload(const HostCsr& h0, hipStream_t s, bool want_sell, u64 sell_C, u64 sell_sigma, int force_lanes,
-               bool drop_zeros = false, bool allow_d16 = true, bool want_banded = false)
+               double thin = -1.0, bool allow_d16 = true, bool want_banded = false)
     {
       HostCsr compact;
       const HostCsr* hp = &h0;
-      if(drop_zeros)
+      if(thin >= 0.0)
       {
         compact.rows = h0.rows; compact.cols = h0.cols;
         compact.rp.assign(h0.rows + 1, 0);
         for(u64 r = 0; r < h0.rows; ++r)
         {
+          double d = 0.0;
           for(u32 k = h0.rp[r]; k < h0.rp[r + 1]; ++k)
-            if(h0.val[k] != 0.0) { compact.col.push_back(h0.col[k]); compact.val.push_back(h0.val[k]); }
+            if(u64(h0.col[k]) == r) { d = h0.val[k]; break; }
+          for(u32 k = h0.rp[r]; k < h0.rp[r + 1]; ++k)
+            if(band_keeps(h0.val[k], d, thin)) { compact.col.push_back(h0.col[k]); compact.val.push_back(h0.val[k]); }
           compact.rp[r + 1] = u32(compact.col.size());
         }
         hp = &compact;
@@ -296,7 +307,7 @@ namespace
     // compaction come back to the host.  A matrix that needs the SELL-C-sigma copy is brought
     // to the host and takes upload().
     int upload_dev(const HostCsr& h0, hipStream_t s, bool want_sell, u64 sell_C, u64 sell_sigma, int force_lanes,
-                   bool drop_zeros, bool allow_d16, bool want_banded)
+                   double thin, bool allow_d16, bool want_banded)
     {
       rows = h0.rows; cols = h0.cols;
       u64 n_in = h0.dnnz;
@@ -308,16 +319,16 @@ namespace
       F3_CHECK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
       F3_CHECK(hipStreamSynchronize(s));
       if(hbad) return rt::kErrArg;
-      // compaction (exact zeros dropped) into FP64 device arrays
+      // compaction (the thin rule of upload) into FP64 device arrays
       DevMem crp, ccol, cval;
       const u32* srp = h0.drp;
       const u32* scol_ = h0.dcol;
       const double* sval_ = h0.dval;
-      if(drop_zeros && rows)
+      if(thin >= 0.0 && rows)
       {
         DevMem cnt;
         F3_RC(cnt.alloc(sizeof(u32) * rows));
-        k_count_nonzero<<<setup_blocks(rows), 256, 0, s>>>(rows, h0.drp, h0.dval, cnt.template as<u32>());
+        k_count_kept<<<setup_blocks(rows), 256, 0, s>>>(rows, h0.drp, h0.dcol, h0.dval, thin, cnt.template as<u32>());
         std::vector<u32> hc(rows), hrp(rows + 1, 0);
         F3_CHECK(hipMemcpyAsync(hc.data(), cnt.p, sizeof(u32) * rows, hipMemcpyDeviceToHost, s));
         F3_CHECK(hipStreamSynchronize(s));
@@ -327,7 +338,7 @@ namespace
         F3_RC(ccol.alloc(sizeof(u32) * std::max<u64>(n_in, 1)));
         F3_RC(cval.alloc(sizeof(double) * std::max<u64>(n_in, 1)));
         F3_CHECK(hipMemcpyAsync(crp.p, hrp.data(), sizeof(u32) * (rows + 1), hipMemcpyHostToDevice, s));
-        k_compact<<<setup_blocks(rows), 256, 0, s>>>(rows, h0.drp, h0.dcol, h0.dval, crp.template as<u32>(),
+        k_compact<<<setup_blocks(rows), 256, 0, s>>>(rows, h0.drp, h0.dcol, h0.dval, thin, crp.template as<u32>(),
                                                      ccol.template as<u32>(), cval.template as<double>());
         F3_CHECK(hipStreamSynchronize(s));   // hrp leaves scope
         srp = crp.template as<u32>();
@@ -849,6 +860,14 @@ namespace
       return true;
     }
 
+    // thin rule of the level matrices: FP64 keeps every entry, inner FP32 (and binary16 storage, which
+    // computes in FP32) drops the couplings below eps_f32^2 of the row diagonal
+    static double level_thin()
+    {
+      const double e = 1.1920928955078125e-07;
+      return sizeof(VT) == 8 ? -1.0 : e * e;
+    }
+
     int init(const std::vector<HostLevel>& lv, const Config& c, hipStream_t stream) override
     {
       cfg = c;
@@ -865,9 +884,9 @@ namespace
         l->n = h.A.rows;
         l->A.allow_sym = cfg.symmetric_band;
         if(h.A.on_device())
-          F3_RC(l->A.upload_dev(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
+          F3_RC(l->A.upload_dev(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, level_thin(), cfg.index16, want_banded));
         else
-          F3_RC(l->A.upload(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
+          F3_RC(l->A.upload(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, level_thin(), cfg.index16, want_banded));
         l->A.rows_per_block = cfg.rows_per_block;
         if(i + 1 < lv.size())
         {
@@ -875,15 +894,16 @@ namespace
             return rt::kErrArg;
           const bool tsell = want_sell && !cfg.transfer_csr;
           const int tlanes = cfg.transfer_lanes >= 0 ? cfg.transfer_lanes : cfg.lanes;
+          const double tthin = cfg.compact_transfer ? 0.0 : -1.0;
           if(h.P.on_device())
           {
-            F3_RC(l->P.upload_dev(h.P, s, tsell, cfg.sell_C, cfg.transfer_sigma, tlanes, cfg.compact_transfer, cfg.index16, false));
-            F3_RC(l->R.upload_dev(h.R, s, tsell, cfg.sell_C, cfg.transfer_sigma, tlanes, cfg.compact_transfer, cfg.index16, false));
+            F3_RC(l->P.upload_dev(h.P, s, tsell, cfg.sell_C, cfg.transfer_sigma, tlanes, tthin, cfg.index16, false));
+            F3_RC(l->R.upload_dev(h.R, s, tsell, cfg.sell_C, cfg.transfer_sigma, tlanes, tthin, cfg.index16, false));
           }
           else
           {
-            F3_RC(l->P.upload(h.P, s, tsell, cfg.sell_C, cfg.transfer_sigma, tlanes, cfg.compact_transfer, cfg.index16));
-            F3_RC(l->R.upload(h.R, s, tsell, cfg.sell_C, cfg.transfer_sigma, tlanes, cfg.compact_transfer, cfg.index16));
+            F3_RC(l->P.upload(h.P, s, tsell, cfg.sell_C, cfg.transfer_sigma, tlanes, tthin, cfg.index16));
+            F3_RC(l->R.upload(h.R, s, tsell, cfg.sell_C, cfg.transfer_sigma, tlanes, tthin, cfg.index16));
           }
           l->P.rows_per_block = cfg.transfer_rows_per_block;
           l->R.rows_per_block = cfg.transfer_rows_per_block;
@@ -952,9 +972,9 @@ namespace
       {
         A64own.allow_sym = cfg.symmetric_band;
         if(h0.A.on_device())
-          F3_RC(A64own.upload_dev(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
+          F3_RC(A64own.upload_dev(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, -1.0, cfg.index16, want_banded));
         else
-          F3_RC(A64own.upload(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, false, cfg.index16, want_banded));
+          F3_RC(A64own.upload(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, -1.0, cfg.index16, want_banded));
         A64 = &A64own;
       }
       F3_RC(r.alloc(sizeof(double) * n0));

@@ -303,3 +303,39 @@ def test_nonsymmetric_band_keeps_the_full_band(torch):
     sg, ig, xg, _ = solve_gpu(torch, H, b, fmt="banded")
     assert so == sg == 2 and ig == io
     assert rel(xg, xo) <= 1e-10
+
+
+@pytest.mark.parametrize("fmt", ["csr", "sell"])
+def test_fp32_level_copies_drop_unresolvable_couplings(torch, fmt):
+    """The FP32 level matrices of every format keep only the entries above eps_f32^2 of their
+    row's diagonal (3D Q1: the 6 face couplings are roundoff residues); the finest sweep streams
+    exactly that many entries (CSR: kept * (4 + 4) + 4 (n + 1) + 4 n * 4 bytes), the FP64 copy keeps
+    every nonzero, the host- and device-built solvers agree bitwise and the solve keeps the
+    oracle's iteration count."""
+    print(f"case FP32 thinned {fmt} 3D Q1 L4", flush=True)
+    from feat3_amd.solver import MGPCG
+    H = pb.hierarchy(3, 1, 4, 1)
+    A = H.levels[0].A
+    n, nnz = H.n, A.nnz
+    rows = np.repeat(np.arange(A.rows), np.diff(A.row_ptr.astype(np.int64)))
+    diag = np.zeros(A.rows)
+    diag[rows[A.col_ind == rows]] = A.val[A.col_ind == rows]
+    kept = int(np.count_nonzero(np.abs(A.val) > np.finfo(np.float32).eps ** 2 * np.abs(diag[rows])))
+    assert kept < nnz
+    if fmt == "csr":
+        s = MGPCG(H, inner="f32", fmt=fmt).init()
+        assert s.time_kernel(0, 1)[1] == kept * 8 + 4 * (n + 1) + 4 * n * 4
+        s.done()
+        s = MGPCG(H, inner="f64", fmt=fmt).init()
+        assert s.time_kernel(0, 1)[1] == nnz * 12 + 4 * (n + 1) + 8 * n * 4
+        s.done()
+    b = pb.filtered_rhs(H, 0)
+    so, io, xo, _ = solve_orc(H, b)
+    sg, ig, xg, _ = solve_gpu(torch, H, b, inner="f32", fmt=fmt)
+    assert so == sg == 2 and ig == io
+    assert rel(xg, xo) <= 1e-8
+    sd = MGPCG(pb.hierarchy_device(3, 1, 4, 1), inner="f32", fmt=fmt).init()
+    x = torch.zeros(H.n, dtype=torch.float64, device="cuda")
+    st, it = sd.correct(x, torch.from_numpy(b).cuda())
+    sd.done()
+    assert st == 2 and it == ig and np.array_equal(x.cpu().numpy(), xg)
