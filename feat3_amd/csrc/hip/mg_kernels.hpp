@@ -439,9 +439,42 @@ namespace mg
     TailLevel<DT_, VT_> lv[kTailMax];
   };
 
+  // FP32 rows of 8, 16, 24 or 32 entries (the tail ELL pads its width to a multiple of 8 up to
+  // 32) issue every load of the row before the first use: one round trip to the L2 per step
+  // instead of one per group of unrolled entries (the steps of the tail are latency-bound); the
+  // terms are summed in the order of the loop below
+  template<u32 W_, typename DT_, typename VT_>
+  __device__ __forceinline__ DT_ tail_row_w(const TailEll<VT_>& M, u32 r, const DT_* xs)
+  {
+    const u64 n = M.n;
+    u32 c[W_];
+    DT_ v[W_];
+#pragma unroll
+    for(u32 k = 0; k < W_; ++k)
+    {
+      c[k] = M.col[u64(k) * n + r];
+      v[k] = dev::ld<DT_>(M.val, u64(k) * n + r);
+    }
+    DT_ s = DT_(0);
+#pragma unroll
+    for(u32 k = 0; k < W_; ++k) s += v[k] * xs[c[k]];
+    return s;
+  }
+
   template<typename DT_, typename VT_>
   __device__ __forceinline__ DT_ tail_row(const TailEll<VT_>& M, u32 r, const DT_* xs)
   {
+    if constexpr(sizeof(DT_) == 4)
+    {
+      switch(M.w)
+      {
+        case 8: return tail_row_w<8, DT_, VT_>(M, r, xs);
+        case 16: return tail_row_w<16, DT_, VT_>(M, r, xs);
+        case 24: return tail_row_w<24, DT_, VT_>(M, r, xs);
+        case 32: return tail_row_w<32, DT_, VT_>(M, r, xs);
+        default: break;
+      }
+    }
     DT_ s = DT_(0);
     const u64 n = M.n;
 #pragma unroll 9

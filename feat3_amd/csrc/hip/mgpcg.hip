@@ -1017,24 +1017,31 @@ namespace
     u64 tail_lds = 0;
     std::vector<std::unique_ptr<DevMem>> tail_mem;
 
-    // ELL copy for the tail kernel: 16-bit columns, exact zeros dropped, entry k of row r at
-    // k * rows + r, padded with column 0 and value 0
-    int tail_ell(const HostCsr& h, mg::TailEll<VT>& e)
+    // ELL copy for the tail kernel: 16-bit columns, the entries band_keeps(v, row diagonal, thin)
+    // (thin = 0: exact zeros dropped; the level matrices take level_thin()), entry k of row r at
+    // k * rows + r, padded with column 0 and value 0; with FP32 arithmetic a width of at most 32
+    // is padded to a multiple of 8 (the kernel's fixed-width row sums)
+    int tail_ell(const HostCsr& h, mg::TailEll<VT>& e, double thin)
     {
+      std::vector<double> dg(h.rows, 0.0);
+      for(u64 r = 0; r < h.rows; ++r)
+        for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
+          if(u64(h.col[k]) == r) { dg[r] = h.val[k]; break; }
       u64 w = 0;
       for(u64 r = 0; r < h.rows; ++r)
       {
         u64 c = 0;
-        for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k) c += h.val[k] != 0.0 ? 1 : 0;
+        for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k) c += band_keeps(h.val[k], dg[r], thin) ? 1 : 0;
         w = std::max(w, c);
       }
+      if(sizeof(DT) == 4 && w <= 32) w = (w + 7) / 8 * 8;
       std::vector<std::uint16_t> col(w * h.rows, 0);
       std::vector<VT> val(w * h.rows, to_storage<VT>(0.0));
       for(u64 r = 0; r < h.rows; ++r)
       {
         u64 j = 0;
         for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
-          if(h.val[k] != 0.0)
+          if(band_keeps(h.val[k], dg[r], thin))
           {
             col[j * h.rows + r] = std::uint16_t(h.col[k]);
             val[j * h.rows + r] = to_storage<VT>(h.val[k]);
@@ -1087,11 +1094,11 @@ namespace
         t.n = u32(l.n);
         t.off = u32(off);
         off += 5 * l.n;
-        F3_RC(tail_ell(lv[i].A, t.A));
+        F3_RC(tail_ell(lv[i].A, t.A, std::max(0.0, level_thin())));
         if(i + 1 < nl)
         {
-          F3_RC(tail_ell(lv[i].P, t.P));
-          F3_RC(tail_ell(lv[i].R, t.R));
+          F3_RC(tail_ell(lv[i].P, t.P, 0.0));
+          F3_RC(tail_ell(lv[i].R, t.R, 0.0));
         }
         t.mask = l.mask.template as<u32>();
         t.dinv = l.dinv.template as<DT>();

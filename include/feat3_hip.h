@@ -280,7 +280,9 @@ typedef struct feat3_mgpcg feat3_mgpcg;
 /* inner_precision: 0 FP64, 1 FP32 inside FP64 PCG, 2 binary16 matrix values with FP32
  * arithmetic inside FP64 PCG; format: 0 CSR, 1 SELL-C-sigma for the level matrices, 2 banded
  * (SparseMatrixBanded layout, no column indices) for every square matrix with at most 64
- * distinct diagonals and at most 25% zero fill, SELL-C-sigma for the others */
+ * distinct diagonals and at most 25% zero fill, SELL-C-sigma for the others.  The inner-precision
+ * copies (1 and 2) of the level matrices, in every format, leave out the entries with
+ * |a_ij| <= eps_f32^2 |a_ii|, which FP32 sums cannot resolve; FP64 copies keep every entry. */
 int feat3_mgpcg_create(feat3_mgpcg** h, int inner_precision, int format);
 int feat3_mgpcg_destroy(feat3_mgpcg* h);
 /* one level (finest first), host arrays, copied: CSR matrix (n x n, u32 indices, FP64
