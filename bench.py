@@ -429,8 +429,8 @@ def main():
         args.level = {"poisson3d": 7, "poisson2d": 9, "vector32": 6, "coeff16": 10, "spmv": 7}[args.workload]
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(DESIGN.md describes how the solve would shard)", file=sys.stderr)
+        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported "
+              "(no multi-GPU partition of the hierarchy is implemented)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
