@@ -59,6 +59,11 @@ class Hierarchy:
     levels: List[Level] = field(default_factory=list)   # finest first
     level_fine: int = 0
     level_coarse: int = 0
+    # "lex": DOFs numbered lexicographically on the tensor lattice (x fastest); "feat": the
+    # reference's numbering (StandardRefinery + DofMappingUniform), with orders[l][i] = the
+    # lexicographic lattice index of DOF i of level l (finest first)
+    numbering: str = "lex"
+    orders: Optional[List[np.ndarray]] = None
 
     @property
     def n(self) -> int:
@@ -119,9 +124,12 @@ def ref_stiffness(dim: int, degree: int) -> np.ndarray:
 
 
 def hierarchy(dim: int, degree: int, level_fine: int, level_coarse: int = 1,
-              kcell: Optional[np.ndarray] = None) -> Hierarchy:
+              kcell: Optional[np.ndarray] = None, numbering: str = "lex") -> Hierarchy:
     """Levels level_fine (finest, index 0) down to level_coarse.  kcell: per-cell coefficient
-    on the finest level; coarser levels use the mean of the children."""
+    on the finest level (cells x fastest); coarser levels use the mean of the children.
+    numbering "feat" renumbers every level's DOFs as the reference does (feat_order)."""
+    if numbering not in ("lex", "feat"):
+        raise ValueError(f"numbering must be 'lex' or 'feat', not {numbering!r}")
     H = Hierarchy(dim, degree, [], level_fine, level_coarse)
     k = kcell
     for lvl in range(level_fine, level_coarse - 1, -1):
@@ -133,7 +141,94 @@ def hierarchy(dim: int, degree: int, level_fine: int, level_coarse: int = 1,
             if k is not None:
                 k = coarsen_cellfield(dim, lvl, k)
         H.levels.append(L)
-    return H
+    return to_feat_numbering(H) if numbering == "feat" else H
+
+
+# -------------------------------------------------------------------------------------------
+# the reference's numbering: StandardRefinery + Lagrange DOF mapping
+# -------------------------------------------------------------------------------------------
+def feat_order(dim: int, degree: int, level: int) -> np.ndarray:
+    """lex_of_feat[i]: the lexicographic lattice index of DOF i in the reference's numbering of
+    the Lagrange-`degree` space on the unit square/cube refined `level` times
+    (feat3_setup_feat_order)."""
+    m = (degree << level) + 1
+    out = np.empty(m ** dim, U32)
+    capi.call("feat3_setup_feat_order", dim, degree, level, _p(out))
+    return out
+
+
+def inverse_order(order: np.ndarray) -> np.ndarray:
+    inv = np.empty_like(order)
+    inv[order] = np.arange(order.size, dtype=order.dtype)
+    return inv
+
+
+def permute_csr(M: Csr, row_order: Optional[np.ndarray], col_order: Optional[np.ndarray]) -> Csr:
+    """Old row i -> row row_order[i], old column j -> column col_order[j] (None: unchanged),
+    columns ascending (feat3_setup_permute_csr)."""
+    rp = np.empty(M.rows + 1, U32)
+    col = np.empty(M.nnz, U32)
+    val = np.empty(M.nnz, F64)
+    ro = None if row_order is None else np.ascontiguousarray(row_order, U32)
+    co = None if col_order is None else np.ascontiguousarray(col_order, U32)
+    capi.call("feat3_setup_permute_csr", M.rows, M.cols, _p(M.row_ptr), _p(M.col_ind), _p(M.val), _p(ro), _p(co),
+              _p(rp), _p(col), _p(val))
+    return Csr(M.rows, M.cols, rp, col, val)
+
+
+def to_feat_numbering(H: Hierarchy) -> Hierarchy:
+    """The same hierarchy with every level in the reference's DOF numbering: matrices, transfers
+    and Dirichlet lists permuted (values moved unchanged), orders[l] = lex_of_feat of level l."""
+    if H.numbering == "feat":
+        return H
+    orders = [feat_order(H.dim, H.degree, lvl) for lvl in range(H.level_fine, H.level_coarse - 1, -1)]
+    inv = [inverse_order(o) for o in orders]   # feat index of each lexicographic index
+    G = Hierarchy(H.dim, H.degree, [], H.level_fine, H.level_coarse, "feat", orders)
+    for i, L in enumerate(H.levels):
+        A = permute_csr(L.A, inv[i], inv[i])
+        bnd = np.sort(inv[i][L.bnd]).astype(U32)
+        P = R = None
+        if L.P is not None:
+            P = permute_csr(L.P, inv[i], inv[i + 1])
+            R = permute_csr(L.R, inv[i + 1], inv[i])
+        G.levels.append(Level(A, bnd, P, R, kcell=L.kcell))
+    return G
+
+
+def to_feat_vector(H: Hierarchy, v_lex: np.ndarray, level: int = 0) -> np.ndarray:
+    """A vector of the lexicographic numbering in H's numbering (identity for "lex")."""
+    return v_lex if H.numbering == "lex" else np.ascontiguousarray(v_lex[H.orders[level]])
+
+
+def lattice_order(coords: np.ndarray) -> np.ndarray:
+    """order[i] = rank of node i by its coordinates (rows of coords, last coordinate slowest):
+    the lexicographic lattice index of a tensor lattice's nodes (feat3_setup_lattice_order)."""
+    c = np.ascontiguousarray(coords, F64)
+    n, dim = c.shape
+    out = np.empty(n, U32)
+    capi.call("feat3_setup_lattice_order", dim, n, _p(c), _p(out))
+    return out
+
+
+def refine_mesh(dim: int, mesh: dict) -> dict:
+    """One StandardRefinery step of a conformal quad/hexa mesh given as the reference's index
+    sets: keys vtx, v_e, v_q, e_q (+ v_c, e_c, q_c for dim 3), flat arrays."""
+    nv = len(mesh["vtx"]) // dim
+    counts = np.array([nv, len(mesh["v_e"]) // 2, len(mesh["v_q"]) // 4,
+                       len(mesh["v_c"]) // 8 if dim == 3 else 0], np.uint64)
+    out = np.empty(4, np.uint64)
+    capi.call("feat3_mesh_refine_sizes", dim, _p(counts), _p(out))
+    fv, fe, fq, fh = (int(x) for x in out)
+    a = {k: np.ascontiguousarray(mesh[k], F64 if k == "vtx" else U32) for k in mesh}
+    f = {"vtx": np.empty(fv * dim, F64), "v_e": np.empty(2 * fe, U32), "v_q": np.empty(4 * fq, U32),
+         "e_q": np.empty(4 * fq, U32)}
+    if dim == 3:
+        f.update(v_c=np.empty(8 * fh, U32), e_c=np.empty(12 * fh, U32), q_c=np.empty(6 * fh, U32))
+    g = lambda d, k: _p(d.get(k))  # noqa: E731
+    capi.call("feat3_mesh_refine", dim, _p(counts), g(a, "vtx"), g(a, "v_e"), g(a, "v_q"), g(a, "e_q"), g(a, "v_c"),
+              g(a, "e_c"), g(a, "q_c"), g(f, "vtx"), g(f, "v_e"), g(f, "v_q"), g(f, "e_q"), g(f, "v_c"), g(f, "e_c"),
+              g(f, "q_c"))
+    return f
 
 
 # -------------------------------------------------------------------------------------------
@@ -226,8 +321,9 @@ def device_levels_to_host(dim: int, degree: int, level_fine: int, level_coarse: 
 
 
 def filtered_rhs(H: Hierarchy, func: int = 0) -> np.ndarray:
-    """Load vector on the finest level with homogeneous Dirichlet values (filter_rhs)."""
-    b = load_vector(H.dim, H.degree, H.level_fine, func)
+    """Load vector on the finest level with homogeneous Dirichlet values (filter_rhs), in H's
+    numbering."""
+    b = to_feat_vector(H, load_vector(H.dim, H.degree, H.level_fine, func))
     b[H.levels[0].bnd] = 0.0
     return b
 
@@ -244,10 +340,10 @@ CONFIGS = {
 }
 
 
-def config_hierarchy(cfg: int, level: Optional[int] = None) -> Hierarchy:
+def config_hierarchy(cfg: int, level: Optional[int] = None, numbering: str = "lex") -> Hierarchy:
     c = CONFIGS[cfg]
     lvl = c["level"] if level is None else level
-    return hierarchy(c["dim"], c["degree"], lvl, 1)
+    return hierarchy(c["dim"], c["degree"], lvl, 1, numbering=numbering)
 
 
 # -------------------------------------------------------------------------------------------

@@ -499,6 +499,36 @@ int feat3_setup_tile_order(uint64_t rows, const uint32_t* row_ptr, const uint32_
  * 259-280); seed 0 selects the reference's default seed */
 int feat3_setup_random_uniform(uint64_t seed, uint64_t n, double a, double b, double* out);
 
+/* ===================================================================================== */
+/* FEAT's mesh refinement and DOF numbering (host)                                        */
+/* ===================================================================================== */
+/* One StandardRefinery step of a conformal quad (dim 2) or hexa (dim 3) mesh
+ * (geometry/standard_refinery.hpp, intern/standard_vertex_refiner.hpp:65-250,
+ * intern/standard_index_refiner.hpp:1341-2700).  counts = {vertices, edges, quads, hexas}
+ * (hexas 0 for dim 2); index sets as ConformalMesh's get_index_set<cell, face>: v_e (2 per
+ * edge), v_q and e_q (4 per quad), v_c (8), e_c (12) and q_c (6) per hexa (dim 3 only, else
+ * NULL); out_counts / the fine arrays are sized by feat3_mesh_refine_sizes.  Host arrays. */
+int feat3_mesh_refine_sizes(int dim, const uint64_t* counts, uint64_t* out_counts);
+int feat3_mesh_refine(int dim, const uint64_t* counts, const double* vtx, const uint32_t* v_e, const uint32_t* v_q,
+                      const uint32_t* e_q, const uint32_t* v_c, const uint32_t* e_c, const uint32_t* q_c, double* f_vtx,
+                      uint32_t* f_v_e, uint32_t* f_v_q, uint32_t* f_e_q, uint32_t* f_v_c, uint32_t* f_e_c,
+                      uint32_t* f_q_c);
+/* FEAT's DOF numbering of the Lagrange-1 or -2 space (space/lagrange{1,2}, DofMappingUniform of
+ * space/dof_mapping_common.hpp:193-280) on UnitCubeFactory's square / cube
+ * (geometry/common_factories.hpp:70-238) refined `level` times: lex_of_feat[i] = the
+ * lexicographic lattice index (x fastest; the numbering of feat3_setup_laplace) of DOF i */
+int feat3_setup_feat_order(int dim, int degree, int level, uint32_t* lex_of_feat);
+/* order[i] = rank of node i when the nodes are sorted by their coordinates (last coordinate
+ * slowest); coords: n x dim, distinct.  For the nodes of a tensor lattice (mesh vertices for
+ * Lagrange-1; vertices, edge midpoints and cell centres for Lagrange-2) this is the
+ * lexicographic lattice index, the order feat3_mgpcg_set_level_order takes. */
+int feat3_setup_lattice_order(int dim, uint64_t n, const double* coords, uint32_t* order);
+/* permuted copy of a CSR matrix: old row i -> row row_order[i], old column j -> column
+ * col_order[j] (NULL: identity); columns ascending in every row; values moved unchanged */
+int feat3_setup_permute_csr(uint64_t rows, uint64_t cols, const uint32_t* row_ptr, const uint32_t* col_ind,
+                            const double* val, const uint32_t* row_order, const uint32_t* col_order,
+                            uint32_t* out_row_ptr, uint32_t* out_col_ind, double* out_val);
+
 #ifdef __cplusplus
 }
 #endif

@@ -259,19 +259,21 @@ def test_config_sizes(cfg, n, nnz, nbnd):
 
 
 def test_grid_transfer_golden_1_to_4_cells():
-    """grid_transfer-test.cpp:65-140.  The reference numbers the 9 fine vertices corners first
-    (0..3), then edge midpoints (edges (0,1),(2,3),(0,2),(1,3)), then the centre; the
-    lexicographic setup is the permutation below of the same matrix."""
-    P = pb.prolongation(2, 1, 1)
+    """grid_transfer-test.cpp:65-140: the unit square refined once, P from level 0 to level 1 in
+    the reference's numbering (StandardRefinery: corners 0..3, then the midpoints of edges
+    (0,1), (2,3), (0,2), (1,3), then the centre) -- pattern and values compared as stored."""
+    H = pb.hierarchy(2, 1, 1, 0, numbering="feat")
+    P = H.levels[0].P
     assert (P.rows, P.cols, P.nnz) == (9, 4, 36)
     assert np.array_equal(P.row_ptr, 4 * np.arange(10))
     assert np.array_equal(P.col_ind, np.tile(np.arange(4), 9))
     d0, d1, d2, d4 = 0.0, 1.0, 0.5, 0.25
     data_ref = np.array([d1, d0, d0, d0, d0, d1, d0, d0, d0, d0, d1, d0, d0, d0, d0, d1,
-                         d2, d2, d0, d0, d0, d0, d2, d2, d2, d0, d2, d0, d0, d2, d0, d2, d4, d4, d4, d4]).reshape(9, 4)
-    feat_to_lex = [0, 2, 6, 8, 1, 7, 3, 5, 4]
+                         d2, d2, d0, d0, d0, d0, d2, d2, d2, d0, d2, d0, d0, d2, d0, d2, d4, d4, d4, d4])
     eps = EPS ** 0.8
-    assert np.allclose(P.to_dense()[feat_to_lex], data_ref, atol=eps, rtol=0)
+    assert np.max(np.abs(P.val - data_ref)) <= eps
+    # R = P^T in the same numbering (LAFEM transpose)
+    assert np.array_equal(H.levels[0].R.to_dense(), P.to_dense().T)
 
 
 @pytest.mark.parametrize("dim,deg,lvl", [(2, 1, 4), (3, 1, 3), (2, 2, 3)])
