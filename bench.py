@@ -290,6 +290,8 @@ def bench_spmv(args):
     from feat3_amd import capi
     from feat3_amd import problems as pb
 
+    capi.call("feat3_hip_initialise", 0)   # MemoryPool<Mem::HIP>::initialise (reduction workspace)
+
     lvl = args.level
     t = time.time()
     A, _ = pb.laplace(3, 1, lvl)
@@ -417,6 +419,10 @@ def main():
     ap.add_argument("--inner", default="f32", choices=["f64", "f32", "f16"])
     ap.add_argument("--fmt", default="banded", choices=["csr", "sell", "banded"],
                     help="level-matrix format of the V-cycle (banded falls back to SELL-C-sigma where a matrix is not banded)")
+    ap.add_argument("--numbering", default="feat", choices=["feat", "lex"],
+                    help="DOF numbering of the input hierarchy: feat = the reference's (StandardRefinery + "
+                         "Lagrange DOF mapping) with the lattice order as the solver's internal level order; "
+                         "lex = lexicographic input")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP64 / FP16 comparison solves")
     ap.add_argument("--workload", default="poisson3d", choices=["poisson3d", "poisson2d", "vector32", "coeff16", "spmv"],
@@ -449,17 +455,20 @@ def main():
     shape = f"{dim}D Poisson Q{deg} level {args.level}"
 
     t = time.time()
-    H = pb.hierarchy(dim, deg, args.level, 1)
+    H = pb.hierarchy(dim, deg, args.level, 1, numbering=args.numbering)
     host_setup_s = time.time() - t
     b = pb.filtered_rhs(H, 0)
     n, nnz = H.n, H.levels[0].A.nnz
-    log(f"setup: {len(H.levels)} levels, n={n}, nnz={nnz} ({host_setup_s:.1f} s)")
-    # the same setup on the device: assembly (bit-identical to the host's) and the solver's init
-    # from the device levels (formats built on the device); the solves below use the host path
-    MGPCG(pb.hierarchy_device(dim, deg, 3, 1), inner=args.inner, fmt=args.fmt).init().done()   # warm-up
+    log(f"setup: {len(H.levels)} levels, n={n}, nnz={nnz}, {args.numbering} numbering ({host_setup_s:.1f} s)")
+    # the same setup on the device: assembly (bit-identical to the host's, renumbered on the device
+    # for the FEAT numbering; the mesh's numbering itself, feat_order, is computed beforehand) and
+    # the solver's init from the device levels (renumbering and formats on the device); the solves
+    # below use the host path
+    orders = H.orders if args.numbering == "feat" else None
+    MGPCG(pb.hierarchy_device(dim, deg, 3, 1, numbering=args.numbering), inner=args.inner, fmt=args.fmt).init().done()
     torch.cuda.synchronize()
     t = time.perf_counter()
-    levels_dev = pb.hierarchy_device(dim, deg, args.level, 1)
+    levels_dev = pb.hierarchy_device(dim, deg, args.level, 1, numbering=args.numbering, orders=orders)
     torch.cuda.synchronize()
     dev_setup_ms = 1e3 * (time.perf_counter() - t)
     t = time.perf_counter()
@@ -471,7 +480,8 @@ def main():
     log(f"setup on the device: assembly {dev_setup_ms:.1f} ms, solver init {dev_init_ms:.0f} ms "
         f"(host: assembly {host_setup_s:.2f} s, init below)")
 
-    setup = {"host_hierarchy_s": host_setup_s, "device_hierarchy_ms": dev_setup_ms, "device_init_ms": dev_init_ms}
+    setup = {"host_hierarchy_s": host_setup_s, "device_hierarchy_ms": dev_setup_ms, "device_init_ms": dev_init_ms,
+             "numbering": args.numbering}
     db = torch.from_numpy(b).to(dev)
     x = torch.zeros(n, dtype=torch.float64, device=dev)
 
@@ -491,10 +501,10 @@ def main():
             assert st == 2, f"solve did not converge (status {st})"
         return times, iters
 
-    def variant(inner, steps, warmup, fmt=None, cycle="v"):
+    def variant(inner, steps, warmup, fmt=None, cycle="v", level_orders="auto"):
         fmt = fmt or args.fmt
         t0 = time.time()
-        s = MGPCG(H, inner=inner, fmt=fmt, cycle=cycle).init()
+        s = MGPCG(H, inner=inner, fmt=fmt, cycle=cycle, level_orders=level_orders).init()
         init_s = time.time() - t0
         if inner == args.inner and fmt == args.fmt and cycle == "v":
             setup["host_init_s"] = init_s
@@ -538,6 +548,11 @@ def main():
                 continue
             r, _, _, _ = variant(args.inner, max(2, args.steps // 2), 1, fmt=fmt)
             variants[f"{args.inner}/{fmt}"] = (r, None)
+        # FEAT-numbered input without the lattice order: the level matrices keep FEAT's numbering
+        # (no band structure: SELL-C-sigma), what the solver does when the caller gives no order
+        if args.numbering == "feat":
+            r, _, _, _ = variant(args.inner, max(2, args.steps // 2), 1, level_orders=None)
+            variants[f"{args.inner}/{args.fmt}/feat-order-kept"] = (r, None)
         # the reference's F- and W-cycles as the preconditioner (multigrid.hpp:1366-1519)
         for cycle in ("f", "w"):
             r, _, _, _ = variant(args.inner, max(2, args.steps // 2), 1, cycle=cycle)
@@ -590,7 +605,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"f32": "f32", "f64": "f64", "f16": "f16-values/f32"}[args.inner],
-        "data": f"synthetic (assembled {dim}D Q{deg} Poisson system, sine-bubble load)",
+        "data": f"synthetic (assembled {dim}D Q{deg} Poisson system, sine-bubble load, "
+                f"{'FEAT' if args.numbering == 'feat' else 'lexicographic'} DOF numbering)",
         "config": {"workload": f"{shape} ({n} DOF, {nnz} nnz), PCG(FP64) + MG V(4,4) "
                                f"Richardson/Jacobi 0.5, exact coarse {'2x2x2' if dim == 3 else '2x2'} cells, {args.inner} V-cycle, "
                                f"{args.fmt} format, rel. tol 1e-8",

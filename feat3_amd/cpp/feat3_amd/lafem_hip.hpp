@@ -341,8 +341,9 @@ namespace feat3_amd
                       : feat3_io_csr_binary_sizes(filename.c_str(), &r, &c, &z), "SparseMatrixCSR(file)");
         std::vector<std::uint32_t> rp(r + 1), ci(z);
         std::vector<double> v(z);
-        hip_check(mtx ? feat3_io_read_mtx(filename.c_str(), rp.data(), ci.data(), v.data())
-                      : feat3_io_read_csr_binary(filename.c_str(), rp.data(), ci.data(), v.data()), "SparseMatrixCSR(file)");
+        hip_check(mtx ? feat3_io_read_mtx(filename.c_str(), r, z, rp.data(), ci.data(), v.data())
+                      : feat3_io_read_csr_binary(filename.c_str(), r, z, rp.data(), ci.data(), v.data()),
+                  "SparseMatrixCSR(file)");
         *this = SparseMatrixCSR(Index(r), Index(c), rp, ci, v);
       }
       // write_out(FileMode, String) (sparse_matrix_csr.hpp:1266-1340)

@@ -539,15 +539,12 @@ namespace
     if(threadIdx.x == 0) *out = SQRT ? DT(sqrt(t)) : t;
   }
 
+  // partials of x.y (or x.x) from at most 2048 blocks (two per SIMD: each thread reads several
+  // elements, and the finalisation sums few partials), then the one-block finalisation into out
   template<typename DT, bool SQRT>
-  int launch_dot(hipStream_t s, const DT* x, const DT* y, u64 n, DT* out)
+  int enqueue_dot(hipStream_t s, const DT* x, const DT* y, u64 n, DT* part, unsigned max_blocks, DT* out)
   {
-    rt::Workspace* ws = rt::workspace();
-    if(!ws) return rt::kErrRuntime;
-    // at most 2048 blocks (two per SIMD): each thread reads several elements, and the
-    // finalisation sums few partials
-    const unsigned nb = unsigned(std::min<u64>(std::min<u64>(stream_blocks(n), 2048), rt::Workspace::kMaxPartials));
-    DT* part = reinterpret_cast<DT*>(ws->partials);
+    const unsigned nb = unsigned(std::min<u64>(std::min<u64>(stream_blocks(n), 2048), max_blocks));
     constexpr int V = 16 / int(sizeof(DT));
     if(reinterpret_cast<std::uintptr_t>(x) % 16 == 0 && reinterpret_cast<std::uintptr_t>(y) % 16 == 0)
       k_dot_vec<DT, V><<<nb, kBlock, 0, s>>>(x, y, n, part);
@@ -557,15 +554,36 @@ namespace
     return rt::check_launch();
   }
 
+  // the drop-in reductions share the device's workspace (feat3_hip_initialise): the stream waits
+  // for the previous reduction that used it (on any stream) and records the workspace's event
+  // after its own finalisation.  Inside a stream capture the captured graph orders its nodes
+  // itself, and the event is left alone (capture does not take foreign events).
+  template<typename DT, bool SQRT>
+  int launch_dot(hipStream_t s, const DT* x, const DT* y, u64 n, DT* out, DT* host_out = nullptr)
+  {
+    rt::Workspace* ws = rt::workspace();
+    if(!ws) return rt::kErrState;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if(hipStreamIsCapturing(s, &cap) != hipSuccess) return rt::kErrRuntime;
+    const bool order = cap == hipStreamCaptureStatusNone;
+    if(order && hipStreamWaitEvent(s, ws->done, 0) != hipSuccess) return rt::kErrRuntime;
+    const int rc = enqueue_dot<DT, SQRT>(s, x, y, n, reinterpret_cast<DT*>(ws->partials), rt::Workspace::kMaxPartials, out);
+    if(rc) return rc;
+    // a host-returning reduction reads the workspace's scalar before the next one may write it
+    if(host_out && hipMemcpyAsync(host_out, out, sizeof(DT), hipMemcpyDeviceToHost, s) != hipSuccess)
+      return rt::kErrRuntime;
+    if(order && hipEventRecord(ws->done, s) != hipSuccess) return rt::kErrRuntime;
+    return rt::kOk;
+  }
+
   template<typename DT, bool SQRT>
   int dot_value(hipStream_t s, const DT* x, const DT* y, u64 n, DT* host_out)
   {
     rt::Workspace* ws = rt::workspace();
-    if(!ws) return rt::kErrRuntime;
+    if(!ws) return rt::kErrState;
     DT* dres = reinterpret_cast<DT*>(ws->scalar);
-    int rc = launch_dot<DT, SQRT>(s, x, y, n, dres);
+    int rc = launch_dot<DT, SQRT>(s, x, y, n, dres, host_out);
     if(rc) return rc;
-    if(hipMemcpyAsync(host_out, dres, sizeof(DT), hipMemcpyDeviceToHost, s) != hipSuccess) return rt::kErrRuntime;
     if(hipStreamSynchronize(s) != hipSuccess) return rt::kErrRuntime;
     return 0;
   }

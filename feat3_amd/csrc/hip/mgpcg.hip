@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -45,6 +46,9 @@ namespace
   struct DevMem
   {
     void* p = nullptr;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
     ~DevMem() { if(p) (void)hipFree(p); }
     template<typename T> T* as() const { return static_cast<T*>(p); }
     int alloc(u64 bytes)
@@ -86,16 +90,19 @@ namespace
       if(!ok) atomicOr(bad, 1);
     }
   }
-  // Entries the banded copy of an inner-precision level matrix leaves out: |a_ij| <= eps2 |a_ii|
-  // with eps2 the square of the inner precision's epsilon (FP32: 1.4e-14; FP64 keeps every
-  // nonzero).  Such a coupling changes a row sum by less than eps^2 of its diagonal term, far
-  // below the rounding of the inner arithmetic itself (binary16 storage flushes such values to
-  // zero anyway).  The 3D Q1 Laplacian's 6 face couplings per row are roundoff residues of about
-  // 2e-17 of the diagonal, so its FP32 band has 21 diagonals instead of 27 (22% fewer bytes per
-  // sweep).  Host and device builders evaluate the same expression on the same FP64 values.
+  // Entries the copy of an inner-precision level matrix leaves out: |a_ij| <= eps2 |a_ii| with
+  // eps2 the square of the inner precision's epsilon (FP32: 1.4e-14; FP64: 0, so only exact zeros
+  // go).  Such a coupling changes a row sum by less than eps^2 of its diagonal term, far below the
+  // rounding of the inner arithmetic itself (binary16 storage flushes such values to zero anyway).
+  // The 3D Q1 Laplacian's 6 face couplings per row are roundoff residues of about 2e-17 of the
+  // diagonal, so its FP32 band has 21 diagonals instead of 27 (22% fewer bytes per sweep).  The
+  // test is written so that a NaN value, or any value in a row whose diagonal is NaN, is kept: a
+  // non-finite matrix stays non-finite in every copy, and the solve reports it (aborted) instead
+  // of solving another matrix.  Host and device builders evaluate the same expression on the same
+  // FP64 values.
   __host__ __device__ inline bool band_keeps(double v, double diag, double eps2)
   {
-    return (v < 0.0 ? -v : v) > eps2 * (diag < 0.0 ? -diag : diag);
+    return !((v < 0.0 ? -v : v) <= eps2 * (diag < 0.0 ? -diag : diag));
   }
   __device__ inline double row_diagonal(const u32* rp, const u32* col, const double* val, u64 r)
   {
@@ -464,11 +471,12 @@ namespace
     // (col - row) and the zero fill stays small: at most kMaxDiag diagonals and
     // ndiag * rows <= 1.25 * nnz.  Otherwise the matrix keeps its other format.
     static constexpr u32 kMaxDiag = 64;
-    // eps^2 of the inner arithmetic (binary16 storage computes in FP32)
+    // eps^2 of the inner arithmetic (binary16 storage computes in FP32); FP64 bands keep every
+    // nonzero (the outer operator and the FP64 path are the matrix as given)
     static double band_eps2()
     {
-      const double e = sizeof(VT) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
-      return e * e;
+      const double e = 1.1920928955078125e-07;
+      return sizeof(VT) == 8 ? 0.0 : e * e;
     }
     int build_banded(const HostCsr& h, hipStream_t s)
     {
@@ -1677,7 +1685,122 @@ struct feat3_mgpcg
   std::unique_ptr<EngineBase> eng;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // the solver's internal DOF order per level (feat3_mgpcg_set_level_order; empty: the caller's
+  // numbering); applied to the level data at the first init after it was set
+  std::vector<std::vector<u32>> orders;
+  std::vector<char> pending;
+  // level 0's order on the device and the internal FP64 vectors b, x of a permuted solve
+  DevMem order0, xi, bi;
+  u64 n0 = 0;
+  bool permuted0 = false;
 };
+
+namespace
+{
+  // old index i -> new index order[i] (NULL: identity) of a host CSR matrix's rows and columns
+  int permute_host_csr(HostCsr& m, const u32* row_order, const u32* col_order)
+  {
+    if(!row_order && !col_order) return rt::kOk;
+    const u64 nnz = m.nnz();
+    std::vector<u32> rp(m.rows + 1), col(nnz);
+    std::vector<double> val(nnz);
+    F3_RC(feat3_setup_permute_csr(m.rows, m.cols, m.rp.data(), m.col.data(), m.val.data(), row_order, col_order,
+                                  rp.data(), col.data(), val.data()));
+    m.rp.swap(rp); m.col.swap(col); m.val.swap(val);
+    return rt::kOk;
+  }
+
+  // the same for a device-resident source: the permuted copy goes into store (kept until init
+  // returns, like the caller's arrays it replaces)
+  int permute_dev_csr(HostCsr& m, const u32* drow, const u32* dcol_order, hipStream_t s, std::deque<DevMem>& store)
+  {
+    if(!drow && !dcol_order) return rt::kOk;
+    store.emplace_back(); DevMem& rp = store.back();
+    store.emplace_back(); DevMem& col = store.back();
+    store.emplace_back(); DevMem& val = store.back();
+    F3_RC(rp.alloc(sizeof(u32) * (m.rows + 1)));
+    F3_RC(col.alloc(sizeof(u32) * std::max<u64>(m.dnnz, 1)));
+    F3_RC(val.alloc(sizeof(double) * std::max<u64>(m.dnnz, 1)));
+    F3_RC(feat3_device_permute_csr(s, m.rows, m.cols, m.drp, m.dcol, m.dval, drow, dcol_order, rp.as<u32>(),
+                                   col.as<u32>(), val.as<double>()));
+    m.drp = rp.as<u32>(); m.dcol = col.as<u32>(); m.dval = val.as<double>();
+    return rt::kOk;
+  }
+
+  // applies the pending level orders: A_l rows and columns by order_l, the Dirichlet list mapped
+  // and sorted, P_l rows by order_l and columns by order_{l+1}, R_l the other way round
+  int apply_orders(feat3_mgpcg* h, std::deque<DevMem>& store)
+  {
+    const std::size_t nl = h->levels.size();
+    h->orders.resize(nl);
+    h->pending.resize(nl, 0);
+    bool any = false;
+    for(std::size_t l = 0; l < nl; ++l) any = any || h->pending[l];
+    if(!any) return rt::kOk;
+    // device copies of the orders that device-resident levels need
+    std::deque<DevMem> dord(nl);
+    auto need_dev = [&](std::size_t l) {
+      return h->levels[l].A.on_device() || (l > 0 && h->levels[l - 1].P.on_device());
+    };
+    for(std::size_t l = 0; l < nl; ++l)
+      if(h->pending[l] && need_dev(l))
+      {
+        F3_RC(dord[l].alloc(sizeof(u32) * h->orders[l].size()));
+        F3_CHECK(hipMemcpyAsync(dord[l].p, h->orders[l].data(), sizeof(u32) * h->orders[l].size(),
+                                hipMemcpyHostToDevice, h->stream));
+      }
+    auto hord = [&](std::size_t l) -> const u32* { return l < nl && h->pending[l] ? h->orders[l].data() : nullptr; };
+    auto dor = [&](std::size_t l) -> const u32* { return l < nl && h->pending[l] ? dord[l].as<u32>() : nullptr; };
+    for(std::size_t l = 0; l < nl; ++l)
+    {
+      HostLevel& L = h->levels[l];
+      if(L.A.on_device()) F3_RC(permute_dev_csr(L.A, dor(l), dor(l), h->stream, store));
+      else F3_RC(permute_host_csr(L.A, hord(l), hord(l)));
+      if(hord(l))
+      {
+        for(u32& i : L.fidx) i = h->orders[l][i];
+        std::sort(L.fidx.begin(), L.fidx.end());
+      }
+      if(l + 1 < nl && L.P.rows)
+      {
+        if(L.P.on_device())
+        {
+          F3_RC(permute_dev_csr(L.P, dor(l), dor(l + 1), h->stream, store));
+          F3_RC(permute_dev_csr(L.R, dor(l + 1), dor(l), h->stream, store));
+        }
+        else
+        {
+          F3_RC(permute_host_csr(L.P, hord(l), hord(l + 1)));
+          F3_RC(permute_host_csr(L.R, hord(l + 1), hord(l)));
+        }
+      }
+    }
+    F3_CHECK(hipStreamSynchronize(h->stream));
+    if(h->pending[0])
+    {
+      h->n0 = h->levels[0].A.rows;
+      F3_RC(h->order0.alloc(sizeof(u32) * h->n0));
+      F3_RC(h->xi.alloc(sizeof(double) * h->n0));
+      F3_RC(h->bi.alloc(sizeof(double) * h->n0));
+      F3_CHECK(hipMemcpy(h->order0.p, h->orders[0].data(), sizeof(u32) * h->n0, hipMemcpyHostToDevice));
+      h->permuted0 = true;
+    }
+    for(std::size_t l = 0; l < nl; ++l) h->pending[l] = 0;
+    return rt::kOk;
+  }
+
+  // caller vector -> internal order (b, x at entry) and back (x at exit), on the solver's stream
+  int to_internal(feat3_mgpcg* h, const double* src, double* dst)
+  {
+    return feat3_device_permute_vector(h->stream, h->n0, h->order0.as<u32>(), 0, src, dst);
+  }
+  int to_caller(feat3_mgpcg* h, const double* src, double* dst)
+  {
+    F3_RC(feat3_device_permute_vector(h->stream, h->n0, h->order0.as<u32>(), 1, src, dst));
+    F3_CHECK(hipStreamSynchronize(h->stream));
+    return rt::kOk;
+  }
+} // namespace
 
 extern "C" {
 
@@ -1892,9 +2015,33 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
 
 // init_symbolic + init_numeric: uploads, converts to the inner precision, Jacobi setup,
 // coarse inverse (solver/base.hpp:227-260, multigrid.hpp:814-818)
+int feat3_mgpcg_set_level_order(feat3_mgpcg* h, int level, const uint32_t* order)
+{
+  if(!h || level < 0 || std::size_t(level) >= h->levels.size()) return rt::kErrState;
+  if(!order) return rt::kErrArg;
+  h->orders.resize(h->levels.size());
+  h->pending.resize(h->levels.size(), 0);
+  // a level's numbering is fixed once its data has been permuted
+  if(!h->orders[level].empty()) return rt::kErrState;
+  const u64 n = h->levels[level].A.rows;
+  std::vector<u32> o(order, order + n);
+  std::vector<char> hit(n, 0);
+  for(u32 v : o)
+  {
+    if(u64(v) >= n || hit[v]) return rt::kErrArg;
+    hit[v] = 1;
+  }
+  h->orders[level].swap(o);
+  h->pending[level] = 1;
+  h->eng.reset();
+  return rt::kOk;
+}
+
 int feat3_mgpcg_init(feat3_mgpcg* h)
 {
   if(!h || h->levels.empty()) return rt::kErrState;
+  std::deque<DevMem> store;    // permuted copies of device-resident levels, until init returns
+  F3_RC(apply_orders(h, store));
   std::unique_ptr<EngineBase> e;
   if(h->inner == 0) e.reset(new Engine<double, double>());
   else if(h->inner == 1) e.reset(new Engine<float, float>());
@@ -1912,7 +2059,12 @@ int feat3_mgpcg_correct(feat3_mgpcg* h, void* caller_stream, double* x, const do
   // the caller's stream (NULL: the null stream) has produced the inputs: wait for it (the
   // solver's own stream does not synchronise with it implicitly)
   if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime;
-  return h->eng->solve(x, b, true, status, iters);
+  if(!h->permuted0) return h->eng->solve(x, b, true, status, iters);
+  if(!x || !b) return rt::kErrArg;
+  F3_RC(to_internal(h, b, h->bi.as<double>()));
+  F3_RC(to_internal(h, x, h->xi.as<double>()));
+  F3_RC(h->eng->solve(h->xi.as<double>(), h->bi.as<double>(), true, status, iters));
+  return to_caller(h, h->xi.as<double>(), x);
 }
 
 // SolverBase::apply(cor, def): cor = PCG(def) from a zero start
@@ -1922,7 +2074,11 @@ int feat3_mgpcg_apply(feat3_mgpcg* h, void* caller_stream, double* cor, const do
   // the caller's stream (NULL: the null stream) has produced the inputs: wait for it (the
   // solver's own stream does not synchronise with it implicitly)
   if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime;
-  return h->eng->solve(cor, def, false, status, iters);
+  if(!h->permuted0) return h->eng->solve(cor, def, false, status, iters);
+  if(!cor || !def) return rt::kErrArg;
+  F3_RC(to_internal(h, def, h->bi.as<double>()));
+  F3_RC(h->eng->solve(h->xi.as<double>(), h->bi.as<double>(), false, status, iters));
+  return to_caller(h, h->xi.as<double>(), cor);
 }
 
 // the MultiGrid preconditioner alone (one V-cycle): cor = MG(def), FP64 in and out
@@ -1932,7 +2088,11 @@ int feat3_mgpcg_vcycle(feat3_mgpcg* h, void* caller_stream, double* cor, const d
   // the caller's stream (NULL: the null stream) has produced the inputs: wait for it (the
   // solver's own stream does not synchronise with it implicitly)
   if(hipStreamSynchronize(static_cast<hipStream_t>(caller_stream)) != hipSuccess) return rt::kErrRuntime;
-  return h->eng->vcycle(cor, def);
+  if(!h->permuted0) return h->eng->vcycle(cor, def);
+  if(!cor || !def) return rt::kErrArg;
+  F3_RC(to_internal(h, def, h->bi.as<double>()));
+  F3_RC(h->eng->vcycle(h->xi.as<double>(), h->bi.as<double>()));
+  return to_caller(h, h->xi.as<double>(), cor);
 }
 
 int feat3_mgpcg_history(feat3_mgpcg* h, double* out, u64 cap, u64* len)

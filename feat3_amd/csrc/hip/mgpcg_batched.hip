@@ -101,7 +101,9 @@ namespace
   // diagonal block: such a block changes a row sum by less than eps^2 of the diagonal term, far
   // below the rounding of FP32 arithmetic (the face couplings of the 3D Q1 vector Laplacian are
   // roundoff residues of about 6e-17 of the diagonal: 21 of 27 blocks per row stay).  The outer
-  // FP64 product keeps the full graph.
+  // FP64 product keeps the full graph.  A block holding a non-finite value is kept, and so is
+  // every block of a row with a non-finite diagonal value: the inner copy of a non-finite matrix
+  // stays non-finite.
   HostGraph thin_graph(const HostGraph& h, u64 bb, int B)
   {
     const double eps2 = 1.1920928955078125e-07 * 1.1920928955078125e-07;
@@ -111,14 +113,26 @@ namespace
     for(u64 r = 0; r < h.rows; ++r)
     {
       double d = 0.0;
+      bool finite_diag = true;
       for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
         if(u64(h.col[k]) == r)
-          for(int q = 0; q < B; ++q) d = std::max(d, std::fabs(h.val[u64(k) * bb + u64(q) * u64(B) + u64(q)]));
+          for(int q = 0; q < B; ++q)
+          {
+            const double v = h.val[u64(k) * bb + u64(q) * u64(B) + u64(q)];
+            finite_diag = finite_diag && std::isfinite(v);
+            d = std::max(d, std::fabs(v));
+          }
       for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k)
       {
         double m = 0.0;
-        for(u64 j = 0; j < bb; ++j) m = std::max(m, std::fabs(h.val[u64(k) * bb + j]));
-        if(u64(h.col[k]) != r && m <= eps2 * d) continue;
+        bool finite = finite_diag;
+        for(u64 j = 0; j < bb; ++j)
+        {
+          const double v = h.val[u64(k) * bb + j];
+          finite = finite && std::isfinite(v);
+          m = std::max(m, std::fabs(v));
+        }
+        if(u64(h.col[k]) != r && finite && m <= eps2 * d) continue;
         c.col.push_back(h.col[k]);
         for(u64 j = 0; j < bb; ++j) c.val.push_back(h.val[u64(k) * bb + j]);
       }

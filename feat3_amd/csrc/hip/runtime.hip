@@ -30,12 +30,27 @@ namespace rt
     if(hipGetDevice(&dev) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> lock(g_mutex);
     auto it = g_ws.find(dev);
-    if(it != g_ws.end()) return &it->second;
+    return it != g_ws.end() ? &it->second : nullptr;
+  }
+
+  int create_workspace()
+  {
+    int dev = 0;
+    if(hipGetDevice(&dev) != hipSuccess) return kErrRuntime;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if(g_ws.count(dev)) return kOk;
     Workspace w{};
-    if(hipMalloc(&w.partials, sizeof(double) * Workspace::kMaxPartials) != hipSuccess) return nullptr;
-    if(hipMalloc(&w.scalar, 64) != hipSuccess) return nullptr;
-    if(hipDeviceSynchronize() != hipSuccess) return nullptr;
-    return &(g_ws[dev] = w);
+    if(hipMalloc(&w.partials, sizeof(double) * Workspace::kMaxPartials) != hipSuccess) return kErrRuntime;
+    if(hipMalloc(&w.scalar, 64) != hipSuccess) { (void)hipFree(w.partials); return kErrRuntime; }
+    if(hipEventCreateWithFlags(&w.done, hipEventDisableTiming) != hipSuccess)
+    {
+      (void)hipFree(w.partials);
+      (void)hipFree(w.scalar);
+      return kErrRuntime;
+    }
+    if(hipDeviceSynchronize() != hipSuccess) return kErrRuntime;
+    g_ws[dev] = w;
+    return kOk;
   }
 
   int check_launch()
@@ -126,11 +141,11 @@ const char* feat3_hip_error_string(int code)
 }
 
 // MemoryPool<Mem::HIP>::initialise (memory_pool.cu:56-72): select the device, create the
-// reduction workspace.
+// reduction workspace (once per device).
 int feat3_hip_initialise(int device)
 {
   if(hipSetDevice(device) != hipSuccess) return rt::kErrRuntime;
-  return rt::workspace() ? rt::kOk : rt::kErrRuntime;
+  return rt::create_workspace();
 }
 
 // MemoryPool<Mem::HIP>::finalise (memory_pool.cu:74-95): reports leaked allocations

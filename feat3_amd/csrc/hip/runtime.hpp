@@ -20,16 +20,21 @@ namespace rt
     kErrNumeric = 5   // numerical breakdown detected during setup (e.g. singular coarse matrix)
   };
 
-  // per-device scratch for deterministic grid reductions of the drop-in BLAS-1 calls
+  // per-device scratch for deterministic grid reductions of the drop-in BLAS-1 calls; `done` is
+  // recorded after every reduction that used it, and the next reduction's stream waits for it, so
+  // that reductions issued on different streams take turns on the one set of partials
   struct Workspace
   {
     static constexpr unsigned kMaxPartials = 8192;
     void* partials;        // kMaxPartials doubles
     void* scalar;          // one double, target of the host-returning reductions
+    hipEvent_t done;
   };
 
-  // the workspace of the current device (allocated on first use; nullptr on failure)
+  // the workspace of the current device, created by feat3_hip_initialise (nullptr before that)
   Workspace* workspace();
+  // creates the current device's workspace (feat3_hip_initialise); kOk if it exists already
+  int create_workspace();
 
   // returns kOk or kErrLaunch after a kernel launch; with FEAT3_HIP_DEBUG_SYNC=1 in the
   // environment every launch is also synchronised and checked (FEAT_DEBUG_MODE analogue)

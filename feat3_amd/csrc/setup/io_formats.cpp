@@ -119,17 +119,25 @@ namespace
     u64 rows, cols, nnz, val, col, rp;
   };
 
+  // Every size comes from the file: the counts are bounded before any sum is formed (no sum of
+  // them can wrap), and every array must lie inside the words read, in the order val < col < rp.
+  // The header words are those of SparseMatrixCSR's container (sparse_matrix_csr.hpp:320-343):
+  // one elements array, two index arrays, three scalar indices (rows, columns, nnz) and one
+  // scalar value (the zero element).
   bool layout(const std::vector<u64>& w, BinLayout& L)
   {
-    const u64 nsd = w[9];
+    if(w.size() < 18 || w[6] != 1 || w[7] != 2 || w[8] != 4 || w[9] != 1) return false;
     const u64 hdr = 10 + w[6] + w[7] + w[8];
+    const u64 limit = u64(1) << 40;                 // far above any u32-indexed matrix
     L.nnz = w[10];
-    if(w[11] != L.nnz) return false;
     L.rows = w[13 + 1];
     L.cols = w[13 + 2];
-    if(w[12] != L.rows + 1 || w[13 + 3] != L.nnz) return false;
-    L.val = feat_offset(hdr) + nsd;   // DT2 = double: one word each
+    if(L.nnz >= limit || L.rows >= limit || L.cols >= limit) return false;
+    if(w[11] != L.nnz || w[12] != L.rows + 1 || w[13 + 3] != L.nnz) return false;
+    L.val = feat_offset(hdr) + w[9];   // DT2 = double: one word each
+    if(L.val + L.nnz > w.size()) return false;
     L.col = feat_offset(L.val + L.nnz);
+    if(L.col < L.val + L.nnz || L.col + L.nnz > w.size()) return false;
     L.rp = L.col + L.nnz;
     return L.rp + L.rows + 1 <= w.size();
   }
@@ -287,10 +295,15 @@ int feat3_io_mtx_sizes(const char* path, uint64_t* rows, uint64_t* cols, uint64_
   return 0;
 }
 
-int feat3_io_read_mtx(const char* path, uint32_t* row_ptr, uint32_t* col_ind, double* val)
+int feat3_io_read_mtx(const char* path, uint64_t rows, uint64_t nnz, uint32_t* row_ptr, uint32_t* col_ind, double* val)
 {
   Entries E;
   if(!path || !row_ptr || !read_mtx(path, E)) return 1;
+  // the caller's arrays hold rows + 1 row pointers and nnz entries: the file must (still) match
+  u64 z = 0;
+  for(const auto& r : E.rc) z += r.second.size();
+  if(E.rows != rows || z != nnz || E.rows >= (u64(1) << 32) || E.cols >= (u64(1) << 32) || z >= (u64(1) << 32)) return 1;
+  if(z && (!col_ind || !val)) return 1;
   u64 k = 0;
   row_ptr[0] = 0;
   for(u64 r = 0; r < E.rows; ++r)
@@ -340,11 +353,17 @@ int feat3_io_csr_binary_sizes(const char* path, uint64_t* rows, uint64_t* cols, 
   return 0;
 }
 
-int feat3_io_read_csr_binary(const char* path, uint32_t* row_ptr, uint32_t* col_ind, double* val)
+int feat3_io_read_csr_binary(const char* path, uint64_t rows, uint64_t nnz, uint32_t* row_ptr, uint32_t* col_ind,
+                             double* val)
 {
   std::vector<u64> w;
   BinLayout L;
   if(!path || !row_ptr || !read_binary(path, w) || !layout(w, L)) return 1;
+  // the file is read again here: it must (still) have the sizes the caller's arrays were made
+  // for, within the limits of feat3_io_csr_binary_sizes
+  if(L.rows != rows || L.nnz != nnz) return 1;
+  if(L.rows >= (u64(1) << 32) || L.cols >= (u64(1) << 32) || L.nnz >= (u64(1) << 32)) return 1;
+  if(L.nnz && (!col_ind || !val)) return 1;
   for(u64 i = 0; i < L.nnz; ++i)
   {
     const u64 c = w[L.col + i];

@@ -292,9 +292,53 @@ class DeviceLevel:
     R: Optional[DeviceCsr] = None
 
 
-def hierarchy_device(dim: int, degree: int, level_fine: int, level_coarse: int = 1, kcell=None):
+class DeviceLevels(list):
+    """Device levels (finest first) of a FEAT-numbered hierarchy with its level orders."""
+
+    def __init__(self, orders=None):
+        super().__init__()
+        self.orders = orders
+        self.numbering = "feat" if orders is not None else "lex"
+
+
+def permute_device_csr(M: DeviceCsr, row_order, col_order) -> DeviceCsr:
+    """feat3_device_permute_csr: old row i -> row_order[i], old column j -> col_order[j]
+    (device u32 tensors held as int32, or None), columns ascending."""
+    import torch
+    rp = torch.empty_like(M.row_ptr)
+    col = torch.empty_like(M.col_ind)
+    val = torch.empty_like(M.val)
+    capi.call("feat3_device_permute_csr", None, M.rows, M.cols, _dptr(M.row_ptr), _dptr(M.col_ind), _dptr(M.val),
+              _dptr(row_order), _dptr(col_order), _dptr(rp), _dptr(col), _dptr(val))
+    return DeviceCsr(M.rows, M.cols, rp, col, val)
+
+
+def hierarchy_device(dim: int, degree: int, level_fine: int, level_coarse: int = 1, kcell=None,
+                     numbering: str = "lex", orders=None):
     """The levels of hierarchy() assembled on the device (finest first); kcell: device
-    coefficient per finest cell or None.  to_host(levels) gives the Hierarchy the solver takes."""
+    coefficient per finest cell or None.  numbering "feat": every level renumbered on the device
+    into the reference's numbering (the host computes the orders, feat_order); the returned
+    list then carries .orders like a FEAT-numbered Hierarchy (orders: those lists if already
+    computed).  device_levels_to_host gives the Hierarchy of the same arrays."""
+    if numbering == "feat":
+        import torch
+        lev = hierarchy_device(dim, degree, level_fine, level_coarse, kcell)
+        if orders is None:
+            orders = [feat_order(dim, degree, lvl) for lvl in range(level_fine, level_coarse - 1, -1)]
+        inv = [torch.from_numpy(inverse_order(o).view(np.int32)).cuda() for o in orders]
+        out = DeviceLevels(orders=orders)
+        for i, L in enumerate(lev):
+            A = permute_device_csr(L.A, inv[i], inv[i])
+            bnd = torch.sort(inv[i][L.bnd.long()])[0]
+            P = R = None
+            if L.P is not None:
+                P = permute_device_csr(L.P, inv[i], inv[i + 1])
+                R = permute_device_csr(L.R, inv[i + 1], inv[i])
+            out.append(DeviceLevel(A, bnd, P, R))
+        torch.cuda.synchronize()
+        return out
+    if numbering != "lex":
+        raise ValueError(f"numbering must be 'lex' or 'feat', not {numbering!r}")
     import torch
     levels = []
     k = kcell
@@ -313,7 +357,8 @@ def hierarchy_device(dim: int, degree: int, level_fine: int, level_coarse: int =
 
 
 def device_levels_to_host(dim: int, degree: int, level_fine: int, level_coarse: int, levels) -> Hierarchy:
-    H = Hierarchy(dim, degree, [], level_fine, level_coarse)
+    H = Hierarchy(dim, degree, [], level_fine, level_coarse, getattr(levels, "numbering", "lex"),
+                  getattr(levels, "orders", None))
     for L in levels:
         H.levels.append(Level(L.A.to_host(), L.bnd.cpu().numpy().view(U32),
                               None if L.P is None else L.P.to_host(), None if L.R is None else L.R.to_host()))
@@ -507,7 +552,8 @@ def read_matrix(path: str) -> Csr:
     rp = np.empty(rows.value + 1, U32)
     col = np.empty(nnz.value, U32)
     val = np.empty(nnz.value, F64)
-    capi.call("feat3_io_read_mtx" if mtx else "feat3_io_read_csr_binary", path.encode(), _p(rp), _p(col), _p(val))
+    capi.call("feat3_io_read_mtx" if mtx else "feat3_io_read_csr_binary", path.encode(), rows.value, nnz.value, _p(rp),
+              _p(col), _p(val))
     return Csr(rows.value, cols.value, rp, col, val)
 
 

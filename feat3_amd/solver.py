@@ -46,7 +46,11 @@ class MGPCG:
                  tol_rel: float = 1e-8, tol_abs: float = 0.0, tol_abs_low: float = 0.0,
                  min_iter: int = 0, max_iter: int = 1000, precond: str = "mg",
                  graph: bool = True, lanes: int = 0, sell_C: int = 64, sell_sigma: int = 1,
-                 options: Optional[dict] = None, cycle: str = "v"):
+                 options: Optional[dict] = None, cycle: str = "v", level_orders="auto"):
+        """level_orders: the solver's internal DOF order per level (feat3_mgpcg_set_level_order):
+        "auto" takes the hierarchy's own (problems.Hierarchy.orders of a FEAT-numbered hierarchy:
+        the lexicographic lattice order, which makes the level matrices banded), None keeps the
+        given numbering, or a list of arrays (finest first; None entries keep a level's)."""
         self.hierarchy = hierarchy
         h = ctypes.c_void_p()
         capi.call("feat3_mgpcg_create", ctypes.byref(h), INNER[inner], FORMAT[fmt])
@@ -68,6 +72,14 @@ class MGPCG:
             nb = L.bnd.numel() if on_device else L.bnd.size
             capi.call("feat3_mgpcg_push_level_device" if on_device else "feat3_mgpcg_push_level", h, A.rows,
                       _ptr(A.row_ptr), _ptr(A.col_ind), _ptr(A.val), nb, _ptr(L.bnd), nc, *pa)
+        if isinstance(level_orders, str):
+            if level_orders != "auto":
+                raise ValueError("level_orders must be 'auto', None or a list of arrays")
+            level_orders = getattr(hierarchy, "orders", None)
+        for i, o in enumerate(level_orders or []):
+            if o is not None:
+                oa = np.ascontiguousarray(o, np.uint32)
+                capi.call("feat3_mgpcg_set_level_order", h, i, _ptr(oa))
         capi.call("feat3_mgpcg_set_smoother", h, omega_jacobi, omega_richardson, pre, post)
         capi.call("feat3_mgpcg_set_tolerances", h, tol_rel, tol_abs, tol_abs_low, min_iter, max_iter)
         capi.call("feat3_mgpcg_set_precond", h, {"mg": 0, "none": 1}[precond])

@@ -12,9 +12,13 @@
  *  - All array arguments are device pointers unless stated otherwise; sizes are element
  *    counts.  Suffixes: f64/f32 = value type, u32/u64 = index type (FEAT's Index is u64,
  *    kernel/base_header.hpp:94-98), h16f32 = binary16 stored values with FP32 vectors.
- *  - No function allocates device memory per call, except the first call on a device of a
- *    reduction (dot/norm), which creates that device's reduction workspace once; the
- *    solver object allocates in feat3_mgpcg_init only.
+ *  - No function allocates device memory per call: feat3_hip_initialise creates each device's
+ *    reduction workspace (the dot/norm entry points return FEAT3_HIP_ERR_STATE on a device
+ *    that was not initialised); the solver objects allocate in their init only.
+ *  - The dot/norm entry points of one device share its workspace; they may be issued on any
+ *    streams: each waits for the previous one (an event recorded by every reduction), so
+ *    reductions on different streams are serialised against each other (not inside a stream
+ *    capture, whose graph orders its own nodes).
  *  - Not thread-safe by contract (one host thread drives one device), as the reference.
  *
  * The first group mirrors LAFEM::Arch::<Op><Mem::CUDA> (lafem/arch/apply.hpp:267-303 and the
@@ -244,7 +248,8 @@ int lafem_hip_convert_f32_h16(void* stream, float* dst, const void* src, uint64_
 /* ===================================================================================== */
 /* MemoryPool<Mem::HIP> / Runtime (memory_pool.hpp:53-377, runtime.cpp:33-144)             */
 /* ===================================================================================== */
-/* initialise: select the device and create its reduction workspace (memory_pool.cu:56-72) */
+/* initialise: select the device and create its reduction workspace (memory_pool.cu:56-72);
+ * required once per device before the dot/norm entry points */
 int feat3_hip_initialise(int device);
 /* finalise: reports the number of allocations still held (memory_pool.cu:74-95) */
 int feat3_hip_finalise(uint64_t* leaked);
@@ -282,7 +287,8 @@ typedef struct feat3_mgpcg feat3_mgpcg;
  * (SparseMatrixBanded layout, no column indices) for every square matrix with at most 64
  * distinct diagonals and at most 25% zero fill, SELL-C-sigma for the others.  The inner-precision
  * copies (1 and 2) of the level matrices, in every format, leave out the entries with
- * |a_ij| <= eps_f32^2 |a_ii|, which FP32 sums cannot resolve; FP64 copies keep every entry. */
+ * |a_ij| <= eps_f32^2 |a_ii|, which FP32 sums cannot resolve; FP64 copies keep every nonzero
+ * (the banded FP64 layout has no slot for a stored zero).  NaN values are kept in every copy. */
 int feat3_mgpcg_create(feat3_mgpcg** h, int inner_precision, int format);
 int feat3_mgpcg_destroy(feat3_mgpcg* h);
 /* one level (finest first), host arrays, copied: CSR matrix (n x n, u32 indices, FP64
@@ -301,6 +307,15 @@ int feat3_mgpcg_push_level_device(feat3_mgpcg* h, uint64_t n, const uint32_t* ro
                                   const double* val, uint64_t nfilt, const uint32_t* filter_idx, uint64_t nc,
                                   const uint32_t* p_row_ptr, const uint32_t* p_col_ind, const double* p_val,
                                   const uint32_t* r_row_ptr, const uint32_t* r_col_ind, const double* r_val);
+/* The solver's internal order of the DOFs of one level (0 = finest; no reference counterpart):
+ * order[i] = internal position of DOF i, a permutation of [0, n); host array, copied; set after
+ * that level's push_level and before init, at most once per level.  At init the level matrix and
+ * Dirichlet list and the transfers on both sides of the level are renumbered (host or device
+ * arrays alike); correct / apply / vcycle still take and return vectors in the caller's
+ * numbering (level 0's order is applied to b, x and def at entry and to x and cor at exit).
+ * For a structured mesh in FEAT's hierarchical numbering, the lexicographic lattice order
+ * (feat3_setup_lattice_order of the DOF nodes' coordinates) makes the level matrices banded. */
+int feat3_mgpcg_set_level_order(feat3_mgpcg* h, int level, const uint32_t* order);
 /* Jacobi damping, Richardson damping, pre/post smoothing steps (>= 1) */
 int feat3_mgpcg_set_smoother(feat3_mgpcg* h, double omega_jacobi, double omega_richardson, int pre, int post);
 /* IterativeSolver::set_tol_rel/set_tol_abs/set_min_iter/set_max_iter (iterative.hpp);
@@ -462,13 +477,16 @@ int feat3_setup_coarsen_cellfield(int dim, int level_fine, const double* kf, dou
 /* The reference's on-disk formats of SparseMatrixCSR (host, u32 indices): Matrix Market
  * coordinate real general/symmetric (sparse_matrix_csr.hpp:1125-1240, 1296-1340) and the binary
  * fm_csr container (Container::_serialise<double, uint64_t>, container.hpp:320-447).  *_sizes
- * reads the dimensions, read fills caller arrays (row_ptr rows + 1, col_ind and val nnz). */
+ * reads the dimensions, read fills caller arrays (row_ptr rows + 1, col_ind and val nnz) and
+ * fails unless the file still has those rows and nnz; header words and offsets of a binary file
+ * are checked before they are used (a malformed file is an error, never an out-of-range read). */
 int feat3_io_mtx_sizes(const char* path, uint64_t* rows, uint64_t* cols, uint64_t* nnz);
-int feat3_io_read_mtx(const char* path, uint32_t* row_ptr, uint32_t* col_ind, double* val);
+int feat3_io_read_mtx(const char* path, uint64_t rows, uint64_t nnz, uint32_t* row_ptr, uint32_t* col_ind, double* val);
 int feat3_io_write_mtx(const char* path, uint64_t rows, uint64_t cols, const uint32_t* row_ptr, const uint32_t* col_ind,
                        const double* val, int symmetric);
 int feat3_io_csr_binary_sizes(const char* path, uint64_t* rows, uint64_t* cols, uint64_t* nnz);
-int feat3_io_read_csr_binary(const char* path, uint32_t* row_ptr, uint32_t* col_ind, double* val);
+int feat3_io_read_csr_binary(const char* path, uint64_t rows, uint64_t nnz, uint32_t* row_ptr, uint32_t* col_ind,
+                             double* val);
 int feat3_io_write_csr_binary(const char* path, uint64_t rows, uint64_t cols, const uint32_t* row_ptr,
                               const uint32_t* col_ind, const double* val);
 /* DenseVector files (DenseVector::read_from / write_out, dense_vector.hpp:587-780), host memory;
@@ -498,6 +516,19 @@ int feat3_setup_tile_order(uint64_t rows, const uint32_t* row_ptr, const uint32_
 /* n doubles uniform in [a, b] from the xorshift64* stream of Random (util/random.hpp:62-110,
  * 259-280); seed 0 selects the reference's default seed */
 int feat3_setup_random_uniform(uint64_t seed, uint64_t n, double a, double b, double* out);
+
+/* Renumbered copy of a device CSR matrix (setup; no reference counterpart): old row i -> row
+ * row_order[i] (a permutation), old column j -> column col_order[j] (a permutation of the
+ * columns), NULL = identity; columns ascending in every row, values moved unchanged.  Device
+ * arrays; out_* sized like the input; synchronises the stream (the row pointers are summed on the
+ * host).  A row order that is not a permutation or a column out of range is an argument error. */
+int feat3_device_permute_csr(void* stream, uint64_t rows, uint64_t cols, const uint32_t* row_ptr,
+                             const uint32_t* col_ind, const double* val, const uint32_t* row_order,
+                             const uint32_t* col_order, uint32_t* out_row_ptr, uint32_t* out_col_ind, double* out_val);
+/* dst[order[i]] = src[i] (inverse = 0) or dst[i] = src[order[i]] (inverse = 1); device arrays,
+ * src != dst, enqueued on stream */
+int feat3_device_permute_vector(void* stream, uint64_t n, const uint32_t* order, int inverse, const double* src,
+                                double* dst);
 
 /* ===================================================================================== */
 /* FEAT's mesh refinement and DOF numbering (host)                                        */

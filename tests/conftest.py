@@ -17,5 +17,6 @@ def torch_cuda():
     torch = pytest.importorskip("torch")
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    import feat3_amd.capi  # noqa: F401  (loads the HIP library after torch)
+    import feat3_amd.capi as capi  # loads the HIP library after torch
+    capi.call("feat3_hip_initialise", 0)   # MemoryPool<Mem::HIP>::initialise: reduction workspace
     return torch

@@ -484,3 +484,30 @@ def test_memory_pool_refcount(T):
     assert np.array_equal(out, host_buf)
     capi.call("feat3_hip_free", p)
     assert capi.lib.feat3_hip_free(p) == 1  # released: unknown pointer
+
+
+def test_reductions_on_two_streams(T):
+    """Dots and norms issued back to back on two streams share the device's reduction workspace:
+    each waits for the previous one, so both results are those of the sequential oracle sums
+    (integer-valued inputs: exact), also when the first one is still running when the second is
+    issued (a large first reduction)."""
+    torch, capi = T
+    rng = np.random.default_rng(11)
+    n1, n2 = 1 << 24, 1 << 20
+    x1 = rng.integers(-8, 8, n1).astype(np.float64)
+    y1 = rng.integers(-8, 8, n1).astype(np.float64)
+    x2 = rng.integers(-8, 8, n2).astype(np.float64)
+    d1x, d1y, d2x = (torch.from_numpy(v).cuda() for v in (x1, y1, x2))
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for rep in range(3):
+        r1 = torch.zeros(1, dtype=torch.float64, device="cuda")
+        r2 = torch.zeros(1, dtype=torch.float64, device="cuda")
+        capi.call("lafem_hip_dot_f64", ctypes.c_void_p(s1.cuda_stream), vp(d1x), vp(d1y), n1, vp(r1))
+        capi.call("lafem_hip_norm2_f64", ctypes.c_void_p(s2.cuda_stream), vp(d2x), n2, vp(r2))
+        hv = ctypes.c_double()
+        capi.call("lafem_hip_dot_value_f64", ctypes.c_void_p(s1.cuda_stream), vp(d2x), vp(d2x), n2, ctypes.byref(hv))
+        torch.cuda.synchronize()
+        assert r1.item() == float(np.dot(x1.astype(np.int64), y1.astype(np.int64))), rep
+        assert r2.item() == float(np.sqrt(float(np.dot(x2.astype(np.int64), x2.astype(np.int64))))), rep
+        assert hv.value == float(np.dot(x2.astype(np.int64), x2.astype(np.int64))), rep

@@ -339,3 +339,25 @@ def test_fp32_level_copies_drop_unresolvable_couplings(torch, fmt):
     st, it = sd.correct(x, torch.from_numpy(b).cuda())
     sd.done()
     assert st == 2 and it == ig and np.array_equal(x.cpu().numpy(), xg)
+
+
+@pytest.mark.parametrize("fmt", ["csr", "sell", "banded"])
+@pytest.mark.parametrize("inner", ["f64", "f32", "f16"])
+@pytest.mark.parametrize("level", [0, 1])
+def test_nan_matrix_aborts_in_every_format(torch, fmt, inner, level):
+    """A NaN coupling in a level matrix (the finest: the outer operator and the V-cycle; the next
+    coarser: the V-cycle only) makes the solve report 'aborted' (iterative.hpp:757-758,794-795;
+    pcg.hpp:220-224,266-269), as the oracle does, in every level format and precision: no copy of
+    a level matrix leaves the non-finite value out."""
+    print(f"case fmt {fmt} inner {inner} level {level}", flush=True)
+    H = pb.hierarchy(2, 1, 5, 1)
+    A = H.levels[level].A
+    m = (2 ** (5 - level) + 1)
+    r = (m // 2) * m + m // 2                      # an interior row
+    ks = [k for k in range(A.row_ptr[r], A.row_ptr[r + 1]) if A.col_ind[k] != r]
+    A.val[ks[0]] = np.nan
+    b = pb.filtered_rhs(H, 0)
+    so, io, _, _ = solve_orc(H, b)
+    sg, ig, xg, _ = solve_gpu(torch, H, b, inner=inner, fmt=fmt)
+    assert so == 3
+    assert sg == 3, f"status {sg} after {ig} iterations"

@@ -2,6 +2,7 @@
 (Container::_serialise<double, uint64_t>, container.hpp:320-447) checked byte for byte against an
 independent restatement of its layout here, and Matrix Market (sparse_matrix_csr.hpp:1125-1340):
 round trips, comments, symmetric expansion, repeated entries (first kept), bad files rejected."""
+import ctypes
 import math
 import struct
 
@@ -153,3 +154,59 @@ def test_vector_mtx_reader_rules(tmp_path):
             pb.read_vector(f, "mtx")
     with pytest.raises(Exception):
         pb.read_vector(str(tmp_path / "missing.dv"), "dv")
+
+
+@pytest.mark.parametrize("word,value", [
+    (9, 2),                 # two scalar values: not a SparseMatrixCSR container
+    (9, 1 << 62),           # an offset that would wrap
+    (10, 1 << 62),          # nnz beyond any array
+    (11, 5),                # index array 0 size differs from nnz
+    (14, (1 << 63) + 3),    # rows near 2^64: rows + 1 wraps
+    (16, 7),                # scalar nnz differs
+])
+def test_binary_reader_rejects_crafted_headers(tmp_path, word, value):
+    """Header words come from the file: each is checked before it is used, so a crafted header is
+    an error (code 1), never a read outside the file's words."""
+    from feat3_amd.capi import Feat3HipError
+    A = pb.hierarchy(2, 1, 2, 2).levels[0].A
+    raw = bytearray(reference_bytes(A))
+    struct.pack_into("<Q", raw, 8 * word, value)
+    f = str(tmp_path / "bad.csr")
+    with open(f, "wb") as fh:
+        fh.write(bytes(raw))
+    with pytest.raises(Feat3HipError):
+        pb.read_matrix(f)
+
+
+def test_binary_reader_rejects_truncated_and_changed_files(tmp_path):
+    from feat3_amd import capi
+    from feat3_amd.capi import Feat3HipError
+    A = pb.hierarchy(2, 1, 2, 2).levels[0].A
+    raw = reference_bytes(A)
+    f = str(tmp_path / "t.csr")
+    for cut in (8, 100, len(raw) - 8):
+        with open(f, "wb") as fh:
+            fh.write(raw[:cut])
+        with pytest.raises(Feat3HipError):
+            pb.read_matrix(f)
+    # sizes read from one file, then the file replaced by a larger matrix before the read: the
+    # read refuses instead of writing past the arrays sized for the first one
+    with open(f, "wb") as fh:
+        fh.write(raw)
+    rows, cols, nnz = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    capi.call("feat3_io_csr_binary_sizes", f.encode(), ctypes.byref(rows), ctypes.byref(cols), ctypes.byref(nnz))
+    B = pb.hierarchy(2, 1, 3, 3).levels[0].A
+    pb.write_matrix(f, B)
+    rp = np.empty(rows.value + 1, np.uint32)
+    col = np.empty(nnz.value, np.uint32)
+    val = np.empty(nnz.value)
+    with pytest.raises(Feat3HipError):
+        capi.call("feat3_io_read_csr_binary", f.encode(), rows.value, nnz.value, ctypes.c_void_p(rp.ctypes.data),
+                  ctypes.c_void_p(col.ctypes.data), ctypes.c_void_p(val.ctypes.data))
+    g = str(tmp_path / "t.mtx")
+    pb.write_matrix(g, A)
+    capi.call("feat3_io_mtx_sizes", g.encode(), ctypes.byref(rows), ctypes.byref(cols), ctypes.byref(nnz))
+    pb.write_matrix(g, B)
+    with pytest.raises(Feat3HipError):
+        capi.call("feat3_io_read_mtx", g.encode(), rows.value, nnz.value, ctypes.c_void_p(rp.ctypes.data),
+                  ctypes.c_void_p(col.ctypes.data), ctypes.c_void_p(val.ctypes.data))

@@ -120,6 +120,21 @@ def test_blas1_exact_on_integers_and_alias_semantics():
     ci = np.zeros(n)
     L.orc_component_invert_f64(po.P(ci), po.P(z), 0.5, n)
     assert np.array_equal(ci, 0.5 / z)
+    # scale (dense_vector.hpp:955-963, scale_generic.hpp: r_i = s * x_i) and component_product
+    # (component_product_generic.hpp:21-51: r_i = x_i * y_i): one rounding each, so numpy's
+    # elementwise products must agree bit for bit on arbitrary values (a dropped or doubled
+    # factor, or a wrong index, would not)
+    u = rng.uniform(-1e3, 1e3, n)
+    w = rng.uniform(-1e3, 1e3, n)
+    sc = np.zeros(n)
+    L.orc_scale_f64(po.P(sc), po.P(u), -3.7, n)
+    assert np.array_equal(sc, u * -3.7)
+    u0 = u.copy()
+    L.orc_scale_f64(po.P(u), po.P(u), 0.5, n)   # in place (r == x)
+    assert np.array_equal(u, u0 * 0.5)
+    cp = np.zeros(n)
+    L.orc_component_product_f64(po.P(cp), po.P(u), po.P(w), n)
+    assert np.array_equal(cp, u * w)
 
 
 def test_unit_filter_and_transpose():
