@@ -365,6 +365,66 @@ namespace
     return rt::check_launch();
   }
 
+  // BCSR with any block shape bh x bw up to 9 x 9 (the reference's generic CUDA kernel takes
+  // BlockHeight < 10 and any width, apply.hpp:277-281, apply.cu:203-231): the block loops are
+  // unrolled to 9 and predicated on the runtime shape, so the partial sums stay in registers
+  constexpr int kMaxBlock = 9;
+  template<int G, typename DT, typename IT>
+  __global__ __launch_bounds__(kBlock) void k_csrb_any(DT* __restrict__ r, DT a, const DT* __restrict__ x, DT b,
+                                                       const DT* y, const DT* __restrict__ val, const IT* __restrict__ col,
+                                                       const IT* __restrict__ rp, u64 rows, int bh, int bw, int use_y)
+  {
+    const unsigned lb = dev::xcd_remap(blockIdx.x, gridDim.x);
+    const u64 row = (u64(lb) * kBlock + threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    if(row >= rows) return;
+    const u64 bb = u64(bh) * u64(bw);
+    DT s[kMaxBlock];
+#pragma unroll
+    for(int h = 0; h < kMaxBlock; ++h) s[h] = DT(0);
+    for(IT i = rp[row] + IT(lane); i < rp[row + 1]; i += IT(G))
+    {
+      const DT* v = val + u64(i) * bb;
+      const DT* xb = x + u64(col[i]) * u64(bw);
+      DT xv[kMaxBlock];
+#pragma unroll
+      for(int w = 0; w < kMaxBlock; ++w) xv[w] = w < bw ? xb[w] : DT(0);
+#pragma unroll
+      for(int h = 0; h < kMaxBlock; ++h)
+#pragma unroll
+        for(int w = 0; w < kMaxBlock; ++w)
+          if(h < bh && w < bw) s[h] += v[h * bw + w] * xv[w];
+    }
+#pragma unroll
+    for(int h = 0; h < kMaxBlock; ++h)
+      s[h] = dev::group_sum<G>(s[h]);
+    if(lane == 0)
+    {
+#pragma unroll
+      for(int h = 0; h < kMaxBlock; ++h)
+        if(h < bh) r[row * u64(bh) + h] = use_y ? (s[h] * a) + (b * y[row * u64(bh) + h]) : (s[h] * a);
+    }
+  }
+
+  template<typename DT, typename IT>
+  int launch_csrb_any(hipStream_t s, DT* r, DT a, const DT* x, DT b, const DT* y, const DT* val, const IT* col,
+                      const IT* rp, u64 rows, u64 nnzb, int bh, int bw)
+  {
+    if(rows == 0) return 0;
+    const int use_y = std::abs(double(b)) < double(std::numeric_limits<DT>::epsilon()) ? 0 : 1;
+    const int g = lanes_for(rows, nnzb);
+    const unsigned nb = blocks_for(rows * u64(g));
+#define F3_CSRB_CASE(GG) \
+  case GG: k_csrb_any<GG, DT, IT><<<nb, kBlock, 0, s>>>(r, a, x, b, y, val, col, rp, rows, bh, bw, use_y); break;
+    switch(g)
+    {
+      F3_CSRB_CASE(4) F3_CSRB_CASE(8) F3_CSRB_CASE(16) F3_CSRB_CASE(32) F3_CSRB_CASE(64)
+      default: return rt::kErrArg;
+    }
+#undef F3_CSRB_CASE
+    return rt::check_launch();
+  }
+
   // scalar CSR times a vector of BS-blocks (csrsb, apply_generic.hpp:156-181)
   template<int G, int BS, typename DT, typename VT, typename IT>
   __global__ __launch_bounds__(kBlock) void k_csrsb(DT* __restrict__ r, DT a, const DT* __restrict__ x, DT b,
@@ -747,15 +807,19 @@ F3_DEF_BANDED(f32_u64, float, u64)
   {                                                                                                           \
     (void)columns;                                                                                            \
     if(!r || !row_ptr || (used_elements > 0 && (!x || !val || !col_ind))) return rt::kErrArg;                 \
-    if(block_height != block_width) return rt::kErrArg;                                                       \
-    switch(block_height)                                                                                      \
-    {                                                                                                         \
-      case 1: return launch_csrb<1, 1, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
-      case 2: return launch_csrb<2, 2, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
-      case 3: return launch_csrb<3, 3, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
-      case 4: return launch_csrb<4, 4, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
-      default: return rt::kErrArg;                                                                            \
-    }                                                                                                         \
+    if(block_height < 1 || block_height > kMaxBlock || block_width < 1 || block_width > kMaxBlock)           \
+      return rt::kErrArg;                                                                                     \
+    if(block_height == block_width)                                                                           \
+      switch(block_height)                                                                                    \
+      {                                                                                                       \
+        case 1: return launch_csrb<1, 1, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
+        case 2: return launch_csrb<2, 2, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
+        case 3: return launch_csrb<3, 3, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
+        case 4: return launch_csrb<4, 4, DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements); \
+        default: break;                                                                                       \
+      }                                                                                                       \
+    return launch_csrb_any<DT, IT>(S_, r, a, x, b, y, val, col_ind, row_ptr, rows, used_elements, block_height, \
+                                   block_width);                                                              \
   }
 F3_DEF_CSRB(f64_u32, double, u32)
 F3_DEF_CSRB(f64_u64, double, u64)
@@ -866,7 +930,9 @@ F3_DEF_BLAS1(f32, float)
                                     const IT* row_ptr, u64 rows, int block_height, int block_width)           \
   {                                                                                                           \
     if(rows == 0) return 0;                                                                                   \
-    if(!diag || !val || !col_ind || !row_ptr || block_height < 1 || block_height > 16) return rt::kErrArg;    \
+    if(!diag || !val || !col_ind || !row_ptr || block_height < 1 || block_height > 16 ||                     \
+       block_width != block_height)                                                                           \
+      return rt::kErrArg;   /* the diagonal of square blocks only */                                          \
     k_diag_csrb<DT, IT><<<blocks_for(rows), kBlock, 0, S_>>>(diag, val, col_ind, row_ptr, rows, block_height, \
                                                              block_width);                                    \
     return rt::check_launch();                                                                                \

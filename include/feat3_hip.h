@@ -126,8 +126,9 @@ int lafem_hip_apply_banded_f32_u64(void* stream, float* r, float a, const float*
                                    const float* val, const uint64_t* offsets, uint64_t num_of_offsets, uint64_t rows,
                                    uint64_t columns);
 
-/* Apply::csrb (apply.hpp:277, CUDA apply.cu:203-231): BCSR with row-major square blocks,
- * block_height == block_width in {1,2,3,4}. */
+/* Apply::csrb (apply.hpp:277-281, CUDA apply.cu:203-231): BCSR with row-major blocks of
+ * block_height x block_width, each in 1..9 (the reference's CUDA kernel takes BlockHeight < 10);
+ * square blocks of size 1..4 run shape-specialised kernels, the other shapes a generic one. */
 int lafem_hip_apply_csrb_f64_u32(void* stream, double* r, double a, const double* x, double b, const double* y,
                                  const double* val, const uint32_t* col_ind, const uint32_t* row_ptr, uint64_t rows,
                                  uint64_t columns, uint64_t used_elements, int block_height, int block_width);
@@ -187,6 +188,8 @@ int lafem_hip_scale_f32(void* stream, float* r, const float* x, float s, uint64_
 /* ===================================================================================== */
 /* Diagonal::{csr,ell,csrb} (diagonal.hpp; CUDA diagonal.cu:20-161)                       */
 /* ===================================================================================== */
+/* a missing diagonal entry reads 0; csrb: the diagonals of the square diagonal blocks
+ * (block_height == block_width, 1..16) */
 int lafem_hip_diagonal_csr_f64_u32(void* stream, double* diag, const double* val, const uint32_t* col_ind,
                                    const uint32_t* row_ptr, uint64_t rows);
 int lafem_hip_diagonal_csr_f64_u64(void* stream, double* diag, const double* val, const uint64_t* col_ind,

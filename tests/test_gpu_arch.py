@@ -511,3 +511,40 @@ def test_reductions_on_two_streams(T):
         assert r1.item() == float(np.dot(x1.astype(np.int64), y1.astype(np.int64))), rep
         assert r2.item() == float(np.sqrt(float(np.dot(x2.astype(np.int64), x2.astype(np.int64))))), rep
         assert hv.value == float(np.dot(x2.astype(np.int64), x2.astype(np.int64))), rep
+
+
+@pytest.mark.parametrize("bh,bw", [(2, 3), (3, 1), (5, 7), (9, 9), (9, 2), (1, 9), (6, 6)])
+@pytest.mark.parametrize("suf", ["f64_u32", "f64_u64", "f32_u32"])
+def test_csrb_any_block_shape(T, bh, bw, suf):
+    """Apply::csrb for every block shape up to 9 x 9 (apply.hpp:277-281) against the oracle."""
+    torch, capi = T
+    rng = np.random.default_rng(bh * 100 + bw)
+    rows, cols = 500, 310
+    itype = np.uint64 if suf.endswith("u64") else np.uint32
+    rp, col, _ = random_csr(rng, rows, cols, 12, itype=itype)
+    nnzb = int(rp[-1])
+    val = rng.uniform(-1, 1, nnzb * bh * bw)
+    x = rng.uniform(-1, 1, cols * bw)
+    y = rng.uniform(-1, 1, rows * bh)
+    ref = y.copy()
+    po.lib().orc_csrb_apply_f64_u32(po.P(ref), 0.75, po.P(x), -2.0, po.P(ref), po.P(val),
+                                    po.P(col.astype(np.uint32)), po.P(rp.astype(np.uint32)), rows, bh, bw)
+    dt = np.float32 if suf.startswith("f32") else np.float64
+    idt = np.int64 if itype == np.uint64 else np.int32
+    dr = dev(torch, y.astype(dt))
+    capi.call(f"lafem_hip_apply_csrb_{suf}", None, vp(dr), 0.75, vp(dev(torch, x.astype(dt))), -2.0, vp(dr),
+              vp(dev(torch, val.astype(dt))), vp(dev(torch, col.astype(idt))), vp(dev(torch, rp.astype(idt))), rows, cols,
+              nnzb, bh, bw)
+    torch.cuda.synchronize()
+    err = rel(host(dr), ref)
+    assert err <= (1e-13 if dt == np.float64 else 1e-5), err
+
+
+def test_csrb_rejects_shapes_beyond_9(T):
+    torch, capi = T
+    d = torch.zeros(16, dtype=torch.float64, device="cuda")
+    i = torch.zeros(2, dtype=torch.int32, device="cuda")
+    for bh, bw in ((10, 2), (2, 10), (0, 3)):
+        with pytest.raises(capi.Feat3HipError):
+            capi.call("lafem_hip_apply_csrb_f64_u32", None, vp(d), 1.0, vp(d), 0.0, vp(d), vp(d), vp(i), vp(i), 1, 1, 0,
+                      bh, bw)

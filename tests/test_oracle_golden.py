@@ -427,3 +427,25 @@ def test_banded_apply_brute_force_and_csr_identity():
     L.orc_banded_apply_f64_u32(po.P(rb), 1.0, po.P(x), 0.0, po.P(x), po.P(val), po.P(o), o.size, A.rows, A.rows)
     po.csr_apply(rc, 1.0, x, 0.0, x, A.val, A.col_ind, A.row_ptr, A.rows, A.rows)
     assert np.array_equal(rb, rc)
+
+
+@pytest.mark.parametrize("bh,bw", [(2, 3), (3, 1), (5, 7), (9, 9), (9, 2), (1, 9)])
+def test_bcsr_non_square_brute_force(bh, bw):
+    """apply_generic.hpp:121-154 for any block shape: the oracle's csrb against the dense product
+    of the assembled block matrix (nb x mb blocks of bh x bw), with alpha/beta."""
+    rng = np.random.default_rng(bh * 10 + bw)
+    nb, mb = 13, 9
+    pat = rng.uniform(size=(nb, mb)) < 0.35
+    rp = np.concatenate([[0], np.cumsum(pat.sum(1))]).astype(np.uint32)
+    col = np.nonzero(pat)[1].astype(np.uint32)
+    blocks = rng.uniform(-1, 1, (col.size, bh, bw))
+    D = np.zeros((nb * bh, mb * bw))
+    for i in range(nb):
+        for k in range(rp[i], rp[i + 1]):
+            D[i * bh:(i + 1) * bh, col[k] * bw:(col[k] + 1) * bw] = blocks[k]
+    x = rng.uniform(-1, 1, mb * bw)
+    y = rng.uniform(-1, 1, nb * bh)
+    r = y.copy()
+    bflat = blocks.ravel().copy()
+    po.lib().orc_csrb_apply_f64_u32(po.P(r), 0.5, po.P(x), -1.5, po.P(r), po.P(bflat), po.P(col), po.P(rp), nb, bh, bw)
+    assert np.allclose(r, 0.5 * (D @ x) - 1.5 * y, rtol=1e-14, atol=1e-14)
