@@ -36,6 +36,7 @@ namespace mg
     const VT_* val;
     u64 rows;
     __device__ __forceinline__ u64 row(u64 p) const { return p; }
+    __device__ __forceinline__ u64 block_of(u64 b) const { return b; }
     __device__ __forceinline__ DT_ row_sum(u64 p, int lane, const DT_* __restrict__ x) const
     {
       const u32 end = rp[p + 1];
@@ -65,6 +66,7 @@ namespace mg
     u64 rows;
     u64 R;
     __device__ __forceinline__ u64 row(u64 p) const { return p; }
+    __device__ __forceinline__ u64 block_of(u64 b) const { return b; }
   };
 
   template<typename T> struct IsStream { static constexpr bool value = false; };
@@ -97,6 +99,7 @@ namespace mg
     const VT_* val;
     u64 rows, C;
     __device__ __forceinline__ u64 row(u64 p) const { return PERM_ ? u64(perm[p]) : p; }
+    __device__ __forceinline__ u64 block_of(u64 b) const { return b; }
     __device__ __forceinline__ DT_ row_sum(u64 p, int, const DT_* __restrict__ x) const
     {
       const u64 ch = p / C;
@@ -144,7 +147,11 @@ namespace mg
     u32 ndiag;
     const u32* sx = nullptr;   // SYM_: stored diagonal of each diagonal
     u32 nds = 0;               // SYM_: stored diagonals
+    // optional processing order of the row ranges (see DevCsr::make_tiles): logical block b of
+    // an XCD works on range tile[b]
+    const u32* tile = nullptr;
     __device__ __forceinline__ u64 row(u64 p) const { return p; }
+    __device__ __forceinline__ u64 block_of(u64 b) const { return tile ? u64(tile[b]) : b; }
     __device__ __forceinline__ DT_ row_sum(u64 p, int, const DT_* __restrict__ x) const
     {
       const dev::cptr<u32> co = dev::to_const(off);
@@ -267,7 +274,7 @@ namespace mg
   __device__ __forceinline__ void for_rows(const Rows_& A, u64 per, const DT_* __restrict__ x, F&& f)
   {
     constexpr int G = Rows_::G;
-    const u64 b = dev::xcd_remap(blockIdx.x, gridDim.x);
+    const u64 b = A.block_of(dev::xcd_remap(blockIdx.x, gridDim.x));
     const u64 beg = b * per;
     const u64 end = beg + per < A.rows ? beg + per : A.rows;
     if constexpr(IsStream<Rows_>::value)

@@ -259,6 +259,10 @@ namespace
     u32 nds = 0;
     DevMem bsx;
     bool allow_sym = true;    // symmetric storage allowed (option "symmetric_band")
+    // processing order of the row ranges (banded matrices): tile[logical block] = the row range
+    // it evaluates, for the grid of tile_blocks blocks (0: ascending order)
+    DevMem tile;
+    unsigned tile_blocks = 0;
 
     // thin >= 0: upload a compact copy with the entries band_keeps(v, row diagonal, thin) only:
     // thin = 0 drops the stored exact zeros (the transfer matrices, whose two-level pattern holds
@@ -628,6 +632,50 @@ namespace
       return rt::kOk;
     }
 
+    // the launch grid of the banded row kernels (with_rows)
+    mg::RowGrid band_grid() const
+    {
+      return rows_per_block ? mg::row_grid_fixed(rows, rows_per_block) : mg::row_grid(rows, 1);
+    }
+
+    // Plane-strip order of the row ranges of a banded matrix.  Each XCD evaluates a contiguous
+    // set of row ranges (dev::xcd_remap), in ascending order by default: with ~256 blocks in
+    // flight per XCD its working set spans several node planes, and a row's reads at the
+    // mirrored (symmetric storage) and gathered positions one plane below miss the 4 MB L2
+    // about one time in five (3D Q1 L7 FP64 outer product: 1.22x its stored bytes).  Here the
+    // ranges of an XCD are taken strip by strip (range index modulo the plane length in ranges)
+    // and, within a strip, plane by plane, so that the range one plane below was evaluated just
+    // before.  The plane length is the band's largest offset.  Rows are the same; only the order
+    // in which blocks take them changes (the reductions sum their block partials in block order).
+    int make_tiles(hipStream_t s)
+    {
+      tile_blocks = 0;
+      if(!banded || ndiag == 0) return rt::kOk;
+      std::vector<u32> ho(ndiag);
+      F3_CHECK(hipMemcpy(ho.data(), boff.p, sizeof(u32) * ndiag, hipMemcpyDeviceToHost));
+      const std::int64_t omax = std::int64_t(ho.back()) - std::int64_t(rows) + 1;
+      const mg::RowGrid g = band_grid();
+      const u64 nb = g.blocks;
+      const u64 tp = std::max<u64>(1, u64((double(omax) + 0.5 * double(g.per)) / double(g.per)));
+      if(omax <= 0 || tp < 2 || nb < 16) return rt::kOk;
+      std::vector<u32> order(nb);
+      const u64 q = nb / 8, r = nb % 8;
+      for(u64 x = 0; x < 8; ++x)
+      {
+        const u64 start = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+        const u64 cnt = q + (x < r ? 1 : 0);
+        std::vector<u32> t(cnt);
+        for(u64 i = 0; i < cnt; ++i) t[i] = u32(start + i);
+        std::stable_sort(t.begin(), t.end(), [&](u32 a, u32 b) { return a % tp < b % tp; });
+        for(u64 i = 0; i < cnt; ++i) order[start + i] = t[i];
+      }
+      F3_RC(tile.alloc(sizeof(u32) * nb));
+      F3_CHECK(hipMemcpyAsync(tile.p, order.data(), sizeof(u32) * nb, hipMemcpyHostToDevice, s));
+      F3_CHECK(hipStreamSynchronize(s));
+      tile_blocks = unsigned(nb);
+      return rt::kOk;
+    }
+
     // bytes of one row-evaluation pass over this matrix as stored (values + indices + row
     // pointers / chunk descriptors, SELL padding included), excluding the vector traffic
     u64 matrix_bytes() const
@@ -649,12 +697,21 @@ namespace
     auto grid = [&](int G) { return m.rows_per_block ? mg::row_grid_fixed(m.rows, m.rows_per_block) : mg::row_grid(m.rows, G); };
     if(m.banded)
     {
+      const mg::RowGrid g = grid(1);
+      const u32* tile = m.tile_blocks == g.blocks ? m.tile.template as<u32>() : nullptr;
       if(m.bsym)
-        f(mg::BandedRows<DT, VT, true>{m.boff.template as<u32>(), m.bval.template as<VT>(), m.rows, m.cols, m.ndiag,
-                                       m.bsx.template as<u32>(), m.nds},
-          grid(1));
+      {
+        mg::BandedRows<DT, VT, true> A{m.boff.template as<u32>(), m.bval.template as<VT>(), m.rows, m.cols, m.ndiag,
+                                       m.bsx.template as<u32>(), m.nds};
+        A.tile = tile;
+        f(A, g);
+      }
       else
-        f(mg::BandedRows<DT, VT>{m.boff.template as<u32>(), m.bval.template as<VT>(), m.rows, m.cols, m.ndiag}, grid(1));
+      {
+        mg::BandedRows<DT, VT> A{m.boff.template as<u32>(), m.bval.template as<VT>(), m.rows, m.cols, m.ndiag};
+        A.tile = tile;
+        f(A, g);
+      }
       return rt::kOk;
     }
     if(m.sell)
@@ -798,6 +855,9 @@ namespace
     // FP64 bands that equal their mirror image bit for bit store only the center and upper
     // diagonals (0: always the full band; the results are identical either way)
     bool symmetric_band = true;
+    // processing order of the row ranges of banded matrices (DevCsr::make_tiles): 0 ascending,
+    // 1 plane strips for the FP64 outer matrix, 2 plane strips for every banded matrix
+    int plane_order = 1;
     double tol_rel = 1e-8, tol_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16), tol_abs_low = 0.0;
     double div_rel = 1.0 / 2.220446049250313e-16, div_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16);
     double stag_rate = 0.95;
@@ -813,6 +873,7 @@ namespace
     virtual int history(double* out, u64 cap, u64* len) = 0;
     virtual int time_kernel(int which, int reps, double* avg_ms, u64* bytes) = 0;
     virtual int info(u64* out, int cap) = 0;
+    virtual int statistics(int reps, double* out, int cap) = 0;
     virtual bool set_tolerances(const Config& c) = 0;
   };
 
@@ -855,6 +916,7 @@ namespace
       for(auto& e : ev) if(e) (void)hipEventDestroy(e);
       if(t0) (void)hipEventDestroy(t0);
       if(t1) (void)hipEventDestroy(t1);
+      for(hipEvent_t e : stat_ev) (void)hipEventDestroy(e);
     }
 
     const DevCsr<double>& outer_matrix() const { return *A64; }
@@ -985,6 +1047,14 @@ namespace
           F3_RC(A64own.upload(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, -1.0, cfg.index16, want_banded));
         A64 = &A64own;
       }
+      // plane-strip order of the banded row ranges: the FP64 outer matrix (1), every level (2)
+      if(cfg.plane_order >= 1)
+      {
+        if(shared) F3_RC(L.front()->A.make_tiles(s));
+        else F3_RC(A64own.make_tiles(s));
+      }
+      if(cfg.plane_order == 2)
+        for(std::size_t i = shared ? 1 : 0; i < L.size(); ++i) F3_RC(L[i]->A.make_tiles(s));
       F3_RC(r.alloc(sizeof(double) * n0));
       F3_RC(p.alloc(sizeof(double) * n0));
       F3_RC(q.alloc(sizeof(double) * n0));
@@ -1286,14 +1356,20 @@ namespace
       return rt::kOk;
     }
 
-    int enqueue_vcycle(const double* def, const int* status)
+    // evs (statistics only): evs[0] is recorded before the first operation and evs[k + 1] after
+    // operation k, so that each operation's time is the span between two events
+    int enqueue_vcycle(const double* def, const int* status, const std::vector<hipEvent_t>* evs = nullptr)
     {
       const std::size_t nl = L.size();
       std::vector<DT*> cur(nl, nullptr), oth(nl, nullptr);
+      if(evs) F3_CHECK(hipEventRecord((*evs)[0], s));
+      std::size_t k = 0;
       for(const CycleOp& c : cycle_ops)
       {
         const std::size_t i = c.lvl;
         Level<DT, VT>& f = *L[i];
+        if(evs && k > 0) F3_CHECK(hipEventRecord((*evs)[k], s));
+        ++k;
         switch(c.op)
         {
           case Op::entry:
@@ -1352,8 +1428,102 @@ namespace
             break;
         }
       }
+      if(evs) F3_CHECK(hipEventRecord((*evs)[k], s));
       if(cur[0] != L[0]->xfinal) return rt::kErrState;
       return rt::check_launch();
+    }
+
+    // ---- statistics (MultiGridHierarchy timings, multigrid.hpp:456-479,858-956) ------------
+    // Per level: time of smoothing, defect, grid transfer (restriction and prolongation, counted
+    // on the finer level as the reference does) and coarse solve per cycle, the number of each
+    // operation per cycle, and the cycle's algorithmic bytes and flops on that level.  Fused
+    // operations count where their main work is: the entry (conversion + first step) and the top
+    // copy as smoothing on level 0, a restriction with the next level's first smoothing step as
+    // transfer (its step is counted as a smoothing step of the coarser level), and a segment of
+    // the single-workgroup coarse tail as the coarse solve of the tail's top level (one kernel:
+    // the levels inside it report counts, bytes and flops, their time is that of the segment).
+    // Cycles are launched operation by operation with an event after each; the solve itself
+    // is not touched (its graphs record no events).
+    std::vector<hipEvent_t> stat_ev;
+    int statistics(int reps, double* out, int cap) override
+    {
+      const std::size_t nl = L.size(), nops = cycle_ops.size();
+      const int need = 3 + 10 * int(nl);
+      if(reps < 1 || !out || cap < need) return rt::kErrArg;
+      while(stat_ev.size() < nops + 1)
+      {
+        hipEvent_t e = nullptr;
+        F3_CHECK(hipEventCreate(&e));
+        stat_ev.push_back(e);
+      }
+      std::vector<double> t(4 * nl, 0.0), cnt(4 * nl, 0.0), by(nl, 0.0), fl(nl, 0.0);
+      F3_RC(enqueue_vcycle(r.template as<double>(), nullptr));   // warm-up
+      F3_CHECK(hipStreamSynchronize(s));
+      double total = 0.0;
+      for(int rep = 0; rep < reps; ++rep)
+      {
+        F3_RC(enqueue_vcycle(r.template as<double>(), nullptr, &stat_ev));
+        F3_CHECK(hipStreamSynchronize(s));
+        float ms = 0.f;
+        F3_CHECK(hipEventElapsedTime(&ms, stat_ev[0], stat_ev[nops]));
+        total += ms;
+        for(std::size_t k = 0; k < nops; ++k)
+        {
+          F3_CHECK(hipEventElapsedTime(&ms, stat_ev[k], stat_ev[k + 1]));
+          const CycleOp& c = cycle_ops[k];
+          int cat = 0;   // 0 smooth, 1 defect, 2 transfer, 3 coarse
+          switch(c.op)
+          {
+            case Op::entry: case Op::sweep: case Op::copy_top: cat = 0; break;
+            case Op::defect: cat = 1; break;
+            case Op::rest: case Op::rest_step: case Op::prol: cat = 2; break;
+            case Op::bottom: cat = 3; break;
+          }
+          t[4 * c.lvl + cat] += double(ms);
+        }
+      }
+      // counts, bytes and flops of the cycle's mathematical operations (the full schedule, tail
+      // levels included)
+      const u64 vb = sizeof(DT);
+      for(const CycleOp& c : cycle_schedule())
+      {
+        const std::size_t i = c.lvl;
+        const Level<DT, VT>& f = *L[i];
+        const double n = double(f.n), nz = double(f.A.nnz);
+        const double nc = i + 1 < nl ? double(L[i + 1]->n) : 0.0;
+        switch(c.op)
+        {
+          case Op::entry: cnt[4 * i] += 1; by[i] += n * (8 + 3 * vb); fl[i] += n; break;
+          case Op::sweep:
+            cnt[4 * i] += 1; by[i] += double(f.A.matrix_bytes()) + n * vb * 4; fl[i] += 2 * nz + 3 * n; break;
+          case Op::defect:
+            cnt[4 * i + 1] += 1; by[i] += double(f.A.matrix_bytes()) + n * vb * 3; fl[i] += 2 * nz + n; break;
+          case Op::rest: case Op::rest_step:
+            cnt[4 * i + 2] += 1;
+            by[i] += double(f.R.matrix_bytes()) + n * vb + nc * vb * (c.op == Op::rest ? 1 : 3);
+            fl[i] += 2 * double(f.R.nnz);
+            if(c.op == Op::rest_step) { cnt[4 * (i + 1)] += 1; fl[i + 1] += nc; }
+            break;
+          case Op::prol:
+            cnt[4 * i + 2] += 1; by[i] += double(f.P.matrix_bytes()) + nc * vb + n * vb * 2; fl[i] += 2 * double(f.P.nnz) + n;
+            break;
+          case Op::bottom:
+            cnt[4 * i + 3] += 1; by[i] += double(ncoarse * ncoarse * 8) + n * vb * 2; fl[i] += 2 * double(ncoarse * ncoarse);
+            break;
+          case Op::copy_top: break;
+        }
+      }
+      out[0] = double(nl);
+      out[1] = total / reps;
+      out[2] = double(nops);
+      for(std::size_t i = 0; i < nl; ++i)
+      {
+        double* o = out + 3 + 10 * i;
+        for(int c = 0; c < 4; ++c) { o[c] = t[4 * i + c] / reps; o[4 + c] = cnt[4 * i + c]; }
+        o[8] = by[i];
+        o[9] = fl[i];
+      }
+      return rt::kOk;
     }
 
     int sweep(Level<DT, VT>& f, DT* xin, DT* xout, const int* status)
@@ -2002,6 +2172,7 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
     h->cfg.transfer_lanes = int(value);
   else if(k == "transfer_rows_per_block" && value >= 0) h->cfg.transfer_rows_per_block = u64(value);
   else if(k == "symmetric_band") h->cfg.symmetric_band = value != 0.0;
+  else if(k == "plane_order" && (value == 0 || value == 1 || value == 2)) h->cfg.plane_order = int(value);
   else if(k == "cycle" && (value == 0 || value == 1 || value == 2)) h->cfg.cycle = int(value);
   else if(k == "rows_per_block" && value >= 0) h->cfg.rows_per_block = u64(value);
   else if(k == "transfer_sigma" && value >= 1) h->cfg.transfer_sigma = u64(value);
@@ -2111,6 +2282,12 @@ int feat3_mgpcg_info(feat3_mgpcg* h, u64* out, int cap)
 {
   if(!h || !h->eng || !out) return rt::kErrState;
   return h->eng->info(out, cap);
+}
+
+int feat3_mgpcg_statistics(feat3_mgpcg* h, int reps, double* out, int cap)
+{
+  if(!h || !h->eng) return rt::kErrState;
+  return h->eng->statistics(reps, out, cap);
 }
 
 } // extern "C"

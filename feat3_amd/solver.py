@@ -142,6 +142,19 @@ class MGPCG:
         capi.call("feat3_mgpcg_time_kernel", self._h, which, reps, ctypes.byref(ms), ctypes.byref(nb))
         return ms.value, nb.value
 
+    def statistics(self, reps: int = 10) -> dict:
+        """Per-level cycle statistics (feat3_mgpcg_statistics): times in ms per cycle and counts,
+        bytes, flops per cycle, lists indexed by level (finest first)."""
+        nl = self.info()["levels"]
+        out = np.zeros(3 + 10 * nl)
+        capi.call("feat3_mgpcg_statistics", self._h, reps, _ptr(out), out.size)
+        per = out[3:].reshape(nl, 10)
+        keys = ["time_smooth", "time_defect", "time_transfer", "time_coarse", "num_smooth", "num_defect",
+                "num_transfer", "num_coarse", "bytes", "flops"]
+        d = {k: per[:, i].copy() for i, k in enumerate(keys)}
+        d.update(levels=nl, ms_per_cycle=float(out[1]), operations=int(out[2]))
+        return d
+
     def info(self) -> dict:
         out = np.zeros(16, np.uint64)
         capi.call("feat3_mgpcg_info", self._h, _ptr(out), 16)

@@ -343,6 +343,8 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
  * image bit for bit stores only its center and upper diagonals; 0: the full band; same results),
  * "cycle" (the multigrid cycle, MultiGridCycle of multigrid.hpp:26-34: 0 V, default, 1 F,
  * 2 W; the coarse-tail kernel runs the cycle's steps on the small levels for each of them);
+ * "plane_order" (order in which the blocks of a banded row kernel take their row ranges: 0
+ * ascending, 1 plane strips for the FP64 outer matrix (default), 2 for every banded matrix);
  * takes effect at the next init */
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value);
 /* init_symbolic + init_numeric (solver/base.hpp:227-260) */
@@ -368,6 +370,18 @@ int feat3_mgpcg_time_kernel(feat3_mgpcg* h, int which, int reps, double* avg_ms,
  *  upper diagonals only), its number of diagonals, levels run by the
  *  single-workgroup coarse tail (0: none), LDS bytes of the tail] */
 int feat3_mgpcg_info(feat3_mgpcg* h, uint64_t* out, int cap);
+/* Per-level statistics of the multigrid cycle (MultiGridHierarchy::get_time_smooth/_defect/
+ * _transfer/_coarse, multigrid.hpp:858-956, and the flop counter of Statistics,
+ * util/statistics.hpp:206-230): `reps` cycles launched operation by operation on the solver's
+ * stream with an event after each (the solve's own graphs are untouched).  out[0] levels L,
+ * out[1] ms per cycle (first to last event), out[2] operations per cycle; then for each level
+ * (finest first) 10 values: ms per cycle in smoothing, defect, grid transfer (restriction and
+ * prolongation, on the finer level as the reference counts them), coarse solve; the number of
+ * smoothing steps, defects, transfers and coarse solves per cycle; algorithmic bytes and flops
+ * per cycle.  A segment of the single-workgroup coarse tail counts as the coarse solve of its
+ * top level (its levels report counts, bytes and flops; their time is the segment's).
+ * cap >= 3 + 10 L.  Creates its events at the first call. */
+int feat3_mgpcg_statistics(feat3_mgpcg* h, int reps, double* out, int cap);
 
 /* ===================================================================================== */
 /* Batched fused solver: K systems that share one sparsity graph, solved in lockstep by   */

@@ -1,0 +1,64 @@
+"""Per-level multigrid statistics (feat3_mgpcg_statistics; the reference's MultiGridHierarchy
+timings, multigrid.hpp:858-956, and flop counter, statistics.hpp:206-230).
+
+Checks: the per-level operation counts are those of the V(4,4)-cycle (and the W-cycle's
+2^(levels-1) coarse solves); the per-level times add up to the cycle time between the first and
+last event; that cycle time is within 10% of the same cycle launched without events
+(feat3_mgpcg_time_kernel); bytes and flops are positive on every level that does work.
+"""
+import numpy as np
+import pytest
+
+from feat3_amd import problems as pb
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch(torch_cuda):
+    return torch_cuda
+
+
+@pytest.mark.parametrize("dim,deg,lvl", [(3, 1, 6), (2, 2, 7)])
+def test_vcycle_statistics(torch, dim, deg, lvl):
+    from feat3_amd.solver import MGPCG
+    print(f"case statistics {dim}D Q{deg} L{lvl}", flush=True)
+    H = pb.hierarchy(dim, deg, lvl, 1)
+    s = MGPCG(H, inner="f32", fmt="banded").init()
+    st = s.statistics(20)
+    nl = st["levels"]
+    assert nl == len(H.levels)
+    pre = post = 4
+    for i in range(nl):
+        if i < nl - 1:
+            assert st["num_smooth"][i] == pre + post, (i, st["num_smooth"])
+            assert st["num_defect"][i] == 1 and st["num_transfer"][i] == 2 and st["num_coarse"][i] == 0
+        else:
+            assert st["num_smooth"][i] == 0 and st["num_defect"][i] == 0
+            assert st["num_transfer"][i] == 0 and st["num_coarse"][i] == 1
+        assert st["bytes"][i] > 0 and st["flops"][i] > 0
+    times = np.stack([st["time_smooth"], st["time_defect"], st["time_transfer"], st["time_coarse"]])
+    assert np.all(times >= 0)
+    tot = float(times.sum())
+    assert abs(tot - st["ms_per_cycle"]) <= 1e-3 * st["ms_per_cycle"] + 1e-4
+    ref_ms, _ = s.time_kernel(3, 20)
+    print(f"  cycle with events {st['ms_per_cycle']:.4f} ms, without {ref_ms:.4f} ms; "
+          f"level 0: smooth {st['time_smooth'][0]:.4f} defect {st['time_defect'][0]:.4f} "
+          f"transfer {st['time_transfer'][0]:.4f} ms", flush=True)
+    assert abs(st["ms_per_cycle"] - ref_ms) <= 0.10 * ref_ms
+    # the finest level's work dominates its time: smoothing is 8 of its 11 matrix passes
+    assert st["time_smooth"][0] > st["time_defect"][0]
+    s.done()
+
+
+def test_wcycle_coarse_solves(torch):
+    from feat3_amd.solver import MGPCG
+    H = pb.hierarchy(2, 1, 6, 1)
+    s = MGPCG(H, inner="f64", cycle="w").init()
+    st = s.statistics(3)
+    nl = st["levels"]
+    assert st["num_coarse"][nl - 1] == 2 ** (nl - 1)
+    # every level above the coarsest: 2^i visits, each with pre + post smoothing steps
+    for i in range(nl - 1):
+        assert st["num_smooth"][i] >= (2 ** i) * 8
+    s.done()
