@@ -600,7 +600,7 @@ namespace batch
     if(k < K && sys_active(st, k))
     {
       const double alpha = st[k].alpha;
-      const bool upd = fabs(alpha) >= 2.220446049250313e-16;
+      const bool upd = !(fabs(alpha) < 2.220446049250313e-16);   // a NaN alpha updates (dense_vector.hpp:891)
       for(u64 i = u64(blockIdx.x) * ngrp + threadIdx.x / G_; i < m; i += u64(gridDim.x) * ngrp)
       {
         const u64 t = i * u64(K) + u64(k);

@@ -733,7 +733,9 @@ namespace mg
   {
     if(st->status != kProgress) return;
     const double alpha = st->alpha;
-    const bool upd = fabs(alpha) >= 2.220446049250313e-16; // axpy with |alpha| < eps is a copy
+    // axpy with |alpha| < eps is a copy; written as a negation so that a NaN alpha updates (and the
+    // NaN reaches the defect norm) as in dense_vector.hpp:891
+    const bool upd = !(fabs(alpha) < 2.220446049250313e-16);
     double v = 0.0;
     for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
     {

@@ -1356,19 +1356,20 @@ namespace
       return rt::kOk;
     }
 
-    // evs (statistics only): evs[0] is recorded before the first operation and evs[k + 1] after
-    // operation k, so that each operation's time is the span between two events
-    int enqueue_vcycle(const double* def, const int* status, const std::vector<hipEvent_t>* evs = nullptr)
+    // evs (statistics only): an event is recorded before every operation k with span_start[k]
+    // set (span_start[0] is) and one after the last operation, so that each span of operations
+    // lies between two consecutive events
+    int enqueue_vcycle(const double* def, const int* status, const std::vector<hipEvent_t>* evs = nullptr,
+                       const std::vector<char>* span_start = nullptr)
     {
       const std::size_t nl = L.size();
       std::vector<DT*> cur(nl, nullptr), oth(nl, nullptr);
-      if(evs) F3_CHECK(hipEventRecord((*evs)[0], s));
-      std::size_t k = 0;
+      std::size_t k = 0, e = 0;
       for(const CycleOp& c : cycle_ops)
       {
         const std::size_t i = c.lvl;
         Level<DT, VT>& f = *L[i];
-        if(evs && k > 0) F3_CHECK(hipEventRecord((*evs)[k], s));
+        if(evs && (*span_start)[k]) F3_CHECK(hipEventRecord((*evs)[e++], s));
         ++k;
         switch(c.op)
         {
@@ -1428,7 +1429,7 @@ namespace
             break;
         }
       }
-      if(evs) F3_CHECK(hipEventRecord((*evs)[k], s));
+      if(evs) F3_CHECK(hipEventRecord((*evs)[e], s));
       if(cur[0] != L[0]->xfinal) return rt::kErrState;
       return rt::check_launch();
     }
@@ -1442,18 +1443,48 @@ namespace
     // transfer (its step is counted as a smoothing step of the coarser level), and a segment of
     // the single-workgroup coarse tail as the coarse solve of the tail's top level (one kernel:
     // the levels inside it report counts, bytes and flops, their time is that of the segment).
-    // Cycles are launched operation by operation with an event after each; the solve itself
-    // is not touched (its graphs record no events).
+    // Cycles are launched operation by operation, with an event wherever the level or the
+    // category changes (a run of smoothing steps on one level is one span): an event costs the
+    // stream a few microseconds, as much as a small level's operation, so one per operation
+    // would inflate the coarse levels' times.  The solve itself is not touched (its graphs
+    // record no events).
     std::vector<hipEvent_t> stat_ev;
+    static int op_category(Op op)
+    {
+      switch(op)
+      {
+        case Op::entry: case Op::sweep: case Op::copy_top: return 0;   // smoothing
+        case Op::defect: return 1;
+        case Op::rest: case Op::rest_step: case Op::prol: return 2;     // grid transfer
+        case Op::bottom: return 3;                                       // coarse solve
+      }
+      return 0;
+    }
     int statistics(int reps, double* out, int cap) override
     {
       const std::size_t nl = L.size(), nops = cycle_ops.size();
       const int need = 3 + 10 * int(nl);
       if(reps < 1 || !out || cap < need) return rt::kErrArg;
-      while(stat_ev.size() < nops + 1)
+      // spans: maximal runs of operations with the same level and category
+      std::vector<char> span_start(nops, 0);
+      std::vector<std::size_t> span_slot;   // 4 * level + category of each span
+      for(std::size_t k = 0; k < nops; ++k)
+      {
+        const std::size_t slot = 4 * cycle_ops[k].lvl + std::size_t(op_category(cycle_ops[k].op));
+        if(k == 0 || slot != span_slot.back())
+        {
+          span_start[k] = 1;
+          span_slot.push_back(slot);
+        }
+      }
+      const std::size_t nspan = span_slot.size();
+      // timing-only events: without the system-scope fence (a cache writeback and invalidation
+      // at every record) an event costs the stream less; the host synchronises the stream
+      // before it reads the times
+      while(stat_ev.size() < nspan + 1)
       {
         hipEvent_t e = nullptr;
-        F3_CHECK(hipEventCreate(&e));
+        F3_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         stat_ev.push_back(e);
       }
       std::vector<double> t(4 * nl, 0.0), cnt(4 * nl, 0.0), by(nl, 0.0), fl(nl, 0.0);
@@ -1462,24 +1493,15 @@ namespace
       double total = 0.0;
       for(int rep = 0; rep < reps; ++rep)
       {
-        F3_RC(enqueue_vcycle(r.template as<double>(), nullptr, &stat_ev));
+        F3_RC(enqueue_vcycle(r.template as<double>(), nullptr, &stat_ev, &span_start));
         F3_CHECK(hipStreamSynchronize(s));
         float ms = 0.f;
-        F3_CHECK(hipEventElapsedTime(&ms, stat_ev[0], stat_ev[nops]));
+        F3_CHECK(hipEventElapsedTime(&ms, stat_ev[0], stat_ev[nspan]));
         total += ms;
-        for(std::size_t k = 0; k < nops; ++k)
+        for(std::size_t k = 0; k < nspan; ++k)
         {
           F3_CHECK(hipEventElapsedTime(&ms, stat_ev[k], stat_ev[k + 1]));
-          const CycleOp& c = cycle_ops[k];
-          int cat = 0;   // 0 smooth, 1 defect, 2 transfer, 3 coarse
-          switch(c.op)
-          {
-            case Op::entry: case Op::sweep: case Op::copy_top: cat = 0; break;
-            case Op::defect: cat = 1; break;
-            case Op::rest: case Op::rest_step: case Op::prol: cat = 2; break;
-            case Op::bottom: cat = 3; break;
-          }
-          t[4 * c.lvl + cat] += double(ms);
+          t[span_slot[k]] += double(ms);
         }
       }
       // counts, bytes and flops of the cycle's mathematical operations (the full schedule, tail

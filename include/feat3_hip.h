@@ -373,8 +373,9 @@ int feat3_mgpcg_info(feat3_mgpcg* h, uint64_t* out, int cap);
 /* Per-level statistics of the multigrid cycle (MultiGridHierarchy::get_time_smooth/_defect/
  * _transfer/_coarse, multigrid.hpp:858-956, and the flop counter of Statistics,
  * util/statistics.hpp:206-230): `reps` cycles launched operation by operation on the solver's
- * stream with an event after each (the solve's own graphs are untouched).  out[0] levels L,
- * out[1] ms per cycle (first to last event), out[2] operations per cycle; then for each level
+ * stream with an event wherever the level or the category changes (the solve's own graphs are
+ * untouched).  out[0] levels L, out[1] ms per cycle (first to last event), out[2] operations
+ * per cycle; then for each level
  * (finest first) 10 values: ms per cycle in smoothing, defect, grid transfer (restriction and
  * prolongation, on the finer level as the reference counts them), coarse solve; the number of
  * smoothing steps, defects, transfers and coarse solves per cycle; algorithmic bytes and flops

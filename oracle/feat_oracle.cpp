@@ -835,8 +835,9 @@ namespace orc
         this->A->apply(q, p);
         this->F->filter_def(q);
         const DT_ alpha = gamma / dot(q, p, n);
-        if(std::abs(alpha) >= eps<DT_>()) axpy(sol, alpha, p, sol, n);
-        if(std::abs(alpha) >= eps<DT_>()) axpy(r, -alpha, q, r, n);
+        // dense_vector.hpp:891: |alpha| < eps copies, anything else (a NaN alpha included) updates
+        if(!(std::abs(alpha) < eps<DT_>())) axpy(sol, alpha, p, sol, n);
+        if(!(std::abs(alpha) < eps<DT_>())) axpy(r, -alpha, q, r, n);
         st = this->set_new_defect(r);
         if(st != Status::progress)
           return st;

@@ -527,8 +527,10 @@ def test_csrb_any_block_shape(T, bh, bw, suf):
     x = rng.uniform(-1, 1, cols * bw)
     y = rng.uniform(-1, 1, rows * bh)
     ref = y.copy()
-    po.lib().orc_csrb_apply_f64_u32(po.P(ref), 0.75, po.P(x), -2.0, po.P(ref), po.P(val),
-                                    po.P(col.astype(np.uint32)), po.P(rp.astype(np.uint32)), rows, bh, bw)
+    # the 32-bit copies are held in names: a pointer taken from a temporary would dangle
+    col32, rp32 = col.astype(np.uint32), rp.astype(np.uint32)
+    po.lib().orc_csrb_apply_f64_u32(po.P(ref), 0.75, po.P(x), -2.0, po.P(ref), po.P(val), po.P(col32), po.P(rp32),
+                                    rows, bh, bw)
     dt = np.float32 if suf.startswith("f32") else np.float64
     idt = np.int64 if itype == np.uint64 else np.int32
     dr = dev(torch, y.astype(dt))

@@ -11,8 +11,8 @@ path restated) on the same system.
 
 Bars: FP64 path identical iterations and relative solution difference <= 1e-10; FP32 / FP16-stored
 V-cycle iterations +-1 of the oracle's FP64 count and <= 1e-8 from the FP64 oracle solution;
-gradients as in test_gpu_autograd (1e-8 vs the CPU gradient, 1e-6 vs finite differences and for
-the identity).
+gradients: 1e-8 vs the CPU gradient, 1e-6 for the identity, and vs central finite differences
+1e-5 at h = 1e-6 (the solves' noise floor at 1M DOF, measured 2.7e-6) and 1e-6 at h = 1e-5.
 """
 import numpy as np
 import pytest
@@ -155,5 +155,10 @@ def test_config4_gradient_16_fields(torch):
             return float(0.5 * ((up - uo) ** 2).sum())
     fd = (loss(K + h * d) - loss(K - h * d)) / (2 * h)
     ad = float(np.sum(g * d))
-    print(f"  FD {fd:.12e} adjoint {ad:.12e}", flush=True)
-    assert abs(fd - ad) <= 1e-6 * abs(fd)
+    # at 1M DOF the quotient's noise floor is the solves' own error (relative residual 1e-13 on a
+    # matrix of condition ~1e6) divided by 2h; the h = 1e-5 quotient has a tenth of that noise and
+    # still a negligible O(h^2) truncation term, so it must come closer to the adjoint value
+    fd5 = (loss(K + 1e-5 * d) - loss(K - 1e-5 * d)) / 2e-5
+    print(f"  FD(h=1e-6) {fd:.12e} FD(h=1e-5) {fd5:.12e} adjoint {ad:.12e}", flush=True)
+    assert abs(fd - ad) <= 1e-5 * abs(fd)
+    assert abs(fd5 - ad) <= 1e-6 * abs(fd5)

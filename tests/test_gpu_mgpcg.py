@@ -361,3 +361,4 @@ def test_nan_matrix_aborts_in_every_format(torch, fmt, inner, level):
     sg, ig, xg, _ = solve_gpu(torch, H, b, inner=inner, fmt=fmt)
     assert so == 3
     assert sg == 3, f"status {sg} after {ig} iterations"
+    assert ig == io

@@ -3,8 +3,9 @@ timings, multigrid.hpp:858-956, and flop counter, statistics.hpp:206-230).
 
 Checks: the per-level operation counts are those of the V(4,4)-cycle (and the W-cycle's
 2^(levels-1) coarse solves); the per-level times add up to the cycle time between the first and
-last event; that cycle time is within 10% of the same cycle launched without events
-(feat3_mgpcg_time_kernel); bytes and flops are positive on every level that does work.
+last event; at the benchmark size (3D Q1 L7) that cycle time is within 10% of the same cycle
+launched without events (feat3_mgpcg_time_kernel); bytes and flops are positive on every level
+that does work.
 """
 import numpy as np
 import pytest
@@ -45,9 +46,30 @@ def test_vcycle_statistics(torch, dim, deg, lvl):
     print(f"  cycle with events {st['ms_per_cycle']:.4f} ms, without {ref_ms:.4f} ms; "
           f"level 0: smooth {st['time_smooth'][0]:.4f} defect {st['time_defect'][0]:.4f} "
           f"transfer {st['time_transfer'][0]:.4f} ms", flush=True)
-    assert abs(st["ms_per_cycle"] - ref_ms) <= 0.10 * ref_ms
+    # the events (one per span of operations) add a few microseconds each: at these sizes the
+    # instrumented cycle stays within 1.5x of the plain one; at the benchmark size within 10%
+    # (test_vcycle_statistics_time_at_benchmark_size)
+    assert st["ms_per_cycle"] <= 1.5 * ref_ms
     # the finest level's work dominates its time: smoothing is 8 of its 11 matrix passes
     assert st["time_smooth"][0] > st["time_defect"][0]
+    s.done()
+
+
+def test_vcycle_statistics_time_at_benchmark_size(torch):
+    """3D Q1 level 7 (2,146,689 DOF), FP32 V-cycle on the banded format: the per-level times add
+    up to the instrumented cycle, which is within 10% of the cycle launched without events."""
+    from feat3_amd.solver import MGPCG
+    print("case statistics 3D Q1 L7", flush=True)
+    H = pb.hierarchy(3, 1, 7, 1)
+    s = MGPCG(H, inner="f32", fmt="banded").init()
+    st = s.statistics(20)
+    times = np.stack([st["time_smooth"], st["time_defect"], st["time_transfer"], st["time_coarse"]])
+    assert np.all(times >= 0)
+    assert abs(float(times.sum()) - st["ms_per_cycle"]) <= 1e-3 * st["ms_per_cycle"] + 1e-4
+    ref_ms, _ = s.time_kernel(3, 20)
+    print(f"  cycle with events {st['ms_per_cycle']:.4f} ms, without {ref_ms:.4f} ms; per level "
+          + " ".join(f"{float(times[:, i].sum()):.4f}" for i in range(st["levels"])), flush=True)
+    assert abs(st["ms_per_cycle"] - ref_ms) <= 0.10 * ref_ms
     s.done()
 
 

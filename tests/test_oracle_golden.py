@@ -230,6 +230,25 @@ def test_solver_status_semantics():
     assert st == 4 and it < 100  # diverged
 
 
+@pytest.mark.parametrize("level", [1, 2])
+def test_nan_in_a_coarser_level_aborts_after_one_iteration(level):
+    """A NaN coupling in a coarser level matrix leaves the initial defect finite but makes the
+    V-cycle's correction NaN, so gamma and alpha are NaN.  DenseVector::axpy copies only when
+    |alpha| < eps (dense_vector.hpp:891); a NaN alpha fails that test, updates x and r, and the
+    first new defect norm is NaN: aborted after one iteration (iterative.hpp:794-795).  Skipping
+    the update for a NaN alpha (an `|alpha| >= eps` test) would instead run to max_iter."""
+    H = pb.hierarchy(2, 1, 5, 1)
+    A = H.levels[level].A
+    m = 2 ** (5 - level) + 1
+    r = (m // 2) * m + m // 2
+    ks = [k for k in range(A.row_ptr[r], A.row_ptr[r + 1]) if A.col_ind[k] != r]
+    A.val[ks[0]] = np.nan
+    S = po.System.from_hierarchy(H)
+    st, it, hist = S.solve(np.zeros(H.n), pb.filtered_rhs(H, 0), po.params(max_iter=50))
+    assert (st, it) == (3, 1)
+    assert np.isfinite(hist[0]) and np.isnan(hist[1])
+
+
 # ---------------------------------------------------------------------------------------------
 # host setup against closed forms and the reference's golden prolongation
 # ---------------------------------------------------------------------------------------------
