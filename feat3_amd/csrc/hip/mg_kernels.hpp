@@ -188,6 +188,93 @@ namespace mg
     }
   };
 
+  // Tensor-product transfer matrix on lattices numbered lexicographically (x fastest):
+  // M[(oz,oy,ox),(iz,iy,ix)] = (wz * wy) * wx with the 1D factors of the three axes (a 2D
+  // matrix has the 1x1 identity as its z factor).  Entry k of output index o along an axis is
+  // (idx[o*kTensorW + k], w[o*kTensorW + k]), k < cnt[o], ascending in idx.  The solver uses it
+  // for the grid transfers of structured hierarchies only after checking every stored entry of
+  // the CSR matrix against it (column, count and the value in the arithmetic type bit for bit),
+  // so a row sum reads no matrix data; the terms are the CSR row's, in its column order, each
+  // product rounded before it is added as in the streaming CSR evaluation (CsrStreamRows):
+  // the results are identical to it.
+  constexpr u32 kTensorW = 8;
+  template<typename DT_>
+  struct TensorAxis
+  {
+    const u32* cnt;
+    const u32* idx;
+    const DT_* w;
+  };
+  // K_: bound of the entries per axis (every cnt <= K_), D3_: the z factor is not the 1x1 identity.
+  // The loops are unrolled to the bound and the terms predicated, so all factor and x loads of a
+  // row are issued before the first addition; an absent term loads x[0] and is selected away.
+  template<typename DT_, int K_ = int(kTensorW), bool D3_ = true>
+  struct TensorRows
+  {
+    static constexpr int G = 1;
+    TensorAxis<DT_> ax[3];
+    u32 mo[3], mi[3];
+    u64 rows;
+    __device__ __forceinline__ u64 row(u64 p) const { return p; }
+    __device__ __forceinline__ u64 block_of(u64 b) const { return b; }
+    __device__ __forceinline__ DT_ row_sum(u64 p, int, const DT_* __restrict__ x) const
+    {
+#pragma clang fp contract(off)
+      constexpr int KZ = D3_ ? K_ : 1;
+      const u32 q = u32(p);
+      const u32 ox = q % mo[0], t = q / mo[0];
+      const u32 oy = t % mo[1], oz = t / mo[1];
+      const u32 nz = D3_ ? ax[2].cnt[oz] : 1u, ny = ax[1].cnt[oy], nx = ax[0].cnt[ox];
+      u32 iz[KZ], iy[K_], ix[K_];
+      DT_ wz[KZ], wy[K_], wx[K_];
+#pragma unroll
+      for(int k = 0; k < KZ; ++k)
+      {
+        iz[k] = D3_ && u32(k) < nz ? ax[2].idx[oz * kTensorW + k] : 0u;
+        wz[k] = D3_ ? (u32(k) < nz ? ax[2].w[oz * kTensorW + k] : DT_(0)) : DT_(1);
+      }
+#pragma unroll
+      for(int k = 0; k < K_; ++k)
+      {
+        iy[k] = u32(k) < ny ? ax[1].idx[oy * kTensorW + k] : 0u;
+        wy[k] = u32(k) < ny ? ax[1].w[oy * kTensorW + k] : DT_(0);
+        ix[k] = u32(k) < nx ? ax[0].idx[ox * kTensorW + k] : 0u;
+        wx[k] = u32(k) < nx ? ax[0].w[ox * kTensorW + k] : DT_(0);
+      }
+      DT_ xv[KZ][K_][K_];
+#pragma unroll
+      for(int a = 0; a < KZ; ++a)
+#pragma unroll
+        for(int b = 0; b < K_; ++b)
+        {
+          const u64 base = (u64(iz[a]) * mi[1] + iy[b]) * u64(mi[0]);
+#pragma unroll
+          for(int c = 0; c < K_; ++c)
+          {
+            const bool in = u32(a) < nz && u32(b) < ny && u32(c) < nx;
+            xv[a][b][c] = x[in ? base + ix[c] : 0];
+          }
+        }
+      DT_ s = DT_(0);
+#pragma unroll
+      for(int a = 0; a < KZ; ++a)
+#pragma unroll
+        for(int b = 0; b < K_; ++b)
+        {
+          const DT_ wzy = D3_ ? wz[a] * wy[b] : wy[b];
+#pragma unroll
+          for(int c = 0; c < K_; ++c)
+          {
+            const DT_ v = wzy * wx[c];
+            const DT_ pr = v * xv[a][b][c];
+            const DT_ s2 = s + pr;
+            s = (u32(a) < nz && u32(b) < ny && u32(c) < nx) ? s2 : s;
+          }
+        }
+      return s;
+    }
+  };
+
   // Launch shape of the row kernels: every block owns a contiguous range of `per` row
   // positions (a multiple of 64), its 256/G lane groups stride through that range, and the
   // blocks are XCD-remapped so that neighbouring ranges share an L2.

@@ -233,6 +233,65 @@ namespace
     return unsigned(b < 1 ? 1 : (b > 16384 ? 16384 : b));
   }
 
+  // ---- tensor-product transfers (mg::TensorRows) -------------------------------------------
+  // the entries of the listed rows in the arithmetic type, kTensorW per row: cnt[i] the row's
+  // length (kTensorW + 1 for a longer row, whose entries are not copied)
+  template<typename DT, typename VT>
+  __global__ void k_tensor_gather(u32 nlist, const u32* __restrict__ list, const u32* __restrict__ rp,
+                                  const u32* __restrict__ col, const VT* __restrict__ val, u32* __restrict__ cnt,
+                                  u32* __restrict__ ocol, DT* __restrict__ oval)
+  {
+    constexpr u32 W = mg::kTensorW;
+    for(u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nlist; i += gridDim.x * blockDim.x)
+    {
+      const u32 a = rp[list[i]], n = rp[list[i] + 1] - a;
+      cnt[i] = n > W ? W + 1 : n;
+      for(u32 k = 0; k < n && k < W; ++k)
+      {
+        ocol[i * W + k] = col[a + k];
+        oval[i * W + k] = dev::ld<DT, VT>(val, a + k);
+      }
+    }
+  }
+  // every CSR row equals its tensor-product row: the same number of entries, the same columns in
+  // the same order and each value, converted to the arithmetic type, bit for bit the product
+  // (wz * wy) * wx that TensorRows::row_sum forms; any difference raises *bad
+  template<typename DT, typename VT>
+  __global__ void k_tensor_check(mg::TensorRows<DT> T, const u32* __restrict__ rp, const u32* __restrict__ col,
+                                 const VT* __restrict__ val, int* bad)
+  {
+#pragma clang fp contract(off)
+    constexpr u32 W = mg::kTensorW;
+    typedef typename Bits<DT>::T B;
+    for(u64 p = u64(blockIdx.x) * blockDim.x + threadIdx.x; p < T.rows; p += u64(gridDim.x) * blockDim.x)
+    {
+      const u32 q = u32(p);
+      const u32 ox = q % T.mo[0], t = q / T.mo[0];
+      const u32 oy = t % T.mo[1], oz = t / T.mo[1];
+      const u32 nz = T.ax[2].cnt[oz], ny = T.ax[1].cnt[oy], nx = T.ax[0].cnt[ox];
+      u64 k = rp[p];
+      const u64 end = rp[p + 1];
+      bool ok = u64(nz) * ny * nx == end - k;
+      for(u32 a = 0; ok && a < nz; ++a)
+      {
+        const u32 iz = T.ax[2].idx[oz * W + a];
+        const DT wz = T.ax[2].w[oz * W + a];
+        for(u32 b = 0; ok && b < ny; ++b)
+        {
+          const DT wzy = wz * T.ax[1].w[oy * W + b];
+          const u64 base = (u64(iz) * T.mi[1] + T.ax[1].idx[oy * W + b]) * u64(T.mi[0]);
+          for(u32 c = 0; ok && c < nx; ++c, ++k)
+          {
+            const DT v = wzy * T.ax[0].w[ox * W + c];
+            ok = u64(col[k]) == base + T.ax[0].idx[ox * W + c] &&
+                 __builtin_bit_cast(B, dev::ld<DT, VT>(val, k)) == __builtin_bit_cast(B, v);
+          }
+        }
+      }
+      if(!ok) atomicOr(bad, 1);
+    }
+  }
+
   struct HostLevel
   {
     HostCsr A, P, R;
@@ -263,6 +322,14 @@ namespace
     // it evaluates, for the grid of tile_blocks blocks (0: ascending order)
     DevMem tile;
     unsigned tile_blocks = 0;
+    // tensor-product evaluation (mg::TensorRows, transfers of structured hierarchies; set by
+    // build_tensor_pair after every stored entry was checked): output and input lattice extents
+    // per axis, per axis the factor tables cnt / idx / w (w in the arithmetic type, tw_bytes wide)
+    bool tensor = false;
+    u32 tmo[3] = {1, 1, 1}, tmi[3] = {1, 1, 1};
+    u32 tw_bytes = 0;
+    u32 tk = 0;   // the largest number of entries of a factor row
+    DevMem tcnt[3], tidx[3], tw[3];
 
     // thin >= 0: upload a compact copy with the entries band_keeps(v, row diagonal, thin) only:
     // thin = 0 drops the stored exact zeros (the transfer matrices, whose two-level pattern holds
@@ -680,6 +747,12 @@ namespace
     // pointers / chunk descriptors, SELL padding included), excluding the vector traffic
     u64 matrix_bytes() const
     {
+      if(tensor)
+      {
+        u64 b = 0;
+        for(int a = 0; a < 3; ++a) b += u64(tmo[a]) * (4 + u64(mg::kTensorW) * (4 + tw_bytes));
+        return b;
+      }
       if(banded) return u64(bsym ? nds : ndiag) * rows * sizeof(VT) + u64(ndiag) * 4 * (bsym ? 2 : 1);
       if(sell)
       {
@@ -688,13 +761,175 @@ namespace
       }
       return nnz * (sizeof(VT) + 4) + (rows + 1) * 4;
     }
+
+    // the TensorRows view of the factor tables (tw_bytes == sizeof(DT))
+    template<typename DT>
+    mg::TensorRows<DT> tensor_rows() const
+    {
+      mg::TensorRows<DT> T;
+      for(int a = 0; a < 3; ++a)
+      {
+        T.ax[a] = mg::TensorAxis<DT>{tcnt[a].template as<u32>(), tidx[a].template as<u32>(), tw[a].template as<DT>()};
+        T.mo[a] = tmo[a];
+        T.mi[a] = tmi[a];
+      }
+      T.rows = rows;
+      return T;
+    }
+
+    // factor tables (per axis: cnt[o], idx/w[o * kTensorW + k]) to the device, then every stored
+    // entry checked against their product; *ok = the check passed
+    template<typename DT>
+    int set_tensor(const u32 mo[3], const u32 mi[3], const std::vector<u32> cnt[3], const std::vector<u32> idx[3],
+                   const std::vector<DT> w[3], hipStream_t s, bool* ok)
+    {
+      *ok = false;
+      for(int a = 0; a < 3; ++a)
+      {
+        tmo[a] = mo[a];
+        tmi[a] = mi[a];
+        F3_RC(tcnt[a].alloc(sizeof(u32) * cnt[a].size()));
+        F3_RC(tidx[a].alloc(sizeof(u32) * idx[a].size()));
+        F3_RC(tw[a].alloc(sizeof(DT) * w[a].size()));
+        F3_CHECK(hipMemcpyAsync(tcnt[a].p, cnt[a].data(), sizeof(u32) * cnt[a].size(), hipMemcpyHostToDevice, s));
+        F3_CHECK(hipMemcpyAsync(tidx[a].p, idx[a].data(), sizeof(u32) * idx[a].size(), hipMemcpyHostToDevice, s));
+        F3_CHECK(hipMemcpyAsync(tw[a].p, w[a].data(), sizeof(DT) * w[a].size(), hipMemcpyHostToDevice, s));
+      }
+      tw_bytes = sizeof(DT);
+      tk = 0;
+      for(int a = 0; a < 3; ++a)
+        for(u32 c : cnt[a]) tk = std::max(tk, c);
+      DevMem bad;
+      F3_RC(bad.alloc(sizeof(int)));
+      F3_CHECK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+      k_tensor_check<DT, VT><<<setup_blocks(rows), 256, 0, s>>>(tensor_rows<DT>(), rp.template as<u32>(), col.template as<u32>(),
+                                                               val.template as<VT>(), bad.template as<int>());
+      F3_RC(rt::check_launch());
+      int hbad = 1;
+      F3_CHECK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      F3_CHECK(hipStreamSynchronize(s));   // the host tables leave scope
+      *ok = hbad == 0;
+      return rt::kOk;
+    }
   };
+
+  // Tensor-product structure of a transfer pair on structured hierarchies.  P (fine x coarse)
+  // of a Q1/Q2 hierarchy on the unit square or cube, numbered lexicographically on its node
+  // lattice (the solver's lattice order), is Pz (x) Py (x) Px with 1D factors of 1-3 entries per
+  // row, and R = P^T is the product of their transposes.  The lattices are taken as m^3 (then
+  // m^2) nodes; P's factors are read from the rows of its first line along each axis (rows
+  // (0,0,x), (0,y,0), (z,0,0), whose entries must lie on the matching coarse line), R's are
+  // their transposes; then every stored entry of P and R is checked on the device against the
+  // product in the arithmetic type, bit for bit.  Only if all of that holds are both evaluated
+  // by mg::TensorRows (no matrix data read, identical sums); otherwise they keep their format.
+  template<typename DT, typename VT>
+  int build_tensor_pair(DevCsr<VT>& P, DevCsr<VT>& R, hipStream_t s)
+  {
+    constexpr u32 W = mg::kTensorW;
+    P.tensor = R.tensor = false;
+    if(P.rows == 0 || P.cols == 0 || P.rows >= (u64(1) << 32) || P.cols >= (u64(1) << 32) || R.rows != P.cols ||
+       R.cols != P.rows || P.sell || R.sell || P.banded || R.banded)
+      return rt::kOk;
+    auto lattice = [](u64 n, int d, u32& m) {
+      const double g = std::round(std::pow(double(n), 1.0 / d));
+      for(u64 c = g > 1.0 ? u64(g) - 1 : 1; c <= u64(g) + 1; ++c)
+      {
+        u64 p = 1;
+        for(int i = 0; i < d; ++i) p *= c;
+        if(p == n) { m = u32(c); return true; }
+      }
+      return false;
+    };
+    for(int d = 3; d >= 2; --d)
+    {
+      u32 mf = 0, mc = 0;
+      if(!lattice(P.rows, d, mf) || !lattice(P.cols, d, mc)) continue;
+      const u32 mo[3] = {mf, mf, d == 3 ? mf : 1u}, mi[3] = {mc, mc, d == 3 ? mc : 1u};
+      // the rows of the first line along each axis (a 2D z factor is the 1x1 identity)
+      std::vector<u32> list;
+      for(int a = 0; a < d; ++a)
+        for(u32 i = 0; i < mo[a]; ++i) list.push_back(a == 0 ? i : (a == 1 ? i * mo[0] : i * mo[0] * mo[1]));
+      const u32 nl = u32(list.size());
+      DevMem dl, dc, dcol, dval;
+      F3_RC(dl.alloc(sizeof(u32) * nl));
+      F3_RC(dc.alloc(sizeof(u32) * nl));
+      F3_RC(dcol.alloc(sizeof(u32) * nl * W));
+      F3_RC(dval.alloc(sizeof(DT) * nl * W));
+      F3_CHECK(hipMemcpyAsync(dl.p, list.data(), sizeof(u32) * nl, hipMemcpyHostToDevice, s));
+      k_tensor_gather<DT, VT><<<setup_blocks(nl), 256, 0, s>>>(nl, dl.template as<u32>(), P.rp.template as<u32>(),
+                                                             P.col.template as<u32>(), P.val.template as<VT>(),
+                                                             dc.template as<u32>(), dcol.template as<u32>(), dval.template as<DT>());
+      F3_RC(rt::check_launch());
+      std::vector<u32> hc(nl), hcol(u64(nl) * W);
+      std::vector<DT> hval(u64(nl) * W);
+      F3_CHECK(hipMemcpyAsync(hc.data(), dc.p, sizeof(u32) * nl, hipMemcpyDeviceToHost, s));
+      F3_CHECK(hipMemcpyAsync(hcol.data(), dcol.p, sizeof(u32) * nl * W, hipMemcpyDeviceToHost, s));
+      F3_CHECK(hipMemcpyAsync(hval.data(), dval.p, sizeof(DT) * nl * W, hipMemcpyDeviceToHost, s));
+      F3_CHECK(hipStreamSynchronize(s));
+      std::vector<u32> cnt[3], idx[3];
+      std::vector<DT> w[3];
+      bool good = true;
+      u32 li = 0;
+      for(int a = 0; a < 3; ++a)
+      {
+        cnt[a].assign(mo[a], 0);
+        idx[a].assign(u64(mo[a]) * W, 0);
+        w[a].assign(u64(mo[a]) * W, DT(0));
+        if(a >= d) { cnt[a][0] = 1; w[a][0] = DT(1); continue; }
+        const u64 stride = a == 0 ? 1 : (a == 1 ? u64(mi[0]) : u64(mi[0]) * mi[1]);
+        for(u32 i = 0; i < mo[a]; ++i, ++li)
+        {
+          if(hc[li] > W) { good = false; break; }
+          cnt[a][i] = hc[li];
+          for(u32 k = 0; k < hc[li]; ++k)
+          {
+            const u64 c = hcol[u64(li) * W + k];
+            if(c % stride != 0 || c / stride >= mi[a]) { good = false; break; }
+            idx[a][u64(i) * W + k] = u32(c / stride);
+            w[a][u64(i) * W + k] = hval[u64(li) * W + k];
+          }
+          if(!good) break;
+        }
+        if(!good) break;
+      }
+      if(!good) continue;
+      bool ok = false;
+      F3_RC(P.template set_tensor<DT>(mo, mi, cnt, idx, w, s, &ok));
+      if(!ok) continue;
+      // R's factors: the transposes (coarse index j lists the fine indices i with a P entry
+      // (i, j), ascending in i)
+      std::vector<u32> rcnt[3], ridx[3];
+      std::vector<DT> rw[3];
+      for(int a = 0; a < 3 && good; ++a)
+      {
+        rcnt[a].assign(mi[a], 0);
+        ridx[a].assign(u64(mi[a]) * W, 0);
+        rw[a].assign(u64(mi[a]) * W, DT(0));
+        for(u32 i = 0; i < mo[a] && good; ++i)
+          for(u32 k = 0; k < cnt[a][i]; ++k)
+          {
+            const u32 j = idx[a][u64(i) * W + k];
+            if(rcnt[a][j] >= W) { good = false; break; }
+            ridx[a][u64(j) * W + rcnt[a][j]] = i;
+            rw[a][u64(j) * W + rcnt[a][j]] = w[a][u64(i) * W + k];
+            ++rcnt[a][j];
+          }
+      }
+      if(!good) return rt::kOk;
+      F3_RC(R.template set_tensor<DT>(mi, mo, rcnt, ridx, rw, s, &ok));
+      if(!ok) return rt::kOk;
+      P.tensor = R.tensor = true;
+      return rt::kOk;
+    }
+    return rt::kOk;
+  }
 
   // calls f(rows_evaluator, launch grid) for the matrix's format and lane count
   template<typename DT, typename VT, typename F>
   int with_rows(const DevCsr<VT>& m, F&& f)
   {
     auto grid = [&](int G) { return m.rows_per_block ? mg::row_grid_fixed(m.rows, m.rows_per_block) : mg::row_grid(m.rows, G); };
+    if(m.tensor) return rt::kErrState;   // transfers only: with_transfer_rows
     if(m.banded)
     {
       const mg::RowGrid g = grid(1);
@@ -766,6 +1001,32 @@ namespace
       case 64: f(mg::CsrRows<DT, VT, 64>{rp, col, val, m.rows}, grid(64)); break;
       default: return rt::kErrArg;
     }
+    return rt::kOk;
+  }
+
+  // the same for a transfer matrix, which may also be evaluated as a tensor product: the row
+  // evaluator is instantiated for the bound of its factors' entries per axis (2, 3, 5 or 8) and
+  // for 2D (z factor the identity) or 3D
+  template<typename DT, int K, bool D3>
+  mg::TensorRows<DT, K, D3> tensor_as(const mg::TensorRows<DT>& t)
+  {
+    mg::TensorRows<DT, K, D3> r;
+    for(int a = 0; a < 3; ++a) { r.ax[a] = t.ax[a]; r.mo[a] = t.mo[a]; r.mi[a] = t.mi[a]; }
+    r.rows = t.rows;
+    return r;
+  }
+  template<typename DT, typename VT, typename F>
+  int with_transfer_rows(const DevCsr<VT>& m, F&& f)
+  {
+    if(!m.tensor) return with_rows<DT, VT>(m, f);
+    if(m.tw_bytes != sizeof(DT)) return rt::kErrState;
+    const mg::TensorRows<DT> t = m.template tensor_rows<DT>();
+    const mg::RowGrid g = m.rows_per_block ? mg::row_grid_fixed(m.rows, m.rows_per_block) : mg::row_grid(m.rows, 1);
+    const bool d3 = m.tmo[2] > 1;
+    if(m.tk <= 2) { if(d3) f(tensor_as<DT, 2, true>(t), g); else f(tensor_as<DT, 2, false>(t), g); }
+    else if(m.tk <= 3) { if(d3) f(tensor_as<DT, 3, true>(t), g); else f(tensor_as<DT, 3, false>(t), g); }
+    else if(m.tk <= 5) { if(d3) f(tensor_as<DT, 5, true>(t), g); else f(tensor_as<DT, 5, false>(t), g); }
+    else { if(d3) f(tensor_as<DT, 8, true>(t), g); else f(tensor_as<DT, 8, false>(t), g); }
     return rt::kOk;
   }
 
@@ -846,6 +1107,9 @@ namespace
     // prolongation rows hold 1..8 entries (Q1), which SELL pads to the chunk's widest row
     // (measured 3D Q1 L7 FP32 V-cycle 0.603 -> 0.583 ms, 2D Q2 L9 0.643 -> 0.610 ms)
     bool transfer_csr = true;
+    // transfers of structured hierarchies evaluated as tensor products of their 1D factors
+    // (build_tensor_pair: detected and checked entry by entry; identical results, no matrix data)
+    bool tensor_transfer = true;
     // launch shape of the transfer matrices: lanes per row (-1: as the level matrices) and row
     // positions per block (0: derived from the row count).  128 rows per block (more, smaller
     // blocks than the default grid on the mid-size levels) measured V-cycle 3D Q1 L7 584 -> 580 us,
@@ -977,6 +1241,7 @@ namespace
           }
           l->P.rows_per_block = cfg.transfer_rows_per_block;
           l->R.rows_per_block = cfg.transfer_rows_per_block;
+          if(cfg.tensor_transfer) F3_RC((build_tensor_pair<DT, VT>(l->P, l->R, s)));
         }
         const std::vector<u32> m = make_mask(l->n, h.fidx);
         F3_RC(l->mask.alloc(sizeof(u32) * m.size()));
@@ -1394,7 +1659,7 @@ namespace
           {
             Level<DT, VT>& cl = *L[i + 1];
             DT* xc = c.op == Op::rest_step ? cl.x.template as<DT>() : nullptr;
-            F3_RC(with_rows<DT, VT>(f.R, [&](auto R, mg::RowGrid g) {
+            F3_RC(with_transfer_rows<DT, VT>(f.R, [&](auto R, mg::RowGrid g) {
               mg::k_restrict<<<g.blocks, kBlock, 0, s>>>(R, g.per, cl.mask.template as<u32>(), f.def.template as<DT>(),
                                                          cl.rhs.template as<DT>(), xc, cl.dinv.template as<DT>(), status);
             }));
@@ -1418,7 +1683,7 @@ namespace
             cur[i] = f.xfinal;
             break;
           case Op::prol:
-            F3_RC(with_rows<DT, VT>(f.P, [&](auto P, mg::RowGrid g) {
+            F3_RC(with_transfer_rows<DT, VT>(f.P, [&](auto P, mg::RowGrid g) {
               mg::k_prolong<<<g.blocks, kBlock, 0, s>>>(P, g.per, f.mask.template as<u32>(), (const DT*)cur[i + 1], cur[i],
                                                         status);
             }));
@@ -1824,7 +2089,7 @@ namespace
     // iteration excluding the V-cycle, [8] sizeof inner DT, [9] sizeof VT
     int info(u64* out, int cap) override
     {
-      std::vector<u64> v(16, 0);
+      std::vector<u64> v(17, 0);
       v[0] = L.size();
       v[1] = L.front()->n;
       v[2] = L.front()->A.nnz;
@@ -1860,7 +2125,8 @@ namespace
       v[13] = L.front()->A.ndiag;
       v[14] = tail0 + 1 < L.size() ? L.size() - tail0 : 0;
       v[15] = tail_lds;
-      for(int i = 0; i < cap && i < 16; ++i) out[i] = v[i];
+      for(const auto& l : L) v[16] += l->P.tensor ? 1 : 0;
+      for(int i = 0; i < cap && i < 17; ++i) out[i] = v[i];
       return rt::kOk;
     }
   };
@@ -2189,6 +2455,7 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
   else if(k == "sell_C" && value >= 1 && value <= 1024) h->cfg.sell_C = u64(value);
   else if(k == "tail_max_n" && value >= 0) h->cfg.tail_max_n = u64(value);
   else if(k == "transfer_csr") h->cfg.transfer_csr = value != 0.0;
+  else if(k == "tensor_transfer") h->cfg.tensor_transfer = value != 0.0;
   else if(k == "transfer_lanes" && (value == -1 || value == 0 || value == 2 || value == 4 || value == 8 || value == 16 ||
                                     value == 32 || value == 64))
     h->cfg.transfer_lanes = int(value);

@@ -156,9 +156,9 @@ class MGPCG:
         return d
 
     def info(self) -> dict:
-        out = np.zeros(16, np.uint64)
-        capi.call("feat3_mgpcg_info", self._h, _ptr(out), 16)
+        out = np.zeros(17, np.uint64)
+        capi.call("feat3_mgpcg_info", self._h, _ptr(out), 17)
         keys = ["levels", "n", "nnz", "lanes", "sell_size", "kernels_per_vcycle", "vcycle_bytes",
                 "outer_bytes", "sizeof_inner", "sizeof_value", "index16", "sell_chunks", "banded", "ndiag",
-                "tail_levels", "tail_lds"]
+                "tail_levels", "tail_lds", "tensor_transfers"]
         return {k: int(v) for k, v in zip(keys, out)}

@@ -337,6 +337,9 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
  * level-matrix kernels, a multiple of 64; 0 = derived from the row count), "transfer_csr" (0/1:
  * transfer matrices in CSR evaluated by streaming, default 1, or in the level format),
  * "transfer_sigma" (SELL sort window of the transfer matrices when they are SELL, default 1),
+ * "tensor_transfer" (1, default: transfer pairs of a structured hierarchy in lattice order whose
+ * every stored entry equals the product of 1D factors are evaluated from those factors, with
+ * the same sums; 0: always from the stored matrices),
  * "transfer_lanes" (lanes per row of CSR transfer matrices: -1 as "lanes", default; 0 streaming),
  * "transfer_rows_per_block" (row positions per workgroup of the transfer kernels, default 128;
  * 0 = derived from the row count), "symmetric_band" (1, default: an FP64 band equal to its mirror
@@ -368,7 +371,8 @@ int feat3_mgpcg_time_kernel(feat3_mgpcg* h, int which, int reps, double* avg_ms,
  *  sizeof inner, sizeof stored value, 16-bit SELL columns on the finest level, SELL chunks
  *  of the finest level, finest level banded (0 no, 1 yes, 2 with symmetric storage: center and
  *  upper diagonals only), its number of diagonals, levels run by the
- *  single-workgroup coarse tail (0: none), LDS bytes of the tail] */
+ *  single-workgroup coarse tail (0: none), LDS bytes of the tail, levels whose transfers are
+ *  evaluated as tensor products of their 1D factors] */
 int feat3_mgpcg_info(feat3_mgpcg* h, uint64_t* out, int cap);
 /* Per-level statistics of the multigrid cycle (MultiGridHierarchy::get_time_smooth/_defect/
  * _transfer/_coarse, multigrid.hpp:858-956, and the flop counter of Statistics,

@@ -468,3 +468,31 @@ def test_bcsr_non_square_brute_force(bh, bw):
     bflat = blocks.ravel().copy()
     po.lib().orc_csrb_apply_f64_u32(po.P(r), 0.5, po.P(x), -1.5, po.P(r), po.P(bflat), po.P(col), po.P(rp), nb, bh, bw)
     assert np.allclose(r, 0.5 * (D @ x) - 1.5 * y, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("dim,deg,lvl", [(2, 1, 5), (3, 1, 4), (2, 2, 4), (3, 2, 3)])
+def test_transfers_are_kronecker_products(dim, deg, lvl):
+    """In lattice order the prolongation of a tensor-product element on the unit square or cube
+    is Pz (x) Py (x) Px with the 1D prolongation on every axis, and R = P^T (the structure the
+    solver's tensor-product transfer evaluation checks for at init).  The 1D factor is the
+    restriction of P to the first line of fine nodes along x; both P and R equal the Kronecker
+    products of it entry for entry (the factors' values are dyadic, so the products are exact)."""
+    import scipy.sparse as sp
+    H = pb.hierarchy(dim, deg, lvl, 1)
+    for L in H.levels[:-1]:
+        P = sp.csr_matrix((L.P.val, L.P.col_ind, L.P.row_ptr), shape=(L.P.rows, L.P.cols))
+        R = sp.csr_matrix((L.R.val, L.R.col_ind, L.R.row_ptr), shape=(L.R.rows, L.R.cols))
+        P.eliminate_zeros()
+        R.eliminate_zeros()
+        mf, mc = round(P.shape[0] ** (1 / dim)), round(P.shape[1] ** (1 / dim))
+        assert mf ** dim == P.shape[0] and mc ** dim == P.shape[1]
+        P1 = P[:mf, :mc]
+        assert P1.nnz == P[:mf, :].nnz          # the first fine x-line only reaches the first coarse x-line
+        K = P1
+        for _ in range(dim - 1):
+            K = sp.kron(P1, K, format="csr")
+        assert (abs(K - P)).max() == 0.0 and K.nnz == P.nnz
+        assert (abs(K.T.tocsr() - R)).max() == 0.0
+        # 1D weights: 1 at coinciding nodes; Q1 halves, Q2 the quadratic interpolation weights
+        w = np.unique(P1.data)
+        assert set(w) <= ({0.5, 1.0} if deg == 1 else {-0.125, 0.375, 0.75, 1.0})
