@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <limits>
+#include <mutex>
 
 using namespace feat3_amd;
 using feat3_amd::dev::half;
@@ -699,6 +700,51 @@ namespace
   }
 } // namespace
 
+// ---- scratch of the no-allocation dot product (lafem_hip_dot_noalloc_f64) -------------------
+// The block partials of that entry point live in this file-scope device array: every GPU the
+// library runs on holds its own copy until the process ends, and calls on one GPU share it.  In
+// host code the array's name is a host-side stand-in, never a device address: host code uses
+// only the address hipGetSymbolAddress returns, taken once per device with that device current
+// (feat3_hip_initialise, or the entry point's first call on the device) and kept per device.
+static constexpr unsigned kNoAllocPartials = 2048;   // the reduction grid's largest block count
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+namespace
+{
+  constexpr int kMaxScratchDevices = 64;
+  std::mutex g_scratch_mutex;
+  double* g_scratch[kMaxScratchDevices] = {};
+
+  // the current device's address of `scratch`, looked up at the device's first use
+  int scratch_address(double** out)
+  {
+    int dev = 0;
+    if(hipGetDevice(&dev) != hipSuccess) return rt::kErrRuntime;
+    if(dev < 0 || dev >= kMaxScratchDevices) return rt::kErrArg;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if(!g_scratch[dev])
+    {
+      void* p = nullptr;
+      if(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return rt::kErrRuntime;
+      g_scratch[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch[dev];
+    return rt::kOk;
+  }
+} // namespace
+
+namespace feat3_amd
+{
+namespace rt
+{
+  int setup_noalloc_scratch()
+  {
+    double* p = nullptr;
+    return scratch_address(&p);
+  }
+} // namespace rt
+} // namespace feat3_amd
+
 // =========================================================================================
 // C ABI
 // =========================================================================================
@@ -995,6 +1041,17 @@ F3_DEF_CONV(f64_f32, double, float)
 F3_DEF_CONV(u32_u64, u32, u64)
 F3_DEF_CONV(u64_u32, u64, u32)
 #undef F3_DEF_CONV
+
+// <x, y> into a device scalar with the kernels and grid of lafem_hip_dot_f64 (identical
+// result), the block partials in `scratch` instead of the device's workspace: no allocation
+int lafem_hip_dot_noalloc_f64(void* stream, const double* x, const double* y, u64 n, double* result_dev)
+{
+  if(!result_dev || (n > 0 && (!x || !y))) return rt::kErrArg;
+  double* part = nullptr;
+  const int rc = scratch_address(&part);
+  if(rc) return rc;
+  return enqueue_dot<double, false>(S_, x, y, n, part, kNoAllocPartials, result_dev);
+}
 
 int lafem_hip_convert_h16_f32(void* stream, void* dst, const float* src, u64 n)
 {

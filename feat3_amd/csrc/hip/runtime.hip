@@ -145,7 +145,9 @@ const char* feat3_hip_error_string(int code)
 int feat3_hip_initialise(int device)
 {
   if(hipSetDevice(device) != hipSuccess) return rt::kErrRuntime;
-  return rt::create_workspace();
+  const int rc = rt::create_workspace();
+  if(rc) return rc;
+  return rt::setup_noalloc_scratch();
 }
 
 // MemoryPool<Mem::HIP>::finalise (memory_pool.cu:74-95): reports leaked allocations

@@ -35,6 +35,9 @@ namespace rt
   Workspace* workspace();
   // creates the current device's workspace (feat3_hip_initialise); kOk if it exists already
   int create_workspace();
+  // looks up the current device's address of the no-allocation dot product's scratch array
+  // (feat3_hip_initialise); kOk if it is known already
+  int setup_noalloc_scratch();
 
   // returns kOk or kErrLaunch after a kernel launch; with FEAT3_HIP_DEBUG_SYNC=1 in the
   // environment every launch is also synchronised and checked (FEAT_DEBUG_MODE analogue)

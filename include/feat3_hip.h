@@ -170,6 +170,17 @@ int lafem_hip_dot_f64(void* stream, const double* x, const double* y, uint64_t n
 int lafem_hip_dot_f32(void* stream, const float* x, const float* y, uint64_t n, float* result_dev);
 int lafem_hip_dot_value_f64(void* stream, const double* x, const double* y, uint64_t n, double* result_host);
 int lafem_hip_dot_value_f32(void* stream, const float* x, const float* y, uint64_t n, float* result_host);
+/* <x,y> into a device scalar without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same kernels, grid and summation order as
+ * lafem_hip_dot_f64, so the same result bit for bit; the block partials go to a static device
+ * array of 2048 doubles (16 KiB per GPU, held until the process ends) instead of the
+ * initialised workspace.  Every n is accepted: the grid has at most 2048 blocks, one partial
+ * each.  Nothing needs clearing between calls (each call writes every partial it reads).  All
+ * calls on one GPU share the array: calls that overlap on one GPU (on different streams, or
+ * from several threads) race and must be ordered by the caller.  feat3_hip_initialise looks up
+ * the array's device address; without it the first call on a device does. */
+int lafem_hip_dot_noalloc_f64(void* stream, const double* x, const double* y, uint64_t n, double* result_dev);
 /* ||x||_2 into a device scalar, or returned to the host (synchronises) */
 int lafem_hip_norm2_f64(void* stream, const double* x, uint64_t n, double* result_dev);
 int lafem_hip_norm2_f32(void* stream, const float* x, uint64_t n, float* result_dev);
