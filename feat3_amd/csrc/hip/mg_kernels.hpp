@@ -152,35 +152,41 @@ namespace mg
     const u32* tile = nullptr;
     __device__ __forceinline__ u64 row(u64 p) const { return p; }
     __device__ __forceinline__ u64 block_of(u64 b) const { return tile ? u64(tile[b]) : b; }
-    __device__ __forceinline__ DT_ row_sum(u64 p, int, const DT_* __restrict__ x) const
+    // Index arithmetic in 32 bits (the banded format is built for fewer than 2^31 rows): the
+    // column c = p + off[d] + 1 - rows wraps past 2^32 below 0, so one unsigned compare with cols
+    // tests both ends, and the chunk index (< 2^26) times the chunk stride uses a 24-bit multiply
+    // (64-bit index arithmetic cost about 36 VALU per diagonal, the FP64 outer product's bound).
+    __device__ __forceinline__ DT_ row_sum(u64 p64, int, const DT_* __restrict__ x) const
     {
       const dev::cptr<u32> co = dev::to_const(off);
+      const u32 p = u32(p64), shift = 1u - u32(rows), nc = u32(cols);
       DT_ s = DT_(0);
       if constexpr(SYM_)
       {
         const dev::cptr<u32> cs = dev::to_const(sx);
+        const u32 stride = nds * 64u;
 #pragma unroll 9
         for(u32 d = 0; d < ndiag; ++d)
         {
-          const u64 c = p + u64(co[d]) + 1 - rows;
-          const bool in = c < cols;
-          const u64 vr = (in && c < p) ? c : p;   // lower diagonal: the value stored at row c
-          const u64 at = (vr >> 6) * u64(nds) * 64 + u64(cs[d]) * 64 + (vr & 63);
+          const u32 c = p + co[d] + shift;
+          const bool in = c < nc;
+          const u32 vr = (in && c < p) ? c : p;   // lower diagonal: the value stored at row c
+          const u32 at = __umul24(vr >> 6, stride) + cs[d] * 64u + (vr & 63u);
           const DT_ a = kNonTemporal ? dev::ld_nt<DT_>(val, at) : dev::ld<DT_>(val, at);
           const DT_ t = a * x[in ? c : p];
           s += in ? t : DT_(0);
         }
         return s;
       }
-      const u64 base = (p >> 6) * u64(ndiag) * 64 + (p & 63);
+      const u32 base = __umul24(p >> 6, ndiag * 64u) + (p & 63u);
 #pragma unroll 9
       for(u32 d = 0; d < ndiag; ++d)
       {
-        // column; wraps past 2^64 below 0.  The load is unconditional (an in-range stand-in
-        // column) and the product selected away, so that no branch splits the batch of loads
-        const u64 c = p + u64(co[d]) + 1 - rows;
-        const bool in = c < cols;
-        const DT_ a = kNonTemporal ? dev::ld_nt<DT_>(val, base + u64(d) * 64) : dev::ld<DT_>(val, base + u64(d) * 64);
+        // The load is unconditional (an in-range stand-in column) and the product selected away,
+        // so that no branch splits the batch of loads
+        const u32 c = p + co[d] + shift;
+        const bool in = c < nc;
+        const DT_ a = kNonTemporal ? dev::ld_nt<DT_>(val, base + d * 64u) : dev::ld<DT_>(val, base + d * 64u);
         const DT_ t = a * x[in ? c : p];
         s += in ? t : DT_(0);
       }
@@ -191,12 +197,14 @@ namespace mg
   // Tensor-product transfer matrix on lattices numbered lexicographically (x fastest):
   // M[(oz,oy,ox),(iz,iy,ix)] = (wz * wy) * wx with the 1D factors of the three axes (a 2D
   // matrix has the 1x1 identity as its z factor).  Entry k of output index o along an axis is
-  // (idx[o*kTensorW + k], w[o*kTensorW + k]), k < cnt[o], ascending in idx.  The solver uses it
-  // for the grid transfers of structured hierarchies only after checking every stored entry of
-  // the CSR matrix against it (column, count and the value in the arithmetic type bit for bit),
-  // so a row sum reads no matrix data; the terms are the CSR row's, in its column order, each
-  // product rounded before it is added as in the streaming CSR evaluation (CsrStreamRows):
-  // the results are identical to it.
+  // (idx[o*kTensorW + k], w[o*kTensorW + k]), k < cnt[o], ascending; the indices are stored
+  // scaled by the axis's stride in the input lattice (x: 1, y: mi[0], z: mi[0]*mi[1]), so a
+  // column is iz + iy + ix, and the slots from cnt[o] to kTensorW hold index 0 and weight 0.  The
+  // solver uses it for the grid transfers of structured hierarchies only after checking every
+  // stored entry of the CSR matrix against it (column, count and the value in the arithmetic
+  // type bit for bit), so a row sum reads no matrix data; the terms are the CSR row's, in its
+  // column order, each product rounded before it is added as in the streaming CSR evaluation
+  // (CsrStreamRows): the results are identical to it.
   constexpr u32 kTensorW = 8;
   template<typename DT_>
   struct TensorAxis
@@ -205,15 +213,43 @@ namespace mg
     const u32* idx;
     const DT_* w;
   };
-  // K_: bound of the entries per axis (every cnt <= K_), D3_: the z factor is not the 1x1 identity.
-  // The loops are unrolled to the bound and the terms predicated, so all factor and x loads of a
-  // row are issued before the first addition; an absent term loads x[0] and is selected away.
+
+  // q / d for q < 2^32 with m = ceil(2^64 / d), 2 <= d < 2^32: floor(q m / 2^64) is exact (the
+  // error q (m - 2^64/d) / 2^64 < 2^-32 <= 1/d cannot carry a quotient past an integer)
+  __host__ __device__ __forceinline__ u32 udiv_magic(u32 q, u64 m)
+  {
+    const u64 lo = u64(q) * (m & 0xffffffffull);
+    const u64 hi = u64(q) * (m >> 32) + (lo >> 32);
+    return u32(hi >> 32);
+  }
+  inline u64 udiv_magic_for(u32 d) { return ~u64(0) / d + 1; }
+
+  // the first K_ entries of a factor row (16-byte loads; the row has kTensorW = 8 slots)
+  template<int K_, typename T>
+  __device__ __forceinline__ void load_entries(const T* row, T (&out)[K_])
+  {
+    constexpr int per = int(16 / sizeof(T));
+    typedef T vt __attribute__((ext_vector_type(per)));
+#pragma unroll
+    for(int k0 = 0; k0 < K_; k0 += per)
+    {
+      const vt v = *reinterpret_cast<const vt*>(row + k0);
+#pragma unroll
+      for(int j = 0; j < per; ++j)
+        if(k0 + j < K_) out[k0 + j] = v[j];
+    }
+  }
+
+  // K_: bound of the entries per axis (every cnt <= K_), D3_: the z factor is not the 1x1
+  // identity.  One lane per row; the factor rows are read whole and the x gathers issued for all
+  // K_^3 (K_^2) terms before the additions, an absent term reading x[0] and being selected away.
   template<typename DT_, int K_ = int(kTensorW), bool D3_ = true>
   struct TensorRows
   {
     static constexpr int G = 1;
     TensorAxis<DT_> ax[3];
     u32 mo[3], mi[3];
+    u64 div0, div1;   // udiv_magic multipliers of mo[0] and mo[1]
     u64 rows;
     __device__ __forceinline__ u64 row(u64 p) const { return p; }
     __device__ __forceinline__ u64 block_of(u64 b) const { return b; }
@@ -222,37 +258,37 @@ namespace mg
 #pragma clang fp contract(off)
       constexpr int KZ = D3_ ? K_ : 1;
       const u32 q = u32(p);
-      const u32 ox = q % mo[0], t = q / mo[0];
-      const u32 oy = t % mo[1], oz = t / mo[1];
+      const u32 t = udiv_magic(q, div0), ox = q - t * mo[0];
+      const u32 oz = D3_ ? udiv_magic(t, div1) : 0u, oy = t - oz * mo[1];
       const u32 nz = D3_ ? ax[2].cnt[oz] : 1u, ny = ax[1].cnt[oy], nx = ax[0].cnt[ox];
       u32 iz[KZ], iy[K_], ix[K_];
       DT_ wz[KZ], wy[K_], wx[K_];
-#pragma unroll
-      for(int k = 0; k < KZ; ++k)
+      if constexpr(D3_)
       {
-        iz[k] = D3_ && u32(k) < nz ? ax[2].idx[oz * kTensorW + k] : 0u;
-        wz[k] = D3_ ? (u32(k) < nz ? ax[2].w[oz * kTensorW + k] : DT_(0)) : DT_(1);
+        load_entries<KZ>(ax[2].idx + oz * kTensorW, iz);
+        load_entries<KZ>(ax[2].w + oz * kTensorW, wz);
       }
-#pragma unroll
-      for(int k = 0; k < K_; ++k)
+      else
       {
-        iy[k] = u32(k) < ny ? ax[1].idx[oy * kTensorW + k] : 0u;
-        wy[k] = u32(k) < ny ? ax[1].w[oy * kTensorW + k] : DT_(0);
-        ix[k] = u32(k) < nx ? ax[0].idx[ox * kTensorW + k] : 0u;
-        wx[k] = u32(k) < nx ? ax[0].w[ox * kTensorW + k] : DT_(0);
+        iz[0] = 0u;
+        wz[0] = DT_(1);
       }
+      load_entries<K_>(ax[1].idx + oy * kTensorW, iy);
+      load_entries<K_>(ax[1].w + oy * kTensorW, wy);
+      load_entries<K_>(ax[0].idx + ox * kTensorW, ix);
+      load_entries<K_>(ax[0].w + ox * kTensorW, wx);
       DT_ xv[KZ][K_][K_];
 #pragma unroll
       for(int a = 0; a < KZ; ++a)
 #pragma unroll
         for(int b = 0; b < K_; ++b)
         {
-          const u64 base = (u64(iz[a]) * mi[1] + iy[b]) * u64(mi[0]);
+          const u32 base = iz[a] + iy[b];
 #pragma unroll
           for(int c = 0; c < K_; ++c)
           {
             const bool in = u32(a) < nz && u32(b) < ny && u32(c) < nx;
-            xv[a][b][c] = x[in ? base + ix[c] : 0];
+            xv[a][b][c] = x[in ? base + ix[c] : 0u];
           }
         }
       DT_ s = DT_(0);
@@ -871,6 +907,21 @@ namespace mg
       st->first = 0;
     }
     else { const double g2 = st->gamma; st->gamma = t; st->beta = t / g2; } // pcg.hpp:273-279
+  }
+
+  // Publication of the solver state to the host (one thread): publication c of a solve (c counted
+  // on the device in *count) goes to slot c % R of a ring in pinned host memory, and after a
+  // system-scope fence its sequence number c + 1 to seq[c % R], which the host polls.  It runs
+  // whatever the status, so every publication the host launched arrives.
+  template<int R>
+  __global__ void k_publish(const PcgState* __restrict__ st, PcgState* ring, unsigned* seq, unsigned* count)
+  {
+    if(blockIdx.x != 0 || threadIdx.x != 0) return;
+    const unsigned c = *count;
+    ring[c % R] = *st;
+    __threadfence_system();
+    reinterpret_cast<volatile unsigned*>(seq)[c % R] = c + 1;
+    *count = c + 1;
   }
 
   // p = beta p + z  (axpy with r == x: r = r*a + y; |beta| < eps: copy)   (pcg.hpp:281-283)

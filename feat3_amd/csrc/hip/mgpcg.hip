@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -279,7 +280,7 @@ namespace
         for(u32 b = 0; ok && b < ny; ++b)
         {
           const DT wzy = wz * T.ax[1].w[oy * W + b];
-          const u64 base = (u64(iz) * T.mi[1] + T.ax[1].idx[oy * W + b]) * u64(T.mi[0]);
+          const u64 base = u64(iz) + u64(T.ax[1].idx[oy * W + b]);
           for(u32 c = 0; ok && c < nx; ++c, ++k)
           {
             const DT v = wzy * T.ax[0].w[ox * W + c];
@@ -461,7 +462,8 @@ namespace
     int build_banded_dev(const u32* drp, const u32* dcol, const double* dval, hipStream_t s)
     {
       banded = false;
-      if(rows != cols || rows == 0 || rows >= (u64(1) << 31)) return rt::kOk;
+      // BandedRows indexes in 32 bits: at most kMaxDiag (64) diagonals of fewer than 2^26 rows
+      if(rows != cols || rows == 0 || rows >= (u64(1) << 26)) return rt::kOk;
       const u64 nbits = 2 * rows - 1, nw = (nbits + 31) / 32;
       DevMem bits;
       F3_RC(bits.alloc(sizeof(u32) * nw));
@@ -552,7 +554,8 @@ namespace
     int build_banded(const HostCsr& h, hipStream_t s)
     {
       banded = false;
-      if(rows != cols || rows == 0 || rows >= (u64(1) << 31)) return rt::kOk;
+      // BandedRows indexes in 32 bits: at most kMaxDiag (64) diagonals of fewer than 2^26 rows
+      if(rows != cols || rows == 0 || rows >= (u64(1) << 26)) return rt::kOk;
       const double eps2 = band_eps2();
       std::vector<double> dg(rows, 0.0);
       for(u64 r = 0; r < rows; ++r)
@@ -773,6 +776,8 @@ namespace
         T.mo[a] = tmo[a];
         T.mi[a] = tmi[a];
       }
+      T.div0 = mg::udiv_magic_for(tmo[0]);
+      T.div1 = mg::udiv_magic_for(tmo[1]);
       T.rows = rows;
       return T;
     }
@@ -784,15 +789,32 @@ namespace
                    const std::vector<DT> w[3], hipStream_t s, bool* ok)
     {
       *ok = false;
+      if(mo[0] < 2 || mo[1] < 2) return rt::kOk;   // the row kernel divides by mo[0] and mo[1]
+      // the row kernel's reciprocal division, checked at every quotient boundary it meets
+      // (q < rows for mo[0], q < mo[1] mo[2] for mo[1])
+      const u64 lim[2] = {rows, u64(mo[1]) * mo[2]};
+      for(int a = 0; a < 2; ++a)
+      {
+        const u64 m = mg::udiv_magic_for(mo[a]);
+        for(u64 k = 1; k * mo[a] <= lim[a]; ++k)
+          if(mg::udiv_magic(u32(k * mo[a] - 1), m) != k - 1 ||
+             (k * mo[a] < lim[a] && mg::udiv_magic(u32(k * mo[a]), m) != k))
+            return rt::kOk;
+      }
+      std::vector<u32> sidx[3];
       for(int a = 0; a < 3; ++a)
       {
         tmo[a] = mo[a];
         tmi[a] = mi[a];
+        // indices scaled by the axis's stride in the input lattice: a column is iz + iy + ix
+        const u64 stride = a == 0 ? 1 : (a == 1 ? u64(mi[0]) : u64(mi[0]) * mi[1]);
+        sidx[a].resize(idx[a].size());
+        for(std::size_t k = 0; k < idx[a].size(); ++k) sidx[a][k] = u32(u64(idx[a][k]) * stride);
         F3_RC(tcnt[a].alloc(sizeof(u32) * cnt[a].size()));
         F3_RC(tidx[a].alloc(sizeof(u32) * idx[a].size()));
         F3_RC(tw[a].alloc(sizeof(DT) * w[a].size()));
         F3_CHECK(hipMemcpyAsync(tcnt[a].p, cnt[a].data(), sizeof(u32) * cnt[a].size(), hipMemcpyHostToDevice, s));
-        F3_CHECK(hipMemcpyAsync(tidx[a].p, idx[a].data(), sizeof(u32) * idx[a].size(), hipMemcpyHostToDevice, s));
+        F3_CHECK(hipMemcpyAsync(tidx[a].p, sidx[a].data(), sizeof(u32) * idx[a].size(), hipMemcpyHostToDevice, s));
         F3_CHECK(hipMemcpyAsync(tw[a].p, w[a].data(), sizeof(DT) * w[a].size(), hipMemcpyHostToDevice, s));
       }
       tw_bytes = sizeof(DT);
@@ -1012,6 +1034,8 @@ namespace
   {
     mg::TensorRows<DT, K, D3> r;
     for(int a = 0; a < 3; ++a) { r.ax[a] = t.ax[a]; r.mo[a] = t.mo[a]; r.mi[a] = t.mi[a]; }
+    r.div0 = t.div0;
+    r.div1 = t.div1;
     r.rows = t.rows;
     return r;
   }
@@ -1110,6 +1134,10 @@ namespace
     // transfers of structured hierarchies evaluated as tensor products of their 1D factors
     // (build_tensor_pair: detected and checked entry by entry; identical results, no matrix data)
     bool tensor_transfer = true;
+    // the V-cycle kernels of a solve return at once when the solver status has left "progress"
+    // (0: they run regardless; the PCG kernels behind them still check it, so the results are the
+    // same, and a cycle launched after the last iteration does its work for nothing)
+    bool vcycle_status = true;
     // launch shape of the transfer matrices: lanes per row (-1: as the level matrices) and row
     // positions per block (0: derived from the row count).  128 rows per block (more, smaller
     // blocks than the default grid on the mid-size levels) measured V-cycle 3D Q1 L7 584 -> 580 us,
@@ -1133,7 +1161,10 @@ namespace
     virtual ~EngineBase() {}
     virtual int init(const std::vector<HostLevel>& lv, const Config& cfg, hipStream_t s) = 0;
     virtual int vcycle(double* cor, const double* def) = 0;
-    virtual int solve(double* x, const double* b, bool correct, int* status, u64* iters) = 0;
+    // epilogue (optional): enqueued on the solver's stream after the solve's last launch, before
+    // the final wait
+    virtual int solve(double* x, const double* b, bool correct, int* status, u64* iters,
+                      const std::function<int()>* epilogue = nullptr) = 0;
     virtual int history(double* out, u64 cap, u64* len) = 0;
     virtual int time_kernel(int which, int reps, double* avg_ms, u64* bytes) = 0;
     virtual int info(u64* out, int cap) = 0;
@@ -1163,21 +1194,30 @@ namespace
     DevMem r, p, q, zcg, state, hist, partials;
     const DevCsr<double>* A64 = nullptr;
     u64 n0 = 0, hist_cap = 0;
-    // an iteration is captured as two graphs: [0] outer half (q = A p, alpha, x and r update,
-    // stopping criterion), [1] preconditioner half (z = M r, beta, p update)
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    static constexpr int kRing = 4;            // host copies of the solver state in flight
-    mg::PcgState* pinned = nullptr;
-    hipEvent_t ev[kRing] = {nullptr, nullptr, nullptr, nullptr};
+    // an iteration is captured as one graph: the preconditioner half of iteration k (z = M r,
+    // beta, p update; for k = 0 gamma and p from the first preconditioner application), the
+    // outer half of iteration k + 1 (q = A p, alpha, x and r update, stopping criterion) and the
+    // publication of the state
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    // published solver states (mg::k_publish): pinned, host-coherent; ring_dev is its device
+    // address, pub_count the device's count of publications in the running solve
+    static constexpr int kRing = 4;
+    struct HostRing
+    {
+      mg::PcgState st[kRing];
+      unsigned seq[kRing];
+    };
+    HostRing* ring = nullptr;
+    HostRing* ring_dev = nullptr;
+    DevMem pub_count;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     DevMem timing_state;   // solver state "in progress" for timing the outer product alone
 
     ~Engine() override
     {
       drop_graphs();
-      if(pinned) (void)hipHostFree(pinned);
-      for(auto& e : ev) if(e) (void)hipEventDestroy(e);
+      if(ring) (void)hipHostFree(ring);
       if(t0) (void)hipEventDestroy(t0);
       if(t1) (void)hipEventDestroy(t1);
       for(hipEvent_t e : stat_ev) (void)hipEventDestroy(e);
@@ -1331,8 +1371,13 @@ namespace
       // 8192 blocks (mg::row_grid), the streaming ones at most 4096
       const u64 max_blocks = 8192;
       F3_RC(partials.alloc(sizeof(double) * max_blocks));
-      if(!pinned) F3_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), sizeof(mg::PcgState) * kRing, hipHostMallocDefault));
-      for(auto& e : ev) if(!e) F3_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      if(!ring)
+      {
+        F3_CHECK(hipHostMalloc(reinterpret_cast<void**>(&ring), sizeof(HostRing), hipHostMallocMapped | hipHostMallocCoherent));
+        std::memset(ring, 0, sizeof(HostRing));
+        F3_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&ring_dev), ring, 0));
+      }
+      F3_RC(pub_count.alloc(sizeof(unsigned)));
       if(!t0) F3_CHECK(hipEventCreate(&t0));
       if(!t1) F3_CHECK(hipEventCreate(&t1));
       F3_CHECK(hipStreamSynchronize(s));
@@ -1342,11 +1387,8 @@ namespace
 
     void drop_graphs()
     {
-      for(int i = 0; i < 2; ++i)
-      {
-        if(gexec[i]) { (void)hipGraphExecDestroy(gexec[i]); gexec[i] = nullptr; }
-        if(graph[i]) { (void)hipGraphDestroy(graph[i]); graph[i] = nullptr; }
-      }
+      if(gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+      if(graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
     }
 
     mg::PcgState* st() const { return state.template as<mg::PcgState>(); }
@@ -1883,7 +1925,7 @@ namespace
     {
       mg::PcgState* S = st();
       double* part = partials.template as<double>();
-      F3_RC(enqueue_precond(status_ptr()));
+      F3_RC(enqueue_precond(cfg.vcycle_status ? status_ptr() : nullptr));
       F3_RC(with_z([&](auto z) {
         mg::k_pcg_gamma_beta<<<stream_blocks(n0), kBlock, 0, s>>>(n0, z, r.template as<double>(), S, part);
         fin<mg::kFinBeta>(stream_blocks(n0));
@@ -1895,7 +1937,22 @@ namespace
     double* x_cur = nullptr;   // the solution vector of the running solve
     double* x_graph = nullptr; // the one the captured graph was built for
 
-    int solve(double* x, const double* b, bool correct, int* status, u64* iters) override
+    // the preconditioner half of iteration k, the outer half of iteration k + 1 and the
+    // publication of its state: one graph
+    int enqueue_iteration()
+    {
+      F3_RC(enqueue_precond_half());
+      F3_RC(enqueue_outer_half());
+      return enqueue_publish();
+    }
+    int enqueue_publish()
+    {
+      mg::k_publish<kRing><<<1, 64, 0, s>>>(st(), ring_dev->st, ring_dev->seq, pub_count.template as<unsigned>());
+      return rt::check_launch();
+    }
+
+    int solve(double* x, const double* b, bool correct, int* status, u64* iters,
+              const std::function<int()>* epilogue) override
     {
       if(!x || !b) return rt::kErrArg;
       mg::PcgState h{};
@@ -1904,30 +1961,22 @@ namespace
       h.min_iter = cfg.min_iter; h.max_iter = cfg.max_iter; h.min_stag_iter = cfg.min_stag_iter;
       h.hist_cap = hist_cap; h.status = mg::kUndefined;
       F3_CHECK(hipMemcpyAsync(state.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
-      mg::PcgState* S = st();
+      F3_CHECK(hipMemsetAsync(pub_count.p, 0, sizeof(unsigned), s));
       double* part = partials.template as<double>();
       x_cur = x;
-      if(cfg.use_graph && (gexec[0] == nullptr || x_graph != x))
+      if(cfg.use_graph && (gexec == nullptr || x_graph != x))
       {
         drop_graphs();
-        for(int half = 0; half < 2; ++half)
-        {
-          F3_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-          const int rc = half == 0 ? enqueue_outer_half() : enqueue_precond_half();
-          hipGraph_t g = nullptr;
-          const hipError_t e = hipStreamEndCapture(s, &g);
-          if(rc) { if(g) (void)hipGraphDestroy(g); return rc; }
-          if(e != hipSuccess) return rt::kErrRuntime;
-          graph[half] = g;
-          F3_CHECK(hipGraphInstantiate(&gexec[half], graph[half], nullptr, nullptr, 0));
-        }
+        F3_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_iteration();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if(rc) { if(g) (void)hipGraphDestroy(g); return rc; }
+        if(e != hipSuccess) return rt::kErrRuntime;
+        graph = g;
+        F3_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
         x_graph = x;
       }
-      auto launch_half = [&](int half) -> int {
-        if(cfg.use_graph) F3_CHECK(hipGraphLaunch(gexec[half], s));
-        else F3_RC(half == 0 ? enqueue_outer_half() : enqueue_precond_half());
-        return rt::kOk;
-      };
       if(correct)
       {
         unsigned nb = 0;
@@ -1945,69 +1994,75 @@ namespace
         fin<mg::kFinInitDefect>(stream_blocks(n0));
       }
       F3_RC(rt::check_launch());
-      // z = M r, gamma = r.z, p = z: the preconditioner half with beta = 0 (state flag "first")
-      F3_RC(launch_half(1));
+      F3_RC(enqueue_publish());   // publication 1: the initial defect
 
-      // host copy of the solver state after iteration k (0: after the initial defect)
-      auto post_state = [&](u64 k) -> int {
-        F3_CHECK(hipMemcpyAsync(&pinned[k % kRing], S, sizeof(mg::PcgState), hipMemcpyDeviceToHost, s));
-        F3_CHECK(hipEventRecord(ev[k % kRing], s));
-        return rt::kOk;
-      };
-      std::vector<double> seen_def;   // defects of iterations 0..known, as read by the host
+      // publications launched and read (in order); the host reads publication j from slot
+      // (j - 1) % kRing once its sequence number reads j
+      unsigned launched = 1, read = 0;
+      std::vector<double> seen_def;   // defects of iterations 0..read-1
       int seen = mg::kProgress;
-      // reads the state of the next iteration not read yet (states are read in order)
+      mg::PcgState last{};
       auto read_state = [&]() -> int {
-        const u64 k = seen_def.size();
-        F3_CHECK(hipEventSynchronize(ev[k % kRing]));
-        const mg::PcgState& h = pinned[k % kRing];
-        seen = h.status;
-        seen_def.push_back(h.def_cur);
+        const unsigned want = read + 1, slot = (want - 1) % kRing;
+        for(unsigned spin = 1; __atomic_load_n(&ring->seq[slot], __ATOMIC_ACQUIRE) != want; ++spin)
+          if(spin % 256 == 0)
+          {
+            // the wait ends: once the stream has drained, the publication has arrived or is lost
+            const hipError_t q = hipStreamQuery(s);
+            if(q == hipSuccess)
+            {
+              if(__atomic_load_n(&ring->seq[slot], __ATOMIC_ACQUIRE) == want) break;
+              return rt::kErrRuntime;
+            }
+            if(q != hipErrorNotReady) return rt::kErrRuntime;
+          }
+        last = ring->st[slot];
+        seen = last.status;
+        seen_def.push_back(last.def_cur);
+        read = want;
         return rt::kOk;
       };
-      // Host loop.  Far from convergence an iteration is launched whole while the previous
-      // one runs, and the state of the one before is read (one iteration in flight ahead).
-      // When the defect history read so far predicts that iteration k may meet the criterion,
-      // only its outer half is launched and its state read before the preconditioner half, so
-      // that no iteration of skipped kernels (each kernel checks the status and returns) is
-      // queued behind the last one.
+      auto launch_iteration = [&]() -> int {
+        if(cfg.use_graph) F3_CHECK(hipGraphLaunch(gexec, s));
+        else F3_RC(enqueue_iteration());
+        ++launched;
+        return rt::kOk;
+      };
+      // Host loop.  Far from convergence the next iteration is launched while the previous one
+      // runs (the host reads states one publication behind).  When the defect history read so far
+      // predicts that the outer half just launched may meet the criterion, its state is read
+      // before the next graph (whose kernels would all return at once) is launched.
       const u64 max_launch = cfg.max_iter + 1;
-      F3_RC(post_state(0));
-      u64 k = 0;   // iterations launched
+      u64 k = 0;          // outer halves launched
+      bool wait = false;  // the latest outer half may meet the criterion
       while(k < max_launch && seen == mg::kProgress)
       {
-        const u64 next = k + 1;
-        bool near = next >= cfg.max_iter;
+        if(wait)
+        {
+          while(read < launched && seen == mg::kProgress) F3_RC(read_state());
+          if(seen != mg::kProgress) break;
+        }
+        F3_RC(launch_iteration());
+        ++k;
+        wait = k >= cfg.max_iter;
         if(seen_def.size() >= 2)
         {
           const std::size_t j = seen_def.size() - 1;   // latest iteration read
           const double d0 = seen_def[0], dj = seen_def[j], dp = seen_def[j - 1];
           const double rate = dp > 0.0 ? dj / dp : 1.0;
           const double goal = std::min(cfg.tol_abs, std::max(cfg.tol_rel * d0, cfg.tol_abs_low));
-          const double pred = dj * std::pow(std::min(rate, 1.0), double(next - j));
-          near = near || pred <= 10.0 * goal;
+          const double pred = dj * std::pow(std::min(rate, 1.0), double(k - j));
+          wait = wait || pred <= 10.0 * goal;
         }
-        F3_RC(launch_half(0));
-        F3_RC(post_state(next));
-        if(near)
-        {
-          // read every state up to this iteration's, then decide on its second half
-          while(seen_def.size() <= next && seen == mg::kProgress) F3_RC(read_state());
-          if(seen != mg::kProgress) { k = next; break; }
-          F3_RC(launch_half(1));
-          k = next;
-          continue;
-        }
-        F3_RC(launch_half(1));
-        k = next;
-        while(seen_def.size() < k && seen == mg::kProgress) F3_RC(read_state());
+        if(!wait)
+          while(read + 1 < launched && seen == mg::kProgress) F3_RC(read_state());
       }
+      if(epilogue) F3_RC((*epilogue)());
+      while(read < launched) F3_RC(read_state());   // the last publication is the final state
       F3_CHECK(hipStreamSynchronize(s));
-      mg::PcgState out{};
-      F3_CHECK(hipMemcpy(&out, state.p, sizeof(out), hipMemcpyDeviceToHost));
-      if(status) *status = out.status;
-      if(iters) *iters = out.num_iter;
-      last_hist_len = std::min<u64>(out.num_iter + 1, hist_cap);
+      if(status) *status = last.status;
+      if(iters) *iters = last.num_iter;
+      last_hist_len = std::min<u64>(last.num_iter + 1, hist_cap);
       return rt::kOk;
     }
 
@@ -2252,6 +2307,10 @@ namespace
   {
     return feat3_device_permute_vector(h->stream, h->n0, h->order0.as<u32>(), 0, src, dst);
   }
+  int to_caller_async(feat3_mgpcg* h, const double* src, double* dst)
+  {
+    return feat3_device_permute_vector(h->stream, h->n0, h->order0.as<u32>(), 1, src, dst);
+  }
   int to_caller(feat3_mgpcg* h, const double* src, double* dst)
   {
     F3_RC(feat3_device_permute_vector(h->stream, h->n0, h->order0.as<u32>(), 1, src, dst));
@@ -2456,6 +2515,7 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
   else if(k == "tail_max_n" && value >= 0) h->cfg.tail_max_n = u64(value);
   else if(k == "transfer_csr") h->cfg.transfer_csr = value != 0.0;
   else if(k == "tensor_transfer") h->cfg.tensor_transfer = value != 0.0;
+  else if(k == "vcycle_status") h->cfg.vcycle_status = value != 0.0;
   else if(k == "transfer_lanes" && (value == -1 || value == 0 || value == 2 || value == 4 || value == 8 || value == 16 ||
                                     value == 32 || value == 64))
     h->cfg.transfer_lanes = int(value);
@@ -2523,8 +2583,9 @@ int feat3_mgpcg_correct(feat3_mgpcg* h, void* caller_stream, double* x, const do
   if(!x || !b) return rt::kErrArg;
   F3_RC(to_internal(h, b, h->bi.as<double>()));
   F3_RC(to_internal(h, x, h->xi.as<double>()));
-  F3_RC(h->eng->solve(h->xi.as<double>(), h->bi.as<double>(), true, status, iters));
-  return to_caller(h, h->xi.as<double>(), x);
+  // the solution goes back to the caller's numbering right behind the solve's last launch
+  const std::function<int()> back = [&]() { return to_caller_async(h, h->xi.as<double>(), x); };
+  return h->eng->solve(h->xi.as<double>(), h->bi.as<double>(), true, status, iters, &back);
 }
 
 // SolverBase::apply(cor, def): cor = PCG(def) from a zero start
@@ -2537,8 +2598,8 @@ int feat3_mgpcg_apply(feat3_mgpcg* h, void* caller_stream, double* cor, const do
   if(!h->permuted0) return h->eng->solve(cor, def, false, status, iters);
   if(!cor || !def) return rt::kErrArg;
   F3_RC(to_internal(h, def, h->bi.as<double>()));
-  F3_RC(h->eng->solve(h->xi.as<double>(), h->bi.as<double>(), false, status, iters));
-  return to_caller(h, h->xi.as<double>(), cor);
+  const std::function<int()> back = [&]() { return to_caller_async(h, h->xi.as<double>(), cor); };
+  return h->eng->solve(h->xi.as<double>(), h->bi.as<double>(), false, status, iters, &back);
 }
 
 // the MultiGrid preconditioner alone (one V-cycle): cor = MG(def), FP64 in and out
