@@ -76,20 +76,34 @@ namespace batch
 #endif
   constexpr int kU = F3_BATCH_UNROLL;
 
-  template<typename DT_, typename VT_, int B_, bool SHARED_, int S_>
+  // DIAG_ (shared values, B > 1): every block is diagonal, and only its diagonal is stored,
+  // [entry][h]; the off-diagonal entries of the blocks are exact zeros, so the row sums take the
+  // terms of the full blocks that can be nonzero, in the same order
+  template<typename DT_, typename VT_, int B_, bool SHARED_, int S_, bool DIAG_ = false>
   struct BlockRows
   {
     const u32* rp;
     const u32* col;
-    const VT_* val;   // [entry][h][w] (shared) or [entry][k] (per system, B = 1)
+    const VT_* val;   // [entry][h][w] (shared), [entry][h] (shared, DIAG_) or [entry][k] (per system, B = 1)
     u64 rows;
     int K;
     const u32* ord;   // processing order of the rows (null: ascending)
     static constexpr bool kShared = SHARED_;
+    static constexpr int kVpe = DIAG_ ? B_ : B_ * B_;   // stored values per entry (shared values)
 
     __device__ __forceinline__ void acc(u32 e, int k0, const DT_ (&xv)[B_][S_], DT_ (&s)[B_][S_]) const
     {
-      if constexpr(SHARED_)
+      if constexpr(SHARED_ && DIAG_)
+      {
+#pragma unroll
+        for(int h = 0; h < B_; ++h)
+        {
+          const DT_ a = dev::ld<DT_>(val, u64(e) * B_ + u64(h));
+#pragma unroll
+          for(int i = 0; i < S_; ++i) s[h][i] += a * xv[h][i];
+        }
+      }
+      else if constexpr(SHARED_)
       {
 #pragma unroll
         for(int h = 0; h < B_; ++h)
@@ -114,6 +128,17 @@ namespace batch
     // shared values of entry e from a block-local copy (entries e0.. staged in LDS)
     __device__ __forceinline__ void acc_lds(u32 e, u32 e0, const VT_* lds, const DT_ (&xv)[B_][S_], DT_ (&s)[B_][S_]) const
     {
+      if constexpr(DIAG_)
+      {
+#pragma unroll
+        for(int h = 0; h < B_; ++h)
+        {
+          const DT_ a = cvt<DT_, VT_>(lds[(e - e0) * u32(B_) + u32(h)]);
+#pragma unroll
+          for(int i = 0; i < S_; ++i) s[h][i] += a * xv[h][i];
+        }
+        return;
+      }
 #pragma unroll
       for(int h = 0; h < B_; ++h)
 #pragma unroll
@@ -256,7 +281,7 @@ namespace batch
   template<int G_, int S_, int B_, int CAP_, typename Rows_, typename DT_, typename VT_, typename F>
   __device__ __forceinline__ void for_block_rows_staged(const Rows_& A, u64 per, const DT_* __restrict__ x, VT_* lds, F&& f)
   {
-    constexpr u32 BB = u32(B_ * B_);
+    constexpr u32 BB = u32(Rows_::kVpe);
     constexpr unsigned R = 256 / G_;
     const u64 b = dev::xcd_remap(blockIdx.x, gridDim.x);
     const u64 beg = b * per;
@@ -329,12 +354,12 @@ namespace batch
   // Measured on the 32-RHS 3x3 sweep (3D Q1 L6): staging FP64 values 725 -> 507 us (outer FP64
   // SpMV 643 -> 490 us); staging FP32 values 260 -> 307 us (the 36 KB of LDS per block cost more
   // occupancy than the broadcast loads they replace), so only 8-byte values are staged.
-  template<int B_, typename VT_>
+  template<int VPE_, typename VT_>
   struct Stage
   {
     static constexpr bool enabled = sizeof(VT_) == 8;
     static constexpr int kBytes = 36 * 1024;
-    static constexpr int CAP = kBytes / int(B_ * B_ * sizeof(VT_));
+    static constexpr int CAP = kBytes / int(VPE_ * sizeof(VT_));   // VPE_: stored values per entry
   };
 
   // runs the row loop of a kernel, staged when the values are shared and the order ascending
@@ -342,10 +367,10 @@ namespace batch
   __device__ __forceinline__ void row_loop(const Rows_& A, u64 per, const DT_* __restrict__ x, F&& f)
   {
     typedef typename std::remove_const<typename std::remove_pointer<decltype(A.val)>::type>::type VT;
-    if constexpr(Rows_::kShared && Stage<B_, VT>::enabled)
+    if constexpr(Rows_::kShared && Stage<Rows_::kVpe, VT>::enabled)
     {
-      constexpr int CAP = Stage<B_, VT>::CAP;
-      __shared__ VT lds[CAP * B_ * B_];
+      constexpr int CAP = Stage<Rows_::kVpe, VT>::CAP;
+      __shared__ VT lds[CAP * Rows_::kVpe];
       if(A.ord == nullptr)
       {
         for_block_rows_staged<G_, S_, B_, CAP>(A, per, x, lds, f);
@@ -478,6 +503,25 @@ namespace batch
     u64 num_iter, num_stag_iter;
     int status, pad;
   };
+
+  // Publication of the K system states to the host (one block of 64 threads): publication c of a
+  // solve (counted on the device in *count) goes to slot c % R of a ring in pinned host memory
+  // (64 states per slot), each thread's copy fenced at system scope, then, after the barrier, the
+  // sequence number c + 1 to seq[c % R], which the host polls.  Runs whatever the statuses.
+  template<int R>
+  __global__ __launch_bounds__(64) void k_publish_states(const SysState* __restrict__ st, int K, SysState* ring,
+                                                         unsigned* seq, unsigned* count)
+  {
+    const unsigned c = *count;   // read by every thread before the barrier below
+    if(int(threadIdx.x) < K) ring[u64(c % R) * 64 + threadIdx.x] = st[threadIdx.x];
+    __threadfence_system();
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+      reinterpret_cast<volatile unsigned*>(seq)[c % R] = c + 1;
+      *count = c + 1;
+    }
+  }
 
   struct Criteria
   {
@@ -655,33 +699,33 @@ namespace batch
     }
   }
 
-  // finalisation of one per-system reduction (one block of 256 threads); also refreshes the
-  // "any system in progress" flag after a status change
+  // finalisation of one per-system reduction, first stage: block k sums system k's nb block
+  // partials (stride K) in a fixed order, eight loads in flight per thread, into sums[k] (one
+  // block for all K systems would read nb * K partials alone: 58-68 us at 8192 x 32)
+  __global__ __launch_bounds__(256) void k_fin_sum(const double* __restrict__ partials, unsigned nb, int K,
+                                                   double* __restrict__ sums)
+  {
+    const u64 k = blockIdx.x;
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const unsigned step = blockDim.x;
+    unsigned b = threadIdx.x;
+    for(; b + 7 * step < nb; b += 8 * step)
+#pragma unroll
+      for(unsigned j = 0; j < 8; ++j) acc[j] += partials[u64(b + j * step) * u64(K) + k];
+    for(; b < nb; b += step) acc[0] += partials[u64(b) * u64(K) + k];
+    const double t = dev::block_sum(((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7])));
+    if(threadIdx.x == 0) sums[k] = t;
+  }
+
+  // second stage (one block): the sums into every system's PCG scalars and criteria; also
+  // refreshes the "any system in progress" flag after a status change
   template<int OP>
-  __global__ __launch_bounds__(256) void k_fin(const double* __restrict__ partials, unsigned nb, int K, SysState* st,
+  __global__ __launch_bounds__(256) void k_fin(const double* __restrict__ sums, int K, SysState* st,
                                                const Criteria* cr, double* hist, int* any)
   {
-    __shared__ double red[256];
-    const int k = int(threadIdx.x % 64);
-    const unsigned slice = threadIdx.x / 64;
-    // eight accumulators per thread: eight loads in flight (the partials come from memory)
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if(k < K)
-    {
-      const unsigned step = blockDim.x / 64;
-      unsigned b = slice;
-      for(; b + 7 * step < nb; b += 8 * step)
-#pragma unroll
-        for(unsigned j = 0; j < 8; ++j) acc[j] += partials[u64(b + j * step) * u64(K) + u64(k)];
-      for(; b < nb; b += step) acc[0] += partials[u64(b) * u64(K) + u64(k)];
-    }
-    const double s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-    red[threadIdx.x] = s;
-    __syncthreads();
     if(int(threadIdx.x) < K)
     {
-      double t = 0.0;
-      for(unsigned sl = 0; sl < blockDim.x / 64; ++sl) t += red[sl * 64 + threadIdx.x];
+      const double t = sums[threadIdx.x];
       SysState& S = st[threadIdx.x];
       double* h = hist + u64(threadIdx.x) * cr->hist_cap;
       if(OP == mg::kFinInitDefect)

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <string>
 #include <type_traits>
@@ -79,6 +80,9 @@ namespace
     u64 min_iter = 0, max_iter = 1000;
     bool use_graph = true;
     u64 tile_rows = 0;     // rows per locality tile of the row kernels (0/1: ascending order)
+    // shared B x B values whose every block is diagonal are stored as block diagonals (the
+    // off-diagonal exact zeros left out, as the transfers' structural zeros are)
+    bool diag_blocks = true;
   };
 
   // drop stored exact zeros of a scalar graph (the transfers' structural zeros)
@@ -265,7 +269,9 @@ namespace
     return rt::kOk;
   }
 
-  template<typename DT, typename VT, int B, bool SHARED>
+  // DIAG (shared 3x3 values whose every block is diagonal, e.g. a vector Laplacian): the device
+  // copies of the level matrices and of the FP64 outer matrix hold only the block diagonals
+  template<typename DT, typename VT, int B, bool SHARED, bool DIAG = false>
   struct Engine : public BatchBase
   {
     struct Level
@@ -282,15 +288,24 @@ namespace
     int K = 1;
     int G = 16, GV = 16, S = 1, GO = 16, SO = 1;   // streaming, V-cycle rows, outer FP64 rows
     std::vector<std::unique_ptr<Level>> L;
-    DevMem ainv, a64, r, p, q, st, cr, hist, partials, any;
+    DevMem ainv, a64, r, p, q, st, cr, hist, partials, sums, any;
     DevMem orp, ocol;   // the outer product's graph where level 0's inner graph is thinned
     u64 m0 = 0, hist_cap = 0, ncoarse_rows = 0;
-    // host copies of the system states (K SysState each) of the last kRing iterations
+    // published system states (batch::k_publish_states): a ring of kRing slots of 64 states in
+    // pinned, host-coherent memory (ring_dev: its device address), pub_count the device's count
     static constexpr int kRing = 4;
-    batch::SysState* pinned = nullptr;
-    hipEvent_t ev[kRing] = {nullptr, nullptr, nullptr, nullptr}, t0 = nullptr, t1 = nullptr;
-    // an iteration as two graphs: [0] outer half (q = A p, alpha, x and r, criteria),
-    // [1] preconditioner half (V-cycle, beta, p)
+    struct HostRing
+    {
+      batch::SysState st[kRing][64];
+      unsigned seq[kRing];
+    };
+    HostRing* ring = nullptr;
+    HostRing* ring_dev = nullptr;
+    DevMem pub_count;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    // graphs: [0] the first outer half (q = A p, alpha, x and r, criteria) and the publication;
+    // [1] the preconditioner half of iteration k (V-cycle, beta, p), the outer half of k + 1 and
+    // the publication
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     void drop_graphs()
@@ -315,14 +330,23 @@ namespace
     ~Engine() override
     {
       drop_graphs();
-      if(pinned) (void)hipHostFree(pinned);
-      for(auto& e : ev) if(e) (void)hipEventDestroy(e);
+      if(ring) (void)hipHostFree(ring);
       if(t0) (void)hipEventDestroy(t0);
       if(t1) (void)hipEventDestroy(t1);
     }
 
-    static constexpr u64 vpe() { return u64(B) * u64(B); }   // values per entry and system
-    u64 vals_per_entry() const { return vpe() * (SHARED ? 1 : u64(K)); }
+    static constexpr u64 vpe() { return u64(B) * u64(B); }   // values per entry and system (as pushed)
+    static constexpr u64 svpe() { return DIAG ? u64(B) : vpe(); }   // as stored on the device
+    u64 vals_per_entry() const { return svpe() * (SHARED ? 1 : u64(K)); }
+    // the block diagonals of full B x B blocks (DIAG); other layouts unchanged
+    static std::vector<double> stored(const std::vector<double>& v)
+    {
+      if(!DIAG) return v;
+      std::vector<double> o(v.size() / vpe() * B);
+      for(std::size_t e = 0; e < v.size() / vpe(); ++e)
+        for(int h = 0; h < B; ++h) o[e * B + h] = v[e * vpe() + u64(h) * B + u64(h)];
+      return o;
+    }
 
     template<typename T> std::vector<T> conv(const std::vector<double>& v) const
     {
@@ -331,7 +355,7 @@ namespace
       return o;
     }
 
-    template<int S_> batch::BlockRows<DT, VT, B, SHARED, S_> rowsA(const Level& l) const
+    template<int S_> batch::BlockRows<DT, VT, B, SHARED, S_, DIAG> rowsA(const Level& l) const
     {
       return {l.arp.template as<u32>(), l.acol.template as<u32>(), l.aval.template as<VT>(), l.nodes, K, l.order()};
     }
@@ -344,7 +368,7 @@ namespace
     {
       return {l.rrp.template as<u32>(), l.rcol.template as<u32>(), l.rval.template as<VT>(), l.nc, K, c.order()};
     }
-    template<int S_> batch::BlockRows<double, double, B, SHARED, S_> rows64() const
+    template<int S_> batch::BlockRows<double, double, B, SHARED, S_, DIAG> rows64() const
     {
       const bool own = L[0]->thinned;
       return {(own ? orp : L[0]->arp).template as<u32>(), (own ? ocol : L[0]->acol).template as<u32>(), a64.template as<double>(),
@@ -370,7 +394,7 @@ namespace
           l->nnz = T.rp[h.nodes];
           F3_RC(l->arp.put(T.rp, s));
           F3_RC(l->acol.put(T.col, s));
-          F3_RC(l->aval.put(conv<VT>(T.val), s));
+          F3_RC(l->aval.put(conv<VT>(stored(T.val)), s));
           F3_CHECK(hipStreamSynchronize(s));   // T leaves scope
         }
         else
@@ -378,7 +402,7 @@ namespace
           l->nnz = h.A.rp[h.nodes];
           F3_RC(l->arp.put(h.A.rp, s));
           F3_RC(l->acol.put(h.A.col, s));
-          F3_RC(l->aval.put(conv<VT>(h.A.val), s));
+          F3_RC(l->aval.put(conv<VT>(stored(h.A.val)), s));
         }
         if(i + 1 < lv.size())
         {
@@ -464,7 +488,7 @@ namespace
     {
       // outer FP64
       m0 = l0.nodes * B;
-      F3_RC(a64.put(l0.A.val, s));
+      F3_RC(a64.put(stored(l0.A.val), s));
       if(L[0]->thinned)
       {
         F3_RC(orp.put(l0.A.rp, s));
@@ -477,10 +501,15 @@ namespace
       hist_cap = cfg.max_iter + 2;
       F3_RC(hist.alloc(sizeof(double) * hist_cap * K));
       F3_RC(partials.alloc(sizeof(double) * 8192 * u64(K)));
+      F3_RC(sums.alloc(sizeof(double) * u64(K)));
       F3_RC(any.alloc(64));
-      if(pinned) { (void)hipHostFree(pinned); pinned = nullptr; }
-      F3_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), sizeof(batch::SysState) * u64(K) * kRing, hipHostMallocDefault));
-      for(auto& e : ev) if(!e) F3_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      if(!ring)
+      {
+        F3_CHECK(hipHostMalloc(reinterpret_cast<void**>(&ring), sizeof(HostRing), hipHostMallocMapped | hipHostMallocCoherent));
+        std::memset(ring, 0, sizeof(HostRing));
+        F3_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&ring_dev), ring, 0));
+      }
+      F3_RC(pub_count.alloc(sizeof(unsigned)));
       if(!t0) F3_CHECK(hipEventCreate(&t0));
       if(!t1) F3_CHECK(hipEventCreate(&t1));
       F3_CHECK(hipStreamSynchronize(s));
@@ -571,10 +600,13 @@ namespace
       return rt::check_launch();
     }
 
+    // finalisation of a per-system reduction whose producing kernel ran with nb blocks: the K
+    // sums (one block per system), then the K systems' scalars (one block)
     template<int OP>
     void fin(unsigned nb)
     {
-      batch::k_fin<OP><<<1, kBlock, 0, s>>>(partials.template as<double>(), nb, K, st.template as<batch::SysState>(),
+      batch::k_fin_sum<<<unsigned(K), kBlock, 0, s>>>(partials.template as<double>(), nb, K, sums.template as<double>());
+      batch::k_fin<OP><<<1, kBlock, 0, s>>>(sums.template as<double>(), K, st.template as<batch::SysState>(),
                                              cr.template as<batch::Criteria>(), hist.template as<double>(), any.template as<int>());
     }
 
@@ -666,97 +698,117 @@ namespace
       }));
       fin<mg::kFinGamma>(sb);
       F3_RC(rt::check_launch());
+      F3_CHECK(hipMemsetAsync(pub_count.p, 0, sizeof(unsigned), s));
+      F3_RC(enqueue_publish());   // publication 1: the initial defects
+      auto enqueue_graph = [&](int g) -> int {
+        if(g == 0) { F3_RC(enqueue_outer_half()); return enqueue_publish(); }
+        F3_RC(enqueue_precond_half());
+        F3_RC(enqueue_outer_half());
+        return enqueue_publish();
+      };
       if(cfg.use_graph && (gexec[0] == nullptr || x_graph != x))
       {
         drop_graphs();
-        for(int half = 0; half < 2; ++half)
+        for(int g = 0; g < 2; ++g)
         {
           F3_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-          const int rc = half == 0 ? enqueue_outer_half() : enqueue_precond_half();
+          const int rc = enqueue_graph(g);
           hipGraph_t gr = nullptr;
           const hipError_t e = hipStreamEndCapture(s, &gr);
           if(rc) { if(gr) (void)hipGraphDestroy(gr); return rc; }
           if(e != hipSuccess) return rt::kErrRuntime;
-          graph[half] = gr;
-          F3_CHECK(hipGraphInstantiate(&gexec[half], graph[half], nullptr, nullptr, 0));
+          graph[g] = gr;
+          F3_CHECK(hipGraphInstantiate(&gexec[g], graph[g], nullptr, nullptr, 0));
         }
         x_graph = x;
       }
-      auto launch_half = [&](int half) -> int {
-        if(cfg.use_graph) F3_CHECK(hipGraphLaunch(gexec[half], s));
-        else F3_RC(half == 0 ? enqueue_outer_half() : enqueue_precond_half());
+      unsigned launched = 1, read = 0;   // publications launched and read (in order)
+      auto launch = [&](int g) -> int {
+        if(cfg.use_graph) F3_CHECK(hipGraphLaunch(gexec[g], s));
+        else F3_RC(enqueue_graph(g));
+        ++launched;
         return rt::kOk;
       };
-      auto post_state = [&](u64 k) -> int {
-        F3_CHECK(hipMemcpyAsync(pinned + (k % kRing) * u64(K), st.p, sizeof(batch::SysState) * u64(K), hipMemcpyDeviceToHost, s));
-        F3_CHECK(hipEventRecord(ev[k % kRing], s));
-        return rt::kOk;
-      };
-      // defects of every system after iterations 0..known, as read by the host
+      // defects of every system after iterations 0..read-1 (-1: stopped), as read by the host
       std::vector<std::vector<double>> seen;
+      std::vector<batch::SysState> last(K);
       bool running = true;
       auto read_state = [&]() -> int {
-        const u64 k = seen.size();
-        F3_CHECK(hipEventSynchronize(ev[k % kRing]));
-        const batch::SysState* h = pinned + (k % kRing) * u64(K);
+        const unsigned want = read + 1, slot = (want - 1) % kRing;
+        for(unsigned spin = 1; __atomic_load_n(&ring->seq[slot], __ATOMIC_ACQUIRE) != want; ++spin)
+          if(spin % 256 == 0)
+          {
+            // the wait ends: once the stream has drained, the publication has arrived or is lost
+            const hipError_t qs = hipStreamQuery(s);
+            if(qs == hipSuccess)
+            {
+              if(__atomic_load_n(&ring->seq[slot], __ATOMIC_ACQUIRE) == want) break;
+              return rt::kErrRuntime;
+            }
+            if(qs != hipErrorNotReady) return rt::kErrRuntime;
+          }
         std::vector<double> d(K);
         running = false;
         for(int j = 0; j < K; ++j)
         {
-          d[j] = h[j].status == mg::kProgress ? h[j].def_cur : -1.0;
-          running = running || h[j].status == mg::kProgress;
+          last[j] = ring->st[slot][j];
+          d[j] = last[j].status == mg::kProgress ? last[j].def_cur : -1.0;
+          running = running || last[j].status == mg::kProgress;
         }
         seen.push_back(std::move(d));
+        read = want;
         return rt::kOk;
       };
-      // the same host loop as the single-system solver: one iteration in flight ahead; when every
-      // running system's defect history predicts that the next iteration may meet its criterion,
-      // that iteration's outer half is read before its preconditioner half is launched
+      // Host loop, as the single-system solver's: the next iteration is launched while the
+      // previous one runs; when every running system's defect history predicts that the outer
+      // half just launched may meet its criterion, its states are read before the next graph is
+      // launched
       const u64 max_launch = cfg.max_iter + 1;
-      F3_RC(post_state(0));
-      u64 k = 0;
+      u64 k = 0;          // outer halves launched
+      bool wait = false;  // the latest outer half may end every running system
       while(k < max_launch && running)
       {
-        const u64 next = k + 1;
-        bool near = next >= cfg.max_iter;
-        if(!near && seen.size() >= 2)
+        if(wait)
+        {
+          while(read < launched && running) F3_RC(read_state());
+          if(!running) break;
+        }
+        F3_RC(launch(k == 0 ? 0 : 1));
+        ++k;
+        wait = k >= cfg.max_iter;
+        if(!wait && seen.size() >= 2)
         {
           const std::size_t jj = seen.size() - 1;
-          near = true;
-          for(int j = 0; j < K && near; ++j)
+          wait = true;
+          for(int j = 0; j < K && wait; ++j)
           {
             const double dj = seen[jj][j], dp = seen[jj - 1][j], d0 = seen[0][j];
             if(dj < 0.0) continue;   // this system has stopped
             const double rate = dp > 0.0 ? dj / dp : 1.0;
             const double goal = std::min(cfg.tol_abs, std::max(cfg.tol_rel * d0, cfg.tol_abs_low));
-            near = dj * std::pow(std::min(rate, 1.0), double(next - jj)) <= 10.0 * goal;
+            wait = dj * std::pow(std::min(rate, 1.0), double(k - jj)) <= 10.0 * goal;
           }
         }
-        F3_RC(launch_half(0));
-        F3_RC(post_state(next));
-        if(near)
-        {
-          while(seen.size() <= next && running) F3_RC(read_state());
-          if(!running) { k = next; break; }
-          F3_RC(launch_half(1));
-          k = next;
-          continue;
-        }
-        F3_RC(launch_half(1));
-        k = next;
-        while(seen.size() < k && running) F3_RC(read_state());
+        if(!wait)
+          while(read + 1 < launched && running) F3_RC(read_state());
       }
+      while(read < launched) F3_RC(read_state());   // the last publication holds the final states
       F3_CHECK(hipStreamSynchronize(s));
-      std::vector<batch::SysState> out(K);
-      F3_CHECK(hipMemcpy(out.data(), st.p, sizeof(batch::SysState) * K, hipMemcpyDeviceToHost));
       last_iters.assign(K, 0);
-      for(int k = 0; k < K; ++k)
+      for(int j = 0; j < K; ++j)
       {
-        if(status) status[k] = out[k].status;
-        if(iters) iters[k] = out[k].num_iter;
-        last_iters[k] = out[k].num_iter;
+        if(status) status[j] = last[j].status;
+        if(iters) iters[j] = last[j].num_iter;
+        last_iters[j] = last[j].num_iter;
       }
       return rt::kOk;
+    }
+
+    int enqueue_publish()
+    {
+      batch::k_publish_states<kRing><<<1, 64, 0, s>>>(st.template as<batch::SysState>(), K, &ring_dev->st[0][0], ring_dev->seq,
+                                                      pub_count.template as<unsigned>());
+      return rt::check_launch();
     }
 
     int history(int k, double* out, u64 cap, u64* len) override
@@ -773,8 +825,9 @@ namespace
     {
       if(level < 0 || level >= int(L.size()) || !dv) return rt::kErrArg;
       Level& l = *L[level];
-      // a thinned graph was chosen for the values given at init; other values need a new solver
-      if(l.thinned) return rt::kErrState;
+      // a thinned graph or the block-diagonal storage was chosen for the values given at init;
+      // other values need a new solver
+      if(l.thinned || DIAG) return rt::kErrState;
       const u64 nval = l.nnz * vals_per_entry();
       const unsigned nb = unsigned(std::max<u64>(1, std::min<u64>((nval + 255) / 256, 65536)));
       k_to_storage<VT><<<nb, 256, 0, s>>>(nval, dv, l.aval.template as<VT>());
@@ -845,12 +898,23 @@ namespace
   };
 
   template<typename DT, typename VT>
-  BatchBase* make_engine(int block, int K, bool shared)
+  BatchBase* make_engine(int block, int K, bool shared, bool diag)
   {
     if(block == 1 && !shared) return new Engine<DT, VT, 1, false>(K);
     if(block == 1 && shared) return new Engine<DT, VT, 1, true>(K);
+    if(block == 3 && shared && diag) return new Engine<DT, VT, 3, true, true>(K);
     if(block == 3 && shared) return new Engine<DT, VT, 3, true>(K);
     return nullptr;
+  }
+
+  // every block of every level matrix (shared B x B values) has exact zeros off its diagonal
+  bool all_blocks_diagonal(const std::vector<HostLevel>& lv, int B)
+  {
+    const u64 bb = u64(B) * u64(B);
+    for(const HostLevel& l : lv)
+      for(std::size_t i = 0; i < l.A.val.size(); ++i)
+        if(i % bb % u64(B + 1) != 0 && l.A.val[i] != 0.0) return false;
+    return true;
   }
 } // namespace
 
@@ -956,6 +1020,7 @@ int feat3_batch_set_option(feat3_batch* h, const char* name, double value)
   if(!h || !name) return rt::kErrArg;
   const std::string n(name);
   if(n == "tile_rows" && value >= 0) h->cfg.tile_rows = u64(value);
+  else if(n == "diag_blocks") h->cfg.diag_blocks = value != 0.0;
   else return rt::kErrArg;
   h->eng.reset();
   return rt::kOk;
@@ -965,9 +1030,10 @@ int feat3_batch_init(feat3_batch* h)
 {
   if(!h || h->levels.empty()) return rt::kErrState;
   std::unique_ptr<BatchBase> e;
-  if(h->inner == 0) e.reset(make_engine<double, double>(h->block, h->K, h->shared));
-  else if(h->inner == 1) e.reset(make_engine<float, float>(h->block, h->K, h->shared));
-  else e.reset(make_engine<float, half>(h->block, h->K, h->shared));
+  const bool diag = h->cfg.diag_blocks && h->block > 1 && h->shared && all_blocks_diagonal(h->levels, h->block);
+  if(h->inner == 0) e.reset(make_engine<double, double>(h->block, h->K, h->shared, diag));
+  else if(h->inner == 1) e.reset(make_engine<float, float>(h->block, h->K, h->shared, diag));
+  else e.reset(make_engine<float, half>(h->block, h->K, h->shared, diag));
   if(!e) return rt::kErrArg;
   const int rc = e->init(h->levels, h->cfg, h->stream);
   if(rc) return rc;

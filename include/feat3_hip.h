@@ -430,7 +430,10 @@ int feat3_batch_set_tolerances(feat3_batch* h, double tol_rel, double tol_abs, d
 /* graph replay of one PCG iteration on/off */
 int feat3_batch_set_graph(feat3_batch* h, int graph);
 /* named options, applied at the next init: "tile_rows" (rows per locality tile of the row
- * kernels, feat3_setup_tile_order; default 0 = ascending row order) */
+ * kernels, feat3_setup_tile_order; default 0 = ascending row order), "diag_blocks" (1, default:
+ * shared 3x3 values whose every block has exact zeros off its diagonal, a vector Laplacian's, are
+ * stored as block diagonals on the device, the FP64 outer copy included; such a solver then
+ * refuses feat3_batch_set_values; 0: full blocks) */
 int feat3_batch_set_option(feat3_batch* h, const char* name, double value);
 int feat3_batch_init(feat3_batch* h);
 /* x, b: FP64 device arrays [node][component][system], 16-byte aligned; status, iters: host

@@ -201,7 +201,44 @@ def test_thinned_graph_refuses_new_values(torch):
     with pytest.raises(Exception):
         capi.call("feat3_batch_set_values", s._h, 0, ctypes.c_void_p(dv.data_ptr()))
     s.done()
-    # FP64 keeps the full graph, so new values are taken
-    s = BatchMGPCG(H, 4, values=vals, block=3, inner="f64").init()
+    # FP64 keeps the full graph, so new values are taken -- with full blocks; the block-diagonal
+    # storage of these diagonal blocks (the default) was chosen for the values given at init too
+    s = BatchMGPCG(H, 4, values=vals, block=3, inner="f64", options={"diag_blocks": 0}).init()
     capi.call("feat3_batch_set_values", s._h, 0, ctypes.c_void_p(dv.data_ptr()))
     s.done()
+    s = BatchMGPCG(H, 4, values=vals, block=3, inner="f64").init()
+    with pytest.raises(Exception):
+        capi.call("feat3_batch_set_values", s._h, 0, ctypes.c_void_p(dv.data_ptr()))
+    s.done()
+
+
+@pytest.mark.parametrize("inner", ["f64", "f32", "f16"])
+@pytest.mark.parametrize("nsys", [4, 32])
+def test_diagonal_block_storage_is_identical(torch, inner, nsys):
+    """The vector Laplacian's 3x3 blocks are a_ij * I: stored as block diagonals (option
+    diag_blocks, default) the solve is bit for bit the full-block solve -- the left-out terms are
+    products with exact zeros, which change no partial sum."""
+    print(f"case diag blocks {inner} K={nsys}", flush=True)
+    H = pb.hierarchy(3, 1, 4, 1)
+    vals = [pb.block_identity_values(L.A, 3) for L in H.levels]
+    B = pb.random_rhs(H.n, 3, nsys, H.levels[0].bnd, seed=3)
+    s1, i1, x1, _ = gpu_batch(torch, H, B, nsys, values=vals, block=3, inner=inner)
+    s0, i0, x0, _ = gpu_batch(torch, H, B, nsys, values=vals, block=3, inner=inner, options={"diag_blocks": 0})
+    assert list(s1) == list(s0) and list(i1) == list(i0)
+    assert np.array_equal(x1, x0)
+
+
+@pytest.mark.parametrize("inner", ["f64", "f32"])
+def test_coupled_blocks_keep_full_storage(torch, inner):
+    """Blocks a_ij * M with a coupling M (SPD, tridiagonal) are not diagonal: the full 3x3 blocks
+    are kept, and every system meets its oracle solve (apply_generic.hpp:121-154 block products)."""
+    print(f"case coupled blocks {inner}", flush=True)
+    H = pb.hierarchy(3, 1, 3, 1)
+    M = np.array([[1.0, 0.25, 0.0], [0.25, 1.0, 0.25], [0.0, 0.25, 1.0]])
+    vals = [np.kron(L.A.val, M.ravel()) for L in H.levels]
+    nsys = 4
+    B = pb.random_rhs(H.n, 3, nsys, H.levels[0].bnd, seed=9)
+    g = gpu_batch(torch, H, B, nsys, values=vals, block=3, inner=inner)
+    for k in range(nsys):
+        o = orc_solve(H, B[:, :, k].reshape(-1).copy(), block=3, values=vals)
+        check_system(k, (g[0][k], g[1][k], g[2][:, :, k].reshape(-1), g[3][k]), o, inner)
