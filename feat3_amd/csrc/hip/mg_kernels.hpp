@@ -24,6 +24,23 @@ namespace mg
 
   enum : int { kUndefined = 0, kProgress = 1, kSuccess = 2, kAborted = 3, kDiverged = 4, kMaxIter = 5, kStagnated = 6 };
 
+  // Launch-shape knobs of the row kernels for measurements (make EXTRA=-D...): F3_BAND_UNROLL
+  // diagonals per batch of loads in the banded row sums, F3_ROW_WAVES an upper bound on the waves
+  // per SIMD of the matrix row kernels (0: the compiler's choice)
+#ifndef F3_BAND_UNROLL
+#define F3_BAND_UNROLL 9
+#endif
+#ifndef F3_ROW_WAVES
+#define F3_ROW_WAVES 0
+#endif
+#define F3_STR2(x) #x
+#define F3_STR(x) F3_STR2(x)
+#if F3_ROW_WAVES > 0
+#define F3_ROW_ATTR __attribute__((amdgpu_waves_per_eu(1, F3_ROW_WAVES)))
+#else
+#define F3_ROW_ATTR
+#endif
+
   // Row evaluators.  row_sum(p, lane, x) returns sum_j a_pj x_j for the row at position p,
   // reduced over the G lanes of the calling group (valid in every lane); row(p) maps the
   // position to the vector index (identity for CSR, the sigma permutation for SELL).
@@ -71,6 +88,7 @@ namespace mg
 
   template<typename T> struct IsStream { static constexpr bool value = false; };
   template<typename DT_, typename VT_> struct IsStream<CsrStreamRows<DT_, VT_>> { static constexpr bool value = true; };
+  template<typename T> struct IsTensor { static constexpr bool value = false; };
 
   // column storage of the derived SELL format: absolute u32 indices, or 16-bit signed offsets
   // from the row index (usable when every |col - row| < 2^15, e.g. lexicographic 3D grids up
@@ -131,10 +149,31 @@ namespace mg
   // the upper one stored at row c (its column), the same number, so the row sum takes the same
   // terms in the same order as the full band; the half of the values read a second time come
   // from the L2 (row c lies at most one node plane before p).
-  template<typename DT_, typename VT_, bool SYM_ = false>
+  // x of a banded matrix staged in LDS (BandedRows with STAGE_): a block evaluates its rows in
+  // rounds of kStageRows consecutive rows r0.., and the columns those rows reach form a few
+  // windows (3D Q1 in lattice order: 516 columns each in the node plane below, its own and the one
+  // above), which all threads copy into LDS with contiguous loads before the round's row sums read
+  // x from there: one global load per staged value instead of one gather per matrix entry (3D Q1:
+  // 1548 loads instead of 6912 per round of 256 rows).  Window w holds the columns
+  // r0 + wstart[w] .. r0 + wstart[w] + wlen[w] - 1 at LDS offset wbase[w] (a column outside the
+  // matrix is staged as 0, and its product is selected away as in the unstaged sum); diagonal d
+  // reads the column of row p at xoff[d] + (p - r0).
+  constexpr u32 kStageRows = 256;
+  constexpr u32 kStageWin = 8;
+  constexpr u32 kStageBytes = 32 * 1024;
+  struct BandStage
+  {
+    u32 nwin;
+    int wstart[kStageWin];
+    u32 wlen[kStageWin], wbase[kStageWin];
+    const u32* xoff;   // per diagonal, device memory
+  };
+
+  template<typename DT_, typename VT_, bool SYM_ = false, bool STAGE_ = false>
   struct BandedRows
   {
     static constexpr int G = 1;
+    static constexpr bool kStaged = STAGE_;
     // FP64 values are read with nontemporal loads: at 3D Q1 level 7 (464 MB, larger than the
     // 256 MB Infinity Cache) the FP64 sweep went 117 -> 107 us and the outer PCG product
     // 112 -> 106 us (FP32-inner solve 3.31 -> 3.26 ms, same GPU, alternating builds).  FP32
@@ -150,6 +189,7 @@ namespace mg
     // optional processing order of the row ranges (see DevCsr::make_tiles): logical block b of
     // an XCD works on range tile[b]
     const u32* tile = nullptr;
+    BandStage st{};   // STAGE_: the LDS windows of x
     __device__ __forceinline__ u64 row(u64 p) const { return p; }
     __device__ __forceinline__ u64 block_of(u64 b) const { return tile ? u64(tile[b]) : b; }
     // Index arithmetic in 32 bits (the banded format is built for fewer than 2^31 rows): the
@@ -165,7 +205,7 @@ namespace mg
       {
         const dev::cptr<u32> cs = dev::to_const(sx);
         const u32 stride = nds * 64u;
-#pragma unroll 9
+_Pragma(F3_STR(unroll F3_BAND_UNROLL))
         for(u32 d = 0; d < ndiag; ++d)
         {
           const u32 c = p + co[d] + shift;
@@ -179,7 +219,7 @@ namespace mg
         return s;
       }
       const u32 base = __umul24(p >> 6, ndiag * 64u) + (p & 63u);
-#pragma unroll 9
+_Pragma(F3_STR(unroll F3_BAND_UNROLL))
       for(u32 d = 0; d < ndiag; ++d)
       {
         // The load is unconditional (an in-range stand-in column) and the product selected away,
@@ -192,6 +232,49 @@ namespace mg
       }
       return s;
     }
+
+    // the same row sum with x from the LDS windows of the round that starts at row r0 (the same
+    // terms in the same order)
+    __device__ __forceinline__ DT_ row_sum_staged(u32 p, u32 r0, const DT_* xs) const
+    {
+      const dev::cptr<u32> co = dev::to_const(off);
+      const dev::cptr<u32> xo = dev::to_const(st.xoff);
+      const u32 shift = 1u - u32(rows), nc = u32(cols), q = p - r0;
+      DT_ s = DT_(0);
+      if constexpr(SYM_)
+      {
+        const dev::cptr<u32> cs = dev::to_const(sx);
+        const u32 stride = nds * 64u;
+_Pragma(F3_STR(unroll F3_BAND_UNROLL))
+        for(u32 d = 0; d < ndiag; ++d)
+        {
+          const u32 c = p + co[d] + shift;
+          const bool in = c < nc;
+          const u32 vr = (in && c < p) ? c : p;
+          const u32 at = __umul24(vr >> 6, stride) + cs[d] * 64u + (vr & 63u);
+          const DT_ a = dev::ld<DT_>(val, at);
+          const DT_ t = a * xs[xo[d] + q];
+          s += in ? t : DT_(0);
+        }
+        return s;
+      }
+      const u32 base = __umul24(p >> 6, ndiag * 64u) + (p & 63u);
+_Pragma(F3_STR(unroll F3_BAND_UNROLL))
+      for(u32 d = 0; d < ndiag; ++d)
+      {
+        const u32 c = p + co[d] + shift;
+        const bool in = c < nc;
+        const DT_ a = kNonTemporal ? dev::ld_nt<DT_>(val, base + d * 64u) : dev::ld<DT_>(val, base + d * 64u);
+        const DT_ t = a * xs[xo[d] + q];
+        s += in ? t : DT_(0);
+      }
+      return s;
+    }
+  };
+  template<typename T> struct IsStagedBand { static constexpr bool value = false; };
+  template<typename DT_, typename VT_, bool SYM_> struct IsStagedBand<BandedRows<DT_, VT_, SYM_, true>>
+  {
+    static constexpr bool value = true;
   };
 
   // Tensor-product transfer matrix on lattices numbered lexicographically (x fastest):
@@ -243,10 +326,11 @@ namespace mg
   // K_: bound of the entries per axis (every cnt <= K_), D3_: the z factor is not the 1x1
   // identity.  One lane per row; the factor rows are read whole and the x gathers issued for all
   // K_^3 (K_^2) terms before the additions, an absent term reading x[0] and being selected away.
-  template<typename DT_, int K_ = int(kTensorW), bool D3_ = true>
+  template<typename DT_, int K_ = int(kTensorW), bool D3_ = true, int RT_ = 1>
   struct TensorRows
   {
     static constexpr int G = 1;
+    static constexpr int RT = RT_;   // rows per thread of mg::for_rows
     TensorAxis<DT_> ax[3];
     u32 mo[3], mi[3];
     u64 div0, div1;   // udiv_magic multipliers of mo[0] and mo[1]
@@ -309,6 +393,11 @@ namespace mg
         }
       return s;
     }
+  };
+
+  template<typename DT_, int K_, bool D3_, int RT_> struct IsTensor<TensorRows<DT_, K_, D3_, RT_>>
+  {
+    static constexpr bool value = true;
   };
 
   // Launch shape of the row kernels: every block owns a contiguous range of `per` row
@@ -402,6 +491,49 @@ namespace mg
     const u64 end = beg + per < A.rows ? beg + per : A.rows;
     if constexpr(IsStream<Rows_>::value)
       for_rows_stream(A, beg, end, x, f);
+    else if constexpr(IsStagedBand<Rows_>::value)
+    {
+      constexpr u32 CAP = kStageBytes / u32(sizeof(DT_));
+      __shared__ DT_ xs[CAP];
+      const BandStage& S = A.st;
+      for(u64 r0 = beg; r0 < end; r0 += kStageRows)
+      {
+        __syncthreads();   // the previous round's readers are done with xs
+        for(u32 w = 0; w < S.nwin; ++w)
+          for(u32 t = threadIdx.x; t < S.wlen[w]; t += blockDim.x)
+          {
+            const std::int64_t c = std::int64_t(r0) + S.wstart[w] + std::int64_t(t);
+            xs[S.wbase[w] + t] = (c >= 0 && u64(c) < A.cols) ? x[c] : DT_(0);
+          }
+        __syncthreads();
+        const u64 p = r0 + threadIdx.x;
+        if(threadIdx.x < kStageRows && p < end) f(p, A.row_sum_staged(u32(p), u32(r0), xs), true);
+      }
+    }
+    else if constexpr(IsTensor<Rows_>::value)
+    {
+      // RT rows per thread, 256 apart (each wave's accesses stay contiguous), their row sums formed
+      // before the RT epilogues so that the loads of all of them are in flight together; a
+      // position past the range evaluates the range's first row and is not written (so RT > 1 is
+      // used only where every block has RT full rounds of rows)
+      constexpr int RT = Rows_::RT;
+      for(u64 p0 = beg + threadIdx.x; p0 < end; p0 += u64(blockDim.x) * RT)
+      {
+        DT_ s[RT];
+#pragma unroll
+        for(int j = 0; j < RT; ++j)
+        {
+          const u64 p = p0 + u64(j) * blockDim.x;
+          s[j] = A.row_sum(p < end ? p : beg, 0, x);
+        }
+#pragma unroll
+        for(int j = 0; j < RT; ++j)
+        {
+          const u64 p = p0 + u64(j) * blockDim.x;
+          if(p < end) f(p, s[j], true);
+        }
+      }
+    }
     else
     {
       const int lane = int(threadIdx.x % G);
@@ -435,7 +567,7 @@ namespace mg
 
   // damped Jacobi step  x_out = x_in + dinv .* (rhs - A x_in)
   template<typename Rows_, typename DT_>
-  __global__ __launch_bounds__(256) void k_sweep(Rows_ A, u64 per, const DT_* __restrict__ dinv, const DT_* __restrict__ rhs,
+  __global__ __launch_bounds__(256) F3_ROW_ATTR void k_sweep(Rows_ A, u64 per, const DT_* __restrict__ dinv, const DT_* __restrict__ rhs,
                                                  const DT_* __restrict__ xin, DT_* __restrict__ xout, const int* status)
   {
     if(stopped(status)) return;
@@ -447,7 +579,7 @@ namespace mg
 
   // defect  def = filter(rhs - A x)
   template<typename Rows_, typename DT_>
-  __global__ __launch_bounds__(256) void k_defect(Rows_ A, u64 per, const u32* __restrict__ mask, const DT_* __restrict__ rhs,
+  __global__ __launch_bounds__(256) F3_ROW_ATTR void k_defect(Rows_ A, u64 per, const u32* __restrict__ mask, const DT_* __restrict__ rhs,
                                                   const DT_* __restrict__ x, DT_* __restrict__ def, const int* status)
   {
     if(stopped(status)) return;
@@ -784,7 +916,7 @@ namespace mg
 
   // r = filter(b - A x)  (pcg.hpp:159-173 correct), partial sums of r.r
   template<typename Rows_>
-  __global__ __launch_bounds__(256) void k_pcg_defect_norm(Rows_ A, u64 per, const u32* __restrict__ mask, const double* __restrict__ b,
+  __global__ __launch_bounds__(256) F3_ROW_ATTR void k_pcg_defect_norm(Rows_ A, u64 per, const u32* __restrict__ mask, const double* __restrict__ b,
                                                            const double* __restrict__ x, double* __restrict__ r, double* partials)
   {
     double v = 0.0;
@@ -832,7 +964,7 @@ namespace mg
 
   // q = filter(A p), partial sums of q.p   (pcg.hpp:236-242)
   template<typename Rows_>
-  __global__ __launch_bounds__(256) void k_pcg_spmv_dot(Rows_ A, u64 per, const u32* __restrict__ mask, const double* __restrict__ p,
+  __global__ __launch_bounds__(256) F3_ROW_ATTR void k_pcg_spmv_dot(Rows_ A, u64 per, const u32* __restrict__ mask, const double* __restrict__ p,
                                                         double* __restrict__ q, const PcgState* st, double* partials)
   {
     if(st->status != kProgress) return;

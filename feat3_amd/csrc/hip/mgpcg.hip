@@ -323,6 +323,11 @@ namespace
     // it evaluates, for the grid of tile_blocks blocks (0: ascending order)
     DevMem tile;
     unsigned tile_blocks = 0;
+    // x staged in LDS by the banded row kernels (mg::BandStage, option band_stage_x): set by
+    // build_stage when the windows of a round fit
+    bool stage = false;
+    mg::BandStage stg{};
+    DevMem stg_xoff;
     // tensor-product evaluation (mg::TensorRows, transfers of structured hierarchies; set by
     // build_tensor_pair after every stored entry was checked): output and input lattice extents
     // per axis, per axis the factor tables cnt / idx / w (w in the arithmetic type, tw_bytes wide)
@@ -702,10 +707,56 @@ namespace
       return rt::kOk;
     }
 
-    // the launch grid of the banded row kernels (with_rows)
+    // the launch grid of the banded row kernels (with_rows); staged: 512 rows per block (two
+    // rounds of mg::kStageRows)
     mg::RowGrid band_grid() const
     {
-      return rows_per_block ? mg::row_grid_fixed(rows, rows_per_block) : mg::row_grid(rows, 1);
+      if(rows_per_block) return mg::row_grid_fixed(rows, rows_per_block);
+      return stage ? mg::row_grid_fixed(rows, 2 * mg::kStageRows) : mg::row_grid(rows, 1);
+    }
+
+    // The LDS windows of a banded matrix for rounds of kStageRows rows: the column ranges
+    // [o_d, o_d + R) of the diagonals (offsets o_d ascending) merged where they overlap or lie
+    // less than 64 columns apart; staged only if at most kStageWin windows of in total at most
+    // kStageBytes / 8 values result (FP64-sized, so the same windows serve FP32 arithmetic)
+    int build_stage(hipStream_t s)
+    {
+      stage = false;
+      if(!banded || ndiag == 0) return rt::kOk;
+      std::vector<u32> ho(ndiag);
+      F3_CHECK(hipMemcpy(ho.data(), boff.p, sizeof(u32) * ndiag, hipMemcpyDeviceToHost));
+      const std::int64_t R = mg::kStageRows;
+      std::vector<std::int64_t> ws, we;   // windows [ws, we)
+      std::vector<u32> win(ndiag);
+      for(u32 d = 0; d < ndiag; ++d)
+      {
+        const std::int64_t o = std::int64_t(ho[d]) + 1 - std::int64_t(rows);
+        if(ws.empty() || o > we.back() + 64) { ws.push_back(o); we.push_back(o + R); }
+        else we.back() = std::max(we.back(), o + R);
+        win[d] = u32(ws.size() - 1);
+      }
+      if(ws.size() > mg::kStageWin) return rt::kOk;
+      u64 total = 0;
+      mg::BandStage b{};
+      b.nwin = u32(ws.size());
+      for(std::size_t w = 0; w < ws.size(); ++w)
+      {
+        b.wstart[w] = int(ws[w]);
+        b.wlen[w] = u32(we[w] - ws[w]);
+        b.wbase[w] = u32(total);
+        total += b.wlen[w];
+      }
+      if(total * 8 > mg::kStageBytes) return rt::kOk;
+      std::vector<u32> xo(ndiag);
+      for(u32 d = 0; d < ndiag; ++d)
+        xo[d] = b.wbase[win[d]] + u32(std::int64_t(ho[d]) + 1 - std::int64_t(rows) - ws[win[d]]);
+      F3_RC(stg_xoff.alloc(sizeof(u32) * ndiag));
+      F3_CHECK(hipMemcpyAsync(stg_xoff.p, xo.data(), sizeof(u32) * ndiag, hipMemcpyHostToDevice, s));
+      F3_CHECK(hipStreamSynchronize(s));
+      b.xoff = stg_xoff.template as<u32>();
+      stg = b;
+      stage = true;
+      return rt::kOk;
     }
 
     // Plane-strip order of the row ranges of a banded matrix.  Each XCD evaluates a contiguous
@@ -952,6 +1003,26 @@ namespace
   {
     auto grid = [&](int G) { return m.rows_per_block ? mg::row_grid_fixed(m.rows, m.rows_per_block) : mg::row_grid(m.rows, G); };
     if(m.tensor) return rt::kErrState;   // transfers only: with_transfer_rows
+    if(m.banded && m.stage)
+    {
+      const mg::RowGrid g = m.band_grid();
+      const u32* tile = m.tile_blocks == g.blocks ? m.tile.template as<u32>() : nullptr;
+      if(m.bsym)
+      {
+        mg::BandedRows<DT, VT, true, true> A{};
+        A.off = m.boff.template as<u32>(); A.val = m.bval.template as<VT>(); A.rows = m.rows; A.cols = m.cols;
+        A.ndiag = m.ndiag; A.sx = m.bsx.template as<u32>(); A.nds = m.nds; A.tile = tile; A.st = m.stg;
+        f(A, g);
+      }
+      else
+      {
+        mg::BandedRows<DT, VT, false, true> A{};
+        A.off = m.boff.template as<u32>(); A.val = m.bval.template as<VT>(); A.rows = m.rows; A.cols = m.cols;
+        A.ndiag = m.ndiag; A.tile = tile; A.st = m.stg;
+        f(A, g);
+      }
+      return rt::kOk;
+    }
     if(m.banded)
     {
       const mg::RowGrid g = grid(1);
@@ -1029,10 +1100,10 @@ namespace
   // the same for a transfer matrix, which may also be evaluated as a tensor product: the row
   // evaluator is instantiated for the bound of its factors' entries per axis (2, 3, 5 or 8) and
   // for 2D (z factor the identity) or 3D
-  template<typename DT, int K, bool D3>
-  mg::TensorRows<DT, K, D3> tensor_as(const mg::TensorRows<DT>& t)
+  template<typename DT, int K, bool D3, int RT>
+  mg::TensorRows<DT, K, D3, RT> tensor_as(const mg::TensorRows<DT>& t)
   {
-    mg::TensorRows<DT, K, D3> r;
+    mg::TensorRows<DT, K, D3, RT> r;
     for(int a = 0; a < 3; ++a) { r.ax[a] = t.ax[a]; r.mo[a] = t.mo[a]; r.mi[a] = t.mi[a]; }
     r.div0 = t.div0;
     r.div1 = t.div1;
@@ -1045,12 +1116,21 @@ namespace
     if(!m.tensor) return with_rows<DT, VT>(m, f);
     if(m.tw_bytes != sizeof(DT)) return rt::kErrState;
     const mg::TensorRows<DT> t = m.template tensor_rows<DT>();
-    const mg::RowGrid g = m.rows_per_block ? mg::row_grid_fixed(m.rows, m.rows_per_block) : mg::row_grid(m.rows, 1);
+    // levels of at least 2^20 rows: 1024 rows per block, four per thread (mg::for_rows: their
+    // loads in flight together); smaller ones one row per thread in the transfers' block size
+    const bool big = m.rows >= (u64(1) << 20);
+    const mg::RowGrid g = big ? mg::row_grid_fixed(m.rows, 1024)
+                              : (m.rows_per_block ? mg::row_grid_fixed(m.rows, m.rows_per_block) : mg::row_grid(m.rows, 1));
     const bool d3 = m.tmo[2] > 1;
-    if(m.tk <= 2) { if(d3) f(tensor_as<DT, 2, true>(t), g); else f(tensor_as<DT, 2, false>(t), g); }
-    else if(m.tk <= 3) { if(d3) f(tensor_as<DT, 3, true>(t), g); else f(tensor_as<DT, 3, false>(t), g); }
-    else if(m.tk <= 5) { if(d3) f(tensor_as<DT, 5, true>(t), g); else f(tensor_as<DT, 5, false>(t), g); }
-    else { if(d3) f(tensor_as<DT, 8, true>(t), g); else f(tensor_as<DT, 8, false>(t), g); }
+    auto go = [&](auto k) {
+      constexpr int K = decltype(k)::value;
+      if(big) { if(d3) f(tensor_as<DT, K, true, 4>(t), g); else f(tensor_as<DT, K, false, 4>(t), g); }
+      else { if(d3) f(tensor_as<DT, K, true, 1>(t), g); else f(tensor_as<DT, K, false, 1>(t), g); }
+    };
+    if(m.tk <= 2) go(std::integral_constant<int, 2>());
+    else if(m.tk <= 3) go(std::integral_constant<int, 3>());
+    else if(m.tk <= 5) go(std::integral_constant<int, 5>());
+    else go(std::integral_constant<int, 8>());
     return rt::kOk;
   }
 
@@ -1150,6 +1230,9 @@ namespace
     // processing order of the row ranges of banded matrices (DevCsr::make_tiles): 0 ascending,
     // 1 plane strips for the FP64 outer matrix, 2 plane strips for every banded matrix
     int plane_order = 1;
+    // x of the banded row kernels staged in LDS per round of rows (mg::BandStage): 0 no, 1 the
+    // FP64 outer matrix, 2 every banded matrix
+    int band_stage_x = 0;
     double tol_rel = 1e-8, tol_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16), tol_abs_low = 0.0;
     double div_rel = 1.0 / 2.220446049250313e-16, div_abs = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16);
     double stag_rate = 0.95;
@@ -1352,6 +1435,14 @@ namespace
           F3_RC(A64own.upload(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, -1.0, cfg.index16, want_banded));
         A64 = &A64own;
       }
+      // x staged in LDS by the banded kernels: the FP64 outer matrix (1), every banded matrix (2)
+      if(cfg.band_stage_x >= 1)
+      {
+        if(shared) F3_RC(L.front()->A.build_stage(s));
+        else F3_RC(A64own.build_stage(s));
+      }
+      if(cfg.band_stage_x == 2)
+        for(std::size_t i = shared ? 1 : 0; i < L.size(); ++i) F3_RC(L[i]->A.build_stage(s));
       // plane-strip order of the banded row ranges: the FP64 outer matrix (1), every level (2)
       if(cfg.plane_order >= 1)
       {
@@ -2522,6 +2613,7 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
   else if(k == "transfer_rows_per_block" && value >= 0) h->cfg.transfer_rows_per_block = u64(value);
   else if(k == "symmetric_band") h->cfg.symmetric_band = value != 0.0;
   else if(k == "plane_order" && (value == 0 || value == 1 || value == 2)) h->cfg.plane_order = int(value);
+  else if(k == "band_stage_x" && (value == 0 || value == 1 || value == 2)) h->cfg.band_stage_x = int(value);
   else if(k == "cycle" && (value == 0 || value == 1 || value == 2)) h->cfg.cycle = int(value);
   else if(k == "rows_per_block" && value >= 0) h->cfg.rows_per_block = u64(value);
   else if(k == "transfer_sigma" && value >= 1) h->cfg.transfer_sigma = u64(value);

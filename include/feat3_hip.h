@@ -359,6 +359,9 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
  * 2 W; the coarse-tail kernel runs the cycle's steps on the small levels for each of them);
  * "plane_order" (order in which the blocks of a banded row kernel take their row ranges: 0
  * ascending, 1 plane strips for the FP64 outer matrix (default), 2 for every banded matrix);
+ * "band_stage_x" (x of the banded row kernels staged in LDS per round of 256 rows, the columns
+ * the round reaches copied once instead of gathered per entry: 0 no (default), 1 the FP64 outer
+ * matrix, 2 every banded matrix; the same row sums, slower on gfx950);
  * takes effect at the next init */
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value);
 /* init_symbolic + init_numeric (solver/base.hpp:227-260) */

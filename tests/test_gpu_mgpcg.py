@@ -362,3 +362,32 @@ def test_nan_matrix_aborts_in_every_format(torch, fmt, inner, level):
     assert so == 3
     assert sg == 3, f"status {sg} after {ig} iterations"
     assert ig == io
+
+
+@pytest.mark.parametrize("dim,deg,lvl", [(3, 1, 5), (2, 2, 5), (2, 1, 6)])
+@pytest.mark.parametrize("inner", ["f64", "f32", "f16"])
+def test_band_x_staged_in_lds_is_bitwise_identical(torch, dim, deg, lvl, inner):
+    """Option band_stage_x: the banded row kernels read x from LDS windows staged per round of
+    256 rows (1: the FP64 outer matrix, 2: every banded matrix).  The row sums take the same terms
+    in the same order, so a V-cycle (no reductions) is bit for bit the unstaged one; a solve, whose
+    dot products sum block partials of a grid of 512-row blocks instead, keeps the iteration count
+    and agrees to 1e-12."""
+    from feat3_amd.solver import MGPCG
+    print(f"case band stage {dim}D Q{deg} L{lvl} {inner}", flush=True)
+    H = pb.hierarchy(dim, deg, lvl, 1)
+    b = pb.filtered_rhs(H, 0)
+    db = torch.from_numpy(b).cuda()
+    ref = None
+    for stage in (0, 1, 2):
+        s = MGPCG(H, inner=inner, fmt="banded", options={"band_stage_x": stage}).init()
+        c = torch.zeros_like(db)
+        s.vcycle(c, db)
+        torch.cuda.synchronize()
+        s.done()
+        st, it, x, _ = solve_gpu(torch, H, b, inner=inner, fmt="banded", options={"band_stage_x": stage})
+        assert st == 2
+        if ref is None:
+            ref = (c.cpu().numpy(), it, x)
+        else:
+            assert np.array_equal(c.cpu().numpy(), ref[0]), stage
+            assert it == ref[1] and rel(x, ref[2]) <= 1e-12, stage
