@@ -61,7 +61,9 @@ def sweep_algorithmic_bytes(info, n, nnz, nchunks):
     sizes -- for SELL-C the chunk descriptors (cs, cl) replace row_ptr, the banded format stores no
     indices at all (its offsets, 4 bytes per diagonal, replace them; for the symmetric band storage
     nnz counts the entries on and above the diagonal: the lower half is read from the L2) -- x
-    counted once, plus dinv, b and x_out."""
+    counted once, plus dinv, b and x_out.  The value-coded band reads 4-bit codes instead of the
+    values; its bytes are the solver's own count (time_kernel), this formula then gives the
+    value-stored equivalent."""
     s_v = info["sizeof_value"]
     s_x = info["sizeof_inner"]
     if info.get("banded"):
@@ -512,17 +514,22 @@ def main():
         times, iters = run(s, steps, warmup)
         hist = s.history()
         sol = x.cpu().numpy().copy()
-        sweep_ms, _ = s.time_kernel(0, 50)
+        sweep_ms, sweep_stored_bytes = s.time_kernel(0, 50)
         outer_ms, outer_bytes = s.time_kernel(2, 50)
         vc_ms, _ = s.time_kernel(3, 10)
         info = s.info()
         s.done()
         ms = 1e3 * sum(times) / len(times)
-        sb = sweep_algorithmic_bytes(info, n, band_nnz(H.levels[0].A, inner, upper=info["banded"] == 2),
-                                     info["sell_chunks"])
+        # the value-stored equivalent (the entries' values) beside the bytes the format reads
+        sb_values = sweep_algorithmic_bytes(dict(info, banded=1) if info["banded"] == 3 else info, n,
+                                            band_nnz(H.levels[0].A, inner, upper=info["banded"] == 2),
+                                            info["sell_chunks"])
+        sb = sweep_stored_bytes if info["banded"] == 3 else sb_values
         r = dict(ms_per_solve=ms, ms_min=1e3 * min(times), iterations=iters,
                  dof_iter_per_s=n * sum(iters) / sum(times), final_rel_residual=float(hist[-1] / hist[0]),
                  sweep_us=sweep_ms * 1e3, sweep_bytes=sb, sweep_GBs=sb / (sweep_ms * 1e-3) / 1e9,
+                 level_format={0: "csr/sell", 1: "band", 2: "symmetric band", 3: "value-coded band"}[info["banded"]],
+                 sweep_value_bytes=sb_values, sweep_value_GBs=sb_values / (sweep_ms * 1e-3) / 1e9,
                  vcycle_ms=vc_ms, kernels_per_vcycle=info["kernels_per_vcycle"],
                  outer_product_us=outer_ms * 1e3, outer_product_bytes=outer_bytes,
                  outer_product_GBs=outer_bytes / (outer_ms * 1e-3) / 1e9)

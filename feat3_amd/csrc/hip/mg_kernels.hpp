@@ -277,6 +277,123 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
     static constexpr bool value = true;
   };
 
+  // Value-coded band: a banded matrix whose every diagonal holds at most kDictSlots distinct
+  // values (bit patterns) stores, per row, one 4-bit code per diagonal into its diagonal's table
+  // of values instead of the values themselves: the codes of 8 diagonals in one 32-bit word,
+  // chunk-major (word w of row p at idx[(p / 64) * nw * 64 + w * 64 + p % 64]), the tables
+  // [nw * 8][kDictSlots] in the storage type.  A block copies the tables into LDS (DT, as
+  // BandedRows converts the stored values) before its rows; the row sum then takes the same
+  // terms, in the same order and with the same arithmetic (each product rounded, then added) as
+  // BandedRows, so the results are bit for bit those of the band it was built from (3D Q1:
+  // 4 bytes of codes per 8 diagonals instead of 32 of FP32 or 64 of FP64 values).
+  // ND_ > 0: exactly ND_ diagonals, every loop unrolled in full (all loads of a row in flight
+  // together), and blocks whose rows reach only columns inside the matrix take the row sums
+  // without the range tests.  ND_ = 0: any count, the offsets padded to nw * 8 with stand-ins
+  // whose column is outside the matrix for every row (their terms are selected away).
+  constexpr u32 kDictSlots = 16;
+  constexpr u32 kDictDiag = 32;   // diagonals at most (4 code words per row)
+  template<typename DT_, typename VT_, int ND_ = 0>
+  struct DictBandRows
+  {
+    static constexpr int G = 1;
+    static constexpr int kND = ND_;
+    const u32* off;    // offsets (off + rows - 1), ascending, padded to nw * 8
+    const u32* idx;    // code words, chunk-major
+    const VT_* tab;    // [nw * 8][kDictSlots] values
+    u64 rows, cols;
+    u32 ndiag, nw;     // diagonals, code words per row
+    int omin, omax;    // the smallest and largest offset col - row
+    const u32* tile = nullptr;
+    const DT_* lt = nullptr;   // the tables in LDS (set by for_rows)
+    __device__ __forceinline__ u64 row(u64 p) const { return p; }
+    __device__ __forceinline__ u64 block_of(u64 b) const { return tile ? u64(tile[b]) : b; }
+    // x at column c through a 32-bit byte offset (the band's rows are fewer than 2^26)
+    __device__ __forceinline__ static DT_ xat(const DT_* __restrict__ x, u32 c)
+    {
+      return *reinterpret_cast<const DT_*>(reinterpret_cast<const char*>(x) + c * u32(sizeof(DT_)));
+    }
+    __device__ __forceinline__ DT_ term_value(u32 d, u32 code, u32 j) const
+    {
+      return lt[d * kDictSlots + __builtin_amdgcn_ubfe(code, 4u * j, 4u)];
+    }
+    // BandedRows' range test and selection per term
+    __device__ __forceinline__ DT_ row_sum(u64 p64, int, const DT_* __restrict__ x) const
+    {
+#pragma clang fp contract(off)
+      const dev::cptr<u32> co = dev::to_const(off);
+      const u32 p = u32(p64), shift = 1u - u32(rows), nc = u32(cols);
+      DT_ s = DT_(0);
+      if constexpr(ND_ > 0)
+      {
+        // every load of the row first (code words, then x), then the terms in order
+        constexpr u32 NW = (u32(ND_) + 7u) / 8u;
+        const u32 base = __umul24(p >> 6, NW * 64u) + (p & 63u);
+        u32 code[NW];
+#pragma unroll
+        for(u32 w = 0; w < NW; ++w) code[w] = idx[base + w * 64u];
+        DT_ xv[ND_];
+        bool in[ND_];
+#pragma unroll
+        for(u32 d = 0; d < u32(ND_); ++d)
+        {
+          const u32 c = p + co[d] + shift;
+          in[d] = c < nc;
+          xv[d] = xat(x, in[d] ? c : p);
+        }
+#pragma unroll
+        for(u32 d = 0; d < u32(ND_); ++d)
+        {
+          const DT_ t = term_value(d, code[d >> 3], d & 7u) * xv[d];
+          s += in[d] ? t : DT_(0);
+        }
+      }
+      else
+      {
+        const u32 base = __umul24(p >> 6, nw * 64u) + (p & 63u);
+        for(u32 w = 0; w < nw; ++w)
+        {
+          const u32 code = idx[base + w * 64u];
+#pragma unroll
+          for(u32 j = 0; j < 8u; ++j)
+          {
+            const u32 d = w * 8u + j;
+            const DT_ a = term_value(d, code, j);
+            const u32 c = p + co[d] + shift;
+            const bool in = c < nc;
+            const DT_ t = a * xat(x, in ? c : p);
+            s += in ? t : DT_(0);
+          }
+        }
+      }
+      return s;
+    }
+    // ND_ > 0, rows whose every column is inside the matrix (the caller checks the block's range
+    // against omin / omax): the same products and sums without the range test
+    __device__ __forceinline__ DT_ row_sum_interior(u32 p, const DT_* __restrict__ x) const
+    {
+#pragma clang fp contract(off)
+      const dev::cptr<u32> co = dev::to_const(off);
+      const u32 shift = 1u - u32(rows);
+      constexpr u32 NW = (u32(ND_ > 0 ? ND_ : 1) + 7u) / 8u;
+      const u32 base = __umul24(p >> 6, NW * 64u) + (p & 63u);
+      u32 code[NW];
+#pragma unroll
+      for(u32 w = 0; w < NW; ++w) code[w] = idx[base + w * 64u];
+      DT_ xv[ND_ > 0 ? ND_ : 1];
+#pragma unroll
+      for(u32 d = 0; d < u32(ND_); ++d) xv[d] = xat(x, p + (co[d] + shift));
+      DT_ s = DT_(0);
+#pragma unroll
+      for(u32 d = 0; d < u32(ND_); ++d) s += term_value(d, code[d >> 3], d & 7u) * xv[d];
+      return s;
+    }
+  };
+  template<typename T> struct IsDictBand { static constexpr bool value = false; };
+  template<typename DT_, typename VT_, int ND_> struct IsDictBand<DictBandRows<DT_, VT_, ND_>>
+  {
+    static constexpr bool value = true;
+  };
+
   // Tensor-product transfer matrix on lattices numbered lexicographically (x fastest):
   // M[(oz,oy,ox),(iz,iy,ix)] = (wz * wy) * wx with the 1D factors of the three axes (a 2D
   // matrix has the 1x1 identity as its z factor).  Entry k of output index o along an axis is
@@ -509,6 +626,19 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
         const u64 p = r0 + threadIdx.x;
         if(threadIdx.x < kStageRows && p < end) f(p, A.row_sum_staged(u32(p), u32(r0), xs), true);
       }
+    }
+    else if constexpr(IsDictBand<Rows_>::value)
+    {
+      __shared__ DT_ lt[kDictDiag * kDictSlots];
+      for(u32 i = threadIdx.x; i < A.nw * 8u * kDictSlots; i += blockDim.x) lt[i] = dev::ld<DT_>(A.tab, i);
+      __syncthreads();
+      Rows_ B = A;
+      B.lt = lt;
+      // a block whose rows reach only columns inside the matrix skips the range tests
+      if(Rows_::kND > 0 && std::int64_t(beg) + A.omin >= 0 && std::int64_t(end) - 1 + A.omax < std::int64_t(A.cols))
+        for(u64 p = beg + threadIdx.x; p < end; p += blockDim.x) f(p, B.row_sum_interior(u32(p), x), true);
+      else
+        for(u64 p = beg + threadIdx.x; p < end; p += blockDim.x) f(p, B.row_sum(p, 0, x), true);
     }
     else if constexpr(IsTensor<Rows_>::value)
     {

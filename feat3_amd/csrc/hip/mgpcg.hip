@@ -228,6 +228,79 @@ namespace
     }
   }
 
+  // ---- value-coded band (mg::DictBandRows) ------------------------------------------------
+  // The distinct values of each diagonal of a chunk-major band as bit patterns (zero-extended to
+  // 64 bits): kDictSlots slots per diagonal, kDictEmpty where unused.  Each block collects into
+  // its own LDS tables, then merges them into the global ones; a diagonal with more distinct
+  // values than slots raises *over.  Slot order is that of first insertion (it does not change
+  // any value the row sums read).
+  constexpr unsigned long long kDictEmpty = ~0ull;
+  __device__ inline bool dict_insert(unsigned long long* slots, unsigned long long key)
+  {
+    for(u32 j = 0; j < mg::kDictSlots; ++j)
+    {
+      const unsigned long long cur = __atomic_load_n(&slots[j], __ATOMIC_RELAXED);
+      if(cur == key) return true;
+      if(cur == kDictEmpty)
+      {
+        const unsigned long long prev = atomicCAS(&slots[j], kDictEmpty, key);
+        if(prev == kDictEmpty || prev == key) return true;
+      }
+    }
+    return false;
+  }
+  template<typename VT>
+  __global__ void k_dict_collect(u64 rows, u32 nd, const VT* __restrict__ bval, unsigned long long* keys, int* over)
+  {
+    typedef typename Bits<VT>::T B;
+    __shared__ unsigned long long lk[mg::kDictDiag * mg::kDictSlots];
+    for(u32 i = threadIdx.x; i < nd * mg::kDictSlots; i += blockDim.x) lk[i] = kDictEmpty;
+    __syncthreads();
+    const u64 padded = (rows + 63) / 64 * 64, total = padded * nd;
+    bool ok = true;
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += u64(gridDim.x) * blockDim.x)
+    {
+      const u64 r = (i / (u64(nd) * 64)) * 64 + (i & 63);
+      if(r >= rows) continue;
+      const u32 d = u32((i >> 6) % nd);
+      ok = dict_insert(&lk[d * mg::kDictSlots], (unsigned long long)(reinterpret_cast<const B*>(bval)[i])) && ok;
+    }
+    __syncthreads();
+    for(u32 t = threadIdx.x; t < nd * mg::kDictSlots; t += blockDim.x)
+      if(lk[t] != kDictEmpty) ok = dict_insert(&keys[(t / mg::kDictSlots) * mg::kDictSlots], lk[t]) && ok;
+    if(!ok) atomicOr(over, 1);
+  }
+  // the code words of every row: code k of diagonal d is the slot holding the row's value; a
+  // value without a slot raises *over; rows past the matrix (chunk padding) get code 0
+  template<typename VT>
+  __global__ void k_dict_encode(u64 rows, u32 nd, u32 nw, const VT* __restrict__ bval, const unsigned long long* __restrict__ keys,
+                                u32* __restrict__ idx, int* over)
+  {
+    typedef typename Bits<VT>::T B;
+    __shared__ unsigned long long lk[mg::kDictDiag * mg::kDictSlots];
+    for(u32 i = threadIdx.x; i < nd * mg::kDictSlots; i += blockDim.x) lk[i] = keys[i];
+    __syncthreads();
+    const u64 padded = (rows + 63) / 64 * 64;
+    bool ok = true;
+    for(u64 r = u64(blockIdx.x) * blockDim.x + threadIdx.x; r < padded; r += u64(gridDim.x) * blockDim.x)
+      for(u32 w = 0; w < nw; ++w)
+      {
+        u32 code = 0;
+        for(u32 j = 0; j < 8u && w * 8u + j < nd && r < rows; ++j)
+        {
+          const u32 d = w * 8u + j;
+          const unsigned long long key = (unsigned long long)(reinterpret_cast<const B*>(bval)[(r >> 6) * u64(nd) * 64 + u64(d) * 64 + (r & 63)]);
+          u32 k = mg::kDictSlots;
+          for(u32 q = 0; q < mg::kDictSlots; ++q)
+            if(lk[d * mg::kDictSlots + q] == key) { k = q; break; }
+          if(k == mg::kDictSlots || key == kDictEmpty) { ok = false; k = 0; }
+          code |= k << (4u * j);
+        }
+        idx[(r >> 6) * u64(nw) * 64 + u64(w) * 64 + (r & 63)] = code;
+      }
+    if(!ok) atomicOr(over, 1);
+  }
+
   inline unsigned setup_blocks(u64 n)
   {
     const u64 b = (n + 255) / 256;
@@ -319,6 +392,14 @@ namespace
     u32 nds = 0;
     DevMem bsx;
     bool allow_sym = true;    // symmetric storage allowed (option "symmetric_band")
+    // value-coded band (mg::DictBandRows, option "band_dict"): dnw code words per row in didx,
+    // the tables in dtab, the offsets padded to dnw * 8 diagonals in doff, the smallest and
+    // largest offset col - row in domin / domax; the band's values are then not kept
+    bool allow_dict = true;
+    bool dict = false;
+    u32 dnw = 0;
+    int domin = 0, domax = 0;
+    DevMem didx, dtab, doff;
     // processing order of the row ranges (banded matrices): tile[logical block] = the row range
     // it evaluates, for the grid of tile_blocks blocks (0: ascending order)
     DevMem tile;
@@ -496,9 +577,11 @@ namespace
       F3_CHECK(hipStreamSynchronize(s));   // ho leaves scope
       ndiag = u32(nd);
       banded = true;
+      bsym = false;
+      F3_RC(build_dict(s));
+      if(dict) return rt::kOk;
       // symmetric storage when the offsets are symmetric and every value equals its mirror image
       // (FP64 values, as build_banded)
-      bsym = false;
       if(sizeof(VT) != 8 || !allow_sym) return rt::kOk;
       std::vector<std::int64_t> offs(nd);
       for(u64 d = 0; d < nd; ++d) offs[d] = std::int64_t(ho[d]) - std::int64_t(rows) + 1;
@@ -599,6 +682,21 @@ namespace
         }
       std::vector<u32> ho(nd);
       for(u64 d = 0; d < nd; ++d) ho[d] = u32(offs[d] + std::int64_t(rows) - 1);
+      // the value-coded copy of the full band where it applies
+      if(allow_dict)
+      {
+        F3_RC(boff.alloc(sizeof(u32) * nd));
+        F3_RC(bval.alloc(sizeof(VT) * hv.size()));
+        F3_CHECK(hipMemcpyAsync(boff.p, ho.data(), sizeof(u32) * nd, hipMemcpyHostToDevice, s));
+        F3_CHECK(hipMemcpyAsync(bval.p, hv.data(), sizeof(VT) * hv.size(), hipMemcpyHostToDevice, s));
+        F3_CHECK(hipStreamSynchronize(s));
+        ndiag = u32(nd);
+        bsym = false;
+        nds = 0;
+        banded = true;
+        F3_RC(build_dict(s));
+        if(dict) return rt::kOk;
+      }
       // symmetric storage when the band equals its mirror image bit for bit, for FP64 values
       // (3D Q1 L7: FP64 sweep 107 -> 85 us, outer PCG product likewise).  FP32 and binary16 bands
       // stay full: their values are served partly from the Infinity Cache, and the mirrored
@@ -707,6 +805,69 @@ namespace
       return rt::kOk;
     }
 
+    // The value-coded copy of the full band in bval (ndiag diagonals, not the symmetric storage)
+    // when every diagonal holds at most kDictSlots distinct values and the diagonals, padded to a
+    // multiple of 8, fit kDictDiag; on success the band's values are released.  The tables take
+    // the values' bit patterns, so the row sums read exactly the band's values.
+    int build_dict(hipStream_t s)
+    {
+      dict = false;
+      if(!allow_dict || !banded || bsym || ndiag == 0) return rt::kOk;
+      const u32 nw = (ndiag + 7) / 8;
+      if(nw * 8 > mg::kDictDiag) return rt::kOk;
+      const u64 padded = (rows + 63) / 64 * 64, nk = u64(ndiag) * mg::kDictSlots;
+      DevMem keys, over, ix;
+      F3_RC(keys.alloc(sizeof(unsigned long long) * nk));
+      F3_RC(over.alloc(sizeof(int)));
+      F3_CHECK(hipMemsetAsync(keys.p, 0xFF, sizeof(unsigned long long) * nk, s));
+      F3_CHECK(hipMemsetAsync(over.p, 0, sizeof(int), s));
+      k_dict_collect<VT><<<setup_blocks(padded * ndiag), 256, 0, s>>>(rows, ndiag, bval.template as<VT>(),
+                                                                      keys.template as<unsigned long long>(), over.template as<int>());
+      F3_RC(rt::check_launch());
+      int hover = 0;
+      F3_CHECK(hipMemcpyAsync(&hover, over.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      F3_CHECK(hipStreamSynchronize(s));
+      if(hover) return rt::kOk;
+      F3_RC(ix.alloc(sizeof(u32) * padded * nw));
+      k_dict_encode<VT><<<setup_blocks(padded), 256, 0, s>>>(rows, ndiag, nw, bval.template as<VT>(),
+                                                             keys.template as<unsigned long long>(), ix.template as<u32>(),
+                                                             over.template as<int>());
+      F3_RC(rt::check_launch());
+      std::vector<unsigned long long> hk(nk);
+      std::vector<u32> ho(ndiag);
+      F3_CHECK(hipMemcpyAsync(&hover, over.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      F3_CHECK(hipMemcpyAsync(hk.data(), keys.p, sizeof(unsigned long long) * nk, hipMemcpyDeviceToHost, s));
+      F3_CHECK(hipMemcpyAsync(ho.data(), boff.p, sizeof(u32) * ndiag, hipMemcpyDeviceToHost, s));
+      F3_CHECK(hipStreamSynchronize(s));
+      if(hover) return rt::kOk;
+      // tables (unused slots and the slots of the diagonals past ndiag hold 0)
+      std::vector<VT> tab(u64(nw) * 8 * mg::kDictSlots, to_storage<VT>(0.0));
+      for(u64 i = 0; i < nk; ++i)
+        if(hk[i] != kDictEmpty)
+        {
+          const typename Bits<VT>::T b = typename Bits<VT>::T(hk[i]);
+          std::memcpy(&tab[i], &b, sizeof(VT));
+        }
+      // the offsets padded with stand-ins whose column p + cols is outside the matrix for every row
+      std::vector<u32> po(u64(nw) * 8, u32(rows - 1 + cols));
+      std::copy(ho.begin(), ho.end(), po.begin());
+      F3_RC(dtab.alloc(sizeof(VT) * tab.size()));
+      F3_RC(doff.alloc(sizeof(u32) * po.size()));
+      F3_CHECK(hipMemcpyAsync(dtab.p, tab.data(), sizeof(VT) * tab.size(), hipMemcpyHostToDevice, s));
+      F3_CHECK(hipMemcpyAsync(doff.p, po.data(), sizeof(u32) * po.size(), hipMemcpyHostToDevice, s));
+      F3_CHECK(hipStreamSynchronize(s));   // tab and po leave scope
+      domin = int(std::int64_t(ho.front()) + 1 - std::int64_t(rows));
+      domax = int(std::int64_t(ho.back()) + 1 - std::int64_t(rows));
+      std::swap(didx.p, ix.p);
+      {
+        DevMem released;
+        std::swap(released.p, bval.p);
+      }
+      dnw = nw;
+      dict = true;
+      return rt::kOk;
+    }
+
     // the launch grid of the banded row kernels (with_rows); staged: 512 rows per block (two
     // rounds of mg::kStageRows)
     mg::RowGrid band_grid() const
@@ -722,7 +883,7 @@ namespace
     int build_stage(hipStream_t s)
     {
       stage = false;
-      if(!banded || ndiag == 0) return rt::kOk;
+      if(!banded || dict || ndiag == 0) return rt::kOk;
       std::vector<u32> ho(ndiag);
       F3_CHECK(hipMemcpy(ho.data(), boff.p, sizeof(u32) * ndiag, hipMemcpyDeviceToHost));
       const std::int64_t R = mg::kStageRows;
@@ -807,6 +968,7 @@ namespace
         for(int a = 0; a < 3; ++a) b += u64(tmo[a]) * (4 + u64(mg::kTensorW) * (4 + tw_bytes));
         return b;
       }
+      if(banded && dict) return u64(dnw) * rows * 4 + u64(ndiag) * 4 + u64(dnw) * 8 * mg::kDictSlots * sizeof(VT);
       if(banded) return u64(bsym ? nds : ndiag) * rows * sizeof(VT) + u64(ndiag) * 4 * (bsym ? 2 : 1);
       if(sell)
       {
@@ -1023,6 +1185,25 @@ namespace
       }
       return rt::kOk;
     }
+    if(m.banded && m.dict)
+    {
+      const mg::RowGrid g = grid(1);
+      const u32* tile = m.tile_blocks == g.blocks ? m.tile.template as<u32>() : nullptr;
+      // unrolled in full for the diagonal counts of Q1 operators (2D: 9; 3D: 27, 21 without the
+      // face couplings below eps_f32^2)
+      auto go = [&](auto nd) {
+        mg::DictBandRows<DT, VT, decltype(nd)::value> A{m.doff.template as<u32>(), m.didx.template as<u32>(),
+                                                        m.dtab.template as<VT>(), m.rows, m.cols, m.ndiag, m.dnw,
+                                                        m.domin, m.domax};
+        A.tile = tile;
+        f(A, g);
+      };
+      if(m.ndiag == 27) go(std::integral_constant<int, 27>());
+      else if(m.ndiag == 21) go(std::integral_constant<int, 21>());
+      else if(m.ndiag == 9) go(std::integral_constant<int, 9>());
+      else go(std::integral_constant<int, 0>());
+      return rt::kOk;
+    }
     if(m.banded)
     {
       const mg::RowGrid g = grid(1);
@@ -1224,6 +1405,7 @@ namespace
     // FP64 bands that equal their mirror image bit for bit store only the center and upper
     // diagonals (0: always the full band; the results are identical either way)
     bool symmetric_band = true;
+    bool band_dict = true;   // value-coded bands (mg::DictBandRows) where every diagonal has few values
     // processing order of the row ranges of banded matrices (DevCsr::make_tiles): 0 ascending,
     // 1 plane strips for the FP64 outer matrix, 2 plane strips for every banded matrix
     int plane_order = 1;
@@ -1337,6 +1519,7 @@ namespace
         const HostLevel& h = lv[i];
         l->n = h.A.rows;
         l->A.allow_sym = cfg.symmetric_band;
+        l->A.allow_dict = cfg.band_dict;
         if(h.A.on_device())
           F3_RC(l->A.upload_dev(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, level_thin(), cfg.index16, want_banded));
         else
@@ -1426,6 +1609,7 @@ namespace
       if(!shared)
       {
         A64own.allow_sym = cfg.symmetric_band;
+        A64own.allow_dict = cfg.band_dict;
         if(h0.A.on_device())
           F3_RC(A64own.upload_dev(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, -1.0, cfg.index16, want_banded));
         else
@@ -2264,7 +2448,7 @@ namespace
       v[9] = sizeof(VT);
       v[10] = L.front()->A.sell_d16 ? 1 : 0;
       v[11] = L.front()->A.sell ? (L.front()->n + L.front()->A.C - 1) / L.front()->A.C : 0;
-      v[12] = L.front()->A.banded ? (L.front()->A.bsym ? 2 : 1) : 0;
+      v[12] = L.front()->A.banded ? (L.front()->A.dict ? 3 : (L.front()->A.bsym ? 2 : 1)) : 0;
       v[13] = L.front()->A.ndiag;
       v[14] = tail0 + 1 < L.size() ? L.size() - tail0 : 0;
       v[15] = tail_lds;
@@ -2609,6 +2793,7 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
     h->cfg.transfer_lanes = int(value);
   else if(k == "transfer_rows_per_block" && value >= 0) h->cfg.transfer_rows_per_block = u64(value);
   else if(k == "symmetric_band") h->cfg.symmetric_band = value != 0.0;
+  else if(k == "band_dict") h->cfg.band_dict = value != 0.0;
   else if(k == "plane_order" && (value == 0 || value == 1 || value == 2)) h->cfg.plane_order = int(value);
   else if(k == "band_stage_x" && (value == 0 || value == 1 || value == 2)) h->cfg.band_stage_x = int(value);
   else if(k == "cycle" && (value == 0 || value == 1 || value == 2)) h->cfg.cycle = int(value);

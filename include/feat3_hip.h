@@ -361,7 +361,11 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
  * ascending, 1 plane strips for the FP64 outer matrix (default), 2 for every banded matrix);
  * "band_stage_x" (x of the banded row kernels staged in LDS per round of 256 rows, the columns
  * the round reaches copied once instead of gathered per entry: 0 no (default), 1 the FP64 outer
- * matrix, 2 every banded matrix; the same row sums, slower on gfx950);
+ * matrix, 2 every banded matrix; the same row sums, slower on gfx950; not used with
+ * value-coded bands); "band_dict" (1, default: a band whose every diagonal holds at most 16
+ * distinct values, bit patterns, is stored as 4-bit codes per row and diagonal into per-diagonal
+ * tables of those values, which the row kernels read from LDS: the same values and row sums,
+ * bit for bit, with 0.5 instead of 4 (FP32) or 8 (FP64) bytes per entry; 0: the values);
  * takes effect at the next init */
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value);
 /* init_symbolic + init_numeric (solver/base.hpp:227-260) */
@@ -384,7 +388,7 @@ int feat3_mgpcg_time_kernel(feat3_mgpcg* h, int which, int reps, double* avg_ms,
 /* [levels, n, nnz, lanes, sell size, kernels per cycle, cycle bytes, outer bytes,
  *  sizeof inner, sizeof stored value, 16-bit SELL columns on the finest level, SELL chunks
  *  of the finest level, finest level banded (0 no, 1 yes, 2 with symmetric storage: center and
- *  upper diagonals only), its number of diagonals, levels run by the
+ *  upper diagonals only, 3 value-coded), its number of diagonals, levels run by the
  *  single-workgroup coarse tail (0: none), LDS bytes of the tail, levels whose transfers are
  *  evaluated as tensor products of their 1D factors] */
 int feat3_mgpcg_info(feat3_mgpcg* h, uint64_t* out, int cap);

@@ -187,13 +187,18 @@ def test_banded_format_matches_oracle(torch, dim, deg, lvl, inner):
     so, io, xo, _ = solve_orc(H, b)
     s = MGPCG(H, inner=inner, fmt="banded").init()
     info = s.info()
-    if deg == 1:
-        # the assembled Laplacians are symmetric bit for bit: the FP64 band stores its upper half
-        assert info["banded"] == (2 if inner == "f64" else 1)
-        assert info["ndiag"] == ((27 if inner == "f64" else 21) if dim == 3 else 9)
-    else:
-        assert info["banded"] == 0 and info["sell_chunks"] > 0
     s.done()
+    s = MGPCG(H, inner=inner, fmt="banded", options={"band_dict": 0}).init()
+    info0 = s.info()
+    s.done()
+    if deg == 1:
+        # every diagonal holds at most 16 distinct values: value-coded band (3); without it the
+        # assembled Laplacians, symmetric bit for bit, keep the upper half of the FP64 band (2)
+        assert info["banded"] == 3
+        assert info0["banded"] == (2 if inner == "f64" else 1)
+        assert info["ndiag"] == info0["ndiag"] == ((27 if inner == "f64" else 21) if dim == 3 else 9)
+    else:
+        assert info["banded"] == info0["banded"] == 0 and info["sell_chunks"] > 0
     sg, ig, xg, _ = solve_gpu(torch, H, b, inner=inner, fmt="banded")
     assert so == 2 and sg == 2
     if inner == "f64":
@@ -237,7 +242,7 @@ def test_time_kernel_entries(torch):
     H = pb.hierarchy(3, 1, 4, 1)
     b = pb.filtered_rhs(H, 0)
     ref = solve_gpu(torch, H, b, inner="f32", fmt="banded")
-    s = MGPCG(H, inner="f32", fmt="banded").init()
+    s = MGPCG(H, inner="f32", fmt="banded", options={"band_dict": 0}).init()
     A = H.levels[0].A
     n, nnz = H.n, A.nnz
     # the FP32 band keeps the entries above eps_f32^2 of their row's diagonal (21 of 27 per row)
@@ -258,6 +263,21 @@ def test_time_kernel_entries(torch):
     s.done()
     assert st == 2 and it == ref[1]
     assert np.array_equal(x.cpu().numpy(), ref[2])
+    # value-coded bands: 4-bit codes, 8 diagonals per word (FP32: 21 diagonals, 3 words; FP64:
+    # 27, 4 words), the offsets and per word 8 tables of 16 values, then the vectors
+    s = MGPCG(H, inner="f32", fmt="banded").init()
+    assert s.info()["banded"] == 3
+    mask = (n + 31) // 32 * 4
+    band32 = 3 * n * 4 + 21 * 4 + 3 * 8 * 16 * 4
+    band64 = 4 * n * 4 + 27 * 4 + 4 * 8 * 16 * 8
+    for which, nbytes in ((0, band32 + 4 * n * 4), (1, band32 + 3 * n * 4 + mask), (2, band64 + 2 * n * 8 + mask)):
+        ms, nb = s.time_kernel(which, 3)
+        assert ms > 0.0 and nb == nbytes, (which, nb, nbytes)
+    x.zero_()
+    st, it = s.correct(x, torch.from_numpy(b).cuda())
+    s.done()
+    assert st == 2 and it == ref[1]
+    assert np.array_equal(x.cpu().numpy(), ref[2])
     with pytest.raises(Exception):
         MGPCG(H).init().time_kernel(7, 1)
 
@@ -272,14 +292,14 @@ def test_symmetric_band_is_bitwise_the_full_band(torch, dim, lvl):
     from feat3_amd.solver import MGPCG
     H = pb.hierarchy(dim, 1, lvl, 1)
     b = pb.filtered_rhs(H, 0)
-    s = MGPCG(H, fmt="banded").init()
+    s = MGPCG(H, fmt="banded", options=dict(band_dict=0)).init()
     assert s.info()["banded"] == 2
     s.done()
-    ref = solve_gpu(torch, H, b, fmt="banded", options=dict(symmetric_band=0))
-    sym = solve_gpu(torch, H, b, fmt="banded")
+    ref = solve_gpu(torch, H, b, fmt="banded", options=dict(symmetric_band=0, band_dict=0))
+    sym = solve_gpu(torch, H, b, fmt="banded", options=dict(band_dict=0))
     assert ref[1] == sym[1] and np.array_equal(ref[2], sym[2])
     levels = pb.hierarchy_device(dim, 1, lvl, 1)
-    sd = MGPCG(levels, fmt="banded").init()
+    sd = MGPCG(levels, fmt="banded", options=dict(band_dict=0)).init()
     assert sd.info()["banded"] == 2
     x = torch.zeros(H.n, dtype=torch.float64, device="cuda")
     st, it = sd.correct(x, torch.from_numpy(b).cuda())
@@ -296,13 +316,14 @@ def test_nonsymmetric_band_keeps_the_full_band(torch):
     rows = np.repeat(np.arange(A.rows), np.diff(A.row_ptr.astype(np.int64)))
     A.val = np.where(A.col_ind > rows, A.val * (1.0 + 1e-3), A.val)
     b = pb.filtered_rhs(H, 0)
-    s = MGPCG(H, fmt="banded").init()
+    s = MGPCG(H, fmt="banded", options=dict(band_dict=0)).init()
     assert s.info()["banded"] == 1
     s.done()
     so, io, xo, _ = solve_orc(H, b)
-    sg, ig, xg, _ = solve_gpu(torch, H, b, fmt="banded")
-    assert so == sg == 2 and ig == io
-    assert rel(xg, xo) <= 1e-10
+    for opts in (dict(band_dict=0), dict(band_dict=1)):
+        sg, ig, xg, _ = solve_gpu(torch, H, b, fmt="banded", options=opts)
+        assert so == sg == 2 and ig == io
+        assert rel(xg, xo) <= 1e-10
 
 
 @pytest.mark.parametrize("fmt", ["csr", "sell"])
