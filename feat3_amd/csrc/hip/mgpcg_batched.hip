@@ -259,6 +259,22 @@ namespace
     return done ? rt::kOk : rt::kErrArg;
   }
 
+  // grid of the V-cycle's row kernels: one pass of 256 / G rows per block, so that a block pays
+  // its rows' dependent round trips (row pointer, columns, gathers) once instead of once per
+  // 64-row pass (mg::row_grid), which left the small levels latency-bound.  Only the V-cycle's
+  // kernels use it; they reduce nothing (no per-block partials), so the results do not depend
+  // on it.
+  inline mg::RowGrid vcycle_grid(u64 rows, int G)
+  {
+    const u64 per = 256 / u64(G);
+    const u64 nb = (rows + per - 1) / per;
+    if(nb >= (u64(1) << 31)) return mg::row_grid(rows, G);
+    mg::RowGrid g;
+    g.per = per;
+    g.blocks = unsigned(nb < 1 ? 1 : nb);
+    return g;
+  }
+
   template<typename F>
   int with_stream(int G, F&& f)
   {
@@ -531,7 +547,7 @@ namespace
     {
       return with_layout<DT>(GV, S, [&](auto gv, auto sv) {
         constexpr int Gc = decltype(gv)::value, Sc = decltype(sv)::value;
-        const mg::RowGrid g = mg::row_grid(f.nodes, Gc);
+        const mg::RowGrid g = vcycle_grid(f.nodes, Gc);
         batch::k_sweep<Gc, Sc, B, SHARED><<<g.blocks, kBlock, 0, s>>>(rowsA<Sc>(f), g.per, f.dinv.template as<DT>(),
                                                                       f.rhs.template as<DT>(), (const DT*)xin, xout, a);
       });
@@ -541,7 +557,7 @@ namespace
     {
       return with_layout<DT>(GV, S, [&](auto gv, auto sv) {
         constexpr int Gc = decltype(gv)::value, Sc = decltype(sv)::value;
-        const mg::RowGrid g = mg::row_grid(f.nodes, Gc);
+        const mg::RowGrid g = vcycle_grid(f.nodes, Gc);
         batch::k_defect<Gc, Sc, B><<<g.blocks, kBlock, 0, s>>>(rowsA<Sc>(f), g.per, f.mask.template as<u32>(),
                                                               f.rhs.template as<DT>(), x, f.def.template as<DT>(), a);
       });
@@ -551,7 +567,7 @@ namespace
     {
       return with_layout<DT>(GV, S, [&](auto gv, auto sv) {
         constexpr int Gc = decltype(gv)::value, Sc = decltype(sv)::value;
-        const mg::RowGrid g = mg::row_grid(f.nc, Gc);
+        const mg::RowGrid g = vcycle_grid(f.nc, Gc);
         batch::k_restrict<Gc, Sc, B, SHARED><<<g.blocks, kBlock, 0, s>>>(rowsR<Sc>(f, c), g.per, c.mask.template as<u32>(),
                                                                          f.def.template as<DT>(), c.rhs.template as<DT>(), xc,
                                                                          c.dinv.template as<DT>(), a);
@@ -562,7 +578,7 @@ namespace
     {
       return with_layout<DT>(GV, S, [&](auto gv, auto sv) {
         constexpr int Gc = decltype(gv)::value, Sc = decltype(sv)::value;
-        const mg::RowGrid g = mg::row_grid(f.nodes, Gc);
+        const mg::RowGrid g = vcycle_grid(f.nodes, Gc);
         batch::k_prolong<Gc, Sc, B><<<g.blocks, kBlock, 0, s>>>(rowsP<Sc>(f), g.per, f.mask.template as<u32>(), xc, xf, a);
       });
     }
