@@ -169,6 +169,10 @@ namespace mg
     const u32* xoff;   // per diagonal, device memory
   };
 
+  // the banded row sums accumulate s = a * x + s with one rounding (fused multiply-add)
+  __device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+  __device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
   template<typename DT_, typename VT_, bool SYM_ = false, bool STAGE_ = false>
   struct BandedRows
   {
@@ -213,8 +217,8 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
           const u32 vr = (in && c < p) ? c : p;   // lower diagonal: the value stored at row c
           const u32 at = __umul24(vr >> 6, stride) + cs[d] * 64u + (vr & 63u);
           const DT_ a = kNonTemporal ? dev::ld_nt<DT_>(val, at) : dev::ld<DT_>(val, at);
-          const DT_ t = a * x[in ? c : p];
-          s += in ? t : DT_(0);
+          const DT_ t = fmadd(a, x[in ? c : p], s);
+          s = in ? t : s;
         }
         return s;
       }
@@ -222,13 +226,13 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
 _Pragma(F3_STR(unroll F3_BAND_UNROLL))
       for(u32 d = 0; d < ndiag; ++d)
       {
-        // The load is unconditional (an in-range stand-in column) and the product selected away,
+        // The load is unconditional (an in-range stand-in column) and the term selected away,
         // so that no branch splits the batch of loads
         const u32 c = p + co[d] + shift;
         const bool in = c < nc;
         const DT_ a = kNonTemporal ? dev::ld_nt<DT_>(val, base + d * 64u) : dev::ld<DT_>(val, base + d * 64u);
-        const DT_ t = a * x[in ? c : p];
-        s += in ? t : DT_(0);
+        const DT_ t = fmadd(a, x[in ? c : p], s);
+        s = in ? t : s;
       }
       return s;
     }
@@ -253,8 +257,8 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
           const u32 vr = (in && c < p) ? c : p;
           const u32 at = __umul24(vr >> 6, stride) + cs[d] * 64u + (vr & 63u);
           const DT_ a = dev::ld<DT_>(val, at);
-          const DT_ t = a * xs[xo[d] + q];
-          s += in ? t : DT_(0);
+          const DT_ t = fmadd(a, xs[xo[d] + q], s);
+          s = in ? t : s;
         }
         return s;
       }
@@ -265,8 +269,8 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
         const u32 c = p + co[d] + shift;
         const bool in = c < nc;
         const DT_ a = kNonTemporal ? dev::ld_nt<DT_>(val, base + d * 64u) : dev::ld<DT_>(val, base + d * 64u);
-        const DT_ t = a * xs[xo[d] + q];
-        s += in ? t : DT_(0);
+        const DT_ t = fmadd(a, xs[xo[d] + q], s);
+        s = in ? t : s;
       }
       return s;
     }
@@ -283,8 +287,8 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
   // chunk-major (word w of row p at idx[(p / 64) * nw * 64 + w * 64 + p % 64]), the tables
   // [nw * 8][kDictSlots] in the storage type.  A block copies the tables into LDS (DT, as
   // BandedRows converts the stored values) before its rows; the row sum then takes the same
-  // terms, in the same order and with the same arithmetic (each product rounded, then added) as
-  // BandedRows, so the results are bit for bit those of the band it was built from (3D Q1:
+  // terms, in the same order and with the same arithmetic (fused multiply-adds) as BandedRows,
+  // so the results are bit for bit those of the band it was built from (3D Q1:
   // 4 bytes of codes per 8 diagonals instead of 32 of FP32 or 64 of FP64 values).
   // ND_ > 0: exactly ND_ diagonals, every loop unrolled in full (all loads of a row in flight
   // together), and blocks whose rows reach only columns inside the matrix take the row sums
@@ -319,7 +323,6 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
     // BandedRows' range test and selection per term
     __device__ __forceinline__ DT_ row_sum(u64 p64, int, const DT_* __restrict__ x) const
     {
-#pragma clang fp contract(off)
       const dev::cptr<u32> co = dev::to_const(off);
       const u32 p = u32(p64), shift = 1u - u32(rows), nc = u32(cols);
       DT_ s = DT_(0);
@@ -343,8 +346,8 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
 #pragma unroll
         for(u32 d = 0; d < u32(ND_); ++d)
         {
-          const DT_ t = term_value(d, code[d >> 3], d & 7u) * xv[d];
-          s += in[d] ? t : DT_(0);
+          const DT_ t = fmadd(term_value(d, code[d >> 3], d & 7u), xv[d], s);
+          s = in[d] ? t : s;
         }
       }
       else
@@ -360,31 +363,46 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
             const DT_ a = term_value(d, code, j);
             const u32 c = p + co[d] + shift;
             const bool in = c < nc;
-            const DT_ t = a * xat(x, in ? c : p);
-            s += in ? t : DT_(0);
+            const DT_ t = fmadd(a, xat(x, in ? c : p), s);
+            s = in ? t : s;
           }
         }
       }
       return s;
     }
     // ND_ > 0, rows whose every column is inside the matrix (the caller checks the block's range
-    // against omin / omax): the same products and sums without the range test
+    // against omin / omax): the same terms without the range test.  FP64 x is read with buffer
+    // loads whose per-lane offset (p + omin) * 8 is the same for every diagonal and whose
+    // diagonal offset (off_d - omin) * 8 is wave-uniform (a scalar register), so the addresses
+    // cost no vector instructions (3D Q1 L7 outer product 50.6 -> 39.8 us); FP32 x keeps global
+    // loads at p + off_d (buffer loads, all issued together, took the FP32 sweep 19.7 -> 21.1 us)
     __device__ __forceinline__ DT_ row_sum_interior(u32 p, const DT_* __restrict__ x) const
     {
-#pragma clang fp contract(off)
       const dev::cptr<u32> co = dev::to_const(off);
-      const u32 shift = 1u - u32(rows);
       constexpr u32 NW = (u32(ND_ > 0 ? ND_ : 1) + 7u) / 8u;
       const u32 base = __umul24(p >> 6, NW * 64u) + (p & 63u);
       u32 code[NW];
 #pragma unroll
       for(u32 w = 0; w < NW; ++w) code[w] = idx[base + w * 64u];
       DT_ xv[ND_ > 0 ? ND_ : 1];
+      if constexpr(sizeof(DT_) == 8)
+      {
+        const u32 shift = 1u - u32(rows) - u32(omin);   // co[d] + shift = off_d - omin >= 0
+        const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(const_cast<DT_*>(x), 0, -1, 0x00020000);
+        const u32 vo = (p + u32(omin)) * 8u;
 #pragma unroll
-      for(u32 d = 0; d < u32(ND_); ++d) xv[d] = xat(x, p + (co[d] + shift));
+        for(u32 d = 0; d < u32(ND_); ++d)
+          xv[d] = __builtin_bit_cast(DT_, __builtin_amdgcn_raw_buffer_load_b64(xr, vo, (co[d] + shift) * 8u, 0));
+      }
+      else
+      {
+        const u32 shift = 1u - u32(rows);
+#pragma unroll
+        for(u32 d = 0; d < u32(ND_); ++d) xv[d] = xat(x, p + (co[d] + shift));
+      }
       DT_ s = DT_(0);
 #pragma unroll
-      for(u32 d = 0; d < u32(ND_); ++d) s += term_value(d, code[d >> 3], d & 7u) * xv[d];
+      for(u32 d = 0; d < u32(ND_); ++d) s = fmadd(term_value(d, code[d >> 3], d & 7u), xv[d], s);
       return s;
     }
   };
