@@ -421,6 +421,9 @@ def main():
     ap.add_argument("--inner", default="f32", choices=["f64", "f32", "f16"])
     ap.add_argument("--fmt", default="banded", choices=["csr", "sell", "banded"],
                     help="level-matrix format of the V-cycle (banded falls back to SELL-C-sigma where a matrix is not banded)")
+    ap.add_argument("--band-dict", type=int, default=1, choices=[0, 1],
+                    help="value-coded bands where every diagonal has at most 16 distinct values (1, default) "
+                         "or the values stored (0); the same results bit for bit")
     ap.add_argument("--numbering", default="feat", choices=["feat", "lex"],
                     help="DOF numbering of the input hierarchy: feat = the reference's (StandardRefinery + "
                          "Lagrange DOF mapping) with the lattice order as the solver's internal level order; "
@@ -506,7 +509,8 @@ def main():
     def variant(inner, steps, warmup, fmt=None, cycle="v", level_orders="auto"):
         fmt = fmt or args.fmt
         t0 = time.time()
-        s = MGPCG(H, inner=inner, fmt=fmt, cycle=cycle, level_orders=level_orders).init()
+        s = MGPCG(H, inner=inner, fmt=fmt, cycle=cycle, level_orders=level_orders,
+                  options={"band_dict": args.band_dict}).init()
         init_s = time.time() - t0
         if inner == args.inner and fmt == args.fmt and cycle == "v":
             setup["host_init_s"] = init_s

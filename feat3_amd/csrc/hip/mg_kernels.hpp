@@ -371,39 +371,62 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
       return s;
     }
     // ND_ > 0, rows whose every column is inside the matrix (the caller checks the block's range
-    // against omin / omax): the same terms without the range test.  FP64 x is read with buffer
+    // against omin / omax): the same terms without the range test, for R rows at once (every
+    // load of the R rows issued before the first term is formed).  FP64 x is read with buffer
     // loads whose per-lane offset (p + omin) * 8 is the same for every diagonal and whose
     // diagonal offset (off_d - omin) * 8 is wave-uniform (a scalar register), so the addresses
     // cost no vector instructions (3D Q1 L7 outer product 50.6 -> 39.8 us); FP32 x keeps global
     // loads at p + off_d (buffer loads, all issued together, took the FP32 sweep 19.7 -> 21.1 us)
-    __device__ __forceinline__ DT_ row_sum_interior(u32 p, const DT_* __restrict__ x) const
+    template<int R>
+    __device__ __forceinline__ void row_sums_interior(const u32 (&p)[R], const DT_* __restrict__ x, DT_ (&s)[R]) const
     {
       const dev::cptr<u32> co = dev::to_const(off);
       constexpr u32 NW = (u32(ND_ > 0 ? ND_ : 1) + 7u) / 8u;
-      const u32 base = __umul24(p >> 6, NW * 64u) + (p & 63u);
-      u32 code[NW];
+      constexpr u32 NDX = u32(ND_ > 0 ? ND_ : 1);
+      u32 code[R][NW];
+      DT_ xv[R][NDX];
 #pragma unroll
-      for(u32 w = 0; w < NW; ++w) code[w] = idx[base + w * 64u];
-      DT_ xv[ND_ > 0 ? ND_ : 1];
+      for(int r = 0; r < R; ++r)
+      {
+        const u32 base = __umul24(p[r] >> 6, NW * 64u) + (p[r] & 63u);
+#pragma unroll
+        for(u32 w = 0; w < NW; ++w) code[r][w] = idx[base + w * 64u];
+      }
       if constexpr(sizeof(DT_) == 8)
       {
         const u32 shift = 1u - u32(rows) - u32(omin);   // co[d] + shift = off_d - omin >= 0
         const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(const_cast<DT_*>(x), 0, -1, 0x00020000);
-        const u32 vo = (p + u32(omin)) * 8u;
 #pragma unroll
-        for(u32 d = 0; d < u32(ND_); ++d)
-          xv[d] = __builtin_bit_cast(DT_, __builtin_amdgcn_raw_buffer_load_b64(xr, vo, (co[d] + shift) * 8u, 0));
+        for(int r = 0; r < R; ++r)
+        {
+          const u32 vo = (p[r] + u32(omin)) * 8u;
+#pragma unroll
+          for(u32 d = 0; d < u32(ND_); ++d)
+            xv[r][d] = __builtin_bit_cast(DT_, __builtin_amdgcn_raw_buffer_load_b64(xr, vo, (co[d] + shift) * 8u, 0));
+        }
       }
       else
       {
         const u32 shift = 1u - u32(rows);
 #pragma unroll
-        for(u32 d = 0; d < u32(ND_); ++d) xv[d] = xat(x, p + (co[d] + shift));
-      }
-      DT_ s = DT_(0);
+        for(int r = 0; r < R; ++r)
 #pragma unroll
-      for(u32 d = 0; d < u32(ND_); ++d) s = fmadd(term_value(d, code[d >> 3], d & 7u), xv[d], s);
-      return s;
+          for(u32 d = 0; d < u32(ND_); ++d) xv[r][d] = xat(x, p[r] + (co[d] + shift));
+      }
+#pragma unroll
+      for(int r = 0; r < R; ++r)
+      {
+        s[r] = DT_(0);
+#pragma unroll
+        for(u32 d = 0; d < u32(ND_); ++d) s[r] = fmadd(term_value(d, code[r][d >> 3], d & 7u), xv[r][d], s[r]);
+      }
+    }
+    __device__ __forceinline__ DT_ row_sum_interior(u32 p, const DT_* __restrict__ x) const
+    {
+      const u32 pp[1] = {p};
+      DT_ s[1];
+      row_sums_interior<1>(pp, x, s);
+      return s[0];
     }
   };
   template<typename T> struct IsDictBand { static constexpr bool value = false; };
@@ -652,9 +675,29 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
       __syncthreads();
       Rows_ B = A;
       B.lt = lt;
-      // a block whose rows reach only columns inside the matrix skips the range tests
+      // a block whose rows reach only columns inside the matrix skips the range tests; there an
+      // FP32 thread takes two rows blockDim.x apart at once, their loads in flight together (3D
+      // Q1 L7 sweep 19.2 -> 15.4 us; FP64, whose 54 registers of x per row then cost occupancy,
+      // one row: outer product 39.4 us against 42.4 with two)
+      constexpr int RT = sizeof(DT_) == 4 ? 2 : 1;
       if(Rows_::kND > 0 && std::int64_t(beg) + A.omin >= 0 && std::int64_t(end) - 1 + A.omax < std::int64_t(A.cols))
-        for(u64 p = beg + threadIdx.x; p < end; p += blockDim.x) f(p, B.row_sum_interior(u32(p), x), true);
+        for(u64 p0 = beg + threadIdx.x; p0 < end; p0 += RT * u64(blockDim.x))
+        {
+          u32 pp[RT];
+          bool has[RT];
+#pragma unroll
+          for(int r = 0; r < RT; ++r)
+          {
+            const u64 q = p0 + u64(r) * blockDim.x;
+            has[r] = q < end;
+            pp[r] = u32(has[r] ? q : p0);
+          }
+          DT_ sr[RT];
+          B.template row_sums_interior<RT>(pp, x, sr);
+#pragma unroll
+          for(int r = 0; r < RT; ++r)
+            if(has[r]) f(pp[r], sr[r], true);
+        }
       else
         for(u64 p = beg + threadIdx.x; p < end; p += blockDim.x) f(p, B.row_sum(p, 0, x), true);
     }
