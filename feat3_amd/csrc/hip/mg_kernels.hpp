@@ -294,6 +294,9 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
   // together), and blocks whose rows reach only columns inside the matrix take the row sums
   // without the range tests.  ND_ = 0: any count, the offsets padded to nw * 8 with stand-ins
   // whose column is outside the matrix for every row (their terms are selected away).
+#ifndef F3_DICT_RT
+#define F3_DICT_RT 2   // FP32 rows per thread in the interior blocks of value-coded levels (3, 4: the same time)
+#endif
   constexpr u32 kDictSlots = 16;
   constexpr u32 kDictDiag = 32;   // diagonals at most (4 code words per row)
   template<typename DT_, typename VT_, int ND_ = 0>
@@ -679,7 +682,7 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
       // FP32 thread takes two rows blockDim.x apart at once, their loads in flight together (3D
       // Q1 L7 sweep 19.2 -> 15.4 us; FP64, whose 54 registers of x per row then cost occupancy,
       // one row: outer product 39.4 us against 42.4 with two)
-      constexpr int RT = sizeof(DT_) == 4 ? 2 : 1;
+      constexpr int RT = sizeof(DT_) == 4 ? F3_DICT_RT : 1;
       if(Rows_::kND > 0 && std::int64_t(beg) + A.omin >= 0 && std::int64_t(end) - 1 + A.omax < std::int64_t(A.cols))
         for(u64 p0 = beg + threadIdx.x; p0 < end; p0 += RT * u64(blockDim.x))
         {

@@ -1189,7 +1189,7 @@ namespace
     {
       // FP32 levels of 2^20 rows or more: 512 rows per block, two per thread in the interior blocks
       const bool two = sizeof(DT) == 4 && !m.rows_per_block && m.rows >= (u64(1) << 20);
-      const mg::RowGrid g = two ? mg::row_grid_fixed(m.rows, 512) : grid(1);
+      const mg::RowGrid g = two ? mg::row_grid_fixed(m.rows, 256 * F3_DICT_RT) : grid(1);
       const u32* tile = m.tile_blocks == g.blocks ? m.tile.template as<u32>() : nullptr;
       // unrolled in full for the diagonal counts of Q1 operators (2D: 9; 3D: 27, 21 without the
       // face couplings below eps_f32^2)
