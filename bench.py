@@ -25,11 +25,12 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 
 # HBM bytes per launch of the finest-level sweep, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (separate runs of this script) and scaled by the same counters on a plain 512 MiB
-# device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); profiles/*_pmc.md hold the commands and raw
-# values.  Keyed by (workload, level, inner, format); absent -> null.
+# device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); profiles/k_sweep_3d_q1_l7_f32.md and
+# profiles/batch_k_sweep_vector32_f32.md hold the commands and raw values (the headline's band is
+# value-coded).  Keyed by (workload, level, inner, format); absent -> null.
 MEASURED_SWEEP_TRAFFIC = {
-    ("poisson3d", 7, "f32", "banded"): (2 * 101519.1 + 8385.5) * 1024,
-    ("vector32", 6, "f32", "banded"): (2 * 395648.5 + 102984.4) * 1024,
+    ("poisson3d", 7, "f32", "banded"): (2 * 25667.0 + 8385.5) * 1024,
+    ("vector32", 6, "f32", "banded"): (2 * 288665.2 + 102984.4) * 1024,
 }
 
 
@@ -603,7 +604,8 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     achieved = head["sweep_GBs"]
-    traffic = MEASURED_SWEEP_TRAFFIC.get((args.workload, args.level, args.inner, args.fmt))
+    # measured with the value-coded band (the default)
+    traffic = MEASURED_SWEEP_TRAFFIC.get((args.workload, args.level, args.inner, args.fmt)) if args.band_dict else None
     out = {
         "metric": "time to rel. residual 1e-8 & DOF·iter/s, FP32-inner vs FP64; SpMV GB/s",
         "value": value,
