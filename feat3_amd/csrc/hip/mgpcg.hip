@@ -565,7 +565,8 @@ namespace
           if(ho.size() >= kMaxDiag) return rt::kOk;
           ho.push_back(u32(w * 32 + u64(__builtin_ctz(b))));
         }
-      if(double(ho.size()) * double(rows) > 1.25 * double(std::max<u64>(nnz, 1))) return rt::kOk;
+      const bool dense = double(ho.size()) * double(rows) <= 1.25 * double(std::max<u64>(nnz, 1));
+      if(!dense && !codable_fill(ho.size())) return rt::kOk;
       const u64 nd = ho.size(), padded = (rows + 63) / 64 * 64;
       F3_RC(boff.alloc(sizeof(u32) * nd));
       F3_RC(bval.alloc(sizeof(VT) * nd * padded));
@@ -580,6 +581,7 @@ namespace
       bsym = false;
       F3_RC(build_dict(s));
       if(dict) return rt::kOk;
+      if(!dense) { drop_band(); return rt::kOk; }
       // symmetric storage when the offsets are symmetric and every value equals its mirror image
       // (FP64 values, as build_banded)
       if(sizeof(VT) != 8 || !allow_sym) return rt::kOk;
@@ -639,6 +641,22 @@ namespace
       const double e = 1.1920928955078125e-07;
       return sizeof(VT) == 8 ? 0.0 : e * e;
     }
+    // A value-coded band stores 4 bits per slot, so it may carry more zero fill than a band of
+    // values: up to 4 slots per stored entry (2D Q2: 25 diagonals at 57% fill, 13 bytes of codes
+    // per row against 16 entries of 6 bytes in SELL-C-sigma); the band is only kept if the codes
+    // are built
+    bool codable_fill(u64 nd) const
+    {
+      return allow_dict && nd <= mg::kDictDiag && double(nd) * double(rows) <= 4.0 * double(std::max<u64>(nnz, 1));
+    }
+    void drop_band()
+    {
+      banded = false;
+      ndiag = 0;
+      DevMem a, b;
+      std::swap(a.p, bval.p);
+      std::swap(b.p, boff.p);
+    }
     int build_banded(const HostCsr& h, hipStream_t s)
     {
       banded = false;
@@ -663,7 +681,8 @@ namespace
             offs.insert(it, o);
           }
         }
-      if(double(offs.size()) * double(rows) > 1.25 * double(std::max<u64>(nnz, 1))) return rt::kOk;
+      const bool dense = double(offs.size()) * double(rows) <= 1.25 * double(std::max<u64>(nnz, 1));
+      if(!dense && !codable_fill(offs.size())) return rt::kOk;
       const u64 nd = offs.size();
       // chunk-major: the nd diagonals of 64 consecutive rows are contiguous, so that a wave
       // reads one contiguous region (val[(p / 64) * nd * 64 + d * 64 + p % 64])
@@ -697,6 +716,7 @@ namespace
         F3_RC(build_dict(s));
         if(dict) return rt::kOk;
       }
+      if(!dense) { drop_band(); return rt::kOk; }
       // symmetric storage when the band equals its mirror image bit for bit, for FP64 values
       // (3D Q1 L7: FP64 sweep 107 -> 85 us, outer PCG product likewise).  FP32 and binary16 bands
       // stay full: their values are served partly from the Infinity Cache, and the mirrored
@@ -1192,7 +1212,7 @@ namespace
       const mg::RowGrid g = two ? mg::row_grid_fixed(m.rows, 256 * F3_DICT_RT) : grid(1);
       const u32* tile = m.tile_blocks == g.blocks ? m.tile.template as<u32>() : nullptr;
       // unrolled in full for the diagonal counts of Q1 operators (2D: 9; 3D: 27, 21 without the
-      // face couplings below eps_f32^2)
+      // face couplings below eps_f32^2) and of the 2D Q2 operator (25)
       auto go = [&](auto nd) {
         mg::DictBandRows<DT, VT, decltype(nd)::value> A{m.doff.template as<u32>(), m.didx.template as<u32>(),
                                                         m.dtab.template as<VT>(), m.rows, m.cols, m.ndiag, m.dnw,
@@ -1201,6 +1221,7 @@ namespace
         f(A, g);
       };
       if(m.ndiag == 27) go(std::integral_constant<int, 27>());
+      else if(m.ndiag == 25) go(std::integral_constant<int, 25>());
       else if(m.ndiag == 21) go(std::integral_constant<int, 21>());
       else if(m.ndiag == 9) go(std::integral_constant<int, 9>());
       else go(std::integral_constant<int, 0>());

@@ -178,8 +178,8 @@ def test_banded_format_matches_oracle(torch, dim, deg, lvl, inner):
     FP64 oracle solution.  In lexicographic numbering Q1 matrices are banded (27 / 9 diagonals,
     little fill); the FP32 / binary16 band of the 3D matrix leaves out the 6 face diagonals, whose
     values are roundoff residues of about 2e-17 of the diagonal (below eps_f32^2), so it has 21.
-    The 2D Q2 matrix has 25 diagonals but rows of 9..25 entries (56% fill), so the format falls
-    back to SELL-C-sigma there."""
+    The 2D Q2 matrix has 25 diagonals but rows of 9..25 entries (56% fill), so a band of values
+    falls back to SELL-C-sigma there; the value-coded band (default) takes it."""
     print(f"case banded {dim}D Q{deg} L{lvl} {inner}", flush=True)
     from feat3_amd.solver import MGPCG
     H = pb.hierarchy(dim, deg, lvl, 1)
@@ -198,7 +198,10 @@ def test_banded_format_matches_oracle(torch, dim, deg, lvl, inner):
         assert info0["banded"] == (2 if inner == "f64" else 1)
         assert info["ndiag"] == info0["ndiag"] == ((27 if inner == "f64" else 21) if dim == 3 else 9)
     else:
-        assert info["banded"] == info0["banded"] == 0 and info["sell_chunks"] > 0
+        # 25 diagonals at 57% fill: too much fill for a band of values (SELL-C-sigma), within the
+        # 4 slots per entry a value-coded band may carry
+        assert info["banded"] == 3 and info["ndiag"] == 25
+        assert info0["banded"] == 0 and info0["sell_chunks"] > 0
     sg, ig, xg, _ = solve_gpu(torch, H, b, inner=inner, fmt="banded")
     assert so == 2 and sg == 2
     if inner == "f64":
