@@ -630,21 +630,25 @@ namespace batch
     block_partials<G_, 1>(v, partials, K);
   }
 
-  // x += alpha p, r -= alpha q (active systems); partial sums of r.r
-  template<int G_, int B_>
+  // x += alpha p, r -= alpha q (active systems); partial sums of r.r; and the next V-cycle's
+  // entry for every system, as k_entry computes it from the new r: rhs = (inner) r, x0 = dinv rhs
+  // (the V-cycle that follows then starts without k_entry, which would read r again)
+  template<int G_, int B_, bool SHARED_, typename DT_>
   __global__ __launch_bounds__(256) void k_pcg_update(u64 nodes, int K, const double* __restrict__ p, const double* __restrict__ q,
                                                       double* __restrict__ x, double* __restrict__ r, const SysState* st,
-                                                      const int* any, double* partials)
+                                                      const int* any, double* partials, DT_* __restrict__ rhs,
+                                                      DT_* __restrict__ x0, const DT_* __restrict__ dinv)
   {
     if(idle(any)) return;
     double v[1] = {0.0};
     const int k = int(threadIdx.x % G_);
     const u64 ngrp = blockDim.x / G_;
     const u64 m = nodes * B_;
-    if(k < K && sys_active(st, k))
+    if(k < K)
     {
+      const bool act = sys_active(st, k);
       const double alpha = st[k].alpha;
-      const bool upd = !(fabs(alpha) < 2.220446049250313e-16);   // a NaN alpha updates (dense_vector.hpp:891)
+      const bool upd = act && !(fabs(alpha) < 2.220446049250313e-16);   // a NaN alpha updates (dense_vector.hpp:891)
       for(u64 i = u64(blockIdx.x) * ngrp + threadIdx.x / G_; i < m; i += u64(gridDim.x) * ngrp)
       {
         const u64 t = i * u64(K) + u64(k);
@@ -655,7 +659,10 @@ namespace batch
           ri = ri + (-alpha) * q[t];
           r[t] = ri;
         }
-        v[0] += ri * ri;
+        if(act) v[0] += ri * ri;
+        const DT_ rv = DT_(ri);
+        rhs[t] = rv;
+        x0[t] = dinv[SHARED_ ? i : t] * rv;
       }
     }
     block_partials<G_, 1>(v, partials, K);

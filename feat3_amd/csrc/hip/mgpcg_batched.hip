@@ -583,12 +583,14 @@ namespace
       });
     }
 
-    int enqueue_vcycle(const double* def, const int* a)
+    // entered: rhs and x of the top level already hold the cycle's entry (written by k_pcg_update)
+    int enqueue_vcycle(const double* def, const int* a, bool entered = false)
     {
       const std::size_t nl = L.size();
       Level& top = *L[0];
-      batch::k_entry<DT, B, SHARED><<<4096, kBlock, 0, s>>>(top.nodes, K, def, top.rhs.template as<DT>(), top.x.template as<DT>(),
-                                                           top.dinv.template as<DT>(), a);
+      if(!entered)
+        batch::k_entry<DT, B, SHARED><<<4096, kBlock, 0, s>>>(top.nodes, K, def, top.rhs.template as<DT>(), top.x.template as<DT>(),
+                                                             top.dinv.template as<DT>(), a);
       std::vector<DT*> cur(nl, nullptr), oth(nl, nullptr);
       for(std::size_t i = 0; i + 1 < nl; ++i)
       {
@@ -662,8 +664,10 @@ namespace
       const unsigned sb = stream_blocks(m0, G);
       F3_RC(with_stream(G, [&](auto gg) {
         constexpr int Gc = decltype(gg)::value;
-        batch::k_pcg_update<Gc, B><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, p.template as<double>(), q.template as<double>(), x_cur,
-                                                         r.template as<double>(), Sst, anyp(), part);
+        batch::k_pcg_update<Gc, B, SHARED, DT><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, p.template as<double>(), q.template as<double>(),
+                                                                     x_cur, r.template as<double>(), Sst, anyp(), part,
+                                                                     L[0]->rhs.template as<DT>(), L[0]->x.template as<DT>(),
+                                                                     L[0]->dinv.template as<DT>());
       }));
       fin<mg::kFinNewDefect>(sb);
       return rt::check_launch();
@@ -676,7 +680,8 @@ namespace
       batch::SysState* Sst = st.template as<batch::SysState>();
       const unsigned sb = stream_blocks(m0, G);
       const DT* z = L[0]->xfinal;
-      F3_RC(enqueue_vcycle(r.template as<double>(), anyp()));
+      // the outer half before it (k_pcg_update) wrote the cycle's entry
+      F3_RC(enqueue_vcycle(r.template as<double>(), anyp(), true));
       F3_RC(with_stream(G, [&](auto gg) {
         constexpr int Gc = decltype(gg)::value;
         batch::k_pcg_rz<Gc, B, DT><<<sb, kBlock, 0, s>>>(L[0]->nodes, K, z, r.template as<double>(), Sst, anyp(), part);
