@@ -25,12 +25,13 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 
 # HBM bytes per launch of the finest-level sweep, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (separate runs of this script) and scaled by the same counters on a plain 512 MiB
-# device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); profiles/k_sweep_3d_q1_l7_f32.md and
-# profiles/batch_k_sweep_vector32_f32.md hold the commands and raw values (the headline's band is
-# value-coded).  Keyed by (workload, level, inner, format); absent -> null.
+# device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); profiles/k_sweep_3d_q1_l7_f32.md,
+# profiles/solve_kernels_2d_q2_l9.md and profiles/batch_k_sweep_vector32_f32.md hold the commands
+# and raw values (the two scalar workloads' bands are value-coded).  Keyed by (workload, level, inner, format); absent -> null.
 MEASURED_SWEEP_TRAFFIC = {
     ("poisson3d", 7, "f32", "banded"): (2 * 25667.0 + 8385.5) * 1024,
-    ("vector32", 6, "f32", "banded"): (2 * 288665.2 + 102984.4) * 1024,
+    ("poisson2d", 9, "f32", "banded"): (2 * 14478.4 + 4104.0) * 1024,
+    ("vector32", 6, "f32", "banded"): (2 * 288774.8 + 102984.4) * 1024,
 }
 
 
