@@ -491,9 +491,296 @@ namespace batch
       const int k = int(t % u64(K));
       const double* a = ainv + (per_system ? u64(k) * m * m : 0) + i * m;
       double s = 0.0;
+      // unrolled so that a group of the row's loads is in flight at once (the sum keeps its order)
+#pragma unroll 8
       for(u64 j = 0; j < m; ++j) s += a[j] * double(rhs[j * u64(K) + u64(k)]);
       x[i * u64(K) + u64(k)] = masked(mask, i / u64(B)) ? DT_(0) : DT_(s);
     }
+  }
+
+  // ---- coarse tail of the shared-values solver: one workgroup per system ----------------------
+  //
+  // Below about a thousand nodes a level step is a launch of 5-7 us that moves a few hundred KB.
+  // k_tail_sys runs the V-cycle's part on the levels tail[0] (entered from the restriction above,
+  // which wrote tail[0]'s right-hand side) down to the coarsest and back in one workgroup of
+  // kTailSysThreads per system, with barriers between the steps.  The system's vectors on these
+  // levels live in LDS ([node][h]: x0, x1, rhs, def, dinv; 5 * nodes * B values per level); the
+  // shared matrices are read from global memory in ELL order (TailSysEll: the entries of a wave's
+  // rows coalesced), where the K workgroups keep them L2-resident.
+  // A thread takes a node and its B components.  Every step sums a row's terms in the order of
+  // k_sweep / k_defect / k_restrict / k_prolong / k_coarse with the same expressions, so the
+  // results are bit for bit those of the per-level launches.
+  constexpr int kTailSysMax = 8;
+  constexpr int kTailSysThreads = 1024;
+
+  // A matrix of the tail in ELL order: entry k of row r at col[k * rows + r] and values
+  // val[(k * V + t) * rows + r] (V values per entry as stored: [h] or [h][w]), k < w; rp gives
+  // the row lengths (entries past a row's end are padding, column 0 and value 0, left out)
+  template<typename VT_>
+  struct TailSysEll
+  {
+    const u32* rp;
+    const u32* col;
+    const VT_* val;
+    u32 rows, w;
+  };
+
+  template<typename DT_, typename VT_>
+  struct TailSysLevel
+  {
+    u32 nodes, off;                // nodes; offset of the level's vectors in LDS (DT_ elements)
+    TailSysEll<VT_> A, P, R;       // P: rows = this level's nodes; R: rows = the next coarser level's
+    const u32* mask;
+    const DT_* dinv;               // [node][h]
+  };
+
+  // ELL copy of a CSR matrix with V values per entry (TailSysEll), one thread per (row, k)
+  template<typename VT_>
+  __global__ void k_tail_ell(u32 rows, u32 w, u32 V, const u32* __restrict__ rp, const u32* __restrict__ col,
+                             const VT_* __restrict__ val, u32* __restrict__ ecol, VT_* __restrict__ eval)
+  {
+    const u64 n = u64(rows) * w;
+    for(u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+    {
+      const u32 k = u32(i / rows), r = u32(i % rows);
+      const u32 e = rp[r] + k;
+      const bool in = e < rp[r + 1];
+      ecol[i] = in ? col[e] : 0u;
+      for(u32 t = 0; t < V; ++t) eval[(u64(k) * V + t) * rows + r] = in ? val[u64(e) * V + t] : VT_(0.0f);
+    }
+  }
+
+  template<typename DT_, typename VT_>
+  struct TailSysDesc
+  {
+    int nlev, pre, post;
+    u32 ncoarse;                   // rows of the coarsest level (nodes * B)
+    const double* ainv;            // FP64 inverse of the filtered coarsest matrix (shared)
+    const DT_* rhs_in;             // tail[0]'s right-hand side, [node][h][k]
+    DT_* x_out;                    // tail[0]'s final iterate, [node][h][k]
+    TailSysLevel<DT_, VT_> lv[kTailSysMax];
+  };
+
+  // s[h] = sum_e A_e[h][.] x[col_e][.] over the row's entries in ascending order (BlockRows with
+  // shared values); x in LDS, [node][w].  The entries are taken in batches of NB whose loads (one
+  // coalesced access per entry and value across the rows of a wave) are all issued before the
+  // first use; loads past the ELL width read its last column, and terms past the row's end are
+  // left out.
+  template<typename DT_, typename VT_, int B_, bool DIAG_>
+  __device__ __forceinline__ void tail_sys_block_row(const TailSysEll<VT_>& M, u32 p, const DT_* x, DT_ (&s)[B_])
+  {
+    constexpr u32 V = DIAG_ ? u32(B_) : u32(B_ * B_);
+    constexpr u32 NB = V * sizeof(DT_) <= 12 ? 16 : V * sizeof(DT_) <= 36 ? 8 : 4;   // registers of a batch
+#pragma unroll
+    for(int h = 0; h < B_; ++h) s[h] = DT_(0);
+    const u32 len = M.rp[p + 1] - M.rp[p];
+    const u64 n = M.rows;
+    for(u32 k0 = 0; k0 < M.w; k0 += NB)
+    {
+      u32 j[NB];
+      DT_ a[NB][V];
+#pragma unroll
+      for(u32 u = 0; u < NB; ++u)
+      {
+        const u64 k = k0 + u < M.w ? k0 + u : M.w - 1;
+        j[u] = M.col[k * n + p];
+#pragma unroll
+        for(u32 t = 0; t < V; ++t) a[u][t] = dev::ld<DT_>(M.val, (k * V + t) * n + p);
+      }
+#pragma unroll
+      for(u32 u = 0; u < NB; ++u)
+      {
+        if(k0 + u < len)
+        {
+          DT_ xv[B_];
+#pragma unroll
+          for(int w = 0; w < B_; ++w) xv[w] = x[j[u] * u32(B_) + u32(w)];
+          if constexpr(DIAG_)
+          {
+#pragma unroll
+            for(int h = 0; h < B_; ++h) s[h] += a[u][h] * xv[h];
+          }
+          else
+          {
+#pragma unroll
+            for(int h = 0; h < B_; ++h)
+#pragma unroll
+              for(int w = 0; w < B_; ++w) s[h] += a[u][h * B_ + w] * xv[w];
+          }
+        }
+      }
+    }
+  }
+
+  // s[h] = sum_e v_e x[col_e][h] (a scalar transfer graph applied per component, CompRows), in
+  // batches of 16 entries as above
+  template<typename DT_, typename VT_, int B_>
+  __device__ __forceinline__ void tail_sys_comp_row(const TailSysEll<VT_>& M, u32 p, const DT_* x, DT_ (&s)[B_])
+  {
+    constexpr u32 NB = 16;
+#pragma unroll
+    for(int h = 0; h < B_; ++h) s[h] = DT_(0);
+    const u32 len = M.rp[p + 1] - M.rp[p];
+    const u64 n = M.rows;
+    for(u32 k0 = 0; k0 < M.w; k0 += NB)
+    {
+      u32 j[NB];
+      DT_ v[NB];
+#pragma unroll
+      for(u32 u = 0; u < NB; ++u)
+      {
+        const u64 k = k0 + u < M.w ? k0 + u : M.w - 1;
+        j[u] = M.col[k * n + p];
+        v[u] = dev::ld<DT_>(M.val, k * n + p);
+      }
+#pragma unroll
+      for(u32 u = 0; u < NB; ++u)
+      {
+        if(k0 + u < len)
+        {
+#pragma unroll
+          for(int h = 0; h < B_; ++h) s[h] += v[u] * x[j[u] * u32(B_) + u32(h)];
+        }
+      }
+    }
+  }
+
+  // grid: K workgroups of kTailSysThreads (workgroup k = system k); dynamic LDS: the levels'
+  // vectors.  Level li's iterate alternates between its buffers x0 and x1; the coarsest level's
+  // solution is x0.
+  template<typename DT_, typename VT_, int B_, bool DIAG_>
+  __global__ __launch_bounds__(kTailSysThreads) void k_tail_sys(TailSysDesc<DT_, VT_> d, int K, const int* any)
+  {
+    if(idle(any)) return;
+    extern __shared__ double tail_sys_arena_[];
+    DT_* arena = reinterpret_cast<DT_*>(tail_sys_arena_);
+    const u32 tid = threadIdx.x;
+    const u64 k = blockIdx.x;
+    // dinv of every level; tail[0]'s right-hand side and first smoothing step x0 = dinv rhs
+    for(int i = 0; i < d.nlev; ++i)
+    {
+      const TailSysLevel<DT_, VT_>& l = d.lv[i];
+      const u32 n = l.nodes * u32(B_);
+      DT_* v = arena + l.off;
+      for(u32 r = tid; r < n; r += kTailSysThreads)
+      {
+        const DT_ di = l.dinv[r];
+        v[4 * n + r] = di;
+        if(i == 0)
+        {
+          const DT_ b = d.rhs_in[u64(r) * u64(K) + k];
+          v[2 * n + r] = b;
+          v[r] = di * b;
+        }
+      }
+    }
+    __syncthreads();
+    // bit li: the buffer (0: x0, 1: x1) that holds level li's iterate
+    u32 curm = 0u;
+    auto cur = [&](int li) { return (curm >> li) & 1u; };
+    // x_out = x_in + dinv (rhs - A x_in) on level li, from its current buffer into the other one
+    auto sweep = [&](int li) {
+      const TailSysLevel<DT_, VT_>& l = d.lv[li];
+      const u32 n = l.nodes * u32(B_);
+      DT_* v = arena + l.off;
+      const DT_* xi = v + cur(li) * n;
+      DT_* xo = v + (cur(li) ^ 1u) * n;
+      for(u32 p = tid; p < l.nodes; p += kTailSysThreads)
+      {
+        DT_ s[B_];
+        tail_sys_block_row<DT_, VT_, B_, DIAG_>(l.A, p, xi, s);
+#pragma unroll
+        for(int h = 0; h < B_; ++h)
+        {
+          const u32 i = p * u32(B_) + u32(h);
+          xo[i] = xi[i] + v[4 * n + i] * (v[2 * n + i] - s[h]);
+        }
+      }
+      curm ^= 1u << li;
+      __syncthreads();
+    };
+    for(int li = 0; li + 1 < d.nlev; ++li)
+    {
+      for(int j = 1; j < d.pre; ++j) sweep(li);
+      const TailSysLevel<DT_, VT_>& f = d.lv[li];
+      const TailSysLevel<DT_, VT_>& c = d.lv[li + 1];
+      const u32 n = f.nodes * u32(B_), nc = c.nodes * u32(B_);
+      DT_* fv = arena + f.off;
+      DT_* cv = arena + c.off;
+      // def = filter(rhs - A x)
+      for(u32 p = tid; p < f.nodes; p += kTailSysThreads)
+      {
+        DT_ s[B_];
+        tail_sys_block_row<DT_, VT_, B_, DIAG_>(f.A, p, fv + cur(li) * n, s);
+        const bool m = masked(f.mask, p);
+#pragma unroll
+        for(int h = 0; h < B_; ++h)
+        {
+          const u32 i = p * u32(B_) + u32(h);
+          fv[3 * n + i] = m ? DT_(0) : fv[2 * n + i] - s[h];
+        }
+      }
+      __syncthreads();
+      // rhs_c = filter(R def); x_c = dinv_c rhs_c unless c is the coarsest level
+      const bool first_step = li + 2 < d.nlev;
+      for(u32 p = tid; p < c.nodes; p += kTailSysThreads)
+      {
+        DT_ s[B_];
+        tail_sys_comp_row<DT_, VT_, B_>(f.R, p, fv + 3 * n, s);
+        const bool m = masked(c.mask, p);
+#pragma unroll
+        for(int h = 0; h < B_; ++h)
+        {
+          const u32 i = p * u32(B_) + u32(h);
+          const DT_ r = m ? DT_(0) : s[h];
+          cv[2 * nc + i] = r;
+          if(first_step) cv[i] = cv[4 * nc + i] * r;
+        }
+      }
+      __syncthreads();
+    }
+    {
+      // exact coarse solve x0 = Ainv rhs
+      const TailSysLevel<DT_, VT_>& c = d.lv[d.nlev - 1];
+      DT_* cv = arena + c.off;
+      const u32 m = d.ncoarse;
+      for(u32 i = tid; i < m; i += kTailSysThreads)
+      {
+        const double* a = d.ainv + u64(i) * m;
+        double s = 0.0;
+#pragma unroll 8
+        for(u32 j = 0; j < m; ++j) s += a[j] * double(cv[2 * m + j]);
+        cv[i] = masked(c.mask, i / u32(B_)) ? DT_(0) : DT_(s);
+      }
+      __syncthreads();
+    }
+    for(int li = d.nlev - 2; li >= 0; --li)
+    {
+      // x += filter(P x_c)
+      const TailSysLevel<DT_, VT_>& f = d.lv[li];
+      const TailSysLevel<DT_, VT_>& c = d.lv[li + 1];
+      const u32 n = f.nodes * u32(B_), nc = c.nodes * u32(B_);
+      DT_* xf = arena + f.off + cur(li) * n;
+      const DT_* xc = arena + c.off + cur(li + 1) * nc;
+      for(u32 p = tid; p < f.nodes; p += kTailSysThreads)
+      {
+        DT_ s[B_];
+        tail_sys_comp_row<DT_, VT_, B_>(f.P, p, xc, s);
+        if(masked(f.mask, p)) continue;
+#pragma unroll
+        for(int h = 0; h < B_; ++h)
+        {
+          const u32 i = p * u32(B_) + u32(h);
+          xf[i] = xf[i] + s[h];
+        }
+      }
+      __syncthreads();
+      for(int j = 0; j < d.post; ++j) sweep(li);
+    }
+    const TailSysLevel<DT_, VT_>& t = d.lv[0];
+    const u32 n = t.nodes * u32(B_);
+    const DT_* xf = arena + t.off + cur(0) * n;
+    for(u32 r = tid; r < n; r += kTailSysThreads) d.x_out[u64(r) * u64(K) + k] = xf[r];
   }
 
   // ---- PCG -----------------------------------------------------------------------------------

@@ -835,6 +835,8 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
     for(u64 i = threadIdx.x; i < n; i += blockDim.x)
     {
       double s = 0.0;
+      // unrolled so that a group of the row's loads is in flight at once (the sum keeps its order)
+#pragma unroll 8
       for(u64 j = 0; j < n; ++j) s += ainv[i * n + j] * sr[j];
       x[i] = dev::masked(mask, i) ? DT_(0) : DT_(s);
     }
@@ -1012,6 +1014,7 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
         for(u32 i = tid; i < d.ncoarse; i += kTailThreads)
         {
           double s = 0.0;
+#pragma unroll 8
           for(u32 j = 0; j < d.ncoarse; ++j) s += d.ainv[u64(i) * d.ncoarse + j] * double(fv[2 * n + j]);
           fv[i] = dev::masked(f.mask, i) ? DT_(0) : DT_(s);
         }

@@ -83,6 +83,9 @@ namespace
     // shared B x B values whose every block is diagonal are stored as block diagonals (the
     // off-diagonal exact zeros left out, as the transfers' structural zeros are)
     bool diag_blocks = true;
+    // shared values: the levels of at most this many nodes (below the finest) run in one
+    // LDS-resident workgroup per system (batch::k_tail_sys); 0 = every level keeps its launches
+    u64 tail_nodes = 1000;
   };
 
   // drop stored exact zeros of a scalar graph (the transfers' structural zeros)
@@ -296,6 +299,9 @@ namespace
       DevMem arp, acol, aval, prp, pcol, pval, rrp, rcol, rval, mask, dinv, rhs, x, t, def, ord;
       u64 nc = 0;
       bool thinned = false;   // the graph holds fewer entries than the pushed one (thin_graph)
+      // ELL copies of A, P, R for the coarse tail (levels tail0 and below), widths aw, pw, rw
+      DevMem ea_col, ea_val, ep_col, ep_val, er_col, er_val;
+      u32 aw = 0, pw = 0, rw = 0;
       DT* xfinal = nullptr;
       const u32* order() const { return ord.p ? ord.template as<u32>() : nullptr; }
     };
@@ -307,6 +313,10 @@ namespace
     DevMem ainv, a64, r, p, q, st, cr, hist, partials, sums, any;
     DevMem orp, ocol;   // the outer product's graph where level 0's inner graph is thinned
     u64 m0 = 0, hist_cap = 0, ncoarse_rows = 0;
+    // coarse tail: levels tail0..nl-1 run in k_tail_sys when tail0 + 1 < nl (shared values);
+    // tail_lds bytes of LDS hold one system's vectors on them
+    std::size_t tail0 = 0;
+    u64 tail_lds = 0;
     // published system states (batch::k_publish_states): a ring of kRing slots of 64 states in
     // pinned, host-coherent memory (ring_dev: its device address), pub_count the device's count
     static constexpr int kRing = 4;
@@ -464,7 +474,100 @@ namespace
       for(std::size_t i = 0; i + 1 < L.size(); ++i)
         L[i]->xfinal = ((cfg.pre - 1 + cfg.post) % 2 == 0) ? L[i]->x.template as<DT>() : L[i]->t.template as<DT>();
       L.back()->xfinal = L.back()->x.template as<DT>();
+      F3_RC(setup_tail());
       return init_outer(lv[0]);
+    }
+
+    // the deepest run of levels (the coarsest included) whose every level has at most
+    // cfg.tail_nodes nodes and whose vectors fit one workgroup's LDS
+    int setup_tail()
+    {
+      const std::size_t nl = L.size();
+      tail0 = nl - 1;
+      tail_lds = 0;
+      if(!SHARED || cfg.tail_nodes == 0) return rt::kOk;
+      int dev = 0, cap = 65536;
+      if(hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+      auto bytes = [&](std::size_t t0) {
+        u64 b = 0;
+        for(std::size_t i = t0; i < nl; ++i) b += 5 * L[i]->nodes * u64(B) * sizeof(DT);
+        return b;
+      };
+      while(tail0 > 0 && L[tail0 - 1]->nodes <= cfg.tail_nodes && nl - (tail0 - 1) <= std::size_t(batch::kTailSysMax) &&
+            bytes(tail0 - 1) <= u64(cap))
+        --tail0;
+      if(tail0 + 1 >= nl) return rt::kOk;
+      tail_lds = bytes(tail0);
+      for(std::size_t i = tail0; i < nl; ++i)
+      {
+        Level& l = *L[i];
+        F3_RC(tail_ell(l.arp, l.acol, l.aval, l.nodes, u32(svpe()), l.ea_col, l.ea_val, l.aw));
+        if(i + 1 < nl)
+        {
+          F3_RC(tail_ell(l.prp, l.pcol, l.pval, l.nodes, 1u, l.ep_col, l.ep_val, l.pw));
+          F3_RC(tail_ell(l.rrp, l.rcol, l.rval, l.nc, 1u, l.er_col, l.er_val, l.rw));
+        }
+      }
+      F3_CHECK(hipStreamSynchronize(s));
+      if constexpr(SHARED)
+      {
+        if(tail_lds > 65536)
+          F3_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch::k_tail_sys<DT, VT, B, DIAG>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(tail_lds)));
+      }
+      return rt::kOk;
+    }
+
+    // ELL copy (batch::TailSysEll) of a device CSR matrix of `rows` rows with V values per
+    // entry; w = its longest row.  Rebuilt from the CSR values when they change (set_values).
+    int tail_ell(const DevMem& rp, const DevMem& col, const DevMem& val, u64 rows, u32 V, DevMem& ecol, DevMem& eval, u32& w)
+    {
+      std::vector<u32> h(rows + 1);
+      F3_CHECK(hipMemcpyAsync(h.data(), rp.p, sizeof(u32) * (rows + 1), hipMemcpyDeviceToHost, s));
+      F3_CHECK(hipStreamSynchronize(s));
+      w = 0;
+      for(u64 r = 0; r < rows; ++r) w = std::max(w, h[r + 1] - h[r]);
+      if(w == 0) w = 1;
+      if(!ecol.p) F3_RC(ecol.alloc(sizeof(u32) * rows * w));
+      if(!eval.p) F3_RC(eval.alloc(sizeof(VT) * rows * w * V));
+      const u64 n = rows * w;
+      const unsigned nb = unsigned(std::max<u64>(1, std::min<u64>((n + 255) / 256, 65536)));
+      batch::k_tail_ell<VT><<<nb, 256, 0, s>>>(u32(rows), w, V, rp.template as<u32>(), col.template as<u32>(),
+                                               val.template as<VT>(), ecol.template as<u32>(), eval.template as<VT>());
+      return rt::check_launch();
+    }
+
+    // the tail's levels as the kernel reads them (current device pointers: set_values may
+    // replace the values and the coarse inverse)
+    batch::TailSysDesc<DT, VT> tail_desc() const
+    {
+      batch::TailSysDesc<DT, VT> d{};
+      const std::size_t nl = L.size();
+      d.nlev = int(nl - tail0);
+      d.pre = cfg.pre;
+      d.post = cfg.post;
+      d.ncoarse = u32(ncoarse_rows);
+      d.ainv = ainv.template as<double>();
+      d.rhs_in = L[tail0]->rhs.template as<DT>();
+      d.x_out = L[tail0]->xfinal;
+      u64 off = 0;
+      for(std::size_t i = tail0; i < nl; ++i)
+      {
+        const Level& l = *L[i];
+        batch::TailSysLevel<DT, VT>& t = d.lv[i - tail0];
+        t.nodes = u32(l.nodes);
+        t.off = u32(off);
+        off += 5 * l.nodes * u64(B);
+        t.A = {l.arp.template as<u32>(), l.ea_col.template as<u32>(), l.ea_val.template as<VT>(), u32(l.nodes), l.aw};
+        if(i + 1 < nl)
+        {
+          t.P = {l.prp.template as<u32>(), l.ep_col.template as<u32>(), l.ep_val.template as<VT>(), u32(l.nodes), l.pw};
+          t.R = {l.rrp.template as<u32>(), l.er_col.template as<u32>(), l.er_val.template as<VT>(), u32(l.nc), l.rw};
+        }
+        t.mask = l.mask.template as<u32>();
+        t.dinv = l.dinv.template as<DT>();
+      }
+      return d;
     }
 
     // FP64 inverses of the unit-filtered coarse matrices from the coarsest level's values
@@ -592,7 +695,9 @@ namespace
         batch::k_entry<DT, B, SHARED><<<4096, kBlock, 0, s>>>(top.nodes, K, def, top.rhs.template as<DT>(), top.x.template as<DT>(),
                                                              top.dinv.template as<DT>(), a);
       std::vector<DT*> cur(nl, nullptr), oth(nl, nullptr);
-      for(std::size_t i = 0; i + 1 < nl; ++i)
+      // levels above the coarse tail (every level but the coarsest when there is none); the
+      // tail takes its first level's first smoothing step itself
+      for(std::size_t i = 0; i < tail0; ++i)
       {
         Level& f = *L[i];
         Level& c = *L[i + 1];
@@ -600,13 +705,21 @@ namespace
         oth[i] = f.t.template as<DT>();
         for(int k = 1; k < cfg.pre; ++k) { F3_RC(sweep(f, cur[i], oth[i], a)); std::swap(cur[i], oth[i]); }
         F3_RC(defect(f, cur[i], a));
-        F3_RC(restrict_to(f, c, (i + 2 < nl) ? c.x.template as<DT>() : nullptr, a));
+        F3_RC(restrict_to(f, c, (i + 1 < tail0) ? c.x.template as<DT>() : nullptr, a));
       }
-      Level& cl = *L.back();
-      const unsigned cb = unsigned(std::max<u64>(1, (ncoarse_rows * u64(K) + kBlock - 1) / kBlock));
-      batch::k_coarse<DT><<<cb, kBlock, 0, s>>>(ncoarse_rows, K, SHARED ? 0 : 1, ainv.template as<double>(),
-                                               cl.mask.template as<u32>(), B, cl.rhs.template as<DT>(), cl.x.template as<DT>(), a);
-      for(std::size_t ii = nl - 1; ii > 0; --ii)
+      if constexpr(SHARED)
+      {
+        if(tail0 + 1 < nl)
+          batch::k_tail_sys<DT, VT, B, DIAG><<<unsigned(K), batch::kTailSysThreads, tail_lds, s>>>(tail_desc(), K, a);
+      }
+      if(tail0 + 1 >= nl)
+      {
+        Level& cl = *L.back();
+        const unsigned cb = unsigned(std::max<u64>(1, (ncoarse_rows * u64(K) + kBlock - 1) / kBlock));
+        batch::k_coarse<DT><<<cb, kBlock, 0, s>>>(ncoarse_rows, K, SHARED ? 0 : 1, ainv.template as<double>(),
+                                                 cl.mask.template as<u32>(), B, cl.rhs.template as<DT>(), cl.x.template as<DT>(), a);
+      }
+      for(std::size_t ii = tail0; ii > 0; --ii)
       {
         const std::size_t i = ii - 1;
         Level& f = *L[i];
@@ -859,6 +972,8 @@ namespace
                                                    l.mask.template as<u32>(), DT(cfg.omega_j), DT(cfg.omega_r),
                                                    l.dinv.template as<DT>());
       F3_RC(rt::check_launch());
+      if(tail0 + 1 < L.size() && std::size_t(level) >= tail0)
+        F3_RC(tail_ell(l.arp, l.acol, l.aval, l.nodes, u32(svpe()), l.ea_col, l.ea_val, l.aw));
       if(level == 0) F3_CHECK(hipMemcpyAsync(a64.p, dv, sizeof(double) * nval, hipMemcpyDeviceToDevice, s));
       if(level + 1 == int(L.size()))
       {
@@ -1042,6 +1157,7 @@ int feat3_batch_set_option(feat3_batch* h, const char* name, double value)
   const std::string n(name);
   if(n == "tile_rows" && value >= 0) h->cfg.tile_rows = u64(value);
   else if(n == "diag_blocks") h->cfg.diag_blocks = value != 0.0;
+  else if(n == "tail_max_n" && value >= 0) h->cfg.tail_nodes = u64(value);
   else return rt::kErrArg;
   h->eng.reset();
   return rt::kOk;

@@ -440,7 +440,9 @@ int feat3_batch_set_graph(feat3_batch* h, int graph);
  * kernels, feat3_setup_tile_order; default 0 = ascending row order), "diag_blocks" (1, default:
  * shared 3x3 values whose every block has exact zeros off its diagonal, a vector Laplacian's, are
  * stored as block diagonals on the device, the FP64 outer copy included; such a solver then
- * refuses feat3_batch_set_values; 0: full blocks) */
+ * refuses feat3_batch_set_values; 0: full blocks), "tail_max_n" (shared values: the deepest
+ * levels of at most this many nodes each, the coarsest included, run as one LDS-resident
+ * workgroup per system, bit for bit the per-level launches; default 1000, 0 = off) */
 int feat3_batch_set_option(feat3_batch* h, const char* name, double value);
 int feat3_batch_init(feat3_batch* h);
 /* x, b: FP64 device arrays [node][component][system], 16-byte aligned; status, iters: host

@@ -228,6 +228,25 @@ def test_diagonal_block_storage_is_identical(torch, inner, nsys):
     assert np.array_equal(x1, x0)
 
 
+@pytest.mark.parametrize("inner", ["f64", "f32", "f16"])
+@pytest.mark.parametrize("dim,level,block,nsys,tail", [(3, 5, 3, 32, 1000), (3, 4, 3, 5, 1000), (3, 3, 3, 4, 1000),
+                                                        (2, 6, 1, 8, 1000), (2, 5, 1, 3, 2000)])
+def test_coarse_tail_is_identical(torch, inner, dim, level, block, nsys, tail):
+    """Shared values: the levels of at most tail_max_n nodes (3D L5: the levels of 729, 125 and
+    27 nodes; 3D L3 and 2D L5 with 2000: the whole hierarchy, the finest level included)
+    run in one LDS-resident workgroup per system; the solve is bit for bit that of the per-level
+    launches (tail_max_n 0)."""
+    print(f"case coarse tail {inner} {dim}D L{level} B={block} K={nsys} tail {tail}", flush=True)
+    H = pb.hierarchy(dim, 1, level, 1)
+    vals = [pb.block_identity_values(L.A, 3) for L in H.levels] if block == 3 else None
+    B = pb.random_rhs(H.n, block, nsys, H.levels[0].bnd, seed=17)
+    s1, i1, x1, h1 = gpu_batch(torch, H, B, nsys, values=vals, block=block, inner=inner, options={"tail_max_n": tail})
+    s0, i0, x0, h0 = gpu_batch(torch, H, B, nsys, values=vals, block=block, inner=inner, options={"tail_max_n": 0})
+    assert list(s1) == list(s0) and list(i1) == list(i0)
+    assert np.array_equal(x1, x0)
+    assert all(np.array_equal(a, b) for a, b in zip(h1, h0))
+
+
 @pytest.mark.parametrize("inner", ["f64", "f32"])
 def test_coupled_blocks_keep_full_storage(torch, inner):
     """Blocks a_ij * M with a coupling M (SPD, tridiagonal) are not diagonal: the full 3x3 blocks
