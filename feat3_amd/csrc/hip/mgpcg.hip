@@ -1207,9 +1207,12 @@ namespace
     }
     if(m.banded && m.dict)
     {
-      // FP32 levels of 2^20 rows or more: 512 rows per block, two per thread in the interior blocks
-      const bool two = sizeof(DT) == 4 && !m.rows_per_block && m.rows >= (u64(1) << 20);
-      const mg::RowGrid g = two ? mg::row_grid_fixed(m.rows, 256 * F3_DICT_RT) : grid(1);
+      // matrices of 2^20 rows or more: FP32 512 rows per block, two per thread in the interior
+      // blocks; FP64 768 rows per block, three per thread one after another (every thread the same
+      // count: 3D Q1 L7 outer product 40.0 us with row_grid's 320-row blocks, where a quarter of
+      // the threads take a second row, 36.5 with 512, 35.3 with 768, 37.3 with 1024)
+      const bool big = !m.rows_per_block && m.rows >= (u64(1) << 20);
+      const mg::RowGrid g = big ? mg::row_grid_fixed(m.rows, sizeof(DT) == 4 ? 256 * F3_DICT_RT : 768) : grid(1);
       const u32* tile = m.tile_blocks == g.blocks ? m.tile.template as<u32>() : nullptr;
       // unrolled in full for the diagonal counts of Q1 operators (2D: 9; 3D: 27, 21 without the
       // face couplings below eps_f32^2) and of the 2D Q2 operator (25)
@@ -1405,6 +1408,7 @@ namespace
     // the single-workgroup coarse-tail kernel k_tail; 0 = every level keeps its own launches
     u64 tail_max_n = 1100;
     u64 rows_per_block = 0;   // row positions per block of the row kernels (0: by row_grid)
+    u64 outer_rows_per_block = 0;   // the same for the FP64 outer matrix when it is a copy of its own
     // SELL sort window of the transfer matrices: prolongation rows have 1/2/4/8 entries in an
     // alternating pattern, so sorting by length inside windows removes most of the padding
     u64 transfer_sigma = 1;
@@ -1637,6 +1641,7 @@ namespace
           F3_RC(A64own.upload_dev(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, -1.0, cfg.index16, want_banded));
         else
           F3_RC(A64own.upload(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, -1.0, cfg.index16, want_banded));
+        A64own.rows_per_block = cfg.outer_rows_per_block;
         A64 = &A64own;
       }
       // x staged in LDS by the banded kernels: the FP64 outer matrix (1), every banded matrix (2)
@@ -2799,6 +2804,7 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, u64 sell_C, u64
 
 // named options: "index16" (0/1), "compact_transfer" (0/1), "sell_C", "sell_sigma", "lanes",
 // "tail_max_n", "rows_per_block" (level matrices; 0 = derived from the row count),
+// "outer_rows_per_block" (the FP64 outer matrix's copy; 0 = derived),
 // "transfer_sigma" (SELL sort window of P and R), "transfer_csr" (P and R in CSR, default 1)
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
 {
@@ -2821,6 +2827,7 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
   else if(k == "band_stage_x" && (value == 0 || value == 1 || value == 2)) h->cfg.band_stage_x = int(value);
   else if(k == "cycle" && (value == 0 || value == 1 || value == 2)) h->cfg.cycle = int(value);
   else if(k == "rows_per_block" && value >= 0) h->cfg.rows_per_block = u64(value);
+  else if(k == "outer_rows_per_block" && value >= 0) h->cfg.outer_rows_per_block = u64(value);
   else if(k == "transfer_sigma" && value >= 1) h->cfg.transfer_sigma = u64(value);
   else if(k == "sell_sigma" && value >= 1) h->cfg.sell_sigma = u64(value);
   else if(k == "lanes" && (value == 0 || value == 2 || value == 4 || value == 8 || value == 16 || value == 32 || value == 64))

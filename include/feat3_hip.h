@@ -345,7 +345,9 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
  * they fit), "compact_transfer" 0/1 (device P/R without stored structural zeros), "sell_C",
  * "sell_sigma", "lanes", "tail_max_n" (levels of at most this many rows run in one LDS-resident
  * workgroup; 0 = none; default 1100), "rows_per_block" (row positions per workgroup of the
- * level-matrix kernels, a multiple of 64; 0 = derived from the row count), "transfer_csr" (0/1:
+ * level-matrix kernels, a multiple of 64; 0 = derived from the row count),
+ * "outer_rows_per_block" (the same for the outer FP64 matrix when it is a copy of its own),
+ * "transfer_csr" (0/1:
  * transfer matrices in CSR evaluated by streaming, default 1, or in the level format),
  * "transfer_sigma" (SELL sort window of the transfer matrices when they are SELL, default 1),
  * "tensor_transfer" (1, default: transfer pairs of a structured hierarchy in lattice order whose
