@@ -779,7 +779,7 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
     if(stopped(status)) return;
     for_rows(A, per, x, [&](u64 p, DT_ s, bool w) {
       const u64 row = A.row(p);
-      if(w) def[row] = dev::masked(mask, row) ? DT_(0) : rhs[row] - s;
+      if(w) def[row] = dev::masked_nn(mask, row) ? DT_(0) : rhs[row] - s;
     });
   }
 
@@ -1121,7 +1121,7 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
       const u64 row = A.row(p);
       if(w)
       {
-        const double d = dev::masked(mask, row) ? 0.0 : b[row] - s;
+        const double d = dev::masked_nn(mask, row) ? 0.0 : b[row] - s;
         r[row] = d;
         v += d * d;
       }
@@ -1170,7 +1170,7 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
       const u64 row = A.row(pos);
       if(w)
       {
-        const double qi = dev::masked(mask, row) ? 0.0 : s;
+        const double qi = dev::masked_nn(mask, row) ? 0.0 : s;
         q[row] = qi;
         v += qi * p[row];
       }
