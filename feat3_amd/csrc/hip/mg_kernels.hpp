@@ -897,10 +897,11 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
     TailLevel<DT_, VT_> lv[kTailMax];
   };
 
-  // FP32 rows of 8, 16, 24 or 32 entries (the tail ELL pads its width to a multiple of 8 up to
-  // 32) issue every load of the row before the first use: one round trip to the L2 per step
-  // instead of one per group of unrolled entries (the steps of the tail are latency-bound); the
-  // terms are summed in the order of the loop below
+  // FP32 rows of 8, 9, 16, 21, 24, 25, 27 or 32 entries (the tail ELL keeps the widths of the Q1
+  // and Q2 stencils, 9, 21, 25 and 27, and pads others to a multiple of 8 up to 32) issue every
+  // load of the row before the first use: one round trip to the L2 per step instead of one per
+  // group of unrolled entries (the steps of the tail are latency-bound); the terms are summed in
+  // the order of the loop below
   template<u32 W_, typename DT_, typename VT_>
   __device__ __forceinline__ DT_ tail_row_w(const TailEll<VT_>& M, u32 r, const DT_* xs)
   {
@@ -927,8 +928,12 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
       switch(M.w)
       {
         case 8: return tail_row_w<8, DT_, VT_>(M, r, xs);
+        case 9: return tail_row_w<9, DT_, VT_>(M, r, xs);
         case 16: return tail_row_w<16, DT_, VT_>(M, r, xs);
+        case 21: return tail_row_w<21, DT_, VT_>(M, r, xs);
         case 24: return tail_row_w<24, DT_, VT_>(M, r, xs);
+        case 25: return tail_row_w<25, DT_, VT_>(M, r, xs);
+        case 27: return tail_row_w<27, DT_, VT_>(M, r, xs);
         case 32: return tail_row_w<32, DT_, VT_>(M, r, xs);
         default: break;
       }

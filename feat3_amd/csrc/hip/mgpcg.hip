@@ -1705,7 +1705,7 @@ namespace
     // ELL copy for the tail kernel: 16-bit columns, the entries band_keeps(v, row diagonal, thin)
     // (thin = 0: exact zeros dropped; the level matrices take level_thin()), entry k of row r at
     // k * rows + r, padded with column 0 and value 0; with FP32 arithmetic a width of at most 32
-    // is padded to a multiple of 8 (the kernel's fixed-width row sums)
+    // other than 9, 21, 25 or 27 is padded to a multiple of 8 (the kernel's fixed-width row sums)
     int tail_ell(const HostCsr& h, mg::TailEll<VT>& e, double thin)
     {
       std::vector<double> dg(h.rows, 0.0);
@@ -1719,7 +1719,7 @@ namespace
         for(u32 k = h.rp[r]; k < h.rp[r + 1]; ++k) c += band_keeps(h.val[k], dg[r], thin) ? 1 : 0;
         w = std::max(w, c);
       }
-      if(sizeof(DT) == 4 && w <= 32) w = (w + 7) / 8 * 8;
+      if(sizeof(DT) == 4 && w <= 32 && w != 9 && w != 21 && w != 25 && w != 27) w = (w + 7) / 8 * 8;
       std::vector<std::uint16_t> col(w * h.rows, 0);
       std::vector<VT> val(w * h.rows, to_storage<VT>(0.0));
       for(u64 r = 0; r < h.rows; ++r)
