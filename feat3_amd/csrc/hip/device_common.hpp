@@ -161,10 +161,12 @@ namespace dev
   {
     return mask != nullptr && ((mask[i >> 5] >> (i & 31u)) & 1u);
   }
-  // the same for a mask that is never null: without the pointer test the compiler keeps one copy
-  // of a row loop instead of a copy per case, whose scalar state overflowed the SGPRs of the
-  // banded row kernels (spills to VGPR lanes, 119 instead of 76 VGPRs for the FP32 defect)
-  __device__ __forceinline__ bool masked_nn(const std::uint32_t* __restrict__ mask, std::uint64_t i)
+  // the same for a mask that is never null and a 32-bit row (row pointers are 32-bit, so every
+  // row index is): without the pointer test the compiler keeps one copy of a row loop instead of a
+  // copy per case, whose scalar state overflowed the SGPRs of the banded row kernels, and 32-bit
+  // indices keep the epilogue's address arithmetic off 64-bit registers (FP32 value-coded defect
+  // 117 -> 76 VGPRs, FP64 outer product 79 -> 61)
+  __device__ __forceinline__ bool masked_nn(const std::uint32_t* __restrict__ mask, std::uint32_t i)
   {
     return (mask[i >> 5] >> (i & 31u)) & 1u;
   }

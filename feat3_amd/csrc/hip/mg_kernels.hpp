@@ -766,8 +766,8 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
   {
     if(stopped(status)) return;
     for_rows(A, per, xin, [&](u64 p, DT_ s, bool w) {
-      const u64 row = A.row(p);
-      if(w) xout[row] = xin[row] + dinv[row] * (rhs[row] - s);
+      const u32 i = u32(A.row(p));
+      if(w) xout[i] = xin[i] + dinv[i] * (rhs[i] - s);
     });
   }
 
@@ -779,7 +779,14 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
     if(stopped(status)) return;
     for_rows(A, per, x, [&](u64 p, DT_ s, bool w) {
       const u64 row = A.row(p);
-      if(w) def[row] = dev::masked_nn(mask, row) ? DT_(0) : rhs[row] - s;
+      if(w)
+      {
+        // rhs read whatever the mask says: a load in one arm of the selection became a branch
+        // around it, which doubled the live registers of the two-row loop (117 against 76 VGPRs)
+        const u32 i = u32(row);
+        const DT_ d = rhs[i] - s;
+        def[i] = dev::masked_nn(mask, i) ? DT_(0) : d;
+      }
     });
   }
 
@@ -1123,11 +1130,12 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
   {
     double v = 0.0;
     for_rows(A, per, x, [&](u64 p, double s, bool w) {
-      const u64 row = A.row(p);
+      const u32 i = u32(A.row(p));
       if(w)
       {
-        const double d = dev::masked_nn(mask, row) ? 0.0 : b[row] - s;
-        r[row] = d;
+        const double bs = b[i] - s;
+        const double d = dev::masked_nn(mask, i) ? 0.0 : bs;
+        r[i] = d;
         v += d * d;
       }
     });
@@ -1175,9 +1183,10 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
       const u64 row = A.row(pos);
       if(w)
       {
-        const double qi = dev::masked_nn(mask, row) ? 0.0 : s;
-        q[row] = qi;
-        v += qi * p[row];
+        const u32 i = u32(row);
+        const double qi = dev::masked_nn(mask, i) ? 0.0 : s;
+        q[i] = qi;
+        v += qi * p[i];
       }
     });
     dev::block_partial(v, partials);
