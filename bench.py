@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 MEASURED_SWEEP_TRAFFIC = {
     ("poisson3d", 7, "f32", "banded"): (2 * 25667.0 + 8385.5) * 1024,
     ("poisson2d", 9, "f32", "banded"): (2 * 14478.4 + 4104.0) * 1024,
-    ("vector32", 6, "f32", "banded"): (2 * 288774.8 + 102984.4) * 1024,
+    ("vector32", 6, "f32", "banded"): (2 * 288673.2 + 102984.4) * 1024,
 }
 
 
