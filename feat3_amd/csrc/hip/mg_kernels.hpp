@@ -299,7 +299,7 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
 #endif
   constexpr u32 kDictSlots = 16;
   constexpr u32 kDictDiag = 32;   // diagonals at most (4 code words per row)
-  template<typename DT_, typename VT_, int ND_ = 0>
+  template<typename DT_, typename VT_, int ND_ = 0, bool TRIP_ = false>
   struct DictBandRows
   {
     static constexpr int G = 1;
@@ -312,12 +312,32 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
     int omin, omax;    // the smallest and largest offset col - row
     const u32* tile = nullptr;
     const DT_* lt = nullptr;   // the tables in LDS (set by for_rows)
+    // TRIP_: 27 diagonals in nine runs of three consecutive offsets (off_{3g} + 1 = off_{3g+1} =
+    // off_{3g+2} - 1: the 3D Q1 stencil in lattice order), FP64
+    static_assert(!TRIP_ || (ND_ == 27 && sizeof(DT_) == 8), "runs of three: 27 FP64 diagonals");
     __device__ __forceinline__ u64 row(u64 p) const { return p; }
     __device__ __forceinline__ u64 block_of(u64 b) const { return tile ? u64(tile[b]) : b; }
     // x at column c through a 32-bit byte offset (the band's rows are fewer than 2^26)
     __device__ __forceinline__ static DT_ xat(const DT_* __restrict__ x, u32 c)
     {
       return *reinterpret_cast<const DT_*>(reinterpret_cast<const char*>(x) + c * u32(sizeof(DT_)));
+    }
+    // a lane's value from the lane before it (DPP control 0x138, wave_shr:1) or after it (0x130,
+    // wave_shl:1); the first / last lane keeps its own (its caller loads that one itself)
+    template<int CTRL>
+    __device__ __forceinline__ static DT_ lane_shift(DT_ v)
+    {
+      if constexpr(sizeof(DT_) == 8)
+      {
+        const u64 b = __builtin_bit_cast(u64, v);
+        const u32 lo = u32(b), hi = u32(b >> 32);
+        const u32 slo = u32(__builtin_amdgcn_update_dpp(int(lo), int(lo), CTRL, 0xF, 0xF, false));
+        const u32 shi = u32(__builtin_amdgcn_update_dpp(int(hi), int(hi), CTRL, 0xF, 0xF, false));
+        return __builtin_bit_cast(DT_, (u64(shi) << 32) | u64(slo));
+      }
+      else
+        return __builtin_bit_cast(DT_, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v), __builtin_bit_cast(int, v), CTRL,
+                                                                   0xF, 0xF, false));
     }
     __device__ __forceinline__ DT_ term_value(u32 d, u32 code, u32 j) const
     {
@@ -399,13 +419,54 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
       {
         const u32 shift = 1u - u32(rows) - u32(omin);   // co[d] + shift = off_d - omin >= 0
         const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(const_cast<DT_*>(x), 0, -1, 0x00020000);
-#pragma unroll
-        for(int r = 0; r < R; ++r)
+        auto ldx = [&](u32 vo, u32 d) {
+          return __builtin_bit_cast(DT_, __builtin_amdgcn_raw_buffer_load_b64(xr, vo, (co[d] + shift) * 8u, 0));
+        };
+        if constexpr(TRIP_ && R == 1)
         {
-          const u32 vo = (p[r] + u32(omin)) * 8u;
+          {
+            // the lanes of a wave hold consecutive rows, so x at off - 1 and off + 1 of a run is
+            // the middle diagonal's x of the lane before and after: one load per run and two
+            // lane shifts (DPP wave_shr:1 / wave_shl:1) instead of three loads; the first and
+            // last lane, and a lane next to an inactive one, load their own (two branch regions,
+            // issued with the middle loads)
+            const u32 vo = (p[0] + u32(omin)) * 8u;
+            const u32 lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+            const u64 ex = __builtin_amdgcn_read_exec();
+            const bool own_l = lane == 0u || ((ex >> (lane - 1u)) & 1u) == 0u;
+            const bool own_r = lane == 63u || ((ex >> (lane + 1u)) & 1u) == 0u;
+            DT_ xc[9], bl[9], br[9];
 #pragma unroll
-          for(u32 d = 0; d < u32(ND_); ++d)
-            xv[r][d] = __builtin_bit_cast(DT_, __builtin_amdgcn_raw_buffer_load_b64(xr, vo, (co[d] + shift) * 8u, 0));
+            for(u32 g = 0; g < 9u; ++g) { xc[g] = ldx(vo, 3u * g + 1u); bl[g] = DT_(0); br[g] = DT_(0); }
+            if(own_l)
+            {
+#pragma unroll
+              for(u32 g = 0; g < 9u; ++g) bl[g] = ldx(vo, 3u * g);
+            }
+            if(own_r)
+            {
+#pragma unroll
+              for(u32 g = 0; g < 9u; ++g) br[g] = ldx(vo, 3u * g + 2u);
+            }
+#pragma unroll
+            for(u32 g = 0; g < 9u; ++g)
+            {
+              const DT_ sl = lane_shift<0x138>(xc[g]), sn = lane_shift<0x130>(xc[g]);
+              xv[0][3u * g] = own_l ? bl[g] : sl;
+              xv[0][3u * g + 1u] = xc[g];
+              xv[0][3u * g + 2u] = own_r ? br[g] : sn;
+            }
+          }
+        }
+        else
+        {
+#pragma unroll
+          for(int r = 0; r < R; ++r)
+          {
+            const u32 vo = (p[r] + u32(omin)) * 8u;
+#pragma unroll
+            for(u32 d = 0; d < u32(ND_); ++d) xv[r][d] = ldx(vo, d);
+          }
         }
       }
       else
@@ -433,7 +494,7 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
     }
   };
   template<typename T> struct IsDictBand { static constexpr bool value = false; };
-  template<typename DT_, typename VT_, int ND_> struct IsDictBand<DictBandRows<DT_, VT_, ND_>>
+  template<typename DT_, typename VT_, int ND_, bool T_> struct IsDictBand<DictBandRows<DT_, VT_, ND_, T_>>
   {
     static constexpr bool value = true;
   };

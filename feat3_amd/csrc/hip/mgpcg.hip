@@ -397,6 +397,8 @@ namespace
     // largest offset col - row in domin / domax; the band's values are then not kept
     bool allow_dict = true;
     bool dict = false;
+    bool dtrip = false;              // 27 offsets in nine runs of three (DictBandRows::trip)
+    bool allow_lane_shift = true;    // take those runs' outer x from the neighbouring lanes
     u32 dnw = 0;
     int domin = 0, domax = 0;
     DevMem didx, dtab, doff;
@@ -876,6 +878,8 @@ namespace
       F3_CHECK(hipMemcpyAsync(dtab.p, tab.data(), sizeof(VT) * tab.size(), hipMemcpyHostToDevice, s));
       F3_CHECK(hipMemcpyAsync(doff.p, po.data(), sizeof(u32) * po.size(), hipMemcpyHostToDevice, s));
       F3_CHECK(hipStreamSynchronize(s));   // tab and po leave scope
+      dtrip = ndiag == 27;
+      for(u32 g = 0; g < 9 && dtrip; ++g) dtrip = ho[3 * g] + 1 == ho[3 * g + 1] && ho[3 * g + 1] + 1 == ho[3 * g + 2];
       domin = int(std::int64_t(ho.front()) + 1 - std::int64_t(rows));
       domax = int(std::int64_t(ho.back()) + 1 - std::int64_t(rows));
       std::swap(didx.p, ix.p);
@@ -1216,18 +1220,29 @@ namespace
       const u32* tile = m.tile_blocks == g.blocks ? m.tile.template as<u32>() : nullptr;
       // unrolled in full for the diagonal counts of Q1 operators (2D: 9; 3D: 27, 21 without the
       // face couplings below eps_f32^2) and of the 2D Q2 operator (25)
-      auto go = [&](auto nd) {
-        mg::DictBandRows<DT, VT, decltype(nd)::value> A{m.doff.template as<u32>(), m.didx.template as<u32>(),
-                                                        m.dtab.template as<VT>(), m.rows, m.cols, m.ndiag, m.dnw,
-                                                        m.domin, m.domax};
+      auto go = [&](auto nd, auto trip) {
+        mg::DictBandRows<DT, VT, decltype(nd)::value, decltype(trip)::value> A{
+          m.doff.template as<u32>(), m.didx.template as<u32>(), m.dtab.template as<VT>(), m.rows, m.cols, m.ndiag, m.dnw,
+          m.domin, m.domax};
         A.tile = tile;
         f(A, g);
       };
-      if(m.ndiag == 27) go(std::integral_constant<int, 27>());
-      else if(m.ndiag == 25) go(std::integral_constant<int, 25>());
-      else if(m.ndiag == 21) go(std::integral_constant<int, 21>());
-      else if(m.ndiag == 9) go(std::integral_constant<int, 9>());
-      else go(std::integral_constant<int, 0>());
+      const std::false_type plain;
+      if(m.ndiag == 27)
+      {
+        // the FP64 3D Q1 stencil: one x load per run of three diagonals (DictBandRows TRIP_)
+        if constexpr(sizeof(DT) == 8)
+        {
+          if(m.dtrip && m.allow_lane_shift) go(std::integral_constant<int, 27>(), std::true_type());
+          else go(std::integral_constant<int, 27>(), plain);
+        }
+        else
+          go(std::integral_constant<int, 27>(), plain);
+      }
+      else if(m.ndiag == 25) go(std::integral_constant<int, 25>(), plain);
+      else if(m.ndiag == 21) go(std::integral_constant<int, 21>(), plain);
+      else if(m.ndiag == 9) go(std::integral_constant<int, 9>(), plain);
+      else go(std::integral_constant<int, 0>(), plain);
       return rt::kOk;
     }
     if(m.banded)
@@ -1433,6 +1448,9 @@ namespace
     // diagonals (0: always the full band; the results are identical either way)
     bool symmetric_band = true;
     bool band_dict = true;   // value-coded bands (mg::DictBandRows) where every diagonal has few values
+    // FP64 value-coded rows of the 3D Q1 stencil read one x per run of three diagonals and take
+    // the other two from the neighbouring lanes (identical results; 0: three loads)
+    bool lane_shift = true;
     // processing order of the row ranges of banded matrices (DevCsr::make_tiles): 0 ascending,
     // 1 plane strips for the FP64 outer matrix, 2 plane strips for every banded matrix
     int plane_order = 1;
@@ -1547,6 +1565,7 @@ namespace
         l->n = h.A.rows;
         l->A.allow_sym = cfg.symmetric_band;
         l->A.allow_dict = cfg.band_dict;
+        l->A.allow_lane_shift = cfg.lane_shift;
         if(h.A.on_device())
           F3_RC(l->A.upload_dev(h.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, level_thin(), cfg.index16, want_banded));
         else
@@ -1637,6 +1656,7 @@ namespace
       {
         A64own.allow_sym = cfg.symmetric_band;
         A64own.allow_dict = cfg.band_dict;
+        A64own.allow_lane_shift = cfg.lane_shift;
         if(h0.A.on_device())
           F3_RC(A64own.upload_dev(h0.A, s, want_sell, cfg.sell_C, cfg.sell_sigma, cfg.lanes, -1.0, cfg.index16, want_banded));
         else
@@ -2823,6 +2843,7 @@ int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
   else if(k == "transfer_rows_per_block" && value >= 0) h->cfg.transfer_rows_per_block = u64(value);
   else if(k == "symmetric_band") h->cfg.symmetric_band = value != 0.0;
   else if(k == "band_dict") h->cfg.band_dict = value != 0.0;
+  else if(k == "lane_shift") h->cfg.lane_shift = value != 0.0;
   else if(k == "plane_order" && (value == 0 || value == 1 || value == 2)) h->cfg.plane_order = int(value);
   else if(k == "band_stage_x" && (value == 0 || value == 1 || value == 2)) h->cfg.band_stage_x = int(value);
   else if(k == "cycle" && (value == 0 || value == 1 || value == 2)) h->cfg.cycle = int(value);

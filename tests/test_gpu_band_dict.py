@@ -72,3 +72,21 @@ def test_many_distinct_values_keep_the_value_band(torch):
     so, io, _ = S.solve(xo, b, po.params())
     assert st == so == 2 and it == io
     assert rel(x, xo) <= 1e-10
+
+
+@pytest.mark.parametrize("lvl", [5, 6])
+@pytest.mark.parametrize("inner", ["f64", "f32"])
+def test_lane_shifted_runs_are_bitwise_the_loaded_ones(torch, lvl, inner):
+    """FP64 value-coded rows of the 3D Q1 stencil (27 diagonals in nine runs of three) take a
+    run's outer x from the neighbouring lanes (option lane_shift, default 1): the same x values,
+    so the V-cycle and the solve (the FP64 outer product, and with FP64 inner arithmetic every
+    level) are bit for bit those of three loads per run (lane_shift 0)."""
+    print(f"case lane shift 3D Q1 L{lvl} {inner}", flush=True)
+    H = pb.hierarchy(3, 1, lvl, 1)
+    b = pb.filtered_rhs(H, 0)
+    i1, c1, st1, it1, x1 = run(torch, H, b, inner, {"lane_shift": 1})
+    i0, c0, st0, it0, x0 = run(torch, H, b, inner, {"lane_shift": 0})
+    assert i1["banded"] == 3 and i1 == i0
+    assert np.array_equal(c1, c0)
+    assert st1 == st0 == 2 and it1 == it0
+    assert np.array_equal(x1, x0)
