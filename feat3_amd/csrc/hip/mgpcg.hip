@@ -2824,7 +2824,7 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, u64 sell_C, u64
 
 // named options: "index16" (0/1), "compact_transfer" (0/1), "sell_C", "sell_sigma", "lanes",
 // "tail_max_n", "rows_per_block" (level matrices; 0 = derived from the row count),
-// "outer_rows_per_block" (the FP64 outer matrix's copy; 0 = derived),
+// "outer_rows_per_block" (the FP64 outer matrix's copy; 0 = derived), "lane_shift" (0/1),
 // "transfer_sigma" (SELL sort window of P and R), "transfer_csr" (P and R in CSR, default 1)
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value)
 {

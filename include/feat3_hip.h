@@ -368,7 +368,9 @@ int feat3_mgpcg_set_launch(feat3_mgpcg* h, int graph, int lanes, uint64_t sell_C
  * distinct values, bit patterns, is stored as 4-bit codes per row and diagonal into per-diagonal
  * tables of those values, which the row kernels read from LDS: the same values and row sums,
  * bit for bit, with 0.5 instead of 4 (FP32) or 8 (FP64) bytes per entry; 0: the values);
- * takes effect at the next init */
+ * "lane_shift" (1, default: FP64 value-coded rows of 27 diagonals in nine runs of three
+ * consecutive offsets load one x per run and take the other two from the neighbouring lanes,
+ * the same sums bit for bit; 0: three loads); takes effect at the next init */
 int feat3_mgpcg_set_option(feat3_mgpcg* h, const char* name, double value);
 /* init_symbolic + init_numeric (solver/base.hpp:227-260) */
 int feat3_mgpcg_init(feat3_mgpcg* h);
