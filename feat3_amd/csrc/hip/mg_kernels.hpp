@@ -820,16 +820,76 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
     }
   }
 
+  // The row loop of value-coded bands with the epilogue's operands loaded ahead: pre(row) issues
+  // the loads a row's epilogue needs (they do not depend on the row sum) before its code words
+  // and x, and for a thread's first rows before the block stages the value tables, so they are in
+  // flight during the staging and the row sums; f(row, s, pre(row)) then finishes the row.  The
+  // rows and sums are those of for_rows.
+  template<typename Rows_, typename DT_, typename Pre, typename F>
+  __device__ __forceinline__ void for_dict_rows_pre(const Rows_& A, u64 per, const DT_* __restrict__ x, Pre&& pre, F&& f)
+  {
+    static_assert(IsDictBand<Rows_>::value, "value-coded bands only");
+    const u64 b = A.block_of(dev::xcd_remap(blockIdx.x, gridDim.x));
+    const u64 beg = b * per;
+    const u64 end = beg + per < A.rows ? beg + per : A.rows;
+    constexpr int RT = sizeof(DT_) == 4 ? F3_DICT_RT : 1;
+    const bool interior =
+      Rows_::kND > 0 && std::int64_t(beg) + A.omin >= 0 && std::int64_t(end) - 1 + A.omax < std::int64_t(A.cols);
+    const int rt = interior ? RT : 1;   // rows per thread and round
+    typedef decltype(pre(u32(0))) P;
+    P pv[RT];
+    u32 pp[RT];
+    bool has[RT];
+    auto load = [&](u64 p0) {
+#pragma unroll
+      for(int r = 0; r < RT; ++r)
+      {
+        const u64 q = p0 + u64(r) * blockDim.x;
+        has[r] = r < rt && q < end;
+        pp[r] = u32(has[r] ? q : p0);
+        pv[r] = pre(pp[r]);
+      }
+    };
+    u64 p0 = beg + threadIdx.x;
+    if(p0 < end) load(p0);
+    __shared__ DT_ lt[kDictDiag * kDictSlots];
+    for(u32 i = threadIdx.x; i < A.nw * 8u * kDictSlots; i += blockDim.x) lt[i] = dev::ld<DT_>(A.tab, i);
+    __syncthreads();
+    Rows_ B = A;
+    B.lt = lt;
+    for(bool first = true; p0 < end; p0 += u64(rt) * blockDim.x, first = false)
+    {
+      if(!first) load(p0);
+      if(interior)
+      {
+        DT_ sr[RT];
+        B.template row_sums_interior<RT>(pp, x, sr);
+#pragma unroll
+        for(int r = 0; r < RT; ++r)
+          if(has[r]) f(pp[r], sr[r], pv[r]);
+      }
+      else
+        f(pp[0], B.row_sum(pp[0], 0, x), pv[0]);
+    }
+  }
+
   // damped Jacobi step  x_out = x_in + dinv .* (rhs - A x_in)
   template<typename Rows_, typename DT_>
   __global__ __launch_bounds__(256) F3_ROW_ATTR void k_sweep(Rows_ A, u64 per, const DT_* __restrict__ dinv, const DT_* __restrict__ rhs,
                                                  const DT_* __restrict__ xin, DT_* __restrict__ xout, const int* status)
   {
     if(stopped(status)) return;
-    for_rows(A, per, xin, [&](u64 p, DT_ s, bool w) {
-      const u32 i = u32(A.row(p));
-      if(w) xout[i] = xin[i] + dinv[i] * (rhs[i] - s);
-    });
+    if constexpr(IsDictBand<Rows_>::value)
+    {
+      struct Op { DT_ x, d, b; };
+      for_dict_rows_pre(A, per, xin, [&](u32 i) { return Op{xin[i], dinv[i], rhs[i]}; },
+                        [&](u32 i, DT_ s, const Op& o) { xout[i] = o.x + o.d * (o.b - s); });
+    }
+    else
+      for_rows(A, per, xin, [&](u64 p, DT_ s, bool w) {
+        const u32 i = u32(A.row(p));
+        if(w) xout[i] = xin[i] + dinv[i] * (rhs[i] - s);
+      });
   }
 
   // defect  def = filter(rhs - A x)
@@ -838,17 +898,24 @@ _Pragma(F3_STR(unroll F3_BAND_UNROLL))
                                                   const DT_* __restrict__ x, DT_* __restrict__ def, const int* status)
   {
     if(stopped(status)) return;
-    for_rows(A, per, x, [&](u64 p, DT_ s, bool w) {
-      const u64 row = A.row(p);
-      if(w)
-      {
-        // rhs read whatever the mask says: a load in one arm of the selection became a branch
-        // around it, which doubled the live registers of the two-row loop (117 against 76 VGPRs)
-        const u32 i = u32(row);
-        const DT_ d = rhs[i] - s;
-        def[i] = dev::masked_nn(mask, i) ? DT_(0) : d;
-      }
-    });
+    if constexpr(IsDictBand<Rows_>::value)
+    {
+      struct Op { DT_ b; bool m; };
+      for_dict_rows_pre(A, per, x, [&](u32 i) { return Op{rhs[i], dev::masked_nn(mask, i)}; },
+                        [&](u32 i, DT_ s, const Op& o) { def[i] = o.m ? DT_(0) : o.b - s; });
+    }
+    else
+      for_rows(A, per, x, [&](u64 p, DT_ s, bool w) {
+        const u64 row = A.row(p);
+        if(w)
+        {
+          // rhs read whatever the mask says: a load in one arm of the selection became a branch
+          // around it, which doubled the live registers of the two-row loop (117 against 76 VGPRs)
+          const u32 i = u32(row);
+          const DT_ d = rhs[i] - s;
+          def[i] = dev::masked_nn(mask, i) ? DT_(0) : d;
+        }
+      });
   }
 
   // restriction  rhs_c = filter_c(R def_f);  optional first smoothing step x_c = dinv_c .* rhs_c
