@@ -29,9 +29,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 # profiles/solve_kernels_2d_q2_l9.md and profiles/batch_k_sweep_vector32_f32.md hold the commands
 # and raw values (the two scalar workloads' bands are value-coded).  Keyed by (workload, level, inner, format); absent -> null.
 MEASURED_SWEEP_TRAFFIC = {
-    ("poisson3d", 7, "f32", "banded"): (2 * 25667.0 + 8385.5) * 1024,
-    ("poisson2d", 9, "f32", "banded"): (2 * 14478.4 + 4104.0) * 1024,
-    ("vector32", 6, "f32", "banded"): (2 * 288673.2 + 102984.4) * 1024,
+    ("poisson3d", 7, "f32", "banded"): (2 * 25674.4 + 8385.5) * 1024,
+    ("poisson2d", 9, "f32", "banded"): (2 * 14486.4 + 4104.0) * 1024,
+    ("vector32", 6, "f32", "banded"): (2 * 288738.1 + 102984.4) * 1024,
 }
 
 
